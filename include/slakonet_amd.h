@@ -1,0 +1,139 @@
+/* C-ABI of the SlaKoNet MI355X hot path (libslakonet_amd.so, gfx950).
+ *
+ * Plain device pointers, sizes and a HIP stream; no framework types. All
+ * calls are stream-ordered and allocate nothing; callers own every buffer
+ * (including the eigensolver workspace sized by slk_eigh_workspace_bytes).
+ * Return value: 0 on success, a hipError_t value on launch failure, 2 for an
+ * undersized workspace. Numerical failures are reported per matrix in `info`.
+ *
+ * The reference (atomgptlab/slakonet) has no FFI; each entry point below
+ * replaces the Python operator cited beside it, and the ctypes binding in
+ * slakonet_amd/_lib.py is what the reference-side integration would call.
+ */
+#ifndef SLAKONET_AMD_H
+#define SLAKONET_AMD_H
+
+#include <stddef.h>
+#include <hip/hip_runtime_api.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* One structure inside a build batch. Its k-matrices are consecutive in the
+ * output starting at matrix index mat0; each matrix is ld x ld row-major. */
+typedef struct SlkStructDesc {
+  int atom0, n_atoms, n_orb, ld;
+  long long mat0;
+  int k0, nk, t0, nt;
+} SlkStructDesc;
+
+/* One workgroup of the build: rows of atom i, columns of atoms [j0, j1)
+ * (ncol <= 64 columns starting at orbital c0), k-points [kc0, kc0 + nkc). */
+typedef struct SlkBuildWork {
+  int s, i, j0, j1, c0, ncol, kc0, nkc;
+} SlkBuildWork;
+
+/* Device-resident SK tables: rows of 20 doubles (10 H then 10 S integrals in
+ * the order ss-sigma, sp-sigma, sd-sigma, pp-sigma, pp-pi, pd-sigma, pd-pi,
+ * dd-sigma, dd-pi, dd-delta); row g of a pair holds r = (g + 1) * step. */
+typedef struct SlkTableSet {
+  const double* tables;
+  const int* pair_row0;
+  const int* pair_ngrid;
+  const double* pair_step;
+  const int* pair_index;    /* [nspec * nspec] ordered pair -> pair id, -1 if absent */
+  const int* spec_norb;     /* orbitals per species */
+  const int* spec_slot;     /* [nspec * 9] local orbital -> spd slot (s,py,pz,px,dxy,dyz,dz2,dxz,dx2-y2) */
+  const double* spec_onsite;/* [nspec * 3] eps_s, eps_p, eps_d (Hartree) */
+  int nspec;
+  double tail;              /* 1.0 Bohr, PolyInterpU default */
+} SlkTableSet;
+
+typedef struct SlkBuildArgs {
+  SlkTableSet tab;
+  const SlkStructDesc* structs;
+  const SlkBuildWork* work;
+  int n_work;
+  const double* atom_pos;   /* [atoms][3] Bohr */
+  const int* atom_spec;
+  const int* atom_orb;
+  const double* trans_cart; /* [T][3] Bohr */
+  const int* trans_int;     /* [T][3] */
+  const double* kfrac;      /* [K][3] fractional k */
+  double cutoff;            /* 10.0 Bohr */
+  void* H;
+  void* S;
+  int out_f32;              /* 0: complex128, 1: complex64 */
+} SlkBuildArgs;
+
+/* Fused SK interpolation + rotation + Bloch sum + on-site into H(k), S(k).
+ * Replaces hs_matrix(geometry, basis, sk_feed, cutoff=10.0)
+ * (slakonet/slaterkoster.py:103-441), SkfFeed.off_site/on_site
+ * (slakonet/skfeed.py:285-359) and PolyInterpU.__call__
+ * (slakonet/interpolation.py:107-238). */
+int slk_build_hs_launch(const SlkBuildArgs* args, hipStream_t stream);
+
+/* Identity-padding of rows/columns [n_orb, ld) of a structure's matrices
+ * (S = 1, H = pad_h on the padded diagonal). No reference counterpart: used to
+ * bucket structures of uneven size into one batched solve. */
+int slk_pad_launch(const SlkStructDesc* structs_dev, int s, int nk, int n_orb, int ld, void* H, void* S,
+                   double pad_h, int out_f32, hipStream_t stream);
+
+/* Generalised Hermitian eigenproblem H z = e S z for a batch of matrices.
+ * Replaces eighb(h, s, scheme="chol") (slakonet/utils.py:585-816) and the
+ * per-k loop of SimpleDftb.__call__ (slakonet/main.py:165-196).
+ * w: [batch][wld] ascending; V (nullable): eigenvectors in columns;
+ * info: [2 * batch]; info[b] > 0: S not positive definite at that pivot,
+ * info[batch + b] > 0: QL iteration did not converge. */
+size_t slk_eigh_workspace_bytes(int n, int ld, int batch, int vectors);
+int slk_eigh_f64(const void* H, const void* S, int n, int ld, int batch, double* w, int wld, void* V,
+                 void* ws, size_t ws_bytes, int* info, hipStream_t stream);
+/* Same solve, plus elapsed milliseconds of its six stages (potrf, trinv,
+ * reduce, hetrd, tridiagonal solve, back-transform) from HIP events recorded
+ * on `stream`; returns after the last event completed. Measurement aid with
+ * no reference counterpart. */
+int slk_eigh_f64_timed(const void* H, const void* S, int n, int ld, int batch, double* w, int wld, void* V,
+                       void* ws, size_t ws_bytes, int* info, hipStream_t stream, float* stage_ms);
+
+/* Eigenvalue spectrum of one structure for the reductions below. */
+typedef struct SlkSpectrum {
+  long long w0;
+  int nk, nb, wld;
+  int kw0;
+  double nelec;
+} SlkSpectrum;
+
+/* Fermi level per structure into edges[s * slk_edge_fields() + 0].
+ * mode 0: bisection root of N(mu) = Ne; mode 1: the reference's half-step
+ * Newton with its stopping rule. Replaces fermi_search
+ * (slakonet/fermi.py:139-198). */
+int slk_fermi_launch(const double* W, const double* KW, const SlkSpectrum* spec, int n_spec, double kT, int mode,
+                     double tol, int max_iter, double* edges, hipStream_t stream);
+
+/* Band edges / gap statistics given the Fermi level. Replaces
+ * calculate_band_gap (slakonet/main.py:637-707),
+ * calculate_band_structure_properties (slakonet/main.py:1539-1642) and the
+ * get_fermi gap block (slakonet/optim.py:1011-1059). */
+int slk_edges_launch(const double* W, const SlkSpectrum* spec, int n_spec, double metal_ev, double* edges,
+                     hipStream_t stream);
+int slk_edge_fields(void);
+
+typedef struct SlkDosJob {
+  long long w0;
+  int nk, nb, wld, kw0;
+  int edge;        /* structure index into edges for the Fermi shift, -1: none */
+  long long out0;  /* offset of this structure's DOS row */
+} SlkDosJob;
+
+/* Gaussian DOS on a caller-supplied energy grid (eV) and its statistics
+ * [dos_at_fermi, dos_gap_center, trapz total]. Replaces calculate_dos
+ * (slakonet/main.py:709-797) and the DOS block of get_properties_dict
+ * (slakonet/main.py:1735-1801). */
+int slk_dos_launch(const double* W, const double* KW, const SlkDosJob* jobs, int n_jobs, const double* grid, int ne,
+                   double sigma, const double* edges, double* dos, double* stats, hipStream_t stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SLAKONET_AMD_H */

@@ -1,0 +1,157 @@
+"""Fermi level, band gap, DOS and the properties dict (oracle, test infra only).
+
+* fermi_search: the reference's half-step Newton on L(mu) = (N(mu) - Ne)^2
+  (slakonet/fermi.py:139-198): mu0 = mean of all eigenvalues, stop when
+  |dL/dmu| < tol (checked before the update), mu <- mu - L / (dL/dmu + 1e-12),
+  N(mu) = 2 sum_k w_k sum_b f((e_kb - mu)/kT), f = 1/(exp(x)+1). dL/dmu is
+  written the way autograd evaluates it: df/dmu = f^2 exp(x) / kT.
+* band_gap: calculate_band_gap (slakonet/main.py:637-707).
+* band_structure: calculate_band_structure_properties (slakonet/main.py:1539-1642).
+* dos: calculate_dos (slakonet/main.py:709-797), Fermi-shifted, eV.
+* properties: get_properties_dict(include_dos_data=True)
+  (slakonet/main.py:1657-1819) plus the get_fermi block of
+  compute_multi_element_properties (slakonet/optim.py:1011-1059).
+"""
+import numpy as np
+
+H2E = 27.211
+
+
+def fermi_search(eig, n_electrons, k_weights, kT=0.01, tol=1e-6, max_iter=100, trace=None):
+    eig = np.asarray(eig, dtype=np.float64).reshape(-1, np.shape(eig)[-1])
+    w = np.asarray(k_weights, dtype=np.float64).reshape(-1)
+    mu = eig.mean()
+    for it in range(max_iter):
+        x = (eig - mu) / kT
+        ex = np.exp(x)
+        f = 1.0 / (ex + 1.0)
+        total = 2.0 * (f * w[:, None]).sum()
+        loss = (total - n_electrons) ** 2
+        dn = 2.0 * ((f * f * ex / kT) * w[:, None]).sum()
+        grad = 2.0 * (total - n_electrons) * dn
+        if trace is not None:
+            trace.append((mu, loss, grad))
+        if abs(grad) < tol:
+            break
+        mu = mu - loss / (grad + 1e-12)
+    return mu
+
+
+def electron_count(eig, k_weights, mu, kT=0.01):
+    eig = np.asarray(eig, dtype=np.float64)
+    w = np.asarray(k_weights, dtype=np.float64).reshape(-1)
+    f = 1.0 / (np.exp((eig - mu) / kT) + 1.0)
+    return 2.0 * (f * w[:, None]).sum()
+
+
+def band_gap(eig, mu):
+    ev = np.asarray(eig) * H2E
+    fev = mu * H2E
+    if np.any(np.abs(ev - fev) < 1e-3):
+        return dict(gap=0.0, vbm=fev, cbm=fev, direct=False, vbm_kpoint=0, cbm_kpoint=0, metal=True)
+    occ = ev < fev
+    un = ev >= fev
+    vbm = ev[occ].max()
+    cbm = ev[un].min()
+    vk = int(np.argwhere(ev == vbm)[0][0])
+    ck = int(np.argwhere(ev == cbm)[0][0])
+    return dict(gap=cbm - vbm, vbm=vbm, cbm=cbm, direct=(vk == ck), vbm_kpoint=vk,
+                cbm_kpoint=ck, metal=False)
+
+
+def band_structure(eig, mu):
+    e = (np.asarray(eig) - mu) * H2E
+    val = e[e < 0]
+    con = e[e >= 0]
+    vbm = val.max() if len(val) else -np.inf
+    vbw = (val.max() - val.min()) if len(val) else 0.0
+    cbm = con.min() if len(con) else np.inf
+    cbw = (con.max() - con.min()) if len(con) else 0.0
+    gap = cbm - vbm if (np.isfinite(vbm) and np.isfinite(cbm)) else 0.0
+    dg = []
+    for ik in range(e.shape[0]):
+        kb = e[ik]
+        v = kb[kb < 0]
+        c = kb[kb > 0]
+        if len(v) and len(c):
+            dg.append(c.min() - v.max())
+    mdg = min(dg) if dg else np.inf
+    return dict(band_gap=gap, valence_band_width=vbw, conduction_band_width=cbw,
+                minimum_direct_gap=(mdg if np.isfinite(mdg) else None),
+                is_direct_semiconductor=(abs(gap - mdg) < 0.01) if np.isfinite(mdg) else False)
+
+
+def dos(eig, k_weights, mu, energy_range=(-10.0, 10.0), num_points=5000, sigma=0.1,
+        fermi_shift=True):
+    e = np.asarray(eig, dtype=np.float64)
+    if fermi_shift:
+        e = e - mu
+    e = e * H2E
+    grid = np.linspace(energy_range[0], energy_range[1], num_points)
+    w = np.asarray(k_weights, dtype=np.float64).reshape(-1)
+    norm = sigma * np.sqrt(2.0 * np.pi)
+    out = np.zeros(num_points)
+    for ik in range(e.shape[0]):
+        g = np.exp(-0.5 * ((grid[None, :] - e[ik][:, None]) / sigma) ** 2) / norm
+        out += w[ik] * g.sum(0)
+    return grid, out
+
+
+def properties(eig, k_weights, n_electrons, get_fermi=True, fermi_method="bisection"):
+    """Numeric content of the reference properties dict for one structure."""
+    eig = np.asarray(eig, dtype=np.float64)
+    if fermi_method == "reference":
+        mu = fermi_search(eig, n_electrons, k_weights)
+    else:
+        mu = fermi_bisect(eig, n_electrons, k_weights)
+    bg = band_gap(eig, mu)
+    bs = band_structure(eig, mu)
+    grid, d = dos(eig, k_weights, mu)
+    fi = int(np.argmin(np.abs(grid)))
+    region = np.abs(grid) < 2.0
+    gc = grid[region][int(np.argmin(d[region]))]
+    p = {
+        "fermi_energy_eV": mu * H2E,
+        "fermi_energy_Ha": mu,
+        "band_gap_eV": bg["gap"],
+        "vbm_eV": bg["vbm"],
+        "cbm_eV": bg["cbm"],
+        "is_direct_gap": bg["direct"],
+        "vbm_kpoint": bg["vbm_kpoint"],
+        "cbm_kpoint": bg["cbm_kpoint"],
+        "valence_band_width_eV": bs["valence_band_width"],
+        "conduction_band_width_eV": bs["conduction_band_width"],
+        "minimum_direct_gap_eV": bs["minimum_direct_gap"],
+        "is_direct_semiconductor": bs["is_direct_semiconductor"],
+        "nkpoints": eig.shape[0],
+        "nbands": eig.shape[-1],
+        "nelectrons": int(n_electrons),
+        "dos_at_fermi": d[fi],
+        "dos_gap_center_eV": gc,
+        "dos_total_states": np.trapezoid(d, grid),
+        "dos_energy_grid": grid,
+        "dos_values": d,
+    }
+    if get_fermi:
+        occ = eig <= mu
+        vb = np.where(occ, eig, -np.inf).max()
+        cb = np.where(~occ, eig, np.inf).min()
+        p["efermi"] = mu * H2E
+        p["bandgap"] = max(cb - vb, 0.0) * H2E
+    return p
+
+
+def fermi_bisect(eig, n_electrons, k_weights, kT=0.01):
+    """Root of N(mu) = Ne by bisection to full double resolution."""
+    eig = np.asarray(eig, dtype=np.float64)
+    lo = eig.min() - 50 * kT
+    hi = eig.max() + 50 * kT
+    for _ in range(400):
+        mid = 0.5 * (lo + hi)
+        if mid <= lo or mid >= hi:
+            break
+        if electron_count(eig, k_weights, mid, kT) > n_electrons:
+            hi = mid
+        else:
+            lo = mid
+    return 0.5 * (lo + hi)

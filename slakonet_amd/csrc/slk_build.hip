@@ -1,0 +1,452 @@
+// Fused H(k)/S(k) build for the SlaKoNet hot path on MI355X (gfx950).
+//
+// One workgroup owns a tile: the orbital rows of one atom i, the orbital
+// columns of a run of atoms [j0, j1) (<= 64 columns), and a chunk of <= KC
+// k-points of one structure. Inside the workgroup:
+//   1. screen every (j, T) image of the tile's column atoms for
+//      0 < |R_j + T - R_i| < cutoff (deterministic compaction in LDS);
+//   2. interpolate the 20 H/S integrals of both ordered species tables with
+//      8-point Lagrange weights on the uniform grid (stencil/clamp/tail rules of
+//      slakonet/interpolation.py:107-238);
+//   3. form the 9x9 Slater-Koster blocks with the closed-form two-centre table
+//      (s, p=(y,z,x), d=(xy,yz,z2,xz,x2-y2); slakonet/slaterkoster.py:1013-1372),
+//      choosing per element the direct block or the conjugate-transposed block
+//      of the reversed pair exactly as hs_matrix assigns them
+//      (slakonet/slaterkoster.py:207-405);
+//   4. Bloch-sum with exp(2 pi i k.T) into register accumulators;
+//   5. add the on-site diagonal and write the tile rows with coalesced stores.
+// Every output element is written exactly once; zero blocks included.
+#include "slk_common.h"
+#include "slakonet_amd.h"
+
+#define BUILD_THREADS 256
+#define KC 4            // k-points per workgroup
+#define MAXPI 24        // pair-images per LDS batch
+#define SEG 2048        // candidates screened per segment
+#define MAXCOL 64
+
+__constant__ int c_slot_l[9] = {0, 1, 1, 1, 2, 2, 2, 2, 2};
+__constant__ int c_slot_m[9] = {0, 0, 1, 2, 0, 1, 2, 3, 4};
+// first canonical integral of the (l1<=l2) pair
+__device__ __forceinline__ int lpair_base(int l1, int l2) {
+  // (0,0)0 (0,1)1 (0,2)2 (1,1)3 (1,2)5 (2,2)7
+  return l1 == 0 ? l2 : (l1 == 1 ? (l2 == 1 ? 3 : 5) : 7);
+}
+
+#define SQ3 1.7320508075688772
+
+// Two-centre Slater-Koster element for l1 <= l2: orbital a of shell l1 on the
+// row atom, orbital b of shell l2 on the column atom, direction cosines
+// (x, y, z) from row to column atom, V = {sigma, pi, delta}.
+__device__ double sk_elem(int l1, int l2, int a, int b, double x, double y, double z,
+                          const double* V) {
+  if (l1 == 0 && l2 == 0) return V[0];
+  if (l1 == 0 && l2 == 1) {
+    const double c[3] = {y, z, x};
+    return c[b] * V[0];
+  }
+  if (l1 == 0 && l2 == 2) {
+    double g;
+    switch (b) {
+      case 0: g = SQ3 * x * y; break;
+      case 1: g = SQ3 * y * z; break;
+      case 2: g = z * z - 0.5 * (x * x + y * y); break;
+      case 3: g = SQ3 * x * z; break;
+      default: g = 0.5 * SQ3 * (x * x - y * y); break;
+    }
+    return g * V[0];
+  }
+  if (l1 == 1 && l2 == 1) {
+    const double c[3] = {y, z, x};
+    return c[a] * c[b] * (V[0] - V[1]) + (a == b ? V[1] : 0.0);
+  }
+  const double l = x, m = y, n = z;
+  const double l2s = l * l, m2s = m * m, n2s = n * n;
+  const double s = V[0], p = V[1];
+  if (l1 == 1 && l2 == 2) {
+    // a: 0=y 1=z 2=x ; b: 0=xy 1=yz 2=z2 3=xz 4=x2-y2
+    const int pa = (a == 2) ? 0 : (a == 0 ? 1 : 2);  // 0=x 1=y 2=z
+    switch (pa * 5 + b) {
+      case 0: return SQ3 * l2s * m * s + m * (1.0 - 2.0 * l2s) * p;            // x,xy
+      case 1: return SQ3 * l * m * n * s - 2.0 * l * m * n * p;                  // x,yz
+      case 2: return l * (n2s - 0.5 * (l2s + m2s)) * s - SQ3 * l * n2s * p;      // x,z2
+      case 3: return SQ3 * l2s * n * s + n * (1.0 - 2.0 * l2s) * p;            // x,xz
+      case 4: return 0.5 * SQ3 * l * (l2s - m2s) * s + l * (1.0 - l2s + m2s) * p;  // x,x2-y2
+      case 5: return SQ3 * m2s * l * s + l * (1.0 - 2.0 * m2s) * p;            // y,xy
+      case 6: return SQ3 * m2s * n * s + n * (1.0 - 2.0 * m2s) * p;            // y,yz
+      case 7: return m * (n2s - 0.5 * (l2s + m2s)) * s - SQ3 * m * n2s * p;      // y,z2
+      case 8: return SQ3 * l * m * n * s - 2.0 * l * m * n * p;                  // y,xz
+      case 9: return 0.5 * SQ3 * m * (l2s - m2s) * s - m * (1.0 + l2s - m2s) * p;  // y,x2-y2
+      case 10: return SQ3 * l * m * n * s - 2.0 * l * m * n * p;                 // z,xy
+      case 11: return SQ3 * n2s * m * s + m * (1.0 - 2.0 * n2s) * p;           // z,yz
+      case 12: return n * (n2s - 0.5 * (l2s + m2s)) * s + SQ3 * n * (l2s + m2s) * p;  // z,z2
+      case 13: return SQ3 * n2s * l * s + l * (1.0 - 2.0 * n2s) * p;           // z,xz
+      default: return 0.5 * SQ3 * n * (l2s - m2s) * s - n * (l2s - m2s) * p;   // z,x2-y2
+    }
+  }
+  // d-d, symmetric in (a, b)
+  const double d = V[2];
+  int ia = a < b ? a : b, ib = a < b ? b : a;
+  const double lm = l * m, mn = m * n, nl = n * l;
+  const double q = l2s - m2s;                 // l^2 - m^2
+  const double r = n2s - 0.5 * (l2s + m2s);   // n^2 - (l^2+m^2)/2
+  switch (ia * 5 + ib) {
+    case 0: return 3.0 * l2s * m2s * s + (l2s + m2s - 4.0 * l2s * m2s) * p + (n2s + l2s * m2s) * d;
+    case 1: return 3.0 * l * m2s * n * s + nl * (1.0 - 4.0 * m2s) * p + nl * (m2s - 1.0) * d;
+    case 2: return SQ3 * lm * r * s - 2.0 * SQ3 * lm * n2s * p + 0.5 * SQ3 * lm * (1.0 + n2s) * d;
+    case 3: return 3.0 * l2s * m * n * s + mn * (1.0 - 4.0 * l2s) * p + mn * (l2s - 1.0) * d;
+    case 4: return 1.5 * lm * q * s - 2.0 * lm * q * p + 0.5 * lm * q * d;
+    case 6: return 3.0 * m2s * n2s * s + (m2s + n2s - 4.0 * m2s * n2s) * p + (l2s + m2s * n2s) * d;
+    case 7: return SQ3 * mn * r * s + SQ3 * mn * (l2s + m2s - n2s) * p - 0.5 * SQ3 * mn * (l2s + m2s) * d;
+    case 8: return 3.0 * m * n2s * l * s + lm * (1.0 - 4.0 * n2s) * p + lm * (n2s - 1.0) * d;
+    case 9: return 1.5 * mn * q * s - mn * (1.0 + 2.0 * q) * p + mn * (1.0 + 0.5 * q) * d;
+    case 12: return r * r * s + 3.0 * n2s * (l2s + m2s) * p + 0.75 * (l2s + m2s) * (l2s + m2s) * d;
+    case 13: return SQ3 * nl * r * s + SQ3 * nl * (l2s + m2s - n2s) * p - 0.5 * SQ3 * nl * (l2s + m2s) * d;
+    case 14: return 0.5 * SQ3 * q * r * s - SQ3 * n2s * q * p + 0.25 * SQ3 * (1.0 + n2s) * q * d;
+    case 18: return 3.0 * n2s * l2s * s + (n2s + l2s - 4.0 * n2s * l2s) * p + (m2s + n2s * l2s) * d;
+    case 19: return 1.5 * nl * q * s + nl * (1.0 - 2.0 * q) * p - nl * (1.0 - 0.5 * q) * d;
+    default: return 0.75 * q * q * s + (l2s + m2s - q * q) * p + (n2s + 0.25 * q * q) * d;  // 24
+  }
+}
+
+// Interpolation weights: value = sum_m w[m] * table[base + m]; base = -1 -> 0.
+// Follows slakonet/interpolation.py:107-238 (8-point stencil, clamp, tail).
+__device__ void interp_weights(double r, int ngrid, double step, double tail, int* base, double* w) {
+  const int NI = 8;
+  long ind = (long)floor(r / step);
+  if (ind < ngrid) {
+    long il = ind + 5;
+    if (il > ngrid) il = ngrid;
+    if (il < NI + 1) il = NI + 1;
+    *base = (int)(il - NI - 1);
+    const double t = r / step - (double)(il - NI);   // position in stencil units
+    double left[NI], right[NI];
+    left[0] = 1.0;
+    for (int m = 1; m < NI; ++m) left[m] = left[m - 1] * (t - (m - 1));
+    right[NI - 1] = 1.0;
+    for (int m = NI - 2; m >= 0; --m) right[m] = right[m + 1] * (t - (m + 1));
+    // denominators prod_{j != m} (m - j) = (-1)^(7-m) m! (7-m)!
+    const double fact[8] = {1.0, 1.0, 2.0, 6.0, 24.0, 120.0, 720.0, 5040.0};
+    for (int m = 0; m < NI; ++m) {
+      double den = fact[m] * fact[NI - 1 - m] * (((NI - 1 - m) & 1) ? -1.0 : 1.0);
+      w[m] = left[m] * right[m] / den;
+    }
+    return;
+  }
+  const long max_ind = ngrid - 1 + (long)(tail / step);
+  if (ind > max_ind) { *base = -1; return; }
+  // fifth-order tail to zero (DFTB+ poly5ToZero) from y, y', y'' at (ngrid-1)*step
+  *base = ngrid - NI - 1;
+  const double delta = 1e-5;
+  const double rmax = (ngrid - 1) * step + tail;
+  const double dx = -tail;
+  const double xr = (r - rmax) / dx;
+  const double x3 = xr * xr * xr;
+  const double A = ((6.0 * xr - 15.0) * xr + 10.0) * x3;
+  const double B = ((-3.0 * xr + 7.0) * xr - 4.0) * x3;
+  const double C = ((0.5 * xr - 1.0) * xr + 0.5) * x3;
+  double w0[NI], w2[NI];
+  for (int pass = 0; pass < 2; ++pass) {
+    const double t = (NI - 1) + (pass == 0 ? -delta : delta) / step;
+    double* ww = pass == 0 ? w0 : w2;
+    double left[NI], right[NI];
+    left[0] = 1.0;
+    for (int m = 1; m < NI; ++m) left[m] = left[m - 1] * (t - (m - 1));
+    right[NI - 1] = 1.0;
+    for (int m = NI - 2; m >= 0; --m) right[m] = right[m + 1] * (t - (m + 1));
+    const double fact[8] = {1.0, 1.0, 2.0, 6.0, 24.0, 120.0, 720.0, 5040.0};
+    for (int m = 0; m < NI; ++m) {
+      double den = fact[m] * fact[NI - 1 - m] * (((NI - 1 - m) & 1) ? -1.0 : 1.0);
+      ww[m] = left[m] * right[m] / den;
+    }
+  }
+  for (int m = 0; m < NI; ++m) {
+    const double e7 = (m == NI - 1) ? 1.0 : 0.0;
+    const double yp = (w2[m] - w0[m]) / (2.0 * delta);
+    const double ypp = (w2[m] + w0[m] - 2.0 * e7) / (delta * delta);
+    w[m] = A * e7 + B * dx * yp + C * dx * dx * ypp;
+  }
+}
+
+template <typename OUT>
+__device__ __forceinline__ void store_c(OUT* p, double2 v);
+template <>
+__device__ __forceinline__ void store_c<double2>(double2* p, double2 v) { *p = v; }
+template <>
+__device__ __forceinline__ void store_c<float2>(float2* p, double2 v) { *p = make_float2((float)v.x, (float)v.y); }
+
+template <typename OUT>
+__global__ __launch_bounds__(BUILD_THREADS) void slk_build_kernel(SlkBuildArgs A) {
+  __shared__ int s_list[SEG];            // compacted candidates (packed jrel<<20 | t)
+  __shared__ int s_cnt[BUILD_THREADS];
+  __shared__ int s_total;
+  __shared__ int pi_j[MAXPI];
+  __shared__ int pi_t[MAXPI];
+  __shared__ double pi_u[MAXPI][3];
+  __shared__ int pi_base[MAXPI][2];
+  __shared__ double pi_w[MAXPI][2][8];
+  __shared__ double pi_int[MAXPI][2][20];
+  __shared__ double E[MAXPI][2][81];
+  __shared__ double2 ph[MAXPI][KC];
+  __shared__ int col_atom[MAXCOL];
+  __shared__ int col_slot[MAXCOL];
+  __shared__ int col_orb[MAXCOL];
+
+  const SlkBuildWork wk = A.work[blockIdx.x];
+  const SlkStructDesc sd = A.structs[wk.s];
+  const int tid = threadIdx.x;
+  const int ai = sd.atom0 + wk.i;
+  const int si = A.atom_spec[ai];
+  const int norb_i = A.tab.spec_norb[si];
+  const double rix = A.atom_pos[3 * ai], riy = A.atom_pos[3 * ai + 1], riz = A.atom_pos[3 * ai + 2];
+  const int nja = wk.j1 - wk.j0;
+
+  // column map: tile column -> (atom, canonical slot, orbital index)
+  if (tid == 0) {
+    int c = 0;
+    for (int j = wk.j0; j < wk.j1; ++j) {
+      const int aj = sd.atom0 + j;
+      const int sj = A.atom_spec[aj];
+      const int nb = A.tab.spec_norb[sj];
+      for (int o = 0; o < nb && c < MAXCOL; ++o, ++c) {
+        col_atom[c] = j;
+        col_slot[c] = A.tab.spec_slot[sj * 9 + o];
+        col_orb[c] = A.atom_orb[aj] + o;
+      }
+    }
+  }
+
+  const int nel = norb_i * wk.ncol;
+  double2 accH[3][KC], accS[3][KC];
+#pragma unroll
+  for (int q = 0; q < 3; ++q)
+#pragma unroll
+    for (int k = 0; k < KC; ++k) { accH[q][k] = make_double2(0.0, 0.0); accS[q][k] = make_double2(0.0, 0.0); }
+
+  const long long ncand = (long long)nja * sd.nt;
+  for (long long seg0 = 0; seg0 < ncand; seg0 += SEG) {
+    const int segn = (int)((ncand - seg0) < SEG ? (ncand - seg0) : SEG);
+    // each thread screens a contiguous run of candidates -> deterministic order
+    const int per = (segn + BUILD_THREADS - 1) / BUILD_THREADS;
+    const int beg = tid * per;
+    const int end = min(beg + per, segn);
+    int mine = 0;
+    for (int c = beg; c < end; ++c) {
+      const long long g = seg0 + c;
+      const int jr = (int)(g / sd.nt), t = (int)(g % sd.nt);
+      const int aj = sd.atom0 + wk.j0 + jr;
+      const int tt = sd.t0 + t;
+      const double vx = (A.trans_cart[3 * tt] + A.atom_pos[3 * aj]) - rix;
+      const double vy = (A.trans_cart[3 * tt + 1] + A.atom_pos[3 * aj + 1]) - riy;
+      const double vz = (A.trans_cart[3 * tt + 2] + A.atom_pos[3 * aj + 2]) - riz;
+      const double r = sqrt(vx * vx + vy * vy + vz * vz);
+      if (r < A.cutoff && r > 0.0) ++mine;
+    }
+    s_cnt[tid] = mine;
+    __syncthreads();
+    if (tid == 0) {
+      int acc = 0;
+      for (int q = 0; q < BUILD_THREADS; ++q) { int v = s_cnt[q]; s_cnt[q] = acc; acc += v; }
+      s_total = acc;
+    }
+    __syncthreads();
+    int pos = s_cnt[tid];
+    for (int c = beg; c < end; ++c) {
+      const long long g = seg0 + c;
+      const int jr = (int)(g / sd.nt), t = (int)(g % sd.nt);
+      const int aj = sd.atom0 + wk.j0 + jr;
+      const int tt = sd.t0 + t;
+      const double vx = (A.trans_cart[3 * tt] + A.atom_pos[3 * aj]) - rix;
+      const double vy = (A.trans_cart[3 * tt + 1] + A.atom_pos[3 * aj + 1]) - riy;
+      const double vz = (A.trans_cart[3 * tt + 2] + A.atom_pos[3 * aj + 2]) - riz;
+      const double r = sqrt(vx * vx + vy * vy + vz * vz);
+      if (r < A.cutoff && r > 0.0) s_list[pos++] = (jr << 20) | t;
+    }
+    __syncthreads();
+    const int total = s_total;
+
+    for (int b0 = 0; b0 < total; b0 += MAXPI) {
+      const int np = min(MAXPI, total - b0);
+      // (a) geometry + interpolation weights per pair-image and table direction
+      if (tid < 2 * np) {
+        const int p = tid >> 1, dir = tid & 1;
+        const int packed = s_list[b0 + p];
+        const int jr = packed >> 20, t = packed & 0xFFFFF;
+        const int aj = sd.atom0 + wk.j0 + jr;
+        const int tt = sd.t0 + t;
+        const double vx = (A.trans_cart[3 * tt] + A.atom_pos[3 * aj]) - rix;
+        const double vy = (A.trans_cart[3 * tt + 1] + A.atom_pos[3 * aj + 1]) - riy;
+        const double vz = (A.trans_cart[3 * tt + 2] + A.atom_pos[3 * aj + 2]) - riz;
+        const double r = sqrt(vx * vx + vy * vy + vz * vz);
+        const int sj = A.atom_spec[aj];
+        const int pid = dir == 0 ? A.tab.pair_index[si * A.tab.nspec + sj]
+                                 : A.tab.pair_index[sj * A.tab.nspec + si];
+        if (dir == 0) {
+          pi_j[p] = wk.j0 + jr;
+          pi_t[p] = t;
+          pi_u[p][0] = vx / r; pi_u[p][1] = vy / r; pi_u[p][2] = vz / r;
+        }
+        int base = -1;
+        double w[8];
+        if (pid >= 0) {
+          interp_weights(r, A.tab.pair_ngrid[pid], A.tab.pair_step[pid], A.tab.tail, &base, w);
+          if (base >= 0) base += A.tab.pair_row0[pid];
+        }
+        pi_base[p][dir] = base;
+        for (int m = 0; m < 8; ++m) pi_w[p][dir][m] = (base >= 0) ? w[m] : 0.0;
+      }
+      __syncthreads();
+      // (b) integrals
+      for (int task = tid; task < np * 40; task += BUILD_THREADS) {
+        const int p = task / 40, rem = task % 40, dir = rem / 20, ig = rem % 20;
+        const int base = pi_base[p][dir];
+        double v = 0.0;
+        if (base >= 0) {
+          const double* tb = A.tab.tables + (size_t)base * 20 + ig;
+#pragma unroll
+          for (int m = 0; m < 8; ++m) v += pi_w[p][dir][m] * tb[(size_t)m * 20];
+        }
+        pi_int[p][dir][ig] = v;
+      }
+      // (c) phases exp(2 pi i k.T)
+      for (int task = tid; task < np * KC; task += BUILD_THREADS) {
+        const int p = task / KC, kk = task % KC;
+        double2 v = make_double2(0.0, 0.0);
+        if (kk < wk.nkc) {
+          const int kg = sd.k0 + wk.kc0 + kk;
+          const int tt = sd.t0 + pi_t[p];
+          const double arg = 2.0 * (A.kfrac[3 * kg] * A.trans_int[3 * tt] +
+                                    A.kfrac[3 * kg + 1] * A.trans_int[3 * tt + 1] +
+                                    A.kfrac[3 * kg + 2] * A.trans_int[3 * tt + 2]);
+          double sn, cs;
+          sincospi(arg, &sn, &cs);
+          v = make_double2(cs, sn);
+        }
+        ph[p][kk] = v;
+      }
+      __syncthreads();
+      // (d) effective SK blocks (canonical slots) per pair-image
+      for (int task = tid; task < np * 81; task += BUILD_THREADS) {
+        const int p = task / 81, ab = task % 81, a = ab / 9, b = ab % 9;
+        const int la = c_slot_l[a], lb = c_slot_l[b];
+        const int j = pi_j[p];
+        const bool direct = (la < lb) || (la == lb && wk.i <= j);
+        double eh, es;
+        const double ux = pi_u[p][0], uy = pi_u[p][1], uz = pi_u[p][2];
+        if (direct) {
+          const int ib = lpair_base(la, lb);
+          eh = sk_elem(la, lb, c_slot_m[a], c_slot_m[b], ux, uy, uz, &pi_int[p][0][ib]);
+          es = sk_elem(la, lb, c_slot_m[a], c_slot_m[b], ux, uy, uz, &pi_int[p][0][10 + ib]);
+        } else {
+          // conjugate of the reversed pair's block: table (Z_j, Z_i), direction -u
+          const int ib = lpair_base(lb, la);
+          eh = sk_elem(lb, la, c_slot_m[b], c_slot_m[a], -ux, -uy, -uz, &pi_int[p][1][ib]);
+          es = sk_elem(lb, la, c_slot_m[b], c_slot_m[a], -ux, -uy, -uz, &pi_int[p][1][10 + ib]);
+        }
+        E[p][0][ab] = eh;
+        E[p][1][ab] = es;
+      }
+      __syncthreads();
+      // (e) accumulate into the tile
+#pragma unroll
+      for (int q = 0; q < 3; ++q) {
+        const int e = tid + q * BUILD_THREADS;
+        if (e < nel) {
+          const int ml = e / wk.ncol, c = e % wk.ncol;
+          const int a = A.tab.spec_slot[si * 9 + ml];
+          const int j = col_atom[c], b = col_slot[c];
+          for (int p = 0; p < np; ++p) {
+            if (pi_j[p] != j) continue;
+            const double eh = E[p][0][a * 9 + b], es = E[p][1][a * 9 + b];
+#pragma unroll
+            for (int kk = 0; kk < KC; ++kk) {
+              const double2 f = ph[p][kk];
+              accH[q][kk].x += eh * f.x; accH[q][kk].y += eh * f.y;
+              accS[q][kk].x += es * f.x; accS[q][kk].y += es * f.y;
+            }
+          }
+        }
+      }
+      __syncthreads();
+    }
+  }
+
+  // write the tile (+ on-site on the diagonal)
+  OUT* H = reinterpret_cast<OUT*>(A.H);
+  OUT* S = reinterpret_cast<OUT*>(A.S);
+  const int oi = A.atom_orb[ai];
+  const size_t mat_el = (size_t)sd.ld * sd.ld;
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    const int e = tid + q * BUILD_THREADS;
+    if (e < nel) {
+      const int ml = e / wk.ncol, c = e % wk.ncol;
+      const int row = oi + ml, col = col_orb[c];
+      double onh = 0.0, ons = 0.0;
+      if (row == col) {
+        const int a = A.tab.spec_slot[si * 9 + ml];
+        onh = A.tab.spec_onsite[si * 3 + c_slot_l[a]];
+        ons = 1.0;
+      }
+#pragma unroll
+      for (int kk = 0; kk < KC; ++kk) {
+        if (kk < wk.nkc) {
+          const size_t off = (size_t)(sd.mat0 + wk.kc0 + kk) * mat_el + (size_t)row * sd.ld + col;
+          store_c<OUT>(H + off, make_double2(accH[q][kk].x + onh, accH[q][kk].y));
+          store_c<OUT>(S + off, make_double2(accS[q][kk].x + ons, accS[q][kk].y));
+        }
+      }
+    }
+  }
+}
+
+// Padding rows/columns [n_orb, ld) of every k-matrix: zero off-diagonal,
+// H diagonal = pad_h, S diagonal = 1. Keeps padded eigenvalues at pad_h.
+template <typename OUT>
+__global__ void slk_pad_kernel(const SlkStructDesc* structs, int s, void* Hv, void* Sv, double pad_h) {
+  const SlkStructDesc sd = structs[s];
+  const int n = sd.n_orb, ld = sd.ld;
+  OUT* H = reinterpret_cast<OUT*>(Hv);
+  OUT* S = reinterpret_cast<OUT*>(Sv);
+  const size_t mat = (size_t)(sd.mat0 + blockIdx.y) * ld * ld;
+  const long long padcount = (long long)ld * ld - (long long)n * n;
+  for (long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x; idx < padcount;
+       idx += (long long)gridDim.x * blockDim.x) {
+    int r, c;
+    // rows < n: columns [n, ld); rows >= n: all columns
+    const long long top = (long long)n * (ld - n);
+    if (idx < top) { r = (int)(idx / (ld - n)); c = n + (int)(idx % (ld - n)); }
+    else { long long k = idx - top; r = n + (int)(k / ld); c = (int)(k % ld); }
+    const size_t off = mat + (size_t)r * ld + c;
+    double hv = 0.0, sv = 0.0;
+    if (r == c) { hv = pad_h; sv = 1.0; }
+    store_c<OUT>(H + off, make_double2(hv, 0.0));
+    store_c<OUT>(S + off, make_double2(sv, 0.0));
+  }
+}
+
+extern "C" {
+
+int slk_build_hs_launch(const SlkBuildArgs* args, hipStream_t stream) {
+  if (args->n_work <= 0) return 0;
+  if (args->out_f32)
+    hipLaunchKernelGGL(slk_build_kernel<float2>, dim3(args->n_work), dim3(BUILD_THREADS), 0, stream, *args);
+  else
+    hipLaunchKernelGGL(slk_build_kernel<double2>, dim3(args->n_work), dim3(BUILD_THREADS), 0, stream, *args);
+  return (int)hipGetLastError();
+}
+
+int slk_pad_launch(const SlkStructDesc* structs_dev, int s, int nk, int n_orb, int ld, void* H, void* S,
+                   double pad_h, int out_f32, hipStream_t stream) {
+  if (ld <= n_orb || nk <= 0) return 0;
+  const long long padcount = (long long)ld * ld - (long long)n_orb * n_orb;
+  int blocks = (int)((padcount + 255) / 256);
+  if (blocks > 1024) blocks = 1024;
+  if (out_f32)
+    hipLaunchKernelGGL(slk_pad_kernel<float2>, dim3(blocks, nk), dim3(256), 0, stream, structs_dev, s, H, S, pad_h);
+  else
+    hipLaunchKernelGGL(slk_pad_kernel<double2>, dim3(blocks, nk), dim3(256), 0, stream, structs_dev, s, H, S, pad_h);
+  return (int)hipGetLastError();
+}
+
+}  // extern "C"

@@ -1,0 +1,268 @@
+"""Batched device pipeline: plan -> build H(k)/S(k) -> eigensolve -> Fermi/edges/DOS.
+
+A batch holds any number of structures; every k-point of every structure is
+one matrix of the batch. Host work is O(atoms + images + work items) numpy
+bookkeeping packed into a few device uploads; all O(N^2) and O(n^3) work is in
+the HIP kernels of libslakonet_amd.so.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+from dataclasses import dataclass, field
+from typing import List, Optional, Sequence
+
+import numpy as np
+import torch
+
+from . import _lib
+from .geometry import lattice_translations
+from .tables import TableSet
+from .units import FERMI_KT_HA, FERMI_MAX_ITER, FERMI_TOL, HS_CUTOFF_BOHR, METAL_THRESHOLD_EV
+
+KC = 4          # k-points per build workgroup (slk_build.hip)
+MAXCOL = 64     # columns per build workgroup
+PAD_H = 4.0     # Hartree: H diagonal of identity-padded rows (above every band)
+
+STRUCT_DT = np.dtype([("atom0", "<i4"), ("n_atoms", "<i4"), ("n_orb", "<i4"), ("ld", "<i4"), ("mat0", "<i8"),
+                      ("k0", "<i4"), ("nk", "<i4"), ("t0", "<i4"), ("nt", "<i4")])
+WORK_DT = np.dtype([(f, "<i4") for f in ("s", "i", "j0", "j1", "c0", "ncol", "kc0", "nkc")])
+SPEC_DT = np.dtype([("w0", "<i8"), ("nk", "<i4"), ("nb", "<i4"), ("wld", "<i4"), ("kw0", "<i4"), ("nelec", "<f8")])
+DOS_DT = np.dtype({"names": ["w0", "nk", "nb", "wld", "kw0", "edge", "out0"],
+                   "formats": ["<i8", "<i4", "<i4", "<i4", "<i4", "<i4", "<i8"],
+                   "offsets": [0, 8, 12, 16, 20, 24, 32], "itemsize": 40})
+
+EDGE_FIELDS = ["mu", "metal", "vbm", "cbm", "vbm_k", "cbm_k", "s_vbm", "vb_width", "s_cbm", "cb_width",
+               "min_direct_gap", "vb2", "cb2", "n_iter", "n_val", "n_con"]
+
+
+@dataclass
+class StructureInput:
+    z: np.ndarray            # [N] atomic numbers
+    positions: np.ndarray    # [N, 3] Bohr
+    cell: np.ndarray         # [3, 3] Bohr, rows are lattice vectors
+    kpoints: np.ndarray      # [nk, 3] fractional
+    k_weights: np.ndarray    # [nk]
+    nelec: Optional[float] = None
+
+
+@dataclass
+class BuildPlan:
+    n_struct: int
+    n_mat: int
+    ld: int
+    n_orb: List[int]
+    nk: List[int]
+    mat0: List[int]
+    k_weights: torch.Tensor          # [n_mat] device float64
+    nelec: List[float]
+    dev: dict = field(default_factory=dict)
+    n_work: int = 0
+
+
+def _upload(arr: np.ndarray, device) -> torch.Tensor:
+    a = np.ascontiguousarray(arr)
+    return torch.from_numpy(a.view(np.uint8).reshape(-1).copy()).to(device, non_blocking=True)
+
+
+def column_tiles(norb_atoms: np.ndarray):
+    """Runs of consecutive atoms with <= 64 orbitals: (j0, j1, c0, ncol)."""
+    tiles = []
+    j0, c0, acc = 0, 0, 0
+    off = 0
+    for j, no in enumerate(norb_atoms):
+        if acc + no > MAXCOL and acc > 0:
+            tiles.append((j0, j, c0, acc))
+            j0, c0, acc = j, off, 0
+        acc += int(no)
+        off += int(no)
+    if acc > 0:
+        tiles.append((j0, len(norb_atoms), c0, acc))
+    return np.array(tiles, dtype=np.int32).reshape(-1, 4)
+
+
+def plan_build(tables: TableSet, inputs: Sequence[StructureInput], ld: Optional[int] = None,
+               cutoff: float = HS_CUTOFF_BOHR) -> BuildPlan:
+    device = tables.device
+    structs = np.zeros(len(inputs), dtype=STRUCT_DT)
+    pos_l, spec_l, orb_l, tc_l, ti_l, k_l, kw_l, work_l = [], [], [], [], [], [], [], []
+    n_orb, nks, mat0, nelec = [], [], [], []
+    atom0 = t0 = k0 = m0 = 0
+    for s, inp in enumerate(inputs):
+        z = np.asarray(inp.z, dtype=np.int64)
+        tables.check_pairs(z)
+        sp = tables.species_index(z)
+        norb_atoms = tables.norb[sp]
+        orb = np.concatenate([[0], np.cumsum(norb_atoms)[:-1]]).astype(np.int32)
+        n = int(norb_atoms.sum())
+        pos = np.asarray(inp.positions, dtype=np.float64)
+        cell = np.asarray(inp.cell, dtype=np.float64)
+        ti, tcart = lattice_translations(cell, pos, cutoff)
+        kp = np.asarray(inp.kpoints, dtype=np.float64).reshape(-1, 3)
+        nk = len(kp)
+        na = len(z)
+        structs[s] = (atom0, na, n, 0, m0, k0, nk, t0, len(ti))
+        tiles = column_tiles(norb_atoms)
+        nkc = (nk + KC - 1) // KC
+        ii, tt, kk = np.meshgrid(np.arange(na), np.arange(len(tiles)), np.arange(nkc), indexing="ij")
+        ii, tt, kk = ii.ravel(), tt.ravel(), kk.ravel()
+        w = np.zeros(len(ii), dtype=WORK_DT)
+        w["s"] = s
+        w["i"] = ii
+        w["j0"] = tiles[tt, 0]
+        w["j1"] = tiles[tt, 1]
+        w["c0"] = tiles[tt, 2]
+        w["ncol"] = tiles[tt, 3]
+        w["kc0"] = kk * KC
+        w["nkc"] = np.minimum(KC, nk - kk * KC)
+        work_l.append(w)
+        pos_l.append(pos)
+        spec_l.append(sp)
+        orb_l.append(orb)
+        tc_l.append(tcart)
+        ti_l.append(ti)
+        k_l.append(kp)
+        kw_l.append(np.asarray(inp.k_weights, dtype=np.float64).reshape(-1))
+        n_orb.append(n)
+        nks.append(nk)
+        mat0.append(m0)
+        nelec.append(inp.nelec if inp.nelec is not None else tables.n_electrons(z))
+        atom0 += na
+        t0 += len(ti)
+        k0 += nk
+        m0 += nk
+    ld = int(max(n_orb)) if ld is None else int(ld)
+    if ld < max(n_orb):
+        raise ValueError("ld smaller than the largest structure")
+    structs["ld"] = ld
+    work = np.concatenate(work_l)
+    dev = {
+        "structs": _upload(structs, device),
+        "work": _upload(work, device),
+        "atom_pos": torch.from_numpy(np.concatenate(pos_l).reshape(-1)).to(device),
+        "atom_spec": torch.from_numpy(np.concatenate(spec_l).astype(np.int32)).to(device),
+        "atom_orb": torch.from_numpy(np.concatenate(orb_l).astype(np.int32)).to(device),
+        "trans_cart": torch.from_numpy(np.concatenate(tc_l).reshape(-1)).to(device),
+        "trans_int": torch.from_numpy(np.concatenate(ti_l).astype(np.int32).reshape(-1)).to(device),
+        "kfrac": torch.from_numpy(np.concatenate(k_l).reshape(-1)).to(device),
+    }
+    kw = torch.from_numpy(np.concatenate(kw_l)).to(device)
+    return BuildPlan(len(inputs), m0, ld, n_orb, nks, mat0, kw, nelec, dev, len(work))
+
+
+def run_build(tables: TableSet, plan: BuildPlan, out_f32: bool = False, cutoff: float = HS_CUTOFF_BOHR,
+              table_tensor: Optional[torch.Tensor] = None, stream=None):
+    """H, S [n_mat, ld, ld] complex128 (or complex64)."""
+    lib = _lib.lib()
+    dt = torch.complex64 if out_f32 else torch.complex128
+    H = torch.empty(plan.n_mat, plan.ld, plan.ld, dtype=dt, device=tables.device)
+    S = torch.empty_like(H)
+    d = plan.dev
+    args = _lib.SlkBuildArgs(tables.c_struct(table_tensor), d["structs"].data_ptr(), d["work"].data_ptr(),
+                             plan.n_work, d["atom_pos"].data_ptr(), d["atom_spec"].data_ptr(),
+                             d["atom_orb"].data_ptr(), d["trans_cart"].data_ptr(), d["trans_int"].data_ptr(),
+                             d["kfrac"].data_ptr(), float(cutoff), H.data_ptr(), S.data_ptr(), int(out_f32))
+    sh = _lib.stream_handle(stream)
+    _lib.check(lib.slk_build_hs_launch(C.byref(args), sh), "slk_build_hs_launch")
+    for s in range(plan.n_struct):
+        if plan.n_orb[s] < plan.ld:
+            _lib.check(lib.slk_pad_launch(C.c_void_p(d["structs"].data_ptr()), s, plan.nk[s], plan.n_orb[s],
+                                          plan.ld, C.c_void_p(H.data_ptr()), C.c_void_p(S.data_ptr()), PAD_H,
+                                          int(out_f32), sh), "slk_pad_launch")
+    return H, S
+
+
+class EigenFailure(RuntimeError):
+    pass
+
+
+EIGH_STAGES = ["potrf", "trinv", "reduce", "hetrd", "tridiag_solve", "back_transform"]
+
+
+def eigh(H: torch.Tensor, S: torch.Tensor, vectors: bool = False, stream=None, check: bool = True,
+         stage_ms: Optional[list] = None):
+    """Batched generalized eigensolve (FP64). Returns w [batch, ld], V or None, info.
+
+    With ``stage_ms`` (a list) the per-stage milliseconds measured with HIP
+    events on the launch stream are appended (one synchronisation)."""
+    lib = _lib.lib()
+    batch, ld, _ = H.shape
+    n = ld
+    ws_bytes = int(lib.slk_eigh_workspace_bytes(n, ld, batch, int(vectors)))
+    ws = torch.empty(ws_bytes, dtype=torch.uint8, device=H.device)
+    w = torch.empty(batch, ld, dtype=torch.float64, device=H.device)
+    info = torch.zeros(2 * batch, dtype=torch.int32, device=H.device)
+    V = torch.empty_like(H) if vectors else None
+    args = (C.c_void_p(H.data_ptr()), C.c_void_p(S.data_ptr()), n, ld, batch, C.c_void_p(w.data_ptr()), ld,
+            _lib.ptr(V), C.c_void_p(ws.data_ptr()), ws_bytes, C.c_void_p(info.data_ptr()),
+            _lib.stream_handle(stream))
+    if stage_ms is None:
+        rc = lib.slk_eigh_f64(*args)
+    else:
+        ms = (C.c_float * 6)()
+        rc = lib.slk_eigh_f64_timed(*args, ms)
+        stage_ms.append(dict(zip(EIGH_STAGES, list(ms))))
+    if rc == 3:
+        raise NotImplementedError(f"matrix dimension {n} exceeds the solver's LDS-resident tridiagonal limit")
+    _lib.check(rc, "slk_eigh_f64")
+    if check:
+        check_info(info, batch)
+    return w, V, info
+
+
+def check_info(info: torch.Tensor, batch: int, k_offset: int = 0):
+    inf = info.cpu().numpy()
+    bad = np.nonzero(inf[:batch])[0]
+    if len(bad):
+        k = int(bad[0])
+        raise EigenFailure(f"overlap matrix S(k) is not positive definite at k-point index {k + k_offset} "
+                           f"(Cholesky pivot {int(inf[k])})")
+    bad = np.nonzero(inf[batch:2 * batch])[0]
+    if len(bad):
+        raise EigenFailure(f"tridiagonal QL iteration did not converge at k-point index {int(bad[0]) + k_offset}")
+
+
+def spectra(plan: BuildPlan, w_ld: int) -> np.ndarray:
+    sp = np.zeros(plan.n_struct, dtype=SPEC_DT)
+    for s in range(plan.n_struct):
+        sp[s] = (plan.mat0[s] * w_ld, plan.nk[s], plan.n_orb[s], w_ld, plan.mat0[s], plan.nelec[s])
+    return sp
+
+
+def post(plan: BuildPlan, w: torch.Tensor, fermi_mode: int = 0, kT: float = FERMI_KT_HA,
+         dos_grid: Optional[torch.Tensor] = None, sigma: float = 0.1, stream=None):
+    """Fermi level, band edges and (optionally) DOS for every structure of the plan.
+
+    Returns edges [n_struct, F] (device), dos [n_struct, ne] or None, stats [n_struct, 3] or None.
+    """
+    lib = _lib.lib()
+    dev = w.device
+    nf = int(lib.slk_edge_fields())
+    sp = spectra(plan, w.shape[1])
+    spd = _upload(sp, dev)
+    edges = torch.zeros(plan.n_struct, nf, dtype=torch.float64, device=dev)
+    sh = _lib.stream_handle(stream)
+    kw = plan.k_weights
+    _lib.check(lib.slk_fermi_launch(C.c_void_p(w.data_ptr()), C.c_void_p(kw.data_ptr()), C.c_void_p(spd.data_ptr()),
+                                    plan.n_struct, float(kT), int(fermi_mode), FERMI_TOL, FERMI_MAX_ITER,
+                                    C.c_void_p(edges.data_ptr()), sh), "slk_fermi_launch")
+    _lib.check(lib.slk_edges_launch(C.c_void_p(w.data_ptr()), C.c_void_p(spd.data_ptr()), plan.n_struct,
+                                    METAL_THRESHOLD_EV, C.c_void_p(edges.data_ptr()), sh), "slk_edges_launch")
+    dos = stats = None
+    if dos_grid is not None:
+        ne = int(dos_grid.numel())
+        jobs = np.zeros(plan.n_struct, dtype=DOS_DT)
+        for s in range(plan.n_struct):
+            jobs[s] = (plan.mat0[s] * w.shape[1], plan.nk[s], plan.n_orb[s], w.shape[1], plan.mat0[s], s, s * ne)
+        jd = _upload(jobs, dev)
+        dos = torch.empty(plan.n_struct, ne, dtype=torch.float64, device=dev)
+        stats = torch.empty(plan.n_struct, 3, dtype=torch.float64, device=dev)
+        _lib.check(lib.slk_dos_launch(C.c_void_p(w.data_ptr()), C.c_void_p(kw.data_ptr()), C.c_void_p(jd.data_ptr()),
+                                      plan.n_struct, C.c_void_p(dos_grid.data_ptr()), ne, float(sigma),
+                                      C.c_void_p(edges.data_ptr()), C.c_void_p(dos.data_ptr()),
+                                      C.c_void_p(stats.data_ptr()), sh), "slk_dos_launch")
+    return edges, dos, stats
+
+
+def edges_dict(row: np.ndarray) -> dict:
+    return {k: float(v) for k, v in zip(EDGE_FIELDS, row)}

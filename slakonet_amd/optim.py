@@ -1,0 +1,216 @@
+"""Drop-in model API: trainable SK tables and compute_multi_element_properties.
+
+Mirrors slakonet/optim.py:
+* SkfParameterOptimizer (:1356-1443): every table grid value of one ordered
+  species pair is a parameter, h_params / s_params ParameterDicts keyed
+  "l1-l2" with [n_bond, n_grid] tensors; apply_constraints clamps to
+  [c, 2-c] x original with torch.clamp semantics (which, for negative
+  originals where the bounds cross, returns the upper bound -- kept as is);
+* MultiElementSkfParameterOptimizer (:139-1352) with
+  compute_multi_element_properties(geometry, shell_dict, kpoints, klines,
+  phonons, get_fermi, get_energy, get_forces, get_bulk_mod, device,
+  with_eigenvectors) -> (properties, True) (:949-1074).
+The device tables are packed once from the parameters on every call (no
+per-call rebuild of interpolator objects) and gradients flow through the
+hand-written adjoint when the parameters require grad.
+"""
+from __future__ import annotations
+
+import json
+import os
+from pathlib import Path
+from typing import Dict, Optional
+
+import torch
+from torch import nn
+
+from .calculator import SimpleDftb
+from .geometry import Geometry
+from .skf import Skf
+from .tables import TableSet
+from .units import CHEMICAL_SYMBOLS, H2E, SYMBOL_TO_Z
+
+
+class SkfParameterOptimizer(nn.Module):
+    def __init__(self, skf=None, skf_dict: Optional[dict] = None):
+        super().__init__()
+        if skf is None and skf_dict is None:
+            raise ValueError("an Skf or its dict is required (.skf text files: see slakonet_amd.skf_io)")
+        self.skf = skf if skf is not None else Skf.from_dict(skf_dict)
+        self.skf_dict = self.skf.to_dict()
+        self.original_h_params = {}
+        self.original_s_params = {}
+        h, s = {}, {}
+        for (l1, l2), v in self.skf.hamiltonian.items():
+            key = f"{l1}-{l2}"
+            t = v.detach().clone().to(torch.float64)
+            self.original_h_params[key] = t.clone()
+            h[key] = nn.Parameter(t)
+        for (l1, l2), v in self.skf.overlap.items():
+            key = f"{l1}-{l2}"
+            t = v.detach().clone().to(torch.float64)
+            self.original_s_params[key] = t.clone()
+            s[key] = nn.Parameter(t)
+        self.h_params = nn.ParameterDict(h)
+        self.s_params = nn.ParameterDict(s)
+        self.grid = self.skf_dict.get("grid")
+        self.atomic_data = self.skf_dict.get("atomic_data")
+        self.atom_pair = self.skf_dict.get("atom_pair")
+        self.hs_cutoff = self.skf_dict.get("hs_cutoff")
+
+    def apply_constraints(self, c=(0.9, 0.7, 0.95, 0.9)):
+        """Clamp tables around their originals (slakonet/optim.py:1392-1434)."""
+        with torch.no_grad():
+            for key, p in self.h_params.items():
+                o = self.original_h_params[key].to(p.device)
+                cc = c[0] if key.split("-")[0] == key.split("-")[1] else c[1]
+                p.data = torch.clamp(p.data, o * cc, o * (1 + (1 - cc)))
+            for key, p in self.s_params.items():
+                o = self.original_s_params[key].to(p.device)
+                if key.split("-")[0] == key.split("-")[1]:
+                    p.data = torch.clamp(p.data, torch.maximum(o * c[2], torch.tensor(0.1, device=p.device)),
+                                         o * (1 + (1 - c[2])))
+                else:
+                    p.data = torch.clamp(p.data, o * c[3], o * (1 + (1 - c[3])))
+
+    def overrides(self):
+        h = {tuple(int(x) for x in k.split("-")): v for k, v in self.h_params.items()}
+        s = {tuple(int(x) for x in k.split("-")): v for k, v in self.s_params.items()}
+        return h, s
+
+    def get_updated_skf(self):
+        d = dict(self.skf_dict)
+        d["hamiltonian"] = {k: v for k, v in self.h_params.items()}
+        d["overlap"] = {k: v for k, v in self.s_params.items()}
+        return Skf.from_dict(d)
+
+
+def _pair_key(za, zb):
+    return f"{CHEMICAL_SYMBOLS[za]}-{CHEMICAL_SYMBOLS[zb]}"
+
+
+class MultiElementSkfParameterOptimizer(nn.Module):
+    """Universal SK parameter model over ordered element pairs."""
+
+    def __init__(self, skf_directory=None, geometry=None, vasprun_path=None, available_skf_pairs=None,
+                 universal_params_file=None, elements_in_system=("Si", "C"), skf_dicts=None, device=None):
+        super().__init__()
+        self.skf_directory = skf_directory
+        self.skf_optimizers = nn.ModuleDict()
+        self.atomic_num_to_symbol = {z: CHEMICAL_SYMBOLS[z] for z in range(1, 100)}
+        self.elements_in_system = set(elements_in_system)
+        self._device = torch.device(device) if device is not None else None
+        self._tables = None
+        if skf_dicts is None and skf_directory is not None:
+            uni = universal_params_file or os.path.join(skf_directory, "universal_initial_params.pt")
+            if os.path.exists(uni):
+                skf_dicts = _load_universal(uni)
+            else:
+                from .skf_io import read_skf_directory
+                skf_dicts = read_skf_directory(skf_directory, elements_in_system)
+        if skf_dicts is None:
+            raise ValueError("provide skf_dicts={(Za, Zb): Skf dict} or an skf_directory")
+        for (za, zb), d in sorted(skf_dicts.items()):
+            self.skf_optimizers[_pair_key(int(za), int(zb))] = SkfParameterOptimizer(
+                skf=d if isinstance(d, Skf) else None, skf_dict=None if isinstance(d, Skf) else d)
+        self.element_pairs = {tuple(sorted(k.split("-"))) for k in self.skf_optimizers.keys()}
+        self.elements_in_system = {e for k in self.skf_optimizers.keys() for e in k.split("-")}
+
+    # ------------------------------------------------------------------
+    def _pairs(self):
+        out = {}
+        for key, opt in self.skf_optimizers.items():
+            a, b = key.split("-")
+            out[(SYMBOL_TO_Z[a], SYMBOL_TO_Z[b])] = opt
+        return out
+
+    def get_updated_skfs(self):
+        return {k: opt.get_updated_skf() for k, opt in self.skf_optimizers.items()}
+
+    def apply_constraints(self):
+        for opt in self.skf_optimizers.values():
+            opt.apply_constraints()
+
+    def get_system_elements(self):
+        return sorted(self.elements_in_system)
+
+    def table_set(self, shell_dict, device):
+        """Device tables packed from the current parameters (differentiable)."""
+        pairs = self._pairs()
+        key = (tuple(sorted((int(z), tuple(v)) for z, v in shell_dict.items() if any(z in p for p in pairs))),
+               str(device))
+        if self._tables is None or self._tables_key != key:
+            skfs = {p: opt.skf for p, opt in pairs.items()}
+            zs = {z for p in pairs for z in p}
+            self._tables = TableSet(skfs, {z: shell_dict[z] for z in zs if z in shell_dict}, device=device)
+            self._tables_key = key
+        h = {p: opt.overrides()[0] for p, opt in pairs.items()}
+        s = {p: opt.overrides()[1] for p, opt in pairs.items()}
+        self._tables.update(h, s)
+        return self._tables
+
+    def compute_multi_element_properties(self, geometry=None, shell_dict=None, kpoints=None, klines=None,
+                                         phonons=False, get_fermi=False, get_energy=False, get_forces=False,
+                                         get_bulk_mod=False, device=None, with_eigenvectors=False,
+                                         fermi_method: str = "bisection", kpoints_frac=None, k_weights=None):
+        """Properties of one structure (slakonet/optim.py:949-1074); returns (dict, True)."""
+        if phonons or get_forces or get_bulk_mod:
+            raise NotImplementedError("phonons, forces and bulk modulus are outside the hot path")
+        if device is None:
+            device = "cuda"
+        dev = torch.device(device)
+        if shell_dict is None:
+            from .units import generate_shell_dict_upto_Z65
+            shell_dict = generate_shell_dict_upto_Z65()
+        tables = self.table_set(shell_dict, dev)
+        grad = torch.is_grad_enabled() and any(p.requires_grad for p in self.parameters())
+        calc = SimpleDftb(geometry, shell_dict=shell_dict, tables=tables, kpoints=kpoints, klines=klines,
+                          device=dev, with_eigenvectors=with_eigenvectors or grad, kpoints_frac=kpoints_frac,
+                          k_weights=k_weights, fermi_method=fermi_method,
+                          table_tensor=tables.tables.detach())
+        eigenvalues = calc()
+        if grad:
+            from .adjoint import attach_table_gradient
+            eigenvalues = attach_table_gradient(calc, tables)
+            calc.eigenvalue = eigenvalues
+        properties = calc.get_properties_dict(include_bulk_modulus=False, include_dos_data=True)
+        if grad:
+            from .adjoint import differentiable_properties
+            properties.update(differentiable_properties(calc, eigenvalues))
+        if get_fermi:
+            mu = calc.get_fermi_energy()
+            ef = mu.view(-1, 1, 1)
+            occ = eigenvalues <= ef
+            vbm = torch.where(occ, eigenvalues, torch.full_like(eigenvalues, float("-inf")))
+            cbm = torch.where(~occ, eigenvalues, torch.full_like(eigenvalues, float("inf")))
+            vbm_val = vbm.max(dim=-1)[0].max(dim=-1)[0]
+            cbm_val = cbm.min(dim=-1)[0].min(dim=-1)[0]
+            properties["efermi"] = ef * H2E
+            properties["bandgap"] = (cbm_val - vbm_val).clamp(min=0.0) * H2E
+            properties["calc"] = calc
+        if get_energy:
+            properties["total_energy"] = calc._calculate_electronic_energy()
+        properties["eigenvalues"] = eigenvalues
+        return properties, True
+
+    # ------------------------------------------------------------------ I/O
+    def save_model(self, save_path, method="compact"):
+        from .model_io import save_model
+        return save_model(self, save_path, method)
+
+    @classmethod
+    def load_model(cls, load_path, method="compact", skf_directory=None):
+        from .model_io import load_model
+        return load_model(cls, load_path, method)
+
+
+def _load_universal(path):
+    data = torch.load(path, map_location="cpu", weights_only=True)
+    out = {}
+    for pair_key, entry in data.items():
+        d = dict(entry["skf_dict"])
+        d["hamiltonian"] = entry["h_params"]
+        d["overlap"] = entry["s_params"]
+        za, zb = (SYMBOL_TO_Z[x] for x in pair_key.split("-"))
+        out[(za, zb)] = d
+    return out

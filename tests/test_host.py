@@ -1,0 +1,117 @@
+"""Host-side logic and the C-ABI library surface (no GPU calls)."""
+import os
+import re
+
+import numpy as np
+import pytest
+
+from slakonet_amd import engine, synth
+from slakonet_amd.geometry import Geometry, klines_points, lattice_translations, monkhorst_pack
+from slakonet_amd.units import AA_BOHR
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def header_functions():
+    txt = open(os.path.join(ROOT, "include", "slakonet_amd.h")).read()
+    return sorted(set(re.findall(r"^\s*(?:int|size_t|void)\s+(slk_\w+)\s*\(", txt, re.M)))
+
+
+def test_library_exports_every_header_symbol():
+    from slakonet_amd import _lib
+    lib = _lib.load_library()
+    names = header_functions()
+    assert len(names) >= 8
+    for n in names:
+        assert hasattr(lib, n), n
+    assert set(names) == set(_lib.SIGNATURES), "binding signatures must cover the header exactly"
+
+
+def test_ctypes_struct_sizes_match_numpy_layouts():
+    import ctypes as C
+    from slakonet_amd import _lib
+    assert C.sizeof(_lib.SlkStructDesc) == engine.STRUCT_DT.itemsize == 40
+    assert C.sizeof(_lib.SlkBuildWork) == engine.WORK_DT.itemsize == 32
+    assert C.sizeof(_lib.SlkSpectrum) == engine.SPEC_DT.itemsize == 32
+    assert C.sizeof(_lib.SlkDosJob) == engine.DOS_DT.itemsize == 40
+
+
+def test_monkhorst_pack_matches_reference_rule():
+    k, w = monkhorst_pack([2, 3, 1])
+    assert k.shape == (6, 3) and np.allclose(w, 1 / 6)
+    # (2i+1)/(2N), x slowest, z fastest; [1,1,1] -> (1/2,1/2,1/2)
+    np.testing.assert_allclose(k[0], [0.25, 1 / 6, 0.5])
+    np.testing.assert_allclose(k[1], [0.25, 0.5, 0.5])
+    np.testing.assert_allclose(k[3], [0.75, 1 / 6, 0.5])
+    np.testing.assert_allclose(monkhorst_pack([1, 1, 1])[0], [[0.5, 0.5, 0.5]])
+
+
+def test_klines_inclusive_endpoints_and_error_on_single_point():
+    k, w = klines_points([[0, 0, 0, 0.5, 0, 0, 3], [0.5, 0, 0, 0.5, 0.5, 0, 2]])
+    np.testing.assert_allclose(k, [[0, 0, 0], [0.25, 0, 0], [0.5, 0, 0], [0.5, 0, 0], [0.5, 0.5, 0]])
+    assert np.allclose(w, 0.2)
+    with pytest.raises(ValueError):
+        klines_points([[0, 0, 0, 1, 1, 1, 1]])
+
+
+def test_translations_cover_all_pairs_within_cutoff():
+    rng = np.random.default_rng(0)
+    z, pos, cell = synth.random_cell(rng, 6, [6, 14], 1.8)
+    pb, cb = pos * AA_BOHR, cell * AA_BOHR
+    ti, tc = lattice_translations(cb, pb, 10.0)
+    got = set()
+    for a in range(6):
+        for b in range(6):
+            d = np.linalg.norm(tc + pb[b] - pb[a], axis=1)
+            for t in np.nonzero((d < 10.0) & (d > 0))[0]:
+                got.add((a, b, tuple(ti[t])))
+    # brute force over a much larger box
+    r = 8
+    ax = np.arange(-r, r + 1)
+    big = np.stack(np.meshgrid(ax, ax, ax, indexing="ij"), -1).reshape(-1, 3)
+    bc = big @ cb
+    ref = set()
+    for a in range(6):
+        for b in range(6):
+            d = np.linalg.norm(bc + pb[b] - pb[a], axis=1)
+            for t in np.nonzero((d < 10.0) & (d > 0))[0]:
+                ref.add((a, b, tuple(big[t])))
+    assert got == ref
+
+
+def test_geometry_units_and_fractional():
+    cell = np.eye(3) * 5.0
+    g = Geometry([14, 14], [[0.0, 0, 0], [0.5, 0.5, 0.5]], cell=cell, frac=[1, 1], units="angstrom")
+    np.testing.assert_allclose(g.positions_np[1], [2.5 * AA_BOHR] * 3)
+    np.testing.assert_allclose(g.cell_np, cell * AA_BOHR)
+
+
+def test_column_tiles():
+    t = engine.column_tiles(np.array([9] * 10 + [4, 1]))
+    assert (t[:, 3] <= 64).all() and t[0].tolist() == [0, 7, 0, 63]
+    assert t[:, 3].sum() == 95 and t[-1, 1] == 12
+
+
+def test_synth_tables_deterministic_and_symmetric():
+    a = synth.make_skf_set([6, 14], seed=3)
+    b = synth.make_skf_set([6, 14], seed=3)
+    np.testing.assert_array_equal(a[(6, 14)]["hamiltonian"]["1-2"], b[(6, 14)]["hamiltonian"]["1-2"])
+    # l1 == l2 integrals symmetric in the pair, l1 < l2 independent
+    np.testing.assert_array_equal(a[(6, 14)]["hamiltonian"]["1-1"], a[(14, 6)]["hamiltonian"]["1-1"])
+    assert not np.array_equal(a[(6, 14)]["hamiltonian"]["0-1"], a[(14, 6)]["hamiltonian"]["0-1"])
+    assert a[(6, 6)]["hamiltonian"]["2-2"].shape == (3, 661)
+
+
+def test_stream_structures_respect_min_distance():
+    for i in range(5):
+        st = synth.stream_structure(7, i)
+        pos, cell = st.positions, st.cell
+        inv = np.linalg.inv(cell)
+        for a in range(len(pos)):
+            d = pos - pos[a]
+            f = d @ inv
+            f -= np.round(f)
+            dd = np.linalg.norm(f @ cell, axis=1)
+            dd[a] = 99
+            assert dd.min() >= 1.8 - 1e-9
+        assert 2 <= st.n_atoms <= 256

@@ -1,0 +1,212 @@
+"""Pins of the CPU oracle against what the mathematics fixes (no GPU).
+
+Each test is chosen so that a plausible restatement mistake fails it: a wrong
+stencil row, a swapped sign in a rotation, a dropped conjugate in the
+Hermitian assembly, a missing k-weight or spin factor.
+"""
+import numpy as np
+import pytest
+import scipy.linalg
+
+from oracle import build, interp, post, skrot, solve
+from sk_textbook import block as sk_block
+from slakonet_amd import synth
+from slakonet_amd.units import AA_BOHR
+
+LP = [(0, 0), (0, 1), (0, 2), (1, 1), (1, 2), (2, 2)]
+
+
+# ---------------------------------------------------------------- interpolation
+def test_interp_reproduces_degree7_polynomials():
+    rng = np.random.default_rng(0)
+    step, n = 0.02, 300
+    x = step * (np.arange(n) + 1)
+    coef = rng.normal(size=8)
+    yy = np.polyval(coef, x)[:, None]
+    r = rng.uniform(0.01, (n - 1) * step, 500)   # includes extrapolation below the first point
+    got = interp.poly_interp_u(yy, step, r)[:, 0]
+    np.testing.assert_allclose(got, np.polyval(coef, r), rtol=1e-9, atol=1e-9)
+
+
+def test_interp_uses_rows_up_to_second_last_and_clamps():
+    # the clamp ind_last <= n_grid means the last table row is never read
+    step, n = 0.1, 40
+    yy = np.zeros((n, 1))
+    yy[-1] = 1e6
+    r = np.linspace(0.05, (n - 1) * step - 1e-9, 100)
+    assert np.abs(interp.poly_interp_u(yy, step, r)).max() == 0.0
+    # a change in row n-2 is seen near the end of the table
+    yy2 = np.zeros((n, 1))
+    yy2[-2] = 1.0
+    assert abs(interp.poly_interp_u(yy2, step, [(n - 1) * step])[0, 0] - 1.0) < 1e-12
+
+
+def test_interp_exponential_table_accuracy():
+    x = synth.GRID
+    yy = np.exp(-1.3 * x * 0.529177249)[:, None]
+    r = np.random.default_rng(1).uniform(0.5, 12.0, 1000)
+    got = interp.poly_interp_u(yy, synth.GRID_STEP, r)[:, 0]
+    np.testing.assert_allclose(got, np.exp(-1.3 * r * 0.529177249), rtol=1e-11)
+
+
+def test_interp_tail_endpoints():
+    step, n = 0.02, 400
+    x = step * (np.arange(n) + 1)
+    yy = np.exp(-x)[:, None]
+    tail = 1.0
+    rmax = (n - 1) * step + tail
+    # tail region starts at ind == n_grid; value -> 0 at r_max and beyond -> 0
+    r_in = np.array([n * step + 1e-9, rmax - 1e-9, rmax + 0.5])
+    got = interp.poly_interp_u(yy, step, r_in)[:, 0]
+    assert abs(got[1]) < 1e-12 and got[2] == 0.0
+    # continuity of the tail polynomial with the table value at (n-1)*step
+    y1 = yy[n - 2, 0]
+    val = interp.poly5_zero(np.array([y1]), np.array([0.0]), np.array([0.0]), np.array([-tail]), -tail)
+    assert abs(val[0] - y1) < 1e-15
+
+
+# ---------------------------------------------------------------- SK rotation
+@pytest.mark.parametrize("lp", LP)
+def test_rotation_matches_textbook_sk_table(lp):
+    l1, l2 = lp
+    rng = np.random.default_rng(2)
+    for _ in range(200):
+        u = rng.normal(size=3)
+        u /= np.linalg.norm(u)
+        v = rng.normal(size=l1 + 1)
+        a = skrot.sub_block_rot(l1, l2, u[None], v[None])[0]
+        np.testing.assert_allclose(a, sk_block(l1, l2, u, v), atol=1e-13)
+
+
+def test_rotation_axis_blocks():
+    # bond along z: p_z-p_z is sigma, p_x-p_x is pi; s-d_z2 is sigma
+    u = np.array([[0.0, 0.0, 1.0]])
+    b = skrot.sub_block_rot(1, 1, u, np.array([[2.0, 0.5]]))[0]
+    np.testing.assert_allclose(np.diag(b), [0.5, 2.0, 0.5], atol=1e-15)
+    b = skrot.sub_block_rot(0, 2, u, np.array([[3.0]]))[0]
+    np.testing.assert_allclose(b[0], [0, 0, 3.0, 0, 0], atol=1e-15)
+
+
+@pytest.mark.parametrize("l", [1, 2])
+def test_rotation_matrices_orthogonal(l):
+    rng = np.random.default_rng(3)
+    u = rng.normal(size=(50, 3))
+    u /= np.linalg.norm(u, axis=1, keepdims=True)
+    for rot in ((skrot.rot_p_yz, skrot.rot_p_xy) if l == 1 else (skrot.rot_d_yz, skrot.rot_d_xy)):
+        r = rot(u)
+        np.testing.assert_allclose(np.einsum("mij,mkj->mik", r, r), np.broadcast_to(np.eye(2 * l + 1), r.shape),
+                                   atol=1e-12)
+
+
+# ---------------------------------------------------------------- build
+@pytest.fixture(scope="module")
+def diamond():
+    st = synth.diamond_si()
+    skfs = synth.make_skf_set([14], seed=0)
+    k = st.kpoints[:40]
+    hs = build.build_hs(st.z, st.positions * AA_BOHR, st.cell * AA_BOHR, synth.shell_dict_spd([14]), skfs, k)
+    return st, skfs, k, hs
+
+
+def test_build_hermitian_and_pd(diamond):
+    _, _, _, hs = diamond
+    for m in (hs["H"], hs["S"]):
+        np.testing.assert_allclose(m, np.conj(m.transpose(0, 2, 1)), atol=1e-15)
+    assert min(np.linalg.eigvalsh(s).min() for s in hs["S"]) > 0.5
+
+
+def test_build_periodic_in_k_and_real_at_gamma():
+    st = synth.diamond_si()
+    skfs = synth.make_skf_set([14], seed=0)
+    k = np.array([[0.1, 0.2, 0.3], [1.1, 0.2, -0.7], [0.0, 0.0, 0.0]])
+    hs = build.build_hs(st.z, st.positions * AA_BOHR, st.cell * AA_BOHR, synth.shell_dict_spd([14]), skfs, k)
+    np.testing.assert_allclose(hs["H"][0], hs["H"][1], atol=1e-13)
+    assert np.abs(hs["H"][2].imag).max() < 1e-15
+
+
+def test_build_dimer_block_equals_sk_table():
+    """Two atoms 2.1 A apart in a 30 A cube: the off-diagonal block is the
+    textbook SK block of the ordered table, the reverse block its transpose
+    from the reversed table (hetero pair, asymmetric sp/sd/pd tables)."""
+    za, zb = 6, 14
+    skfs = synth.make_skf_set([za, zb], seed=4)
+    cell = np.eye(3) * 30.0 * AA_BOHR
+    u = np.array([0.3, -0.5, 0.8])
+    u /= np.linalg.norm(u)
+    d = 2.1 * AA_BOHR
+    pos = np.array([[1.0, 1.0, 1.0], [1.0, 1.0, 1.0] + d * u])
+    sd = synth.shell_dict_spd([za, zb])
+    hs = build.build_hs(np.array([za, zb]), pos, cell, sd, skfs, np.zeros((1, 3)))
+    H = hs["H"][0].real
+    for kind, mat in (("hamiltonian", H), ("overlap", hs["S"][0].real)):
+        def integ(a, b, l1, l2):
+            yy = np.asarray(skfs[(a, b)][kind][f"{l1}-{l2}"]).T
+            return interp.poly_interp_u(yy, synth.GRID_STEP, [d])[0]
+        off = {0: 0, 1: 1, 2: 4}
+        for (l1, l2) in LP:
+            ref = sk_block(l1, l2, u, integ(za, zb, l1, l2))
+            got = mat[off[l1]:off[l1] + 2 * l1 + 1, 9 + off[l2]:9 + off[l2] + 2 * l2 + 1]
+            np.testing.assert_allclose(got, ref, atol=1e-14)
+            if l1 != l2:
+                # (l2 on atom a, l1 on atom b) comes from the reversed table, direction -u
+                ref2 = sk_block(l1, l2, -u, integ(zb, za, l1, l2)).T
+                got2 = mat[off[l2]:off[l2] + 2 * l2 + 1, 9 + off[l1]:9 + off[l1] + 2 * l1 + 1]
+                np.testing.assert_allclose(got2, ref2, atol=1e-14)
+
+
+# ---------------------------------------------------------------- solve
+def test_solve_matches_scipy_generalized(diamond):
+    _, _, _, hs = diamond
+    w, v = solve.solve_all(hs["H"][:5], hs["S"][:5], vectors=True)
+    for ik in range(5):
+        ref = scipy.linalg.eigh(hs["H"][ik], hs["S"][ik], eigvals_only=True)
+        np.testing.assert_allclose(w[ik], ref, atol=1e-12)
+        z = v[ik]
+        np.testing.assert_allclose(z.conj().T @ hs["S"][ik] @ z, np.eye(18), atol=1e-11)
+
+
+# ---------------------------------------------------------------- post
+def test_dos_closed_form_and_integral():
+    eig = np.array([[-0.1, 0.0, 0.2], [-0.05, 0.1, 0.3]])
+    w = np.array([0.25, 0.75])
+    grid, d = post.dos(eig, w, 0.0, energy_range=(-20, 20), num_points=20001, sigma=0.1)
+    # integral = sum of weights * bands = 3 (no spin factor)
+    assert abs(np.trapezoid(d, grid) - 3.0) < 1e-8
+    # peak value at an isolated level: w / (sigma sqrt(2 pi)) (grid centred on the level)
+    lv = 0.3 * 27.211
+    grid, d = post.dos(eig, w, 0.0, energy_range=(lv - 1.0, lv + 1.0), num_points=2001, sigma=0.1)
+    assert abs(d[1000] - 0.75 / (0.1 * np.sqrt(2 * np.pi))) < 1e-9
+
+
+def test_fermi_bisection_root():
+    rng = np.random.default_rng(5)
+    eig = np.sort(rng.uniform(-0.5, 0.5, (4, 10)), axis=1)
+    w = np.full(4, 0.25)
+    mu = post.fermi_bisect(eig, 7.0, w)
+    assert abs(post.electron_count(eig, w, mu) - 7.0) < 1e-10
+
+
+def test_fermi_reference_iteration_invariants():
+    # symmetric two-level system: N(mean) = Ne exactly -> returns the mean unchanged
+    eig = np.array([[-0.2, 0.2]])
+    assert post.fermi_search(eig, 2.0, [1.0]) == 0.0
+    # generic case: the returned mu is the first iterate with |dL/dmu| < tol
+    rng = np.random.default_rng(6)
+    eig = np.sort(rng.uniform(-0.3, 0.3, (3, 12)), axis=1)
+    tr = []
+    mu = post.fermi_search(eig, 11.0, np.full(3, 1 / 3), trace=tr)
+    assert tr[-1][0] == mu and abs(tr[-1][2]) < 1e-6
+    assert all(abs(g) >= 1e-6 for (_, _, g) in tr[:-1])
+    assert abs(tr[1][0] - (tr[0][0] - tr[0][1] / (tr[0][2] + 1e-12))) < 1e-15
+
+
+def test_gap_rules():
+    eig = np.array([[-0.3, -0.1, 0.2], [-0.2, -0.05, 0.15]])
+    mu = 0.0
+    g = post.band_gap(eig, mu)
+    assert abs(g["vbm"] - (-0.05 * 27.211)) < 1e-12 and abs(g["cbm"] - 0.15 * 27.211) < 1e-12
+    assert g["vbm_kpoint"] == 1 and g["cbm_kpoint"] == 1 and g["direct"]
+    bs = post.band_structure(eig, mu)
+    assert abs(bs["minimum_direct_gap"] - 0.2 * 27.211) < 1e-12
+    # metal rule: a level within 1e-3 eV of mu
+    assert post.band_gap(np.array([[-0.1, 0.0]]), 0.0)["metal"]
