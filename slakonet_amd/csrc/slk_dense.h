@@ -1,0 +1,33 @@
+// Internal host launchers of the blocked dense kernels (slk_dense.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+
+// op flags for GEMM operands: bit0 transpose, bit1 conjugate.
+enum { OP_N = 0, OP_T = 1, OP_CONJ = 2, OP_C = 3 };
+// K-range limits for triangular operands (element (m,k) of opA, (k,n) of opB)
+enum { KLIM_A_LOWER = 1, KLIM_A_UPPER = 2, KLIM_B_UPPER = 4, KLIM_B_LOWER = 8 };
+
+// C = alpha opA(A) opB(B) + beta C for `batch` problems with strides sA/sB/sC.
+// tri = 1 skips 64x64 output tiles strictly above the diagonal.
+void launch_zgemm(hipStream_t st, int batch, int M, int N, int K, double alpha, const double2* A, int lda,
+                  long long sA, int opA, const double2* B, int ldb, long long sB, int opB, double beta, double2* C,
+                  int ldc, long long sC, int klim, int tri);
+
+// Zero-fill (no memset nodes so that the solve can be graph-captured).
+void launch_zero(hipStream_t st, void* p, size_t bytes);
+
+// Blocked Cholesky S = L L^H into L (copy of S), writing the inverses of the
+// 32x32 diagonal blocks into the diagonal blocks of W; info[b] = first failing
+// pivot (1-based) or 0.
+void launch_potrf_blocked(hipStream_t st, const double2* S, double2* L, double2* W, int n, int ld, int batch,
+                          int* info);
+
+// W = L^-1 (lower) from L and the diagonal-block inverses already in W.
+void launch_trinv_blocked(hipStream_t st, const double2* L, double2* W, double2* T, int n, int ld, int batch);
+
+// Blocked Householder tridiagonalisation of the Hermitian C (in place): d, e
+// (real tridiagonal), tau, reflectors in the strict lower part of C.
+void launch_hetrd_blocked(hipStream_t st, double2* C, int n, int ld, int batch, double* D, double* E, double2* TAU,
+                          double2* Vc, double2* Wc, double2* Y, double2* C12);
+
+constexpr int SLK_NB = 32;   // panel width
