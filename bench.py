@@ -39,7 +39,7 @@ def parse():
     p.add_argument("--warmup", type=int, default=1)
     p.add_argument("--requests", type=int, default=2000)
     p.add_argument("--rate", type=float, default=200.0, help="Poisson arrival rate, requests/s")
-    p.add_argument("--mode", choices=["serve", "plain"], default="plain")
+    p.add_argument("--mode", choices=["serve", "plain"], default="serve")
     p.add_argument("--cpu-seconds", type=float, default=20.0, help="CPU baseline sample budget (0 = skip)")
     p.add_argument("--no-roofline", action="store_true")
     return p.parse_args()

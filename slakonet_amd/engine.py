@@ -61,8 +61,15 @@ class BuildPlan:
 
 
 def _upload(arr: np.ndarray, device) -> torch.Tensor:
+    """Host -> device through pinned staging so the copy is stream-ordered and
+    never blocks the host behind earlier work on the stream."""
     a = np.ascontiguousarray(arr)
-    return torch.from_numpy(a.view(np.uint8).reshape(-1).copy()).to(device, non_blocking=True)
+    return _pinned(a.view(np.uint8).reshape(-1)).to(device, non_blocking=True)
+
+
+def _pinned(a: np.ndarray) -> torch.Tensor:
+    t = torch.from_numpy(np.ascontiguousarray(a))
+    return t.pin_memory() if torch.cuda.is_available() else t
 
 
 def column_tiles(norb_atoms: np.ndarray):
@@ -136,17 +143,30 @@ def plan_build(tables: TableSet, inputs: Sequence[StructureInput], ld: Optional[
         raise ValueError("ld smaller than the largest structure")
     structs["ld"] = ld
     work = np.concatenate(work_l)
-    dev = {
-        "structs": _upload(structs, device),
-        "work": _upload(work, device),
-        "atom_pos": torch.from_numpy(np.concatenate(pos_l).reshape(-1)).to(device),
-        "atom_spec": torch.from_numpy(np.concatenate(spec_l).astype(np.int32)).to(device),
-        "atom_orb": torch.from_numpy(np.concatenate(orb_l).astype(np.int32)).to(device),
-        "trans_cart": torch.from_numpy(np.concatenate(tc_l).reshape(-1)).to(device),
-        "trans_int": torch.from_numpy(np.concatenate(ti_l).astype(np.int32).reshape(-1)).to(device),
-        "kfrac": torch.from_numpy(np.concatenate(k_l).reshape(-1)).to(device),
+    # one pinned staging buffer, one H2D copy, device views per array
+    arrays = {
+        "structs": np.ascontiguousarray(structs).view(np.uint8).reshape(-1),
+        "work": np.ascontiguousarray(work).view(np.uint8).reshape(-1),
+        "atom_pos": np.concatenate(pos_l).reshape(-1).view(np.uint8),
+        "atom_spec": np.concatenate(spec_l).astype(np.int32).view(np.uint8),
+        "atom_orb": np.concatenate(orb_l).astype(np.int32).view(np.uint8),
+        "trans_cart": np.ascontiguousarray(np.concatenate(tc_l).reshape(-1)).view(np.uint8),
+        "trans_int": np.concatenate(ti_l).astype(np.int32).reshape(-1).view(np.uint8),
+        "kfrac": np.ascontiguousarray(np.concatenate(k_l).reshape(-1)).view(np.uint8),
+        "kw": np.ascontiguousarray(np.concatenate(kw_l)).view(np.uint8),
     }
-    kw = torch.from_numpy(np.concatenate(kw_l)).to(device)
+    offs, tot = {}, 0
+    for k, a in arrays.items():
+        offs[k] = tot
+        tot += (a.nbytes + 255) // 256 * 256
+    stage = np.zeros(tot, dtype=np.uint8)
+    for k, a in arrays.items():
+        stage[offs[k]:offs[k] + a.nbytes] = a
+    blob = _pinned(stage).to(device, non_blocking=True)
+    views = {k: blob[offs[k]:offs[k] + arrays[k].nbytes] for k in arrays}
+    dev = {k: v for k, v in views.items() if k != "kw"}
+    dev["blob"] = blob
+    kw = views["kw"].view(torch.float64)
     return BuildPlan(len(inputs), m0, ld, n_orb, nks, mat0, kw, nelec, dev, len(work))
 
 

@@ -1,0 +1,198 @@
+"""Continuous-batching request loop on one MI355X.
+
+Requests (one structure each) are admitted as they arrive, bucketed by
+padded matrix dimension (identity padding keeps the padded eigenvalues at a
+constant far above every band), and launched as batched solves on a small
+set of HIP streams whenever a stream slot and HBM budget are free; requests
+that arrive while others are on the device queue up and join the next batch
+of their bucket. Per batch: one fused build launch, one batched eigensolve,
+Fermi / band-edge / DOS reductions, and one asynchronous copy of the scalar
+results to pinned host memory; completion is observed with events, so the
+host never blocks on the device while work is pending.
+
+SK tables stay in HBM for the lifetime of the engine.
+"""
+from __future__ import annotations
+
+import time
+from collections import deque
+from dataclasses import dataclass, field
+from typing import Dict, List, Optional, Sequence
+
+import numpy as np
+import torch
+
+from . import engine
+from .tables import TableSet
+
+
+def bucket_dim(n: int) -> int:
+    """Padded dimension: multiples of 32 to 256, of 64 to 1024, of 128 beyond."""
+    if n <= 256:
+        step = 32
+    elif n <= 1024:
+        step = 64
+    else:
+        step = 128
+    return ((n + step - 1) // step) * step
+
+
+@dataclass
+class Request:
+    idx: int
+    inp: engine.StructureInput
+    n_orb: int
+    arrival: float
+    ld: int
+
+
+@dataclass
+class Batch:
+    reqs: List[Request]
+    ld: int
+    stream: torch.cuda.Stream
+    event: torch.cuda.Event
+    host: torch.Tensor
+    nbytes: int
+    keep: list = field(default_factory=list)
+    dos: Optional[torch.Tensor] = None
+    w: Optional[torch.Tensor] = None
+    plan: Optional[engine.BuildPlan] = None
+
+
+class ServingEngine:
+    def __init__(self, tables: TableSet, dos_grid: Optional[torch.Tensor] = None, n_streams: int = 4,
+                 max_inflight: int = 8, hbm_budget_bytes: float = 64e9, max_batch_matrices: int = 512,
+                 fermi_mode: int = 0, keep_outputs: bool = False):
+        self.tables = tables
+        self.dev = tables.device
+        self.grid = dos_grid
+        self.streams = [torch.cuda.Stream(device=self.dev) for _ in range(n_streams)]
+        self.max_inflight = max_inflight
+        self.budget = hbm_budget_bytes
+        self.max_batch = max_batch_matrices
+        self.fermi_mode = fermi_mode
+        self.keep_outputs = keep_outputs
+        self.nf = len(engine.EDGE_FIELDS)
+        self._rr = 0
+
+    # ------------------------------------------------------------------
+    def _orbitals(self, inp: engine.StructureInput) -> int:
+        sp = self.tables.species_index(inp.z)
+        return int(self.tables.norb[sp].sum())
+
+    @staticmethod
+    def batch_bytes(ld: int, nmat: int) -> int:
+        # H, S + eigensolver workspace (3 matrices + panels) + eigenvalues
+        return int(nmat * ld * ld * 16 * 5.3) + nmat * ld * 8
+
+    def _launch(self, reqs: List[Request], ld: int) -> Batch:
+        stream = self.streams[self._rr % len(self.streams)]
+        self._rr += 1
+        nmat = sum(len(r.inp.kpoints) for r in reqs)
+        with torch.cuda.stream(stream):
+            plan = engine.plan_build(self.tables, [r.inp for r in reqs], ld=ld)
+            H, S = engine.run_build(self.tables, plan)
+            w, _, info = engine.eigh(H, S, check=False)
+            edges, dos, stats = engine.post(plan, w, fermi_mode=self.fermi_mode, dos_grid=self.grid)
+            parts = [edges.reshape(-1), info.to(torch.float64)]
+            if stats is not None:
+                parts.append(stats.reshape(-1))
+            flat = torch.cat(parts)
+            host = torch.empty(flat.numel(), dtype=torch.float64, pin_memory=True)
+            host.copy_(flat, non_blocking=True)
+            ev = torch.cuda.Event()
+            ev.record(stream)
+        b = Batch(reqs, ld, stream, ev, host, self.batch_bytes(ld, nmat), keep=[H, S, w, edges, info, flat])
+        if self.keep_outputs:
+            b.dos, b.w, b.plan = dos, w, plan
+        return b
+
+    def _finish(self, b: Batch, results: Dict[int, dict], done_t: Dict[int, float], now: float):
+        h = b.host.numpy()
+        ns = len(b.reqs)
+        edges = h[:ns * self.nf].reshape(ns, self.nf)
+        nmat = sum(len(r.inp.kpoints) for r in b.reqs)
+        info = h[ns * self.nf: ns * self.nf + 2 * nmat]
+        stats = h[ns * self.nf + 2 * nmat:].reshape(ns, 3) if self.grid is not None else None
+        m0 = 0
+        for s, r in enumerate(b.reqs):
+            nk = len(r.inp.kpoints)
+            bad = np.nonzero(info[m0:m0 + nk])[0]
+            res = engine.edges_dict(edges[s])
+            res["gap_eV"] = 0.0 if res["metal"] > 0 else res["cbm"] - res["vbm"]
+            res["ok"] = len(bad) == 0
+            if len(bad):
+                res["error"] = f"S(k) not positive definite at k-point index {int(bad[0])}"
+            if stats is not None:
+                res["dos_at_fermi"], res["dos_gap_center_eV"], res["dos_total_states"] = stats[s]
+            if self.keep_outputs:
+                res["eigenvalues"] = b.w[b.plan.mat0[s]:b.plan.mat0[s] + nk, :r.n_orb]
+                if b.dos is not None:
+                    res["dos"] = b.dos[s]
+            results[r.idx] = res
+            done_t[r.idx] = now
+            m0 += nk
+
+    # ------------------------------------------------------------------
+    def replay(self, inputs: Sequence[engine.StructureInput], arrivals: Sequence[float]):
+        """Serve ``inputs`` arriving at ``arrivals`` (seconds from start).
+
+        Returns dict with wall time, per-request latency (arrival -> result on
+        the host) and per-request results."""
+        order = np.argsort(np.asarray(arrivals), kind="stable")
+        pending = deque(int(i) for i in order)
+        queues: Dict[int, deque] = {}
+        inflight: List[Batch] = []
+        used = 0
+        results: Dict[int, dict] = {}
+        done_t: Dict[int, float] = {}
+        t0 = time.perf_counter()
+        n = len(inputs)
+        while len(done_t) < n:
+            now = time.perf_counter() - t0
+            while pending and arrivals[pending[0]] <= now:
+                i = pending.popleft()
+                inp = inputs[i]
+                no = self._orbitals(inp)
+                ld = bucket_dim(no)
+                queues.setdefault(ld, deque()).append(Request(i, inp, no, float(arrivals[i]), ld))
+            progressed = False
+            for b in list(inflight):
+                if b.event.query():
+                    self._finish(b, results, done_t, time.perf_counter() - t0)
+                    inflight.remove(b)
+                    used -= b.nbytes
+                    progressed = True
+            # launch: oldest bucket first while slots and budget allow
+            while len(inflight) < self.max_inflight:
+                live = [(q[0].arrival, ld) for ld, q in queues.items() if q]
+                if not live:
+                    break
+                _, ld = min(live)
+                q = queues[ld]
+                take = []
+                nmat = 0
+                while q and len(take) < self.max_batch:
+                    nk = len(q[0].inp.kpoints)
+                    if take and (nmat + nk > self.max_batch or
+                                 used + self.batch_bytes(ld, nmat + nk) > self.budget):
+                        break
+                    take.append(q.popleft())
+                    nmat += nk
+                if not take:
+                    break
+                b = self._launch(take, ld)
+                used += b.nbytes
+                inflight.append(b)
+                progressed = True
+            if not progressed:
+                if inflight:
+                    time.sleep(5e-5)
+                elif pending:
+                    wait = arrivals[pending[0]] - (time.perf_counter() - t0)
+                    if wait > 0:
+                        time.sleep(min(wait, 1e-3))
+        wall = time.perf_counter() - t0
+        lat = np.array([done_t[i] - float(arrivals[i]) for i in range(n)])
+        return {"wall_s": wall, "latency_s": lat, "results": results}

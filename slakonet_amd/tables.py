@@ -118,6 +118,23 @@ class TableSet:
 
     def update(self, h_override=None, s_override=None):
         """(Re)pack the [rows, 20] table tensor; differentiable w.r.t. overrides."""
+        if h_override is None and s_override is None:
+            # constant tables: pack on the host and upload once
+            host = np.zeros((self.n_rows, 20), dtype=np.float64)
+            row = 0
+            for (za, zb) in self.pairs:
+                skf = self.skfs[(za, zb)]
+                ng = int(skf.grid.numel())
+                col = 0
+                for src in (skf.hamiltonian, skf.overlap):
+                    for (l1, l2) in LKEYS:
+                        t = src.get((l1, l2))
+                        if t is not None:
+                            host[row:row + ng, col:col + l1 + 1] = t.detach().cpu().numpy().T
+                        col += l1 + 1
+                row += ng
+            self.tables = torch.from_numpy(host).to(self.device)
+            return self.tables
         parts = []
         for (za, zb) in self.pairs:
             skf = self.skfs[(za, zb)]

@@ -1,0 +1,51 @@
+"""Continuous-batching engine: results in the mix equal results alone."""
+import numpy as np
+import pytest
+import torch
+
+from slakonet_amd import engine, synth
+from slakonet_amd.serving import ServingEngine, bucket_dim
+from slakonet_amd.tables import TableSet
+from slakonet_amd.units import AA_BOHR
+
+pytestmark = pytest.mark.gpu
+
+TOL_EV = 1e-7      # eigenvalue-derived scalars, eV (FP64 path)
+
+
+def _stream(n, seed=31, n_max=48):
+    sts = [synth.stream_structure(seed, i, n_max=n_max) for i in range(n)]
+    species = sorted({int(z) for st in sts for z in st.z})
+    tab = TableSet(synth.make_skf_set(species, seed=3), synth.shell_dict_spd(species), device="cuda")
+    inputs = [engine.StructureInput(st.z, st.positions * AA_BOHR, st.cell * AA_BOHR, st.kpoints, st.k_weights)
+              for st in sts]
+    return tab, inputs
+
+
+def test_bucket_dims():
+    assert bucket_dim(18) == 32 and bucket_dim(257) == 320 and bucket_dim(2304) == 2304
+    assert all(bucket_dim(n) >= n for n in range(1, 3000, 7))
+
+
+def test_mix_equals_alone():
+    tab, inputs = _stream(24)
+    grid = torch.linspace(-20.0, 10.0, 2001, dtype=torch.float64, device="cuda")
+    eng = ServingEngine(tab, dos_grid=grid, keep_outputs=True)
+    arrivals = np.zeros(len(inputs))
+    arrivals[12:] = 0.002          # a second wave arrives while the first is on the device
+    out = eng.replay(inputs, arrivals)
+    assert len(out["results"]) == len(inputs)
+    for i, inp in enumerate(inputs):
+        res = out["results"][i]
+        assert res["ok"]
+        plan = engine.plan_build(tab, [inp])
+        H, S = engine.run_build(tab, plan)
+        w, _, _ = engine.eigh(H, S)
+        edges, dos, stats = engine.post(plan, w, fermi_mode=0, dos_grid=grid)
+        ed = engine.edges_dict(edges[0].cpu().numpy())
+        for key in ("vbm", "cbm", "vb2", "cb2"):
+            assert abs(res[key] - ed[key]) < TOL_EV, (i, key)
+        assert abs(res["mu"] - ed["mu"]) * 27.211 < TOL_EV
+        n = plan.n_orb[0]
+        assert torch.allclose(res["eigenvalues"], w[:, :n], atol=1e-9, rtol=0)
+        assert torch.allclose(res["dos"], dos[0], atol=1e-7, rtol=1e-7)
