@@ -89,6 +89,11 @@ int slk_pad_launch(const SlkStructDesc* structs_dev, int s, int nk, int n_orb, i
 size_t slk_eigh_workspace_bytes(int n, int ld, int batch, int vectors);
 int slk_eigh_f64(const void* H, const void* S, int n, int ld, int batch, double* w, int wld, void* V,
                  void* ws, size_t ws_bytes, int* info, hipStream_t stream);
+/* Eigenvalues only for n <= 32: the whole solve of one matrix in LDS of one
+ * wave64 workgroup, one launch per batch (slk_eigh_f64 dispatches here for
+ * values-only solves of that size). info: [batch]. */
+int slk_eigh_small_f64(const void* H, const void* S, int n, int ld, int batch, double* w, int wld, int* info,
+                       hipStream_t stream);
 /* Same solve, plus elapsed milliseconds of its six stages (potrf, trinv,
  * reduce, hetrd, tridiagonal solve, back-transform) from HIP events recorded
  * on `stream`; returns after the last event completed. Measurement aid with

@@ -245,9 +245,11 @@ void launch_trinv_blocked(hipStream_t st, const double2* L, double2* W, double2*
 // Vc[k*ld + r], Wc[k*ld + r] for panel column k. Y: trailing mat-vec result.
 // C12: the correction coefficients W^H v (first NB) and V^H v (next NB).
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void latrd_col_kernel(double2* A, int n, int ld, int j, int jj, int finalize,
-                                                        int do_col, double2* Vc, double2* Wc, const double2* Y,
-                                                        double2* C12, double2* TAU, double* D, double* E) {
+#define LATRD_THREADS 1024
+__global__ __launch_bounds__(LATRD_THREADS) void latrd_col_kernel(double2* A, int n, int ld, int j, int jj,
+                                                                  int finalize, int do_col, double2* Vc, double2* Wc,
+                                                                  const double2* Y, double2* C12, double2* TAU,
+                                                                  double* D, double* E) {
   const int b = blockIdx.x;
   double2* a = A + (size_t)b * ld * ld;
   double2* vc = Vc + (size_t)b * NB * ld;
@@ -255,7 +257,8 @@ __global__ __launch_bounds__(256) void latrd_col_kernel(double2* A, int n, int l
   const double2* y = Y + (size_t)b * ld;
   double2* c12 = C12 + (size_t)b * 2 * NB;
   double2* tau = TAU + (size_t)b * ld;
-  __shared__ double red[4];
+  __shared__ double red[LATRD_THREADS / 64];
+  __shared__ double2 s_vj[NB], s_wj[NB];
   __shared__ double2 s_tau, s_scale;
   __shared__ double s_beta;
   __shared__ int s_skip;
@@ -283,12 +286,17 @@ __global__ __launch_bounds__(256) void latrd_col_kernel(double2* A, int n, int l
     __syncthreads();
   }
   if (!do_col) return;
-  // column update with the panel so far
+  // column update with the panel so far (row j of the panel cached in LDS)
+  if (tid < jj) {
+    s_vj[tid] = vc[(size_t)tid * ld + j];
+    s_wj[tid] = wc[(size_t)tid * ld + j];
+  }
+  __syncthreads();
   for (int r = j + tid; r < n; r += blockDim.x) {
     double2 acc = a[(size_t)r * ld + j];
     for (int k = 0; k < jj; ++k) {
-      const double2 t1 = c_mulc(vc[(size_t)k * ld + r], wc[(size_t)k * ld + j]);
-      const double2 t3 = c_mulc(wc[(size_t)k * ld + r], vc[(size_t)k * ld + j]);
+      const double2 t1 = c_mulc(vc[(size_t)k * ld + r], s_wj[k]);
+      const double2 t3 = c_mulc(wc[(size_t)k * ld + r], s_vj[k]);
       acc.x -= t1.x + t3.x;
       acc.y -= t1.y + t3.y;
     }
@@ -334,7 +342,7 @@ __global__ __launch_bounds__(256) void latrd_col_kernel(double2* A, int n, int l
   __syncthreads();
   // correction coefficients, one wave per panel column
   const int lane = tid & 63, wv = tid >> 6;
-  for (int k = wv; k < jj; k += 4) {
+  for (int k = wv; k < jj; k += LATRD_THREADS / 64) {
     double2 s1 = make_double2(0.0, 0.0), s2 = make_double2(0.0, 0.0);
     for (int r = j + 1 + lane; r < n; r += 64) {
       const double2 v = vc[(size_t)jj * ld + r];
@@ -350,6 +358,7 @@ __global__ __launch_bounds__(256) void latrd_col_kernel(double2* A, int n, int l
 }
 
 // y[r] = sum_c A[r][c] v[c] - sum_k (V_k[r] c1_k + W_k[r] c2_k), rows r > j.
+#define HEMV_ROWS 8
 __global__ __launch_bounds__(256) void hemv_kernel(const double2* A, int n, int ld, int j, int jj, const double2* Vc,
                                                    const double2* Wc, double2* Y, const double2* C12) {
   const int b = blockIdx.y;
@@ -360,14 +369,15 @@ __global__ __launch_bounds__(256) void hemv_kernel(const double2* A, int n, int 
   const double2* c12 = C12 + (size_t)b * 2 * NB;
   double2* y = Y + (size_t)b * ld;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  const int rbase = j + 1 + blockIdx.x * 16 + wv * 4;
-  double2 acc[4];
+  // 2 rows per wave, 8 rows per workgroup: enough workgroups to cover the chip
+  const int rbase = j + 1 + blockIdx.x * HEMV_ROWS + wv * 2;
+  double2 acc[2];
 #pragma unroll
-  for (int q = 0; q < 4; ++q) acc[q] = make_double2(0.0, 0.0);
+  for (int q = 0; q < 2; ++q) acc[q] = make_double2(0.0, 0.0);
   for (int c = j + 1 + lane; c < n; c += 64) {
     const double2 vv = v[c];
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
+    for (int q = 0; q < 2; ++q) {
       const int r = rbase + q;
       if (r < n) {
         const double2 x = a[(size_t)r * ld + c];
@@ -377,17 +387,14 @@ __global__ __launch_bounds__(256) void hemv_kernel(const double2* A, int n, int 
     }
   }
 #pragma unroll
-  for (int q = 0; q < 4; ++q) {
+  for (int q = 0; q < 2; ++q) {
     acc[q].x = wave_sum(acc[q].x);
     acc[q].y = wave_sum(acc[q].y);
   }
-  if (lane < 4) {
+  if (lane < 2) {
     const int r = rbase + lane;
     if (r < n) {
-      double2 s = acc[0];
-      if (lane == 1) s = acc[1];
-      if (lane == 2) s = acc[2];
-      if (lane == 3) s = acc[3];
+      double2 s = lane == 0 ? acc[0] : acc[1];
       for (int k = 0; k < jj; ++k) {
         const double2 t1 = c_mul(vc[(size_t)k * ld + r], c12[k]);
         const double2 t2 = c_mul(wc[(size_t)k * ld + r], c12[NB + k]);
@@ -415,14 +422,15 @@ void launch_hetrd_blocked(hipStream_t st, double2* C, int n, int ld, int batch, 
     const int jn = min(j0 + NB, n - 1);
     for (int j = j0; j < jn; ++j) {
       const int jj = j - j0;
-      hipLaunchKernelGGL(latrd_col_kernel, dim3(batch), dim3(256), 0, st, C, n, ld, j, jj, jj > 0 ? 1 : 0, 1, Vc,
-                         Wc, Y, C12, TAU, D, E);
+      hipLaunchKernelGGL(latrd_col_kernel, dim3(batch), dim3(LATRD_THREADS), 0, st, C, n, ld, j, jj,
+                         jj > 0 ? 1 : 0, 1, Vc, Wc, Y, C12, TAU, D, E);
       const int m = n - j - 1;
-      hipLaunchKernelGGL(hemv_kernel, dim3((m + 15) / 16, batch), dim3(256), 0, st, C, n, ld, j, jj, Vc, Wc, Y, C12);
+      hipLaunchKernelGGL(hemv_kernel, dim3((m + HEMV_ROWS - 1) / HEMV_ROWS, batch), dim3(256), 0, st, C, n, ld, j,
+                         jj, Vc, Wc, Y, C12);
     }
     // finalize w of the panel's last column
-    hipLaunchKernelGGL(latrd_col_kernel, dim3(batch), dim3(256), 0, st, C, n, ld, jn, jn - j0, 1, 0, Vc, Wc, Y, C12,
-                       TAU, D, E);
+    hipLaunchKernelGGL(latrd_col_kernel, dim3(batch), dim3(LATRD_THREADS), 0, st, C, n, ld, jn, jn - j0, 1, 0, Vc,
+                       Wc, Y, C12, TAU, D, E);
     if (jn < n) {
       const int m = n - jn, kcols = jn - j0;
       // A22 -= V W^H + W V^H on rows/cols >= jn

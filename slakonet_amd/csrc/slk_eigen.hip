@@ -34,6 +34,9 @@ __device__ __forceinline__ int sturm_count(const double* d, const double* e, int
   return cnt;
 }
 
+// Multisection: one wave per eigenvalue; each step evaluates 64 Sturm counts
+// at equidistant interior points and keeps the sub-interval where the count
+// crosses the eigenvalue index (65x shrink per step instead of 2x).
 __global__ __launch_bounds__(EIG_THREADS) void bisect_kernel(const double* D, const double* Ed, int n, int ld,
                                                              double* Wout, int wld) {
   const int b = blockIdx.y;
@@ -42,35 +45,50 @@ __global__ __launch_bounds__(EIG_THREADS) void bisect_kernel(const double* D, co
   extern __shared__ double sm[];
   double* sd = sm;
   double* se = sm + n;
-  __shared__ double s_lo, s_hi, s_piv;
-  for (int i = threadIdx.x; i < n; i += blockDim.x) {
-    sd[i] = d[i];
-    se[i] = i < n - 1 ? e[i] : 0.0;
+  __shared__ double s_red[3][EIG_THREADS / 64];
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  double lo = 1e300, hi = -1e300, emax = 0.0;
+  for (int i = tid; i < n; i += blockDim.x) {
+    const double di = d[i];
+    const double ei = i < n - 1 ? e[i] : 0.0;
+    const double ep = i > 0 ? e[i - 1] : 0.0;
+    sd[i] = di;
+    se[i] = ei;
+    const double r = fabs(ep) + fabs(ei);
+    lo = fmin(lo, di - r);
+    hi = fmax(hi, di + r);
+    emax = fmax(emax, ei * ei);
   }
+  lo = wave_min(lo);
+  hi = wave_max(hi);
+  emax = wave_max(emax);
+  if (lane == 0) { s_red[0][wv] = lo; s_red[1][wv] = hi; s_red[2][wv] = emax; }
   __syncthreads();
-  if (threadIdx.x == 0) {
-    double lo = 1e300, hi = -1e300, emax = 0.0;
-    for (int i = 0; i < n; ++i) {
-      const double r = (i > 0 ? fabs(se[i - 1]) : 0.0) + (i < n - 1 ? fabs(se[i]) : 0.0);
-      lo = fmin(lo, sd[i] - r);
-      hi = fmax(hi, sd[i] + r);
-      emax = fmax(emax, se[i] * se[i]);
-    }
-    const double span = fmax(fabs(lo), fabs(hi));
-    s_lo = lo - 2.0 * 2.2e-16 * span * n - 1e-300;
-    s_hi = hi + 2.0 * 2.2e-16 * span * n + 1e-300;
-    s_piv = fmax(2.2e-308, 2.2e-308 * emax) * 1e2;
+  lo = 1e300; hi = -1e300; emax = 0.0;
+  for (int q = 0; q < EIG_THREADS / 64; ++q) {
+    lo = fmin(lo, s_red[0][q]);
+    hi = fmax(hi, s_red[1][q]);
+    emax = fmax(emax, s_red[2][q]);
   }
-  __syncthreads();
-  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  const double span = fmax(fabs(lo), fabs(hi));
+  lo = lo - 2.0 * 2.2e-16 * span * n - 1e-300;
+  hi = hi + 2.0 * 2.2e-16 * span * n + 1e-300;
+  const double piv = fmax(2.2e-308, 2.2e-308 * emax) * 1e2;
+  const int idx = blockIdx.x * (EIG_THREADS / 64) + wv;
   if (idx >= n) return;
-  double lo = s_lo, hi = s_hi;
-  for (int it = 0; it < 200; ++it) {
-    const double mid = 0.5 * (lo + hi);
-    if (mid <= lo || mid >= hi) break;
-    if (sturm_count(sd, se, n, mid, s_piv) > idx) hi = mid; else lo = mid;
+  for (int it = 0; it < 64; ++it) {
+    const double width = hi - lo;
+    if (width <= 4.0 * 2.2e-16 * fmax(fabs(lo), fabs(hi)) + piv) break;
+    const double x = lo + width * (double)(lane + 1) / 65.0;
+    const unsigned long long mask = __ballot(sturm_count(sd, se, n, x, piv) > idx);
+    const int first = mask ? (__ffsll((long long)mask) - 1) : 64;
+    const double nlo = first == 0 ? lo : lo + width * (double)first / 65.0;
+    const double nhi = first == 64 ? hi : lo + width * (double)(first + 1) / 65.0;
+    if (nlo == lo && nhi == hi) break;
+    lo = nlo;
+    hi = nhi;
   }
-  Wout[(size_t)b * wld + idx] = 0.5 * (lo + hi);
+  if (lane == 0) Wout[(size_t)b * wld + idx] = 0.5 * (lo + hi);
 }
 
 // ---------------------------------------------------------------------------
@@ -304,6 +322,14 @@ static int eigh_f64_impl(const void* H, const void* S, int n, int ld, int batch,
   if (ws_bytes < slk_eigh_workspace_bytes(n, ld, batch, V != nullptr)) return 2;
   // LDS-resident tridiagonal data: values need 2n doubles, vectors 4n doubles (<= 64 KiB)
   if (n > 4096 || (V != nullptr && n > 2048)) return 3;
+  if (V == nullptr && n <= 32) {
+    // whole solve in LDS, one launch (stages collapse into the first interval)
+    if (ev) (void)hipEventRecord(ev[0], stream);
+    const int rc = slk_eigh_small_f64(H, S, n, ld, batch, w, wld, info, stream);
+    if (ev)
+      for (int i = 1; i < 7; ++i) (void)hipEventRecord(ev[i], stream);
+    return rc;
+  }
   EighWs s = carve(ws, ld, batch);
   const long long sM = (long long)ld * ld;
   const double2* h = reinterpret_cast<const double2*>(H);
@@ -321,7 +347,7 @@ static int eigh_f64_impl(const void* H, const void* S, int n, int ld, int batch,
   launch_hetrd_blocked(stream, s.B0, n, ld, batch, s.D, s.E, s.TAU, s.Vc, s.Wc, s.Y, s.C12);
   if (ev) (void)hipEventRecord(ev[4], stream);
   if (V == nullptr) {
-    hipLaunchKernelGGL(bisect_kernel, dim3((n + 255) / 256, batch), dim3(256), 2 * n * sizeof(double), stream, s.D,
+    hipLaunchKernelGGL(bisect_kernel, dim3((n + 3) / 4, batch), dim3(256), 2 * n * sizeof(double), stream, s.D,
                        s.E, n, ld, w, wld);
     if (ev) (void)hipEventRecord(ev[5], stream);
   } else {
