@@ -27,7 +27,10 @@ void launch_trinv_blocked(hipStream_t st, const double2* L, double2* W, double2*
 
 // Blocked Householder tridiagonalisation of the Hermitian C (in place): d, e
 // (real tridiagonal), tau, reflectors in the strict lower part of C.
+// Workspace: Vc, Wc [NB][ld], Y [ld], Yp [hetrd_partials_elems(ld)], C12
+// [2 NB] per matrix, cnt [batch] ints.
 void launch_hetrd_blocked(hipStream_t st, double2* C, int n, int ld, int batch, double* D, double* E, double2* TAU,
-                          double2* Vc, double2* Wc, double2* Y, double2* C12);
+                          double2* Vc, double2* Wc, double2* Y, double2* Yp, double2* C12, int* cnt);
+size_t hetrd_partials_elems(int ld);
 
 constexpr int SLK_NB = 32;   // panel width

@@ -240,205 +240,3 @@ void launch_trinv_blocked(hipStream_t st, const double2* L, double2* W, double2*
   }
 }
 
-// ---------------------------------------------------------------------------
-// Blocked Householder tridiagonalisation. Panel buffers are column-major:
-// Vc[k*ld + r], Wc[k*ld + r] for panel column k. Y: trailing mat-vec result.
-// C12: the correction coefficients W^H v (first NB) and V^H v (next NB).
-// ---------------------------------------------------------------------------
-#define LATRD_THREADS 1024
-__global__ __launch_bounds__(LATRD_THREADS) void latrd_col_kernel(double2* A, int n, int ld, int j, int jj,
-                                                                  int finalize, int do_col, double2* Vc, double2* Wc,
-                                                                  const double2* Y, double2* C12, double2* TAU,
-                                                                  double* D, double* E) {
-  const int b = blockIdx.x;
-  double2* a = A + (size_t)b * ld * ld;
-  double2* vc = Vc + (size_t)b * NB * ld;
-  double2* wc = Wc + (size_t)b * NB * ld;
-  const double2* y = Y + (size_t)b * ld;
-  double2* c12 = C12 + (size_t)b * 2 * NB;
-  double2* tau = TAU + (size_t)b * ld;
-  __shared__ double red[LATRD_THREADS / 64];
-  __shared__ double2 s_vj[NB], s_wj[NB];
-  __shared__ double2 s_tau, s_scale;
-  __shared__ double s_beta;
-  __shared__ int s_skip;
-  const int tid = threadIdx.x;
-  if (finalize) {
-    // w_{jj-1} = tau y - |tau|^2/2 (v^H y) v over rows >= j (support of v_{j-1})
-    const int kp = jj - 1;
-    const double2 tp = tau[j - 1];
-    double part = 0.0;
-    for (int r = j + tid; r < n; r += blockDim.x) {
-      const double2 v = vc[(size_t)kp * ld + r], yy = y[r];
-      part += v.x * yy.x + v.y * yy.y;
-    }
-    const double vhy = block_sum(part, red);
-    const double t2 = 0.5 * (tp.x * tp.x + tp.y * tp.y) * vhy;
-    for (int r = tid; r < n; r += blockDim.x) {
-      double2 o = make_double2(0.0, 0.0);
-      if (r >= j) {
-        const double2 p = c_mul(tp, y[r]);
-        const double2 v = vc[(size_t)kp * ld + r];
-        o = make_double2(p.x - t2 * v.x, p.y - t2 * v.y);
-      }
-      wc[(size_t)kp * ld + r] = o;
-    }
-    __syncthreads();
-  }
-  if (!do_col) return;
-  // column update with the panel so far (row j of the panel cached in LDS)
-  if (tid < jj) {
-    s_vj[tid] = vc[(size_t)tid * ld + j];
-    s_wj[tid] = wc[(size_t)tid * ld + j];
-  }
-  __syncthreads();
-  for (int r = j + tid; r < n; r += blockDim.x) {
-    double2 acc = a[(size_t)r * ld + j];
-    for (int k = 0; k < jj; ++k) {
-      const double2 t1 = c_mulc(vc[(size_t)k * ld + r], s_wj[k]);
-      const double2 t3 = c_mulc(wc[(size_t)k * ld + r], s_vj[k]);
-      acc.x -= t1.x + t3.x;
-      acc.y -= t1.y + t3.y;
-    }
-    a[(size_t)r * ld + j] = acc;
-  }
-  __syncthreads();
-  double part = 0.0;
-  for (int r = j + 2 + tid; r < n; r += blockDim.x) {
-    const double2 x = a[(size_t)r * ld + j];
-    part += x.x * x.x + x.y * x.y;
-  }
-  const double xn2 = block_sum(part, red);
-  if (tid == 0) {
-    const double2 alpha = a[(size_t)(j + 1) * ld + j];
-    if (xn2 == 0.0 && alpha.y == 0.0) {
-      s_skip = 1;
-      s_tau = make_double2(0.0, 0.0);
-      s_beta = alpha.x;
-      s_scale = make_double2(0.0, 0.0);
-    } else {
-      s_skip = 0;
-      const double nrm = sqrt(alpha.x * alpha.x + alpha.y * alpha.y + xn2);
-      const double beta = alpha.x >= 0.0 ? -nrm : nrm;
-      s_beta = beta;
-      s_tau = make_double2((beta - alpha.x) / beta, -alpha.y / beta);
-      const double2 den = make_double2(alpha.x - beta, alpha.y);
-      const double dd = den.x * den.x + den.y * den.y;
-      s_scale = make_double2(den.x / dd, -den.y / dd);
-    }
-    D[(size_t)b * ld + j] = a[(size_t)j * ld + j].x;
-    E[(size_t)b * ld + j] = s_beta;
-    tau[j] = s_tau;
-  }
-  __syncthreads();
-  const double2 scale = s_scale;
-  for (int r = tid; r < n; r += blockDim.x) {
-    double2 v = make_double2(0.0, 0.0);
-    if (r == j + 1) v = make_double2(1.0, 0.0);
-    else if (r > j + 1) v = s_skip ? make_double2(0.0, 0.0) : c_mul(a[(size_t)r * ld + j], scale);
-    vc[(size_t)jj * ld + r] = v;
-    if (r > j) a[(size_t)r * ld + j] = v;
-  }
-  __syncthreads();
-  // correction coefficients, one wave per panel column
-  const int lane = tid & 63, wv = tid >> 6;
-  for (int k = wv; k < jj; k += LATRD_THREADS / 64) {
-    double2 s1 = make_double2(0.0, 0.0), s2 = make_double2(0.0, 0.0);
-    for (int r = j + 1 + lane; r < n; r += 64) {
-      const double2 v = vc[(size_t)jj * ld + r];
-      const double2 x1 = c_cmul(wc[(size_t)k * ld + r], v);
-      const double2 x2 = c_cmul(vc[(size_t)k * ld + r], v);
-      s1.x += x1.x; s1.y += x1.y;
-      s2.x += x2.x; s2.y += x2.y;
-    }
-    s1.x = wave_sum(s1.x); s1.y = wave_sum(s1.y);
-    s2.x = wave_sum(s2.x); s2.y = wave_sum(s2.y);
-    if (lane == 0) { c12[k] = s1; c12[NB + k] = s2; }
-  }
-}
-
-// y[r] = sum_c A[r][c] v[c] - sum_k (V_k[r] c1_k + W_k[r] c2_k), rows r > j.
-#define HEMV_ROWS 8
-__global__ __launch_bounds__(256) void hemv_kernel(const double2* A, int n, int ld, int j, int jj, const double2* Vc,
-                                                   const double2* Wc, double2* Y, const double2* C12) {
-  const int b = blockIdx.y;
-  const double2* a = A + (size_t)b * ld * ld;
-  const double2* v = Vc + (size_t)b * NB * ld + (size_t)jj * ld;
-  const double2* vc = Vc + (size_t)b * NB * ld;
-  const double2* wc = Wc + (size_t)b * NB * ld;
-  const double2* c12 = C12 + (size_t)b * 2 * NB;
-  double2* y = Y + (size_t)b * ld;
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  // 2 rows per wave, 8 rows per workgroup: enough workgroups to cover the chip
-  const int rbase = j + 1 + blockIdx.x * HEMV_ROWS + wv * 2;
-  double2 acc[2];
-#pragma unroll
-  for (int q = 0; q < 2; ++q) acc[q] = make_double2(0.0, 0.0);
-  for (int c = j + 1 + lane; c < n; c += 64) {
-    const double2 vv = v[c];
-#pragma unroll
-    for (int q = 0; q < 2; ++q) {
-      const int r = rbase + q;
-      if (r < n) {
-        const double2 x = a[(size_t)r * ld + c];
-        acc[q].x += x.x * vv.x - x.y * vv.y;
-        acc[q].y += x.x * vv.y + x.y * vv.x;
-      }
-    }
-  }
-#pragma unroll
-  for (int q = 0; q < 2; ++q) {
-    acc[q].x = wave_sum(acc[q].x);
-    acc[q].y = wave_sum(acc[q].y);
-  }
-  if (lane < 2) {
-    const int r = rbase + lane;
-    if (r < n) {
-      double2 s = lane == 0 ? acc[0] : acc[1];
-      for (int k = 0; k < jj; ++k) {
-        const double2 t1 = c_mul(vc[(size_t)k * ld + r], c12[k]);
-        const double2 t2 = c_mul(wc[(size_t)k * ld + r], c12[NB + k]);
-        s.x -= t1.x + t2.x;
-        s.y -= t1.y + t2.y;
-      }
-      y[r] = s;
-    }
-  }
-}
-
-__global__ void hetrd_last_kernel(const double2* A, int n, int ld, double* D, double* E, double2* TAU) {
-  const int b = blockIdx.x;
-  if (threadIdx.x == 0) {
-    D[(size_t)b * ld + n - 1] = A[(size_t)b * ld * ld + (size_t)(n - 1) * ld + (n - 1)].x;
-    E[(size_t)b * ld + n - 1] = 0.0;
-    TAU[(size_t)b * ld + n - 1] = make_double2(0.0, 0.0);
-  }
-}
-
-void launch_hetrd_blocked(hipStream_t st, double2* C, int n, int ld, int batch, double* D, double* E, double2* TAU,
-                          double2* Vc, double2* Wc, double2* Y, double2* C12) {
-  const long long sP = (long long)NB * ld, sM = (long long)ld * ld;
-  for (int j0 = 0; j0 < n - 1; j0 += NB) {
-    const int jn = min(j0 + NB, n - 1);
-    for (int j = j0; j < jn; ++j) {
-      const int jj = j - j0;
-      hipLaunchKernelGGL(latrd_col_kernel, dim3(batch), dim3(LATRD_THREADS), 0, st, C, n, ld, j, jj,
-                         jj > 0 ? 1 : 0, 1, Vc, Wc, Y, C12, TAU, D, E);
-      const int m = n - j - 1;
-      hipLaunchKernelGGL(hemv_kernel, dim3((m + HEMV_ROWS - 1) / HEMV_ROWS, batch), dim3(256), 0, st, C, n, ld, j,
-                         jj, Vc, Wc, Y, C12);
-    }
-    // finalize w of the panel's last column
-    hipLaunchKernelGGL(latrd_col_kernel, dim3(batch), dim3(LATRD_THREADS), 0, st, C, n, ld, jn, jn - j0, 1, 0, Vc,
-                       Wc, Y, C12, TAU, D, E);
-    if (jn < n) {
-      const int m = n - jn, kcols = jn - j0;
-      // A22 -= V W^H + W V^H on rows/cols >= jn
-      launch_zgemm(st, batch, m, m, kcols, -1.0, Vc + jn, ld, sP, OP_T, Wc + jn, ld, sP, OP_CONJ, 1.0,
-                   C + (size_t)jn * ld + jn, ld, sM, 0, 0);
-      launch_zgemm(st, batch, m, m, kcols, -1.0, Wc + jn, ld, sP, OP_T, Vc + jn, ld, sP, OP_CONJ, 1.0,
-                   C + (size_t)jn * ld + jn, ld, sM, 0, 0);
-    }
-  }
-  hipLaunchKernelGGL(hetrd_last_kernel, dim3(batch), dim3(64), 0, st, C, n, ld, D, E, TAU);
-}

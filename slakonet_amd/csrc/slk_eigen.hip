@@ -18,27 +18,39 @@
 #define EIG_THREADS 256
 
 // ---------------------------------------------------------------------------
-// Eigenvalues of the real symmetric tridiagonal (d, e) by bisection on the
-// Sturm count; thread per eigenvalue index, ascending order.
+// Eigenvalues of the real symmetric tridiagonal (d, e) by bisection /
+// multisection on Sturm counts, ascending order.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ int sturm_count(const double* d, const double* e, int n, double x, double pivmin) {
+// 1/q from the hardware approximation refined by two Newton steps (full
+// double precision for the pivots; |q| >= pivmin keeps it finite).
+__device__ __forceinline__ double rcp_refined(double q) {
+  double r = __builtin_amdgcn_rcp(q);
+  r = fma(fma(-q, r, 1.0), r, r);
+  r = fma(fma(-q, r, 1.0), r, r);
+  return r;
+}
+
+// Number of eigenvalues below x; e2[i] = e[i]^2.
+__device__ __forceinline__ int sturm_count(const double* d, const double* e2, int n, double x, double pivmin) {
   int cnt = 0;
   double q = d[0] - x;
   if (fabs(q) < pivmin) q = -pivmin;
   if (q < 0.0) ++cnt;
   for (int i = 1; i < n; ++i) {
-    q = d[i] - x - e[i - 1] * e[i - 1] / q;
+    q = (d[i] - x) - e2[i - 1] * rcp_refined(q);
     if (fabs(q) < pivmin) q = -pivmin;
     if (q < 0.0) ++cnt;
   }
   return cnt;
 }
 
-// Multisection: one wave per eigenvalue; each step evaluates 64 Sturm counts
-// at equidistant interior points and keeps the sub-interval where the count
-// crosses the eigenvalue index (65x shrink per step instead of 2x).
+// Groups of G lanes (G a power of two <= 64) share one eigenvalue; each step
+// evaluates G Sturm counts at equidistant interior points and keeps the
+// sub-interval where the count crosses the eigenvalue index ((G+1)x shrink per
+// step). G = 1 is plain bisection with 64 eigenvalues per wave; larger G
+// trades total work for parallelism when few eigenvalues are in flight.
 __global__ __launch_bounds__(EIG_THREADS) void bisect_kernel(const double* D, const double* Ed, int n, int ld,
-                                                             double* Wout, int wld) {
+                                                             double* Wout, int wld, int G) {
   const int b = blockIdx.y;
   const double* d = D + (size_t)b * ld;
   const double* e = Ed + (size_t)b * ld;
@@ -53,7 +65,7 @@ __global__ __launch_bounds__(EIG_THREADS) void bisect_kernel(const double* D, co
     const double ei = i < n - 1 ? e[i] : 0.0;
     const double ep = i > 0 ? e[i - 1] : 0.0;
     sd[i] = di;
-    se[i] = ei;
+    se[i] = ei * ei;
     const double r = fabs(ep) + fabs(ei);
     lo = fmin(lo, di - r);
     hi = fmax(hi, di + r);
@@ -74,21 +86,38 @@ __global__ __launch_bounds__(EIG_THREADS) void bisect_kernel(const double* D, co
   lo = lo - 2.0 * 2.2e-16 * span * n - 1e-300;
   hi = hi + 2.0 * 2.2e-16 * span * n + 1e-300;
   const double piv = fmax(2.2e-308, 2.2e-308 * emax) * 1e2;
-  const int idx = blockIdx.x * (EIG_THREADS / 64) + wv;
-  if (idx >= n) return;
-  for (int it = 0; it < 64; ++it) {
+  const int per_wave = 64 / G;
+  const int grp = lane / G, sub = lane % G;
+  const int idx = (blockIdx.x * (EIG_THREADS / 64) + wv) * per_wave + grp;
+  if ((blockIdx.x * (EIG_THREADS / 64) + wv) * per_wave >= n) return;   // whole wave idle
+  const unsigned long long gmask = G == 64 ? ~0ULL : ((1ULL << G) - 1ULL);
+  const double inv = 1.0 / (double)(G + 1);
+  bool done = idx >= n;
+  for (int it = 0; it < 256; ++it) {
     const double width = hi - lo;
-    if (width <= 4.0 * 2.2e-16 * fmax(fabs(lo), fabs(hi)) + piv) break;
-    const double x = lo + width * (double)(lane + 1) / 65.0;
-    const unsigned long long mask = __ballot(sturm_count(sd, se, n, x, piv) > idx);
-    const int first = mask ? (__ffsll((long long)mask) - 1) : 64;
-    const double nlo = first == 0 ? lo : lo + width * (double)first / 65.0;
-    const double nhi = first == 64 ? hi : lo + width * (double)(first + 1) / 65.0;
-    if (nlo == lo && nhi == hi) break;
-    lo = nlo;
-    hi = nhi;
+    if (!done && width <= 4.0 * 2.2e-16 * fmax(fabs(lo), fabs(hi)) + piv) done = true;
+    if (__ballot(!done) == 0ULL) break;
+    const double x = lo + width * (double)(sub + 1) * inv;
+    const bool above = !done && sturm_count(sd, se, n, x, piv) > idx;
+    const unsigned long long mask = (__ballot(above) >> (grp * G)) & gmask;
+    if (!done) {
+      const int first = mask ? (__ffsll((long long)mask) - 1) : G;
+      const double nlo = first == 0 ? lo : lo + width * (double)first * inv;
+      const double nhi = first == G ? hi : lo + width * (double)(first + 1) * inv;
+      if (nlo == lo && nhi == hi) done = true;
+      lo = nlo;
+      hi = nhi;
+    }
   }
-  if (lane == 0) Wout[(size_t)b * wld + idx] = 0.5 * (lo + hi);
+  if (sub == 0 && idx < n) Wout[(size_t)b * wld + idx] = 0.5 * (lo + hi);
+}
+
+// lanes per eigenvalue so that about 2^17 lanes are busy
+static int bisect_group(int n, int batch) {
+  const double total = (double)n * batch;
+  int g = 1;
+  while (g < 64 && total * g * 2 <= 131072.0) g *= 2;
+  return g;
 }
 
 // ---------------------------------------------------------------------------
@@ -277,41 +306,45 @@ __global__ __launch_bounds__(EIG_THREADS) void apply_q_kernel(const double2* __r
 
 extern "C" {
 
-// Workspace (per matrix): L/C, W, W*H (and Y), T, panel V and W, Y, C12,
-// D, E, TAU, and the real Z for the vectors path.
-size_t slk_eigh_workspace_bytes(int n, int ld, int batch, int vectors) {
-  (void)n;
-  const size_t mat = (size_t)ld * ld * sizeof(double2);
-  const size_t panel = (size_t)SLK_NB * ld * sizeof(double2);
-  const size_t vec = (size_t)ld * (2 * sizeof(double) + 2 * sizeof(double2)) + 2 * SLK_NB * sizeof(double2);
-  size_t per = 3 * mat + 3 * panel + vec;
-  if (vectors) per += (size_t)ld * ld * sizeof(double);
-  return per * batch + 1024;
-}
-
+// Workspace (per matrix): L/C, W, W*H (and Y), T, panel V and W, Y, the
+// mat-vec partial sums, C12, D, E, TAU, the real Z for the vectors path,
+// and one ticket counter per matrix.
 struct EighWs {
-  double2 *B0, *B1, *B2, *T, *Vc, *Wc, *Y, *C12, *TAU;
+  double2 *B0, *B1, *B2, *T, *Vc, *Wc, *Y, *Yp, *C12, *TAU;
   double *D, *E, *Z;
+  int* cnt;
 };
 
-static EighWs carve(void* ws, int ld, int batch) {
-  EighWs w;
+// Carves the workspace (base may be null to only size it); returns bytes.
+static size_t carve(char* base, int ld, int batch, int vectors, EighWs* w) {
   const size_t matel = (size_t)ld * ld;
-  char* p = reinterpret_cast<char*>(ws);
-  auto take = [&](size_t bytes) { char* q = p; p += (bytes + 255) / 256 * 256; return q; };
-  w.B0 = reinterpret_cast<double2*>(take(matel * batch * sizeof(double2)));
-  w.B1 = reinterpret_cast<double2*>(take(matel * batch * sizeof(double2)));
-  w.B2 = reinterpret_cast<double2*>(take(matel * batch * sizeof(double2)));
-  w.T = reinterpret_cast<double2*>(take((size_t)SLK_NB * ld * batch * sizeof(double2)));
-  w.Vc = reinterpret_cast<double2*>(take((size_t)SLK_NB * ld * batch * sizeof(double2)));
-  w.Wc = reinterpret_cast<double2*>(take((size_t)SLK_NB * ld * batch * sizeof(double2)));
-  w.Y = reinterpret_cast<double2*>(take((size_t)ld * batch * sizeof(double2)));
-  w.C12 = reinterpret_cast<double2*>(take((size_t)2 * SLK_NB * batch * sizeof(double2)));
-  w.TAU = reinterpret_cast<double2*>(take((size_t)ld * batch * sizeof(double2)));
-  w.D = reinterpret_cast<double*>(take((size_t)ld * batch * sizeof(double)));
-  w.E = reinterpret_cast<double*>(take((size_t)ld * batch * sizeof(double)));
-  w.Z = reinterpret_cast<double*>(take(matel * batch * sizeof(double)));
-  return w;
+  size_t off = 0;
+  auto take = [&](size_t bytes) {
+    char* q = base ? base + off : nullptr;
+    off += (bytes + 255) / 256 * 256;
+    return q;
+  };
+  w->B0 = reinterpret_cast<double2*>(take(matel * batch * sizeof(double2)));
+  w->B1 = reinterpret_cast<double2*>(take(matel * batch * sizeof(double2)));
+  w->B2 = reinterpret_cast<double2*>(take(matel * batch * sizeof(double2)));
+  w->T = reinterpret_cast<double2*>(take((size_t)SLK_NB * ld * batch * sizeof(double2)));
+  w->Vc = reinterpret_cast<double2*>(take((size_t)SLK_NB * ld * batch * sizeof(double2)));
+  w->Wc = reinterpret_cast<double2*>(take((size_t)SLK_NB * ld * batch * sizeof(double2)));
+  w->Y = reinterpret_cast<double2*>(take((size_t)ld * batch * sizeof(double2)));
+  w->Yp = reinterpret_cast<double2*>(take(hetrd_partials_elems(ld) * batch * sizeof(double2)));
+  w->C12 = reinterpret_cast<double2*>(take((size_t)2 * SLK_NB * batch * sizeof(double2)));
+  w->TAU = reinterpret_cast<double2*>(take((size_t)ld * batch * sizeof(double2)));
+  w->D = reinterpret_cast<double*>(take((size_t)ld * batch * sizeof(double)));
+  w->E = reinterpret_cast<double*>(take((size_t)ld * batch * sizeof(double)));
+  w->cnt = reinterpret_cast<int*>(take((size_t)batch * sizeof(int) + 16));
+  w->Z = vectors ? reinterpret_cast<double*>(take(matel * batch * sizeof(double))) : nullptr;
+  return off;
+}
+
+size_t slk_eigh_workspace_bytes(int n, int ld, int batch, int vectors) {
+  (void)n;
+  EighWs w;
+  return carve(nullptr, ld, batch, vectors, &w);
 }
 
 // Stage boundaries for optional timing: potrf | trinv | reduce (2 GEMMs) |
@@ -330,7 +363,8 @@ static int eigh_f64_impl(const void* H, const void* S, int n, int ld, int batch,
       for (int i = 1; i < 7; ++i) (void)hipEventRecord(ev[i], stream);
     return rc;
   }
-  EighWs s = carve(ws, ld, batch);
+  EighWs s;
+  carve(reinterpret_cast<char*>(ws), ld, batch, V != nullptr, &s);
   const long long sM = (long long)ld * ld;
   const double2* h = reinterpret_cast<const double2*>(H);
   const double2* sp = reinterpret_cast<const double2*>(S);
@@ -344,11 +378,13 @@ static int eigh_f64_impl(const void* H, const void* S, int n, int ld, int batch,
   launch_zgemm(stream, batch, n, n, n, 1.0, s.B2, ld, sM, OP_N, s.B1, ld, sM, OP_C, 0.0, s.B0, ld, sM, KLIM_B_UPPER,
                0);
   if (ev) (void)hipEventRecord(ev[3], stream);
-  launch_hetrd_blocked(stream, s.B0, n, ld, batch, s.D, s.E, s.TAU, s.Vc, s.Wc, s.Y, s.C12);
+  launch_hetrd_blocked(stream, s.B0, n, ld, batch, s.D, s.E, s.TAU, s.Vc, s.Wc, s.Y, s.Yp, s.C12, s.cnt);
   if (ev) (void)hipEventRecord(ev[4], stream);
   if (V == nullptr) {
-    hipLaunchKernelGGL(bisect_kernel, dim3((n + 3) / 4, batch), dim3(256), 2 * n * sizeof(double), stream, s.D,
-                       s.E, n, ld, w, wld);
+    const int G = bisect_group(n, batch);
+    const int eig_per_block = (EIG_THREADS / 64) * (64 / G);
+    hipLaunchKernelGGL(bisect_kernel, dim3((n + eig_per_block - 1) / eig_per_block, batch), dim3(EIG_THREADS),
+                       2 * n * sizeof(double), stream, s.D, s.E, n, ld, w, wld, G);
     if (ev) (void)hipEventRecord(ev[5], stream);
   } else {
     hipLaunchKernelGGL(tql2_kernel, dim3(batch), dim3(EIG_THREADS), 4 * n * sizeof(double), stream, s.D, s.E, n, ld,
