@@ -5,18 +5,18 @@
 //   per panel:  latrd_start (first column reflector)
 //               per column j: hetrd_step = 2-D tiled trailing mat-vec
 //                   y = A22 v_j - V (W^H v_j) - W (V^H v_j)
-//                 whose last finishing workgroup (atomic ticket, no waiting)
-//                 sums the partial products, forms w_j and the reflector of
+//                 writing per-tile partial products (enough tiles to cover
+//                 the chip for one matrix), then hetrd_tail (one workgroup
+//                 per matrix) sums them, forms w_j and the reflector of
 //                 column j+1;
 //               rank-2k update A22 -= V W^H + W V^H on the GEMM kernel.
-//
-// One launch per column (plus three per panel) instead of two per column.
 // Panel buffers are column-major: Vc[k*ld + r], Wc[k*ld + r].
 #include "slk_common.h"
 #include "slk_dense.h"
 
 #define NB SLK_NB
 #define HT_THREADS 512
+#define TAIL_THREADS 1024
 #define HT_ROWS 16     // rows per tile: 2 per wave
 #define HT_COLS 256    // columns per tile: 4 per lane
 
@@ -140,8 +140,8 @@ __device__ void latrd_column(const HetrdCtx& c, int b, int j, int jj, double* re
   __syncthreads();
 }
 
-__global__ __launch_bounds__(HT_THREADS) void latrd_start_kernel(HetrdCtx c, int j) {
-  __shared__ double red[HT_THREADS / 64];
+__global__ __launch_bounds__(TAIL_THREADS) void latrd_start_kernel(HetrdCtx c, int j) {
+  __shared__ double red[TAIL_THREADS / 64];
   latrd_column(c, blockIdx.x, j, 0, red);
 }
 
@@ -193,27 +193,23 @@ __global__ __launch_bounds__(HT_THREADS) void hetrd_step_kernel(HetrdCtx c, int 
       yp[(size_t)blockIdx.y * ld + r] = s;
     }
   }
-  // ---- ticket: the last workgroup of this matrix finishes the column ----
-  __shared__ int s_last;
-  __threadfence();
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    const int total = gridDim.x * gridDim.y;
-    const int ticket = atomicAdd(&c.cnt[b], 1);
-    s_last = (ticket == total - 1);
-  }
-  __syncthreads();
-  if (!s_last) return;
-  __threadfence();
-  __shared__ double red[HT_THREADS / 64];
+}
+
+// Column tail (one workgroup per matrix): y = sum of the nch partial
+// products, w_j from y, then the reflector of column j+1 if it is in the panel.
+// A kernel boundary orders it after the mat-vec tiles; a device-scope fence
+// per tile flushes L2 on this multi-XCD part and costs far more.
+__global__ __launch_bounds__(TAIL_THREADS) void hetrd_tail_kernel(HetrdCtx c, int j, int jj, int jn, int nch) {
+  __shared__ double red[TAIL_THREADS / 64];
+  const int b = blockIdx.x;
+  const int n = c.n, ld = c.ld;
+  const double2* yp = c.Yp + (size_t)b * c.ncmax * ld;
   double2* y = c.Y + (size_t)b * ld;
-  const int nch = gridDim.y;
   for (int r = j + 1 + threadIdx.x; r < n; r += blockDim.x) {
     double2 s = make_double2(0.0, 0.0);
     for (int ch = 0; ch < nch; ++ch) s = c_add(s, yp[(size_t)ch * ld + r]);
     y[r] = s;
   }
-  if (threadIdx.x == 0) c.cnt[b] = 0;
   __syncthreads();
   latrd_finalize(c, b, j + 1, jj + 1, red);
   if (j + 1 < jn) latrd_column(c, b, j + 1, jj + 1, red);
@@ -239,11 +235,12 @@ void launch_hetrd_blocked(hipStream_t st, double2* C, int n, int ld, int batch, 
   const long long sP = (long long)NB * ld, sM = (long long)ld * ld;
   for (int j0 = 0; j0 < n - 1; j0 += NB) {
     const int jn = min(j0 + NB, n - 1);
-    hipLaunchKernelGGL(latrd_start_kernel, dim3(batch), dim3(HT_THREADS), 0, st, c, j0);
+    hipLaunchKernelGGL(latrd_start_kernel, dim3(batch), dim3(TAIL_THREADS), 0, st, c, j0);
     for (int j = j0; j < jn; ++j) {
       const int m = n - j - 1;
       dim3 g((m + HT_ROWS - 1) / HT_ROWS, (m + HT_COLS - 1) / HT_COLS, batch);
       hipLaunchKernelGGL(hetrd_step_kernel, g, dim3(HT_THREADS), 0, st, c, j, j - j0, jn);
+      hipLaunchKernelGGL(hetrd_tail_kernel, dim3(batch), dim3(TAIL_THREADS), 0, st, c, j, j - j0, jn, (int)g.y);
     }
     if (jn < n) {
       const int m = n - jn, kcols = jn - j0;
