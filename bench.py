@@ -42,6 +42,8 @@ def parse():
     p.add_argument("--mode", choices=["serve", "plain"], default="serve")
     p.add_argument("--cpu-seconds", type=float, default=20.0, help="CPU baseline sample budget (0 = skip)")
     p.add_argument("--no-roofline", action="store_true")
+    p.add_argument("--precision", choices=["mixed", "fp64"], default="mixed",
+                   help="mixed: FP64 Cholesky/inverse + FP32 reduction/tridiagonalisation + FP64 bisection")
     return p.parse_args()
 
 
@@ -64,6 +66,9 @@ def to_input(st):
     return StructureInput(st.z, st.positions * AA_BOHR, st.cell * AA_BOHR, st.kpoints, st.k_weights)
 
 
+PRECISION = "mixed"
+
+
 def run_plain(tables, inputs, arrivals, grid):
     """Sequential replay: each request waits for its arrival and the previous one."""
     import torch
@@ -76,7 +81,7 @@ def run_plain(tables, inputs, arrivals, grid):
             time.sleep(arrivals[i] - now)
         plan = engine.plan_build(tables, [inp])
         H, S = engine.run_build(tables, plan)
-        w, _, _ = engine.eigh(H, S)
+        w, _, _ = engine.eigh(H, S, precision=PRECISION)
         edges, dos, stats = engine.post(plan, w, fermi_mode=0, dos_grid=grid, sigma=0.1)
         edges.cpu()
         lat[i] = time.perf_counter() - t0 - arrivals[i]
@@ -86,7 +91,7 @@ def run_plain(tables, inputs, arrivals, grid):
 
 def run_serve(tables, inputs, arrivals, grid):
     from slakonet_amd.serving import ServingEngine
-    eng = ServingEngine(tables, dos_grid=grid)
+    eng = ServingEngine(tables, dos_grid=grid, precision=PRECISION)
     res = eng.replay(inputs, arrivals)
     return res["wall_s"], res["latency_s"]
 
@@ -114,7 +119,7 @@ def roofline(tables, inputs):
         n = plan.n_orb[0]
         build_bytes += plan.n_mat * 2 * n * n * 16
         ms = []
-        engine.eigh(H, S, stage_ms=ms)
+        engine.eigh(H, S, stage_ms=ms, precision=PRECISION)
         for k, v in ms[0].items():
             stage_tot[k] = stage_tot.get(k, 0.0) + v
         flops += plan.n_mat * (4.0 / 3.0) * 4 * n ** 3   # complex hetrd, LAPACK count
@@ -152,6 +157,8 @@ def cpu_baseline(structs, skfs, budget_s):
 
 def main():
     args = parse()
+    global PRECISION
+    PRECISION = args.precision
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported (the stream is data-parallel "
               "per request and would shard by request across GPUs)", file=sys.stderr)
@@ -187,7 +194,8 @@ def main():
     out = {
         "metric": METRIC, "value": value, "unit": "structures/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": 1000.0 * total / args.steps, "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64" if PRECISION == "mixed" else "f64",
+        "data": "synthetic",
         "p50_latency_ms": float(np.percentile(lat, 50) * 1000), "p99_latency_ms": float(np.percentile(lat, 99) * 1000),
         "config": {"workload": f"serving stream: {args.requests} requests, atoms log-uniform 2-256, 1-4 species "
                                f"Z<=65, Poisson {args.rate:g}/s, full spectrum + Fermi + gap + 2001-pt DOS",

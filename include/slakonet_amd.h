@@ -89,6 +89,15 @@ int slk_pad_launch(const SlkStructDesc* structs_dev, int s, int nk, int n_orb, i
 size_t slk_eigh_workspace_bytes(int n, int ld, int batch, int vectors);
 int slk_eigh_f64(const void* H, const void* S, int n, int ld, int batch, double* w, int wld, void* V,
                  void* ws, size_t ws_bytes, int* info, hipStream_t stream);
+/* Precision form, eigenvalues only: Cholesky of S and L^-1 in FP64, the
+ * reduction C = L^-1 H L^-H (MFMA GEMMs) and the Householder
+ * tridiagonalisation in FP32, Sturm bisection of T in FP64. Same arguments as
+ * slk_eigh_f64 without V; workspace from slk_eigh_mixed_workspace_bytes. */
+size_t slk_eigh_mixed_workspace_bytes(int n, int ld, int batch);
+int slk_eigh_mixed(const void* H, const void* S, int n, int ld, int batch, double* w, int wld, void* ws,
+                   size_t ws_bytes, int* info, hipStream_t stream);
+int slk_eigh_mixed_timed(const void* H, const void* S, int n, int ld, int batch, double* w, int wld, void* ws,
+                         size_t ws_bytes, int* info, hipStream_t stream, float* stage_ms);
 /* Eigenvalues only for n <= 32: the whole solve of one matrix in LDS of one
  * wave64 workgroup, one launch per batch (slk_eigh_f64 dispatches here for
  * values-only solves of that size). info: [batch]. */
@@ -100,6 +109,17 @@ int slk_eigh_small_f64(const void* H, const void* S, int n, int ld, int batch, d
  * no reference counterpart. */
 int slk_eigh_f64_timed(const void* H, const void* S, int n, int ld, int batch, double* w, int wld, void* V,
                        void* ws, size_t ws_bytes, int* info, hipStream_t stream, float* stage_ms);
+
+/* Batched complex GEMM on MFMA (complex128 / complex64, row-major):
+ * C = alpha opA(A) opB(B) + beta C, op bit0 = transpose, bit1 = conjugate;
+ * sA/sB/sC are batch strides in elements. Building block of the solver (no
+ * reference counterpart), exposed for verification and reuse. */
+int slk_zgemm(int batch, int M, int N, int K, double alpha, const void* A, int lda, long long sA, int opA,
+              const void* B, int ldb, long long sB, int opB, double beta, void* C, int ldc, long long sC,
+              hipStream_t stream);
+int slk_cgemm(int batch, int M, int N, int K, float alpha, const void* A, int lda, long long sA, int opA,
+              const void* B, int ldb, long long sB, int opB, float beta, void* C, int ldc, long long sC,
+              hipStream_t stream);
 
 /* Eigenvalue spectrum of one structure for the reductions below. */
 typedef struct SlkSpectrum {

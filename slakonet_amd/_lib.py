@@ -58,11 +58,22 @@ SIGNATURES = {
     "slk_eigh_workspace_bytes": (C.c_size_t, [C.c_int, C.c_int, C.c_int, C.c_int]),
     "slk_eigh_f64": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_int,
                                C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p, C.c_void_p]),
+    "slk_eigh_mixed_workspace_bytes": (C.c_size_t, [C.c_int, C.c_int, C.c_int]),
+    "slk_eigh_mixed": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_int,
+                                 C.c_void_p, C.c_size_t, C.c_void_p, C.c_void_p]),
+    "slk_eigh_mixed_timed": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_int,
+                                       C.c_void_p, C.c_size_t, C.c_void_p, C.c_void_p, C.POINTER(C.c_float)]),
     "slk_eigh_small_f64": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_int,
                                      C.c_void_p, C.c_void_p]),
     "slk_eigh_f64_timed": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_int,
                                      C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p, C.c_void_p,
                                      C.POINTER(C.c_float)]),
+    "slk_zgemm": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, C.c_void_p, C.c_int, C.c_longlong,
+                            C.c_int, C.c_void_p, C.c_int, C.c_longlong, C.c_int, C.c_double, C.c_void_p, C.c_int,
+                            C.c_longlong, C.c_void_p]),
+    "slk_cgemm": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_float, C.c_void_p, C.c_int, C.c_longlong,
+                            C.c_int, C.c_void_p, C.c_int, C.c_longlong, C.c_int, C.c_float, C.c_void_p, C.c_int,
+                            C.c_longlong, C.c_void_p]),
     "slk_fermi_launch": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_double, C.c_int,
                                    C.c_double, C.c_int, C.c_void_p, C.c_void_p]),
     "slk_edges_launch": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_double, C.c_void_p, C.c_void_p]),

@@ -13,6 +13,10 @@ void launch_zgemm(hipStream_t st, int batch, int M, int N, int K, double alpha, 
                   long long sA, int opA, const double2* B, int ldb, long long sB, int opB, double beta, double2* C,
                   int ldc, long long sC, int klim, int tri);
 
+void launch_cgemm(hipStream_t st, int batch, int M, int N, int K, float alpha, const float2* A, int lda,
+                  long long sA, int opA, const float2* B, int ldb, long long sB, int opB, float beta, float2* C,
+                  int ldc, long long sC, int klim, int tri);
+
 // Zero-fill (no memset nodes so that the solve can be graph-captured).
 void launch_zero(hipStream_t st, void* p, size_t bytes);
 
@@ -28,9 +32,14 @@ void launch_trinv_blocked(hipStream_t st, const double2* L, double2* W, double2*
 // Blocked Householder tridiagonalisation of the Hermitian C (in place): d, e
 // (real tridiagonal), tau, reflectors in the strict lower part of C.
 // Workspace: Vc, Wc [NB][ld], Y [ld], Yp [hetrd_partials_elems(ld)], C12
-// [2 NB] per matrix, cnt [batch] ints.
+// [2 NB] per matrix. D, E are FP64 for both precisions.
 void launch_hetrd_blocked(hipStream_t st, double2* C, int n, int ld, int batch, double* D, double* E, double2* TAU,
-                          double2* Vc, double2* Wc, double2* Y, double2* Yp, double2* C12, int* cnt);
+                          double2* Vc, double2* Wc, double2* Y, double2* Yp, double2* C12);
+void launch_hetrd_blocked_f32(hipStream_t st, float2* C, int n, int ld, int batch, double* D, double* E,
+                              float2* TAU, float2* Vc, float2* Wc, float2* Y, float2* Yp, float2* C12);
 size_t hetrd_partials_elems(int ld);
+
+// complex128 -> complex64 copy of `count` elements.
+void launch_to_c64(hipStream_t st, const double2* src, float2* dst, size_t count);
 
 constexpr int SLK_NB = 32;   // panel width

@@ -11,34 +11,65 @@
 #define NB SLK_NB
 
 // ---------------------------------------------------------------------------
-// complex FP64 GEMM, 64x64 tile, 4x4 outputs per thread
+// Complex GEMM on MFMA: C = alpha opA(A) opB(B) + beta C.
+// 64x64 output tile per 256-thread workgroup (4 waves, 32x32 each as 2x2
+// 16x16 MFMA blocks), K staged through LDS 16 at a time as separate real and
+// imaginary planes. Complex product = 4 real MFMAs per block:
+// Re += Ar Br - Ai Bi, Im += Ar Bi + Ai Br.
+// Operand layout of the 16x16x4 MFMAs (f64 and f32): lane l supplies
+// A[l%16][l/16] and B[l/16][l%16]; it receives D[4*(l/16)+v][l%16], v < 4.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ double2 load_op(const double2* p, int ld, int r, int c, int op) {
+template <typename T> struct CT;
+template <> struct CT<double> {
+  using c2 = double2;
+  using v4 = __attribute__((ext_vector_type(4))) double;
+  static __device__ __forceinline__ v4 mfma(double a, double b, v4 c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+};
+template <> struct CT<float> {
+  using c2 = float2;
+  using v4 = __attribute__((ext_vector_type(4))) float;
+  static __device__ __forceinline__ v4 mfma(float a, float b, v4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+};
+
+template <typename C2>
+__device__ __forceinline__ C2 load_op(const C2* p, int ld, int r, int c, int op) {
   // element (r, c) of op(X)
-  double2 v = (op & OP_T) ? p[(size_t)c * ld + r] : p[(size_t)r * ld + c];
+  C2 v = (op & OP_T) ? p[(size_t)c * ld + r] : p[(size_t)r * ld + c];
   if (op & OP_CONJ) v.y = -v.y;
   return v;
 }
 
-__global__ __launch_bounds__(256) void zgemm_kernel(int M, int N, int K, double alpha, const double2* __restrict__ A,
-                                                    int lda, long long sA, int opA, const double2* __restrict__ B,
-                                                    int ldb, long long sB, int opB, double beta, double2* C, int ldc,
-                                                    long long sC, int klim, int tri) {
+template <typename T>
+__global__ __launch_bounds__(256) void gemm_mfma_kernel(int M, int N, int K, T alpha,
+                                                        const typename CT<T>::c2* __restrict__ A, int lda,
+                                                        long long sA, int opA,
+                                                        const typename CT<T>::c2* __restrict__ B, int ldb,
+                                                        long long sB, int opB, T beta, typename CT<T>::c2* C,
+                                                        int ldc, long long sC, int klim, int tri) {
+  using C2 = typename CT<T>::c2;
+  using V4 = typename CT<T>::v4;
   const int m0 = blockIdx.y * GT, n0 = blockIdx.x * GT;
   if (tri && n0 > m0 + GT - 1) return;
-  __shared__ double2 As[GK][GT + 1];
-  __shared__ double2 Bs[GK][GT + 1];
+  __shared__ T Ar[GK][GT + 4], Ai[GK][GT + 4], Br[GK][GT + 4], Bi[GK][GT + 4];
   const int b = blockIdx.z;
-  const double2* a = A + (size_t)b * sA;
-  const double2* bb = B + (size_t)b * sB;
-  double2* c = C + (size_t)b * sC;
-  const int tid = threadIdx.x;
-  const int tr = tid / 16, tc = tid % 16;
-  double2 acc[4][4];
+  const C2* a = A + (size_t)b * sA;
+  const C2* bb = B + (size_t)b * sB;
+  C2* c = C + (size_t)b * sC;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int wr = (wv >> 1) * 32, wc = (wv & 1) * 32;
+  const int li = lane & 15, lk = lane >> 4;
+  V4 accR[2][2], accI[2][2];
 #pragma unroll
-  for (int i = 0; i < 4; ++i)
+  for (int i = 0; i < 2; ++i)
 #pragma unroll
-    for (int j = 0; j < 4; ++j) acc[i][j] = make_double2(0.0, 0.0);
+    for (int j = 0; j < 2; ++j) {
+      accR[i][j] = V4{0, 0, 0, 0};
+      accI[i][j] = V4{0, 0, 0, 0};
+    }
   int kbeg = 0, kend = K;
   if (klim & KLIM_A_LOWER) kend = min(kend, m0 + GT);
   if (klim & KLIM_A_UPPER) kbeg = max(kbeg, m0);
@@ -50,47 +81,66 @@ __global__ __launch_bounds__(256) void zgemm_kernel(int M, int N, int K, double 
       int mm, kk;
       if (opA & OP_T) { kk = e / GT; mm = e % GT; } else { mm = e / GK; kk = e % GK; }
       const int gm = m0 + mm, gk = k0 + kk;
-      As[kk][mm] = (gm < M && gk < K) ? load_op(a, lda, gm, gk, opA) : make_double2(0.0, 0.0);
+      C2 v;
+      if (gm < M && gk < K) v = load_op(a, lda, gm, gk, opA); else { v.x = 0; v.y = 0; }
+      Ar[kk][mm] = v.x;
+      Ai[kk][mm] = v.y;
     }
     for (int e = tid; e < GT * GK; e += 256) {
       int nn, kk;
       if (opB & OP_T) { nn = e / GK; kk = e % GK; } else { kk = e / GT; nn = e % GT; }
       const int gn = n0 + nn, gk = k0 + kk;
-      Bs[kk][nn] = (gn < N && gk < K) ? load_op(bb, ldb, gk, gn, opB) : make_double2(0.0, 0.0);
+      C2 v;
+      if (gn < N && gk < K) v = load_op(bb, ldb, gk, gn, opB); else { v.x = 0; v.y = 0; }
+      Br[kk][nn] = v.x;
+      Bi[kk][nn] = v.y;
     }
     __syncthreads();
-#pragma unroll 4
-    for (int kk = 0; kk < GK; ++kk) {
-      double2 av[4], bv[4];
 #pragma unroll
-      for (int i = 0; i < 4; ++i) av[i] = As[kk][tr + 16 * i];
+    for (int k4 = 0; k4 < GK; k4 += 4) {
+      T ar[2], ai[2], nai[2], br[2], bi[2];
 #pragma unroll
-      for (int j = 0; j < 4; ++j) bv[j] = Bs[kk][tc + 16 * j];
+      for (int i = 0; i < 2; ++i) {
+        ar[i] = Ar[k4 + lk][wr + 16 * i + li];
+        ai[i] = Ai[k4 + lk][wr + 16 * i + li];
+        nai[i] = -ai[i];
+      }
 #pragma unroll
-      for (int i = 0; i < 4; ++i)
+      for (int j = 0; j < 2; ++j) {
+        br[j] = Br[k4 + lk][wc + 16 * j + li];
+        bi[j] = Bi[k4 + lk][wc + 16 * j + li];
+      }
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-          acc[i][j].x += av[i].x * bv[j].x - av[i].y * bv[j].y;
-          acc[i][j].y += av[i].x * bv[j].y + av[i].y * bv[j].x;
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+          accR[i][j] = CT<T>::mfma(ar[i], br[j], accR[i][j]);
+          accR[i][j] = CT<T>::mfma(nai[i], bi[j], accR[i][j]);
+          accI[i][j] = CT<T>::mfma(ar[i], bi[j], accI[i][j]);
+          accI[i][j] = CT<T>::mfma(ai[i], br[j], accI[i][j]);
         }
     }
     __syncthreads();
   }
 #pragma unroll
-  for (int i = 0; i < 4; ++i)
+  for (int i = 0; i < 2; ++i)
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const int gm = m0 + tr + 16 * i, gn = n0 + tc + 16 * j;
-      if (gm < M && gn < N) {
-        double2 r = make_double2(alpha * acc[i][j].x, alpha * acc[i][j].y);
-        if (beta != 0.0) {
-          const double2 o = c[(size_t)gm * ldc + gn];
-          r.x += beta * o.x;
-          r.y += beta * o.y;
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int gm = m0 + wr + 16 * i + 4 * lk + v, gn = n0 + wc + 16 * j + li;
+        if (gm < M && gn < N) {
+          C2 r;
+          r.x = alpha * accR[i][j][v];
+          r.y = alpha * accI[i][j][v];
+          if (beta != T(0)) {
+            const C2 o = c[(size_t)gm * ldc + gn];
+            r.x += beta * o.x;
+            r.y += beta * o.y;
+          }
+          c[(size_t)gm * ldc + gn] = r;
         }
-        c[(size_t)gm * ldc + gn] = r;
       }
-    }
 }
 
 void launch_zgemm(hipStream_t st, int batch, int M, int N, int K, double alpha, const double2* A, int lda,
@@ -98,8 +148,17 @@ void launch_zgemm(hipStream_t st, int batch, int M, int N, int K, double alpha, 
                   int ldc, long long sC, int klim, int tri) {
   if (M <= 0 || N <= 0 || batch <= 0) return;
   dim3 g((N + GT - 1) / GT, (M + GT - 1) / GT, batch);
-  hipLaunchKernelGGL(zgemm_kernel, g, dim3(256), 0, st, M, N, K, alpha, A, lda, sA, opA, B, ldb, sB, opB, beta, C,
-                     ldc, sC, klim, tri);
+  hipLaunchKernelGGL(gemm_mfma_kernel<double>, g, dim3(256), 0, st, M, N, K, alpha, A, lda, sA, opA, B, ldb, sB,
+                     opB, beta, C, ldc, sC, klim, tri);
+}
+
+void launch_cgemm(hipStream_t st, int batch, int M, int N, int K, float alpha, const float2* A, int lda,
+                  long long sA, int opA, const float2* B, int ldb, long long sB, int opB, float beta, float2* C,
+                  int ldc, long long sC, int klim, int tri) {
+  if (M <= 0 || N <= 0 || batch <= 0) return;
+  dim3 g((N + GT - 1) / GT, (M + GT - 1) / GT, batch);
+  hipLaunchKernelGGL(gemm_mfma_kernel<float>, g, dim3(256), 0, st, M, N, K, alpha, A, lda, sA, opA, B, ldb, sB,
+                     opB, beta, C, ldc, sC, klim, tri);
 }
 
 __global__ void zero_kernel(uint4* p, size_t n16) {
@@ -240,3 +299,40 @@ void launch_trinv_blocked(hipStream_t st, const double2* L, double2* W, double2*
   }
 }
 
+
+extern "C" {
+
+// Batched complex GEMM entry points (C-ABI), used by the eigensolver and
+// exposed for verification: C = alpha opA(A) opB(B) + beta C.
+int slk_zgemm(int batch, int M, int N, int K, double alpha, const void* A, int lda, long long sA, int opA,
+              const void* B, int ldb, long long sB, int opB, double beta, void* C, int ldc, long long sC,
+              hipStream_t stream) {
+  launch_zgemm(stream, batch, M, N, K, alpha, reinterpret_cast<const double2*>(A), lda, sA, opA,
+               reinterpret_cast<const double2*>(B), ldb, sB, opB, beta, reinterpret_cast<double2*>(C), ldc, sC, 0,
+               0);
+  return (int)hipGetLastError();
+}
+
+int slk_cgemm(int batch, int M, int N, int K, float alpha, const void* A, int lda, long long sA, int opA,
+              const void* B, int ldb, long long sB, int opB, float beta, void* C, int ldc, long long sC,
+              hipStream_t stream) {
+  launch_cgemm(stream, batch, M, N, K, alpha, reinterpret_cast<const float2*>(A), lda, sA, opA,
+               reinterpret_cast<const float2*>(B), ldb, sB, opB, beta, reinterpret_cast<float2*>(C), ldc, sC, 0, 0);
+  return (int)hipGetLastError();
+}
+
+}  // extern "C"
+
+__global__ void to_c64_kernel(const double2* __restrict__ src, float2* __restrict__ dst, size_t count) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += (size_t)gridDim.x * blockDim.x) {
+    const double2 v = src[i];
+    dst[i] = make_float2((float)v.x, (float)v.y);
+  }
+}
+
+void launch_to_c64(hipStream_t st, const double2* src, float2* dst, size_t count) {
+  if (count == 0) return;
+  int blocks = (int)((count + 255) / 256);
+  if (blocks > 8192) blocks = 8192;
+  hipLaunchKernelGGL(to_c64_kernel, dim3(blocks), dim3(256), 0, st, src, dst, count);
+}

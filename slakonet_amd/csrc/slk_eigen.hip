@@ -306,17 +306,18 @@ __global__ __launch_bounds__(EIG_THREADS) void apply_q_kernel(const double2* __r
 
 extern "C" {
 
-// Workspace (per matrix): L/C, W, W*H (and Y), T, panel V and W, Y, the
-// mat-vec partial sums, C12, D, E, TAU, the real Z for the vectors path,
-// and one ticket counter per matrix.
+// Workspace (per matrix). FP64 path: L/C, W, W*H (and Y for vectors), the
+// trinv temporary T, panel V and W, Y, the mat-vec partial sums, C12, TAU, D,
+// E, and the real Z for the vectors path. Mixed path: L and W (FP64), T, then
+// FP32 copies W32, H32 (reused for C), W32*H32 and FP32 panels.
 struct EighWs {
   double2 *B0, *B1, *B2, *T, *Vc, *Wc, *Y, *Yp, *C12, *TAU;
+  float2 *F0, *F1, *F2, *fVc, *fWc, *fY, *fYp, *fC12, *fTAU;
   double *D, *E, *Z;
-  int* cnt;
 };
 
 // Carves the workspace (base may be null to only size it); returns bytes.
-static size_t carve(char* base, int ld, int batch, int vectors, EighWs* w) {
+static size_t carve(char* base, int ld, int batch, int vectors, int mixed, EighWs* w) {
   const size_t matel = (size_t)ld * ld;
   size_t off = 0;
   auto take = [&](size_t bytes) {
@@ -324,35 +325,58 @@ static size_t carve(char* base, int ld, int batch, int vectors, EighWs* w) {
     off += (bytes + 255) / 256 * 256;
     return q;
   };
+  *w = EighWs{};
   w->B0 = reinterpret_cast<double2*>(take(matel * batch * sizeof(double2)));
   w->B1 = reinterpret_cast<double2*>(take(matel * batch * sizeof(double2)));
-  w->B2 = reinterpret_cast<double2*>(take(matel * batch * sizeof(double2)));
   w->T = reinterpret_cast<double2*>(take((size_t)SLK_NB * ld * batch * sizeof(double2)));
-  w->Vc = reinterpret_cast<double2*>(take((size_t)SLK_NB * ld * batch * sizeof(double2)));
-  w->Wc = reinterpret_cast<double2*>(take((size_t)SLK_NB * ld * batch * sizeof(double2)));
-  w->Y = reinterpret_cast<double2*>(take((size_t)ld * batch * sizeof(double2)));
-  w->Yp = reinterpret_cast<double2*>(take(hetrd_partials_elems(ld) * batch * sizeof(double2)));
-  w->C12 = reinterpret_cast<double2*>(take((size_t)2 * SLK_NB * batch * sizeof(double2)));
-  w->TAU = reinterpret_cast<double2*>(take((size_t)ld * batch * sizeof(double2)));
   w->D = reinterpret_cast<double*>(take((size_t)ld * batch * sizeof(double)));
   w->E = reinterpret_cast<double*>(take((size_t)ld * batch * sizeof(double)));
-  w->cnt = reinterpret_cast<int*>(take((size_t)batch * sizeof(int) + 16));
-  w->Z = vectors ? reinterpret_cast<double*>(take(matel * batch * sizeof(double))) : nullptr;
+  if (!mixed) {
+    w->B2 = reinterpret_cast<double2*>(take(matel * batch * sizeof(double2)));
+    w->Vc = reinterpret_cast<double2*>(take((size_t)SLK_NB * ld * batch * sizeof(double2)));
+    w->Wc = reinterpret_cast<double2*>(take((size_t)SLK_NB * ld * batch * sizeof(double2)));
+    w->Y = reinterpret_cast<double2*>(take((size_t)ld * batch * sizeof(double2)));
+    w->Yp = reinterpret_cast<double2*>(take(hetrd_partials_elems(ld) * batch * sizeof(double2)));
+    w->C12 = reinterpret_cast<double2*>(take((size_t)2 * SLK_NB * batch * sizeof(double2)));
+    w->TAU = reinterpret_cast<double2*>(take((size_t)ld * batch * sizeof(double2)));
+    w->Z = vectors ? reinterpret_cast<double*>(take(matel * batch * sizeof(double))) : nullptr;
+  } else {
+    w->F0 = reinterpret_cast<float2*>(take(matel * batch * sizeof(float2)));
+    w->F1 = reinterpret_cast<float2*>(take(matel * batch * sizeof(float2)));
+    w->F2 = reinterpret_cast<float2*>(take(matel * batch * sizeof(float2)));
+    w->fVc = reinterpret_cast<float2*>(take((size_t)SLK_NB * ld * batch * sizeof(float2)));
+    w->fWc = reinterpret_cast<float2*>(take((size_t)SLK_NB * ld * batch * sizeof(float2)));
+    w->fY = reinterpret_cast<float2*>(take((size_t)ld * batch * sizeof(float2)));
+    w->fYp = reinterpret_cast<float2*>(take(hetrd_partials_elems(ld) * batch * sizeof(float2)));
+    w->fC12 = reinterpret_cast<float2*>(take((size_t)2 * SLK_NB * batch * sizeof(float2)));
+    w->fTAU = reinterpret_cast<float2*>(take((size_t)ld * batch * sizeof(float2)));
+  }
   return off;
 }
 
 size_t slk_eigh_workspace_bytes(int n, int ld, int batch, int vectors) {
   (void)n;
   EighWs w;
-  return carve(nullptr, ld, batch, vectors, &w);
+  return carve(nullptr, ld, batch, vectors, 0, &w);
+}
+
+size_t slk_eigh_mixed_workspace_bytes(int n, int ld, int batch) {
+  (void)n;
+  EighWs w;
+  return carve(nullptr, ld, batch, 0, 1, &w);
 }
 
 // Stage boundaries for optional timing: potrf | trinv | reduce (2 GEMMs) |
 // hetrd | tridiagonal solve | back-transform. ev (nullable) holds 7 events.
-static int eigh_f64_impl(const void* H, const void* S, int n, int ld, int batch, double* w, int wld, void* V,
-                         void* ws, size_t ws_bytes, int* info, hipStream_t stream, hipEvent_t* ev) {
+// mixed: FP64 Cholesky and inverse, FP32 reduction and tridiagonalisation,
+// FP64 bisection (values only).
+static int eigh_impl(const void* H, const void* S, int n, int ld, int batch, double* w, int wld, void* V,
+                     void* ws, size_t ws_bytes, int* info, hipStream_t stream, hipEvent_t* ev, int mixed) {
   if (batch <= 0 || n <= 0) return 0;
-  if (ws_bytes < slk_eigh_workspace_bytes(n, ld, batch, V != nullptr)) return 2;
+  if (mixed && V != nullptr) return 3;
+  const size_t need = mixed ? slk_eigh_mixed_workspace_bytes(n, ld, batch)
+                            : slk_eigh_workspace_bytes(n, ld, batch, V != nullptr);
+  if (ws_bytes < need) return 2;
   // LDS-resident tridiagonal data: values need 2n doubles, vectors 4n doubles (<= 64 KiB)
   if (n > 4096 || (V != nullptr && n > 2048)) return 3;
   if (V == nullptr && n <= 32) {
@@ -364,7 +388,7 @@ static int eigh_f64_impl(const void* H, const void* S, int n, int ld, int batch,
     return rc;
   }
   EighWs s;
-  carve(reinterpret_cast<char*>(ws), ld, batch, V != nullptr, &s);
+  carve(reinterpret_cast<char*>(ws), ld, batch, V != nullptr, mixed, &s);
   const long long sM = (long long)ld * ld;
   const double2* h = reinterpret_cast<const double2*>(H);
   const double2* sp = reinterpret_cast<const double2*>(S);
@@ -373,12 +397,24 @@ static int eigh_f64_impl(const void* H, const void* S, int n, int ld, int batch,
   if (ev) (void)hipEventRecord(ev[1], stream);
   launch_trinv_blocked(stream, s.B0, s.B1, s.T, n, ld, batch);
   if (ev) (void)hipEventRecord(ev[2], stream);
-  // B2 = W H (W lower), B0 = B2 W^H (W^H upper)
-  launch_zgemm(stream, batch, n, n, n, 1.0, s.B1, ld, sM, OP_N, h, ld, sM, OP_N, 0.0, s.B2, ld, sM, KLIM_A_LOWER, 0);
-  launch_zgemm(stream, batch, n, n, n, 1.0, s.B2, ld, sM, OP_N, s.B1, ld, sM, OP_C, 0.0, s.B0, ld, sM, KLIM_B_UPPER,
-               0);
-  if (ev) (void)hipEventRecord(ev[3], stream);
-  launch_hetrd_blocked(stream, s.B0, n, ld, batch, s.D, s.E, s.TAU, s.Vc, s.Wc, s.Y, s.Yp, s.C12, s.cnt);
+  if (!mixed) {
+    // B2 = W H (W lower), B0 = B2 W^H (W^H upper)
+    launch_zgemm(stream, batch, n, n, n, 1.0, s.B1, ld, sM, OP_N, h, ld, sM, OP_N, 0.0, s.B2, ld, sM, KLIM_A_LOWER,
+                 0);
+    launch_zgemm(stream, batch, n, n, n, 1.0, s.B2, ld, sM, OP_N, s.B1, ld, sM, OP_C, 0.0, s.B0, ld, sM,
+                 KLIM_B_UPPER, 0);
+    if (ev) (void)hipEventRecord(ev[3], stream);
+    launch_hetrd_blocked(stream, s.B0, n, ld, batch, s.D, s.E, s.TAU, s.Vc, s.Wc, s.Y, s.Yp, s.C12);
+  } else {
+    launch_to_c64(stream, s.B1, s.F0, (size_t)batch * ld * ld);
+    launch_to_c64(stream, h, s.F1, (size_t)batch * ld * ld);
+    launch_cgemm(stream, batch, n, n, n, 1.0f, s.F0, ld, sM, OP_N, s.F1, ld, sM, OP_N, 0.0f, s.F2, ld, sM,
+                 KLIM_A_LOWER, 0);
+    launch_cgemm(stream, batch, n, n, n, 1.0f, s.F2, ld, sM, OP_N, s.F0, ld, sM, OP_C, 0.0f, s.F1, ld, sM,
+                 KLIM_B_UPPER, 0);
+    if (ev) (void)hipEventRecord(ev[3], stream);
+    launch_hetrd_blocked_f32(stream, s.F1, n, ld, batch, s.D, s.E, s.fTAU, s.fVc, s.fWc, s.fY, s.fYp, s.fC12);
+  }
   if (ev) (void)hipEventRecord(ev[4], stream);
   if (V == nullptr) {
     const int G = bisect_group(n, batch);
@@ -400,23 +436,38 @@ static int eigh_f64_impl(const void* H, const void* S, int n, int ld, int batch,
   return (int)hipGetLastError();
 }
 
-int slk_eigh_f64(const void* H, const void* S, int n, int ld, int batch, double* w, int wld, void* V,
-                 void* ws, size_t ws_bytes, int* info, hipStream_t stream) {
-  return eigh_f64_impl(H, S, n, ld, batch, w, wld, V, ws, ws_bytes, info, stream, nullptr);
-}
-
-int slk_eigh_f64_timed(const void* H, const void* S, int n, int ld, int batch, double* w, int wld, void* V,
-                       void* ws, size_t ws_bytes, int* info, hipStream_t stream, float* stage_ms) {
+static int eigh_timed(const void* H, const void* S, int n, int ld, int batch, double* w, int wld, void* V, void* ws,
+                      size_t ws_bytes, int* info, hipStream_t stream, float* stage_ms, int mixed) {
   hipEvent_t ev[7];
   for (int i = 0; i < 7; ++i)
     if (hipEventCreate(&ev[i]) != hipSuccess) return 4;
-  int rc = eigh_f64_impl(H, S, n, ld, batch, w, wld, V, ws, ws_bytes, info, stream, ev);
+  int rc = eigh_impl(H, S, n, ld, batch, w, wld, V, ws, ws_bytes, info, stream, ev, mixed);
   if (rc == 0) {
     rc = (int)hipEventSynchronize(ev[6]);
     for (int i = 0; i < 6 && rc == 0; ++i) rc = (int)hipEventElapsedTime(&stage_ms[i], ev[i], ev[i + 1]);
   }
   for (int i = 0; i < 7; ++i) (void)hipEventDestroy(ev[i]);
   return rc;
+}
+
+int slk_eigh_f64(const void* H, const void* S, int n, int ld, int batch, double* w, int wld, void* V,
+                 void* ws, size_t ws_bytes, int* info, hipStream_t stream) {
+  return eigh_impl(H, S, n, ld, batch, w, wld, V, ws, ws_bytes, info, stream, nullptr, 0);
+}
+
+int slk_eigh_f64_timed(const void* H, const void* S, int n, int ld, int batch, double* w, int wld, void* V,
+                       void* ws, size_t ws_bytes, int* info, hipStream_t stream, float* stage_ms) {
+  return eigh_timed(H, S, n, ld, batch, w, wld, V, ws, ws_bytes, info, stream, stage_ms, 0);
+}
+
+int slk_eigh_mixed(const void* H, const void* S, int n, int ld, int batch, double* w, int wld, void* ws,
+                   size_t ws_bytes, int* info, hipStream_t stream) {
+  return eigh_impl(H, S, n, ld, batch, w, wld, nullptr, ws, ws_bytes, info, stream, nullptr, 1);
+}
+
+int slk_eigh_mixed_timed(const void* H, const void* S, int n, int ld, int batch, double* w, int wld, void* ws,
+                         size_t ws_bytes, int* info, hipStream_t stream, float* stage_ms) {
+  return eigh_timed(H, S, n, ld, batch, w, wld, nullptr, ws, ws_bytes, info, stream, stage_ms, 1);
 }
 
 }  // extern "C"

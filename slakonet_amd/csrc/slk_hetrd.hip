@@ -1,6 +1,7 @@
 // Blocked Householder tridiagonalisation C = Q T Q^H of a batch of Hermitian
 // matrices (full storage, row-major, lower reflectors), LAPACK zhetrd/zlatrd
-// structure with 32-column panels:
+// structure with 32-column panels, in FP64 or FP32 (the tridiagonal T is
+// always returned in FP64):
 //
 //   per panel:  latrd_start (first column reflector)
 //               per column j: hetrd_step = 2-D tiled trailing mat-vec
@@ -9,7 +10,9 @@
 //                 the chip for one matrix), then hetrd_tail (one workgroup
 //                 per matrix) sums them, forms w_j and the reflector of
 //                 column j+1;
-//               rank-2k update A22 -= V W^H + W V^H on the GEMM kernel.
+//               rank-2k update A22 -= V W^H + W V^H on the MFMA GEMM.
+// A kernel boundary orders the tail after the tiles; a device-scope fence
+// per tile flushes L2 on this multi-XCD part and costs far more.
 // Panel buffers are column-major: Vc[k*ld + r], Wc[k*ld + r].
 #include "slk_common.h"
 #include "slk_dense.h"
@@ -20,36 +23,51 @@
 #define HT_ROWS 16     // rows per tile: 2 per wave
 #define HT_COLS 256    // columns per tile: 4 per lane
 
+template <typename T> struct C2T;
+template <> struct C2T<double> { using t = double2; };
+template <> struct C2T<float> { using t = float2; };
+
+template <typename T>
+__device__ __forceinline__ typename C2T<T>::t mk(T x, T y) {
+  typename C2T<T>::t r;
+  r.x = x;
+  r.y = y;
+  return r;
+}
+
+template <typename T>
 struct HetrdCtx {
-  double2* A;
+  using C2 = typename C2T<T>::t;
+  C2* A;
   int n, ld;
-  double2 *Vc, *Wc, *Y, *Yp, *C12, *TAU;
+  C2 *Vc, *Wc, *Y, *Yp, *C12, *TAU;
   double *D, *E;
-  int* cnt;
   int ncmax;   // partial-sum chunks per matrix (>= ceil(ld / HT_COLS))
 };
 
 // w_{jj-1} = tau y - |tau|^2/2 (v^H y) v over rows >= j (support of v_{j-1}).
-__device__ void latrd_finalize(const HetrdCtx& c, int b, int j, int jj, double* red) {
+template <typename T>
+__device__ void latrd_finalize(const HetrdCtx<T>& c, int b, int j, int jj, T* red) {
+  using C2 = typename C2T<T>::t;
   const int n = c.n, ld = c.ld;
-  const double2* vc = c.Vc + (size_t)b * NB * ld;
-  double2* wc = c.Wc + (size_t)b * NB * ld;
-  const double2* y = c.Y + (size_t)b * ld;
+  const C2* vc = c.Vc + (size_t)b * NB * ld;
+  C2* wc = c.Wc + (size_t)b * NB * ld;
+  const C2* y = c.Y + (size_t)b * ld;
   const int kp = jj - 1;
-  const double2 tp = c.TAU[(size_t)b * ld + j - 1];
-  double part = 0.0;
+  const C2 tp = c.TAU[(size_t)b * ld + j - 1];
+  T part = 0;
   for (int r = j + threadIdx.x; r < n; r += blockDim.x) {
-    const double2 v = vc[(size_t)kp * ld + r], yy = y[r];
+    const C2 v = vc[(size_t)kp * ld + r], yy = y[r];
     part += v.x * yy.x + v.y * yy.y;
   }
-  const double vhy = block_sum(part, red);
-  const double t2 = 0.5 * (tp.x * tp.x + tp.y * tp.y) * vhy;
+  const T vhy = block_sum(part, red);
+  const T t2 = T(0.5) * (tp.x * tp.x + tp.y * tp.y) * vhy;
   for (int r = threadIdx.x; r < n; r += blockDim.x) {
-    double2 o = make_double2(0.0, 0.0);
+    C2 o = mk<T>(0, 0);
     if (r >= j) {
-      const double2 p = c_mul(tp, y[r]);
-      const double2 v = vc[(size_t)kp * ld + r];
-      o = make_double2(p.x - t2 * v.x, p.y - t2 * v.y);
+      const C2 p = c_mul(tp, y[r]);
+      const C2 v = vc[(size_t)kp * ld + r];
+      o = mk<T>(p.x - t2 * v.x, p.y - t2 * v.y);
     }
     wc[(size_t)kp * ld + r] = o;
   }
@@ -59,15 +77,17 @@ __device__ void latrd_finalize(const HetrdCtx& c, int b, int j, int jj, double* 
 // Column j (panel position jj): apply the panel's pending rank-2 updates,
 // generate the reflector, store v (panel buffer and strict lower column j),
 // d[j], e[j], tau[j], and the correction coefficients W^H v, V^H v.
-__device__ void latrd_column(const HetrdCtx& c, int b, int j, int jj, double* red) {
+template <typename T>
+__device__ void latrd_column(const HetrdCtx<T>& c, int b, int j, int jj, T* red) {
+  using C2 = typename C2T<T>::t;
   const int n = c.n, ld = c.ld;
-  double2* a = c.A + (size_t)b * ld * ld;
-  double2* vc = c.Vc + (size_t)b * NB * ld;
-  const double2* wc = c.Wc + (size_t)b * NB * ld;
-  double2* c12 = c.C12 + (size_t)b * 2 * NB;
-  __shared__ double2 s_vj[NB], s_wj[NB];
-  __shared__ double2 s_tau, s_scale;
-  __shared__ double s_beta;
+  C2* a = c.A + (size_t)b * ld * ld;
+  C2* vc = c.Vc + (size_t)b * NB * ld;
+  const C2* wc = c.Wc + (size_t)b * NB * ld;
+  C2* c12 = c.C12 + (size_t)b * 2 * NB;
+  __shared__ C2 s_vj[NB], s_wj[NB];
+  __shared__ C2 s_tau, s_scale;
+  __shared__ T s_beta;
   __shared__ int s_skip;
   const int tid = threadIdx.x;
   if (tid < jj) {
@@ -75,61 +95,57 @@ __device__ void latrd_column(const HetrdCtx& c, int b, int j, int jj, double* re
     s_wj[tid] = wc[(size_t)tid * ld + j];
   }
   __syncthreads();
+  T part = 0;
   for (int r = j + tid; r < n; r += blockDim.x) {
-    double2 acc = a[(size_t)r * ld + j];
+    C2 acc = a[(size_t)r * ld + j];
     for (int k = 0; k < jj; ++k) {
-      const double2 t1 = c_mulc(vc[(size_t)k * ld + r], s_wj[k]);
-      const double2 t3 = c_mulc(wc[(size_t)k * ld + r], s_vj[k]);
+      const C2 t1 = c_mulc(vc[(size_t)k * ld + r], s_wj[k]);
+      const C2 t3 = c_mulc(wc[(size_t)k * ld + r], s_vj[k]);
       acc.x -= t1.x + t3.x;
       acc.y -= t1.y + t3.y;
     }
     a[(size_t)r * ld + j] = acc;
+    if (r >= j + 2) part += acc.x * acc.x + acc.y * acc.y;
   }
-  __syncthreads();
-  double part = 0.0;
-  for (int r = j + 2 + tid; r < n; r += blockDim.x) {
-    const double2 x = a[(size_t)r * ld + j];
-    part += x.x * x.x + x.y * x.y;
-  }
-  const double xn2 = block_sum(part, red);
+  const T xn2 = block_sum(part, red);
   if (tid == 0) {
-    const double2 alpha = a[(size_t)(j + 1) * ld + j];
-    if (xn2 == 0.0 && alpha.y == 0.0) {
+    const C2 alpha = a[(size_t)(j + 1) * ld + j];
+    if (xn2 == T(0) && alpha.y == T(0)) {
       s_skip = 1;
-      s_tau = make_double2(0.0, 0.0);
+      s_tau = mk<T>(0, 0);
       s_beta = alpha.x;
-      s_scale = make_double2(0.0, 0.0);
+      s_scale = mk<T>(0, 0);
     } else {
       s_skip = 0;
-      const double nrm = sqrt(alpha.x * alpha.x + alpha.y * alpha.y + xn2);
-      const double beta = alpha.x >= 0.0 ? -nrm : nrm;
+      const T nrm = sqrt(alpha.x * alpha.x + alpha.y * alpha.y + xn2);
+      const T beta = alpha.x >= T(0) ? -nrm : nrm;
       s_beta = beta;
-      s_tau = make_double2((beta - alpha.x) / beta, -alpha.y / beta);
-      const double2 den = make_double2(alpha.x - beta, alpha.y);
-      const double dd = den.x * den.x + den.y * den.y;
-      s_scale = make_double2(den.x / dd, -den.y / dd);
+      s_tau = mk<T>((beta - alpha.x) / beta, -alpha.y / beta);
+      const C2 den = mk<T>(alpha.x - beta, alpha.y);
+      const T dd = den.x * den.x + den.y * den.y;
+      s_scale = mk<T>(den.x / dd, -den.y / dd);
     }
-    c.D[(size_t)b * ld + j] = a[(size_t)j * ld + j].x;
-    c.E[(size_t)b * ld + j] = s_beta;
+    c.D[(size_t)b * ld + j] = (double)a[(size_t)j * ld + j].x;
+    c.E[(size_t)b * ld + j] = (double)s_beta;
     c.TAU[(size_t)b * ld + j] = s_tau;
   }
   __syncthreads();
-  const double2 scale = s_scale;
+  const C2 scale = s_scale;
   for (int r = tid; r < n; r += blockDim.x) {
-    double2 v = make_double2(0.0, 0.0);
-    if (r == j + 1) v = make_double2(1.0, 0.0);
-    else if (r > j + 1) v = s_skip ? make_double2(0.0, 0.0) : c_mul(a[(size_t)r * ld + j], scale);
+    C2 v = mk<T>(0, 0);
+    if (r == j + 1) v = mk<T>(1, 0);
+    else if (r > j + 1) v = s_skip ? mk<T>(0, 0) : c_mul(a[(size_t)r * ld + j], scale);
     vc[(size_t)jj * ld + r] = v;
     if (r > j) a[(size_t)r * ld + j] = v;
   }
   __syncthreads();
   const int lane = tid & 63, wv = tid >> 6, nw = blockDim.x >> 6;
   for (int k = wv; k < jj; k += nw) {
-    double2 s1 = make_double2(0.0, 0.0), s2 = make_double2(0.0, 0.0);
+    C2 s1 = mk<T>(0, 0), s2 = mk<T>(0, 0);
     for (int r = j + 1 + lane; r < n; r += 64) {
-      const double2 v = vc[(size_t)jj * ld + r];
-      const double2 x1 = c_cmul(wc[(size_t)k * ld + r], v);
-      const double2 x2 = c_cmul(vc[(size_t)k * ld + r], v);
+      const C2 v = vc[(size_t)jj * ld + r];
+      const C2 x1 = c_cmul(wc[(size_t)k * ld + r], v);
+      const C2 x2 = c_cmul(vc[(size_t)k * ld + r], v);
       s1.x += x1.x; s1.y += x1.y;
       s2.x += x2.x; s2.y += x2.y;
     }
@@ -140,37 +156,40 @@ __device__ void latrd_column(const HetrdCtx& c, int b, int j, int jj, double* re
   __syncthreads();
 }
 
-__global__ __launch_bounds__(TAIL_THREADS) void latrd_start_kernel(HetrdCtx c, int j) {
-  __shared__ double red[TAIL_THREADS / 64];
+template <typename T>
+__global__ __launch_bounds__(TAIL_THREADS) void latrd_start_kernel(HetrdCtx<T> c, int j) {
+  __shared__ T red[TAIL_THREADS / 64];
   latrd_column(c, blockIdx.x, j, 0, red);
 }
 
-// Trailing mat-vec tile for column j + tail (last workgroup of the matrix).
-__global__ __launch_bounds__(HT_THREADS) void hetrd_step_kernel(HetrdCtx c, int j, int jj, int jn) {
+// Trailing mat-vec tile for column j: partial y over HT_COLS columns.
+template <typename T>
+__global__ __launch_bounds__(HT_THREADS) void hetrd_step_kernel(HetrdCtx<T> c, int j, int jj) {
+  using C2 = typename C2T<T>::t;
   const int b = blockIdx.z;
   const int n = c.n, ld = c.ld;
-  const double2* a = c.A + (size_t)b * ld * ld;
-  const double2* vc = c.Vc + (size_t)b * NB * ld;
-  const double2* wc = c.Wc + (size_t)b * NB * ld;
-  const double2* v = vc + (size_t)jj * ld;
-  const double2* c12 = c.C12 + (size_t)b * 2 * NB;
-  double2* yp = c.Yp + (size_t)b * c.ncmax * ld;
+  const C2* a = c.A + (size_t)b * ld * ld;
+  const C2* vc = c.Vc + (size_t)b * NB * ld;
+  const C2* wc = c.Wc + (size_t)b * NB * ld;
+  const C2* v = vc + (size_t)jj * ld;
+  const C2* c12 = c.C12 + (size_t)b * 2 * NB;
+  C2* yp = c.Yp + (size_t)b * c.ncmax * ld;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int r0 = j + 1 + blockIdx.x * HT_ROWS + wv * 2;
   const int cbase = j + 1 + blockIdx.y * HT_COLS;
-  double2 acc0 = make_double2(0.0, 0.0), acc1 = make_double2(0.0, 0.0);
+  C2 acc0 = mk<T>(0, 0), acc1 = mk<T>(0, 0);
 #pragma unroll
   for (int i = 0; i < HT_COLS / 64; ++i) {
     const int col = cbase + lane + 64 * i;
     if (col < n) {
-      const double2 vv = v[col];
+      const C2 vv = v[col];
       if (r0 < n) {
-        const double2 x = a[(size_t)r0 * ld + col];
+        const C2 x = a[(size_t)r0 * ld + col];
         acc0.x += x.x * vv.x - x.y * vv.y;
         acc0.y += x.x * vv.y + x.y * vv.x;
       }
       if (r0 + 1 < n) {
-        const double2 x = a[(size_t)(r0 + 1) * ld + col];
+        const C2 x = a[(size_t)(r0 + 1) * ld + col];
         acc1.x += x.x * vv.x - x.y * vv.y;
         acc1.y += x.x * vv.y + x.y * vv.x;
       }
@@ -181,11 +200,11 @@ __global__ __launch_bounds__(HT_THREADS) void hetrd_step_kernel(HetrdCtx c, int 
   if (lane < 2) {
     const int r = r0 + lane;
     if (r < n) {
-      double2 s = lane == 0 ? acc0 : acc1;
+      C2 s = lane == 0 ? acc0 : acc1;
       if (blockIdx.y == 0) {
         for (int k = 0; k < jj; ++k) {
-          const double2 t1 = c_mul(vc[(size_t)k * ld + r], c12[k]);
-          const double2 t2 = c_mul(wc[(size_t)k * ld + r], c12[NB + k]);
+          const C2 t1 = c_mul(vc[(size_t)k * ld + r], c12[k]);
+          const C2 t2 = c_mul(wc[(size_t)k * ld + r], c12[NB + k]);
           s.x -= t1.x + t2.x;
           s.y -= t1.y + t2.y;
         }
@@ -197,16 +216,16 @@ __global__ __launch_bounds__(HT_THREADS) void hetrd_step_kernel(HetrdCtx c, int 
 
 // Column tail (one workgroup per matrix): y = sum of the nch partial
 // products, w_j from y, then the reflector of column j+1 if it is in the panel.
-// A kernel boundary orders it after the mat-vec tiles; a device-scope fence
-// per tile flushes L2 on this multi-XCD part and costs far more.
-__global__ __launch_bounds__(TAIL_THREADS) void hetrd_tail_kernel(HetrdCtx c, int j, int jj, int jn, int nch) {
-  __shared__ double red[TAIL_THREADS / 64];
+template <typename T>
+__global__ __launch_bounds__(TAIL_THREADS) void hetrd_tail_kernel(HetrdCtx<T> c, int j, int jj, int jn, int nch) {
+  using C2 = typename C2T<T>::t;
+  __shared__ T red[TAIL_THREADS / 64];
   const int b = blockIdx.x;
   const int n = c.n, ld = c.ld;
-  const double2* yp = c.Yp + (size_t)b * c.ncmax * ld;
-  double2* y = c.Y + (size_t)b * ld;
+  const C2* yp = c.Yp + (size_t)b * c.ncmax * ld;
+  C2* y = c.Y + (size_t)b * ld;
   for (int r = j + 1 + threadIdx.x; r < n; r += blockDim.x) {
-    double2 s = make_double2(0.0, 0.0);
+    C2 s = mk<T>(0, 0);
     for (int ch = 0; ch < nch; ++ch) s = c_add(s, yp[(size_t)ch * ld + r]);
     y[r] = s;
   }
@@ -215,40 +234,63 @@ __global__ __launch_bounds__(TAIL_THREADS) void hetrd_tail_kernel(HetrdCtx c, in
   if (j + 1 < jn) latrd_column(c, b, j + 1, jj + 1, red);
 }
 
-__global__ void hetrd_last_kernel(const double2* A, int n, int ld, double* D, double* E, double2* TAU) {
+template <typename T>
+__global__ void hetrd_last_kernel(const typename C2T<T>::t* A, int n, int ld, double* D, double* E,
+                                  typename C2T<T>::t* TAU) {
   const int b = blockIdx.x;
   if (threadIdx.x == 0) {
-    D[(size_t)b * ld + n - 1] = A[(size_t)b * ld * ld + (size_t)(n - 1) * ld + (n - 1)].x;
+    D[(size_t)b * ld + n - 1] = (double)A[(size_t)b * ld * ld + (size_t)(n - 1) * ld + (n - 1)].x;
     E[(size_t)b * ld + n - 1] = 0.0;
-    TAU[(size_t)b * ld + n - 1] = make_double2(0.0, 0.0);
+    TAU[(size_t)b * ld + n - 1] = mk<T>(0, 0);
   }
 }
 
 size_t hetrd_partials_elems(int ld) { return (size_t)((ld + HT_COLS - 1) / HT_COLS) * ld; }
 
-void launch_hetrd_blocked(hipStream_t st, double2* C, int n, int ld, int batch, double* D, double* E, double2* TAU,
-                          double2* Vc, double2* Wc, double2* Y, double2* Yp, double2* C12, int* cnt) {
-  HetrdCtx c;
+template <typename T>
+static void rank2k(hipStream_t st, int batch, int m, int k, typename C2T<T>::t* Vc, typename C2T<T>::t* Wc,
+                   typename C2T<T>::t* C, int ld, long long sP, long long sM);
+template <>
+void rank2k<double>(hipStream_t st, int batch, int m, int k, double2* Vc, double2* Wc, double2* C, int ld,
+                    long long sP, long long sM) {
+  launch_zgemm(st, batch, m, m, k, -1.0, Vc, ld, sP, OP_T, Wc, ld, sP, OP_CONJ, 1.0, C, ld, sM, 0, 0);
+  launch_zgemm(st, batch, m, m, k, -1.0, Wc, ld, sP, OP_T, Vc, ld, sP, OP_CONJ, 1.0, C, ld, sM, 0, 0);
+}
+template <>
+void rank2k<float>(hipStream_t st, int batch, int m, int k, float2* Vc, float2* Wc, float2* C, int ld,
+                   long long sP, long long sM) {
+  launch_cgemm(st, batch, m, m, k, -1.0f, Vc, ld, sP, OP_T, Wc, ld, sP, OP_CONJ, 1.0f, C, ld, sM, 0, 0);
+  launch_cgemm(st, batch, m, m, k, -1.0f, Wc, ld, sP, OP_T, Vc, ld, sP, OP_CONJ, 1.0f, C, ld, sM, 0, 0);
+}
+
+template <typename T>
+static void hetrd_impl(hipStream_t st, typename C2T<T>::t* C, int n, int ld, int batch, double* D, double* E,
+                       typename C2T<T>::t* TAU, typename C2T<T>::t* Vc, typename C2T<T>::t* Wc,
+                       typename C2T<T>::t* Y, typename C2T<T>::t* Yp, typename C2T<T>::t* C12) {
+  HetrdCtx<T> c;
   c.A = C; c.n = n; c.ld = ld; c.Vc = Vc; c.Wc = Wc; c.Y = Y; c.Yp = Yp; c.C12 = C12; c.TAU = TAU;
-  c.D = D; c.E = E; c.cnt = cnt; c.ncmax = (ld + HT_COLS - 1) / HT_COLS;
-  launch_zero(st, cnt, ((size_t)batch * sizeof(int) + 15) / 16 * 16);
+  c.D = D; c.E = E; c.ncmax = (ld + HT_COLS - 1) / HT_COLS;
   const long long sP = (long long)NB * ld, sM = (long long)ld * ld;
   for (int j0 = 0; j0 < n - 1; j0 += NB) {
     const int jn = min(j0 + NB, n - 1);
-    hipLaunchKernelGGL(latrd_start_kernel, dim3(batch), dim3(TAIL_THREADS), 0, st, c, j0);
+    hipLaunchKernelGGL(latrd_start_kernel<T>, dim3(batch), dim3(TAIL_THREADS), 0, st, c, j0);
     for (int j = j0; j < jn; ++j) {
       const int m = n - j - 1;
       dim3 g((m + HT_ROWS - 1) / HT_ROWS, (m + HT_COLS - 1) / HT_COLS, batch);
-      hipLaunchKernelGGL(hetrd_step_kernel, g, dim3(HT_THREADS), 0, st, c, j, j - j0, jn);
-      hipLaunchKernelGGL(hetrd_tail_kernel, dim3(batch), dim3(TAIL_THREADS), 0, st, c, j, j - j0, jn, (int)g.y);
+      hipLaunchKernelGGL(hetrd_step_kernel<T>, g, dim3(HT_THREADS), 0, st, c, j, j - j0);
+      hipLaunchKernelGGL(hetrd_tail_kernel<T>, dim3(batch), dim3(TAIL_THREADS), 0, st, c, j, j - j0, jn, (int)g.y);
     }
-    if (jn < n) {
-      const int m = n - jn, kcols = jn - j0;
-      launch_zgemm(st, batch, m, m, kcols, -1.0, Vc + jn, ld, sP, OP_T, Wc + jn, ld, sP, OP_CONJ, 1.0,
-                   C + (size_t)jn * ld + jn, ld, sM, 0, 0);
-      launch_zgemm(st, batch, m, m, kcols, -1.0, Wc + jn, ld, sP, OP_T, Vc + jn, ld, sP, OP_CONJ, 1.0,
-                   C + (size_t)jn * ld + jn, ld, sM, 0, 0);
-    }
+    if (jn < n) rank2k<T>(st, batch, n - jn, jn - j0, Vc + jn, Wc + jn, C + (size_t)jn * ld + jn, ld, sP, sM);
   }
-  hipLaunchKernelGGL(hetrd_last_kernel, dim3(batch), dim3(64), 0, st, C, n, ld, D, E, TAU);
+  hipLaunchKernelGGL(hetrd_last_kernel<T>, dim3(batch), dim3(64), 0, st, C, n, ld, D, E, TAU);
+}
+
+void launch_hetrd_blocked(hipStream_t st, double2* C, int n, int ld, int batch, double* D, double* E, double2* TAU,
+                          double2* Vc, double2* Wc, double2* Y, double2* Yp, double2* C12) {
+  hetrd_impl<double>(st, C, n, ld, batch, D, E, TAU, Vc, Wc, Y, Yp, C12);
+}
+
+void launch_hetrd_blocked_f32(hipStream_t st, float2* C, int n, int ld, int batch, double* D, double* E,
+                              float2* TAU, float2* Vc, float2* Wc, float2* Y, float2* Yp, float2* C12) {
+  hetrd_impl<float>(st, C, n, ld, batch, D, E, TAU, Vc, Wc, Y, Yp, C12);
 }

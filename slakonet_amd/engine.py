@@ -199,28 +199,39 @@ class EigenFailure(RuntimeError):
 EIGH_STAGES = ["potrf", "trinv", "reduce", "hetrd", "tridiag_solve", "back_transform"]
 
 
-def eigh(H: torch.Tensor, S: torch.Tensor, vectors: bool = False, stream=None, check: bool = True,
-         stage_ms: Optional[list] = None):
-    """Batched generalized eigensolve (FP64). Returns w [batch, ld], V or None, info.
+PRECISIONS = ("fp64", "mixed")
 
+
+def eigh(H: torch.Tensor, S: torch.Tensor, vectors: bool = False, stream=None, check: bool = True,
+         stage_ms: Optional[list] = None, precision: str = "fp64"):
+    """Batched generalized eigensolve. Returns w [batch, ld], V or None, info.
+
+    precision "fp64": every stage in FP64. "mixed": FP64 Cholesky and L^-1,
+    FP32 reduction and tridiagonalisation, FP64 bisection (values only).
     With ``stage_ms`` (a list) the per-stage milliseconds measured with HIP
     events on the launch stream are appended (one synchronisation)."""
+    if precision not in PRECISIONS:
+        raise ValueError(f"precision must be one of {PRECISIONS}")
+    mixed = precision == "mixed"
+    if mixed and vectors:
+        raise ValueError("the mixed-precision solve returns eigenvalues only")
     lib = _lib.lib()
     batch, ld, _ = H.shape
     n = ld
-    ws_bytes = int(lib.slk_eigh_workspace_bytes(n, ld, batch, int(vectors)))
+    ws_bytes = int(lib.slk_eigh_mixed_workspace_bytes(n, ld, batch) if mixed else
+                   lib.slk_eigh_workspace_bytes(n, ld, batch, int(vectors)))
     ws = torch.empty(ws_bytes, dtype=torch.uint8, device=H.device)
     w = torch.empty(batch, ld, dtype=torch.float64, device=H.device)
     info = torch.zeros(2 * batch, dtype=torch.int32, device=H.device)
     V = torch.empty_like(H) if vectors else None
-    args = (C.c_void_p(H.data_ptr()), C.c_void_p(S.data_ptr()), n, ld, batch, C.c_void_p(w.data_ptr()), ld,
-            _lib.ptr(V), C.c_void_p(ws.data_ptr()), ws_bytes, C.c_void_p(info.data_ptr()),
-            _lib.stream_handle(stream))
+    head = (C.c_void_p(H.data_ptr()), C.c_void_p(S.data_ptr()), n, ld, batch, C.c_void_p(w.data_ptr()), ld)
+    tail = (C.c_void_p(ws.data_ptr()), ws_bytes, C.c_void_p(info.data_ptr()), _lib.stream_handle(stream))
+    args = head + tail if mixed else head + (_lib.ptr(V),) + tail
     if stage_ms is None:
-        rc = lib.slk_eigh_f64(*args)
+        rc = (lib.slk_eigh_mixed if mixed else lib.slk_eigh_f64)(*args)
     else:
         ms = (C.c_float * 6)()
-        rc = lib.slk_eigh_f64_timed(*args, ms)
+        rc = (lib.slk_eigh_mixed_timed if mixed else lib.slk_eigh_f64_timed)(*args, ms)
         stage_ms.append(dict(zip(EIGH_STAGES, list(ms))))
     if rc == 3:
         raise NotImplementedError(f"matrix dimension {n} exceeds the solver's LDS-resident tridiagonal limit")

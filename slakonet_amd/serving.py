@@ -63,7 +63,7 @@ class Batch:
 class ServingEngine:
     def __init__(self, tables: TableSet, dos_grid: Optional[torch.Tensor] = None, n_streams: int = 4,
                  max_inflight: int = 8, hbm_budget_bytes: float = 64e9, max_batch_matrices: int = 512,
-                 fermi_mode: int = 0, keep_outputs: bool = False):
+                 fermi_mode: int = 0, keep_outputs: bool = False, precision: str = "mixed"):
         self.tables = tables
         self.dev = tables.device
         self.grid = dos_grid
@@ -73,6 +73,7 @@ class ServingEngine:
         self.max_batch = max_batch_matrices
         self.fermi_mode = fermi_mode
         self.keep_outputs = keep_outputs
+        self.precision = precision
         self.nf = len(engine.EDGE_FIELDS)
         self._rr = 0
 
@@ -93,7 +94,7 @@ class ServingEngine:
         with torch.cuda.stream(stream):
             plan = engine.plan_build(self.tables, [r.inp for r in reqs], ld=ld)
             H, S = engine.run_build(self.tables, plan)
-            w, _, info = engine.eigh(H, S, check=False)
+            w, _, info = engine.eigh(H, S, check=False, precision=self.precision)
             edges, dos, stats = engine.post(plan, w, fermi_mode=self.fermi_mode, dos_grid=self.grid)
             parts = [edges.reshape(-1), info.to(torch.float64)]
             if stats is not None:

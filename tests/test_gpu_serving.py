@@ -30,7 +30,7 @@ def test_bucket_dims():
 def test_mix_equals_alone():
     tab, inputs = _stream(24)
     grid = torch.linspace(-20.0, 10.0, 2001, dtype=torch.float64, device="cuda")
-    eng = ServingEngine(tab, dos_grid=grid, keep_outputs=True)
+    eng = ServingEngine(tab, dos_grid=grid, keep_outputs=True, precision="fp64")
     arrivals = np.zeros(len(inputs))
     arrivals[12:] = 0.002          # a second wave arrives while the first is on the device
     out = eng.replay(inputs, arrivals)
@@ -49,3 +49,29 @@ def test_mix_equals_alone():
         n = plan.n_orb[0]
         assert torch.allclose(res["eigenvalues"], w[:, :n], atol=1e-9, rtol=0)
         assert torch.allclose(res["dos"], dos[0], atol=1e-7, rtol=1e-7)
+
+
+MIXED_TOL_EV = 1e-3   # gap / band-edge tolerance of the precision form (1 meV)
+
+
+def test_mixed_precision_stream_within_1mev():
+    """Precision form in the mix vs the FP64 path alone: band edges and gap
+    within 1 meV, every eigenvalue within 1 meV."""
+    tab, inputs = _stream(16, seed=41, n_max=96)
+    eng = ServingEngine(tab, keep_outputs=True, precision="mixed")
+    out = eng.replay(inputs, np.zeros(len(inputs)))
+    worst = 0.0
+    for i, inp in enumerate(inputs):
+        res = out["results"][i]
+        plan = engine.plan_build(tab, [inp])
+        H, S = engine.run_build(tab, plan)
+        w, _, _ = engine.eigh(H, S, precision="fp64")
+        edges, _, _ = engine.post(plan, w, fermi_mode=0)
+        ed = engine.edges_dict(edges[0].cpu().numpy())
+        if ed["metal"] == 0 and res["metal"] == 0:
+            assert abs(res["vbm"] - ed["vbm"]) < MIXED_TOL_EV
+            assert abs(res["cbm"] - ed["cbm"]) < MIXED_TOL_EV
+        n = plan.n_orb[0]
+        err = float((res["eigenvalues"] - w[:, :n]).abs().max()) * 27.211
+        worst = max(worst, err)
+    assert worst < MIXED_TOL_EV, worst
