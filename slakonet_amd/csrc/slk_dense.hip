@@ -17,15 +17,19 @@
 // imaginary planes. Complex product = 4 real MFMAs per block:
 // Re += Ar Br - Ai Bi, Im += Ar Bi + Ai Br.
 // Operand layout of the 16x16x4 MFMAs (f64 and f32): lane l supplies
-// A[l%16][l/16] and B[l/16][l%16]; it receives D[4*(l/16)+v][l%16], v < 4.
+// A[l%16][l/16] and B[l/16][l%16]; it receives D[row(l/16, v)][l%16], v < 4.
 // ---------------------------------------------------------------------------
 template <typename T> struct CT;
+// Output row of accumulator element v in lane group g = lane/16 (measured on
+// gfx950): f64 16x16x4 interleaves rows (g + 4 v), f32 16x16x4 keeps blocks
+// of four (4 g + v); the column is lane % 16 for both.
 template <> struct CT<double> {
   using c2 = double2;
   using v4 = __attribute__((ext_vector_type(4))) double;
   static __device__ __forceinline__ v4 mfma(double a, double b, v4 c) {
     return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
   }
+  static __device__ __forceinline__ int row(int g, int v) { return g + 4 * v; }
 };
 template <> struct CT<float> {
   using c2 = float2;
@@ -33,6 +37,7 @@ template <> struct CT<float> {
   static __device__ __forceinline__ v4 mfma(float a, float b, v4 c) {
     return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
   }
+  static __device__ __forceinline__ int row(int g, int v) { return 4 * g + v; }
 };
 
 template <typename C2>
@@ -128,7 +133,7 @@ __global__ __launch_bounds__(256) void gemm_mfma_kernel(int M, int N, int K, T a
     for (int j = 0; j < 2; ++j)
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
-        const int gm = m0 + wr + 16 * i + 4 * lk + v, gn = n0 + wc + 16 * j + li;
+        const int gm = m0 + wr + 16 * i + CT<T>::row(lk, v), gn = n0 + wc + 16 * j + li;
         if (gm < M && gn < N) {
           C2 r;
           r.x = alpha * accR[i][j][v];
