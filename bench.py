@@ -38,7 +38,8 @@ def parse():
     p.add_argument("--steps", type=int, default=1)
     p.add_argument("--warmup", type=int, default=1)
     p.add_argument("--requests", type=int, default=2000)
-    p.add_argument("--rate", type=float, default=200.0, help="Poisson arrival rate, requests/s")
+    p.add_argument("--rate", type=float, default=2000.0,
+                   help="Poisson arrival rate, requests/s (default saturates the device: value = capacity)")
     p.add_argument("--mode", choices=["serve", "plain"], default="serve")
     p.add_argument("--cpu-seconds", type=float, default=20.0, help="CPU baseline sample budget (0 = skip)")
     p.add_argument("--no-roofline", action="store_true")
@@ -93,6 +94,15 @@ def run_serve(tables, inputs, arrivals, grid):
     from slakonet_amd.serving import ServingEngine
     eng = ServingEngine(tables, dos_grid=grid, precision=PRECISION)
     res = eng.replay(inputs, arrivals)
+    by_ld = {}
+    for ld, nmat, ns, t0, t1 in eng.batch_log:
+        e = by_ld.setdefault(ld, [0, 0, 0, 0.0])
+        e[0] += 1
+        e[1] += nmat
+        e[2] += ns
+        e[3] += t1 - t0
+    summary = " ".join(f"{ld}:{e[0]}b/{e[2]}s/{e[3]:.2f}s" for ld, e in sorted(by_ld.items()))
+    print(f"# batches by dimension (batches/structures/in-flight seconds): {summary}", file=sys.stderr, flush=True)
     return res["wall_s"], res["latency_s"]
 
 

@@ -58,6 +58,7 @@ class Batch:
     dos: Optional[torch.Tensor] = None
     w: Optional[torch.Tensor] = None
     plan: Optional[engine.BuildPlan] = None
+    t_launch: float = 0.0
 
 
 class ServingEngine:
@@ -110,6 +111,7 @@ class ServingEngine:
         return b
 
     def _finish(self, b: Batch, results: Dict[int, dict], done_t: Dict[int, float], now: float):
+        self.batch_log.append((b.ld, sum(len(r.inp.kpoints) for r in b.reqs), len(b.reqs), b.t_launch, now))
         h = b.host.numpy()
         ns = len(b.reqs)
         edges = h[:ns * self.nf].reshape(ns, self.nf)
@@ -141,6 +143,7 @@ class ServingEngine:
 
         Returns dict with wall time, per-request latency (arrival -> result on
         the host) and per-request results."""
+        self.batch_log = []   # (ld, matrices, structures, launch_s, done_s)
         order = np.argsort(np.asarray(arrivals), kind="stable")
         pending = deque(int(i) for i in order)
         queues: Dict[int, deque] = {}
@@ -184,6 +187,7 @@ class ServingEngine:
                 if not take:
                     break
                 b = self._launch(take, ld)
+                b.t_launch = time.perf_counter() - t0
                 used += b.nbytes
                 inflight.append(b)
                 progressed = True
