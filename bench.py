@@ -40,7 +40,9 @@ def parse():
     p.add_argument("--requests", type=int, default=2000)
     p.add_argument("--rate", type=float, default=2000.0,
                    help="Poisson arrival rate, requests/s (default saturates the device: value = capacity)")
-    p.add_argument("--mode", choices=["serve", "plain"], default="serve")
+    p.add_argument("--mode", choices=["serve", "plain", "fit"], default="serve",
+                   help="fit: the fitting workload (32 cells of 8-64 atoms, 3 species, DOS + gap loss, Adam steps)")
+    p.add_argument("--fit-steps", type=int, default=200, help="Adam steps timed in --mode fit")
     p.add_argument("--cpu-seconds", type=float, default=20.0, help="CPU baseline sample budget (0 = skip)")
     p.add_argument("--no-roofline", action="store_true")
     p.add_argument("--precision", choices=["mixed", "fp64"], default="mixed",
@@ -137,6 +139,47 @@ def roofline(tables, inputs):
     return dom, stage_tot, flops, build_bytes, build_ms
 
 
+def run_fit(args):
+    """Fitting workload: teacher targets, then timed AdamW steps (adjoint backward)."""
+    import torch
+    from slakonet_amd import fit, synth
+    from slakonet_amd.optim import MultiElementSkfParameterOptimizer
+    from slakonet_amd.tables import TableSet
+    dev = torch.device("cuda:0")
+    species = (6, 14, 31)
+    sp = sorted(species)
+    structs = synth.fitting_batch(seed=3, n_cells=32, species=species)
+    grid = torch.linspace(-20.0, 10.0, 2001, dtype=torch.float64, device=dev)
+    teacher = TableSet(synth.make_skf_set(sp, seed=TABLE_SEED, teacher_seed=TABLE_SEED + 1),
+                       synth.shell_dict_spd(sp), device=dev)
+    ds = fit.teacher_datasets(structs, teacher, grid)
+    model = MultiElementSkfParameterOptimizer(skf_dicts=synth.make_skf_set(sp, seed=TABLE_SEED), device=dev).to(dev)
+    shells = synth.shell_dict_spd(sp)
+    fit.train_skf_parameters(model, ds, shells, num_epochs=max(1, args.warmup), learning_rate=1e-3, loss="dos_gap",
+                             log=None)
+    torch.cuda.synchronize()
+    torch.cuda.reset_peak_memory_stats(dev)
+    t0 = time.perf_counter()
+    _, hist = fit.train_skf_parameters(model, ds, shells, num_epochs=args.fit_steps, learning_rate=1e-3,
+                                       loss="dos_gap", early_stopping_patience=10 ** 9,
+                                       log=lambda m: print("# " + m, file=sys.stderr, flush=True), log_every=20)
+    torch.cuda.synchronize()
+    total = time.perf_counter() - t0
+    n_mat = sum(len(st.kpoints) for st in structs)
+    out = {
+        "metric": "fitting Adam steps/s (32 cells, DOS + gap loss, adjoint backward)", "value": len(hist) / total,
+        "unit": "steps/s", "n_gpus": 1, "steps": len(hist), "warmup": args.warmup,
+        "ms_per_step": 1000.0 * total / max(1, len(hist)), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "loss_first": hist[0]["total_loss"], "loss_last": hist[-1]["total_loss"],
+        "peak_hbm_bytes": int(torch.cuda.max_memory_allocated(dev)),
+        "config": {"workload": f"fitting: 32 random cells 8-64 atoms, species {species}, {n_mat} k-matrices, "
+                               "teacher tables (A,b,B,c x (1+N(0,0.1))), loss MSE(DOS 2001 pts) + 0.1 gap err^2, "
+                               "AdamW lr 1e-3, clip 2.0"},
+    }
+    print(json.dumps(out))
+
+
 def cpu_baseline(structs, skfs, budget_s):
     """FP64 oracle (numpy/LAPACK) over the stream in order until the budget is spent."""
     from oracle import build as obuild, post as opost, solve as osolve
@@ -176,6 +219,9 @@ def main():
     import torch
     from slakonet_amd import synth
     torch.cuda.set_device(0)
+    if args.mode == "fit":
+        run_fit(args)
+        return
     dev = torch.device("cuda:0")
     t_gen = time.perf_counter()
     structs = make_stream(args.requests)

@@ -158,6 +158,36 @@ typedef struct SlkDosJob {
 int slk_dos_launch(const double* W, const double* KW, const SlkDosJob* jobs, int n_jobs, const double* grid, int ne,
                    double sigma, const double* edges, double* dos, double* stats, hipStream_t stream);
 
+/* ---- Differentiation form (hand-written adjoints) ----------------------- */
+
+/* Adjoint of slk_dos_launch at fixed Fermi level: gW[w0 + k*wld + b] =
+ * dL/de_kb given gdos = dL/d dos (same layout as dos). Overwrites the
+ * nk x nb entries of every job; max_states >= every job's nk*nb. Replaces
+ * the autograd graph of calculate_dos (slakonet/main.py:709-797). */
+int slk_dos_adjoint_launch(const double* W, const double* KW, const SlkDosJob* jobs, int n_jobs, int max_states,
+                           const double* grid, int ne, double sigma, const double* edges, const double* gdos,
+                           double* gW, hipStream_t stream);
+
+/* Y[b] = C[b] diag(g[b]) (complex128 C [batch, ld, ld] row-major, g real
+ * with row stride gld; columns >= n get 0). With Y C^H on slk_zgemm this
+ * forms the weighted density Q = C diag(g) C^H of the eigenvalue adjoint
+ * de = c^H (dH - e dS) c, which replaces _SymEigB.backward
+ * (slakonet/utils.py:386-473) for eigenvalue gradients. */
+int slk_scale_columns(const void* C, const double* g, int n, int ld, int batch, int gld, void* Y, hipStream_t stream);
+
+/* Adjoint of slk_build_hs_launch with respect to the table values:
+ * dtab[row*20 + q] += sum over k and pair-images of Re(Q(k)[nu][mu] e^{2 pi i k.T})
+ * times the Slater-Koster coefficient and interpolation weight that carried
+ * table entry (row, q) into H(k)[mu][nu] (QH) or S(k)[mu][nu] (QS).
+ * QH, QS: complex128 [n_mat, ld, ld] as H/S. Work items sum over their k
+ * range [kc0, kc0 + nkc). dtab must be zeroed by the caller; FP64 atomic
+ * accumulation (order-dependent last bits). Replaces the autograd path
+ * through hs_matrix / SKF interpolation (slakonet/skfeed.py,
+ * slakonet/interpolation.py) used by the training loop
+ * (slakonet/optim.py:1570-1790). */
+int slk_adjoint_tables_launch(const SlkBuildArgs* args, const void* QH, const void* QS, double* dtab,
+                              hipStream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -85,6 +85,7 @@ class SimpleDftb:
         self.compute_hs_matrices()
         w, V, _ = engine.eigh(self.ham, self.overlap, vectors=self.with_eigenvectors)
         self._w = w
+        self._V = V
         self.eigenvalue = w.unsqueeze(0)                       # [1, nk, nb]
         if self.with_eigenvectors:
             self.eigenvectors = V.permute(1, 2, 0).unsqueeze(0)   # [1, n, nb, nk]

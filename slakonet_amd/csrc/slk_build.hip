@@ -450,3 +450,251 @@ int slk_pad_launch(const SlkStructDesc* structs_dev, int s, int nk, int n_orb, i
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Adjoint of the build with respect to the table values.
+//
+// Given per-k weighted density matrices Q_H(k) = C diag(g) C^H and
+// Q_S(k) = -C diag(g e) C^H (g = dL/de, C the S-orthonormal eigenvectors),
+// dL = sum_k Re sum_{mu,nu} Q(k)[nu][mu] dM(k)[mu][nu] for M = H, S. With
+// M(k)[mu][nu] = sum_T E(i,j,T)[a][b] exp(2 pi i k.T) this is
+// dL = sum_{(i,j,T),a,b} G[a][b] dE[a][b], G = Re sum_k Q(k)[nu][mu] e^{2 pi i k.T}.
+// E is linear in the SK integrals (sk_elem with unit integral vectors gives
+// the coefficients) and the integrals are linear in the table rows
+// (interpolation weights), so G is scattered back onto the table with
+// atomic adds. Work decomposition mirrors the build: one workgroup per
+// (atom row i, column tile, k range [kc0, kc0 + nkc)); k-points of the range
+// are summed inside, ranges and tiles meet in the FP64 atomics (so the last
+// bits of the gradient depend on the order the atomics land in).
+// ---------------------------------------------------------------------------
+#define ADJ_KCH 8
+
+__global__ __launch_bounds__(BUILD_THREADS) void slk_adjoint_kernel(SlkBuildArgs A, const double2* __restrict__ QH,
+                                                                    const double2* __restrict__ QS,
+                                                                    double* __restrict__ dtab) {
+  __shared__ int s_list[SEG];
+  __shared__ int s_cnt[BUILD_THREADS];
+  __shared__ int s_total;
+  __shared__ int pi_j[MAXPI];
+  __shared__ int pi_t[MAXPI];
+  __shared__ double pi_u[MAXPI][3];
+  __shared__ int pi_base[MAXPI][2];
+  __shared__ double pi_w[MAXPI][2][8];
+  __shared__ double G[MAXPI][2][81];
+  __shared__ double dV[MAXPI][2][20];
+  __shared__ double2 ph[MAXPI][ADJ_KCH];
+  __shared__ int col_atom[MAXCOL], col_slot[MAXCOL], col_orb[MAXCOL];
+
+  const SlkBuildWork wk = A.work[blockIdx.x];
+  const SlkStructDesc sd = A.structs[wk.s];
+  const int tid = threadIdx.x;
+  const int ai = sd.atom0 + wk.i;
+  const int si = A.atom_spec[ai];
+  const int norb_i = A.tab.spec_norb[si];
+  const int oi = A.atom_orb[ai];
+  const double rix = A.atom_pos[3 * ai], riy = A.atom_pos[3 * ai + 1], riz = A.atom_pos[3 * ai + 2];
+  const int nja = wk.j1 - wk.j0;
+  const size_t mat_el = (size_t)sd.ld * sd.ld;
+  if (tid == 0) {
+    int c = 0;
+    for (int j = wk.j0; j < wk.j1; ++j) {
+      const int aj = sd.atom0 + j;
+      const int sj = A.atom_spec[aj];
+      const int nb = A.tab.spec_norb[sj];
+      for (int o = 0; o < nb && c < MAXCOL; ++o, ++c) {
+        col_atom[c] = j;
+        col_slot[c] = A.tab.spec_slot[sj * 9 + o];
+        col_orb[c] = A.atom_orb[aj] + o;
+      }
+    }
+  }
+  __syncthreads();
+  const long long ncand = (long long)nja * sd.nt;
+  for (long long seg0 = 0; seg0 < ncand; seg0 += SEG) {
+    const int segn = (int)((ncand - seg0) < SEG ? (ncand - seg0) : SEG);
+    const int per = (segn + BUILD_THREADS - 1) / BUILD_THREADS;
+    const int beg = tid * per;
+    const int end = min(beg + per, segn);
+    int mine = 0;
+    for (int c = beg; c < end; ++c) {
+      const long long g = seg0 + c;
+      const int jr = (int)(g / sd.nt), t = (int)(g % sd.nt);
+      const int aj = sd.atom0 + wk.j0 + jr;
+      const int tt = sd.t0 + t;
+      const double vx = (A.trans_cart[3 * tt] + A.atom_pos[3 * aj]) - rix;
+      const double vy = (A.trans_cart[3 * tt + 1] + A.atom_pos[3 * aj + 1]) - riy;
+      const double vz = (A.trans_cart[3 * tt + 2] + A.atom_pos[3 * aj + 2]) - riz;
+      const double r = sqrt(vx * vx + vy * vy + vz * vz);
+      if (r < A.cutoff && r > 0.0) ++mine;
+    }
+    s_cnt[tid] = mine;
+    __syncthreads();
+    if (tid == 0) {
+      int acc = 0;
+      for (int q = 0; q < BUILD_THREADS; ++q) { int v = s_cnt[q]; s_cnt[q] = acc; acc += v; }
+      s_total = acc;
+    }
+    __syncthreads();
+    int pos = s_cnt[tid];
+    for (int c = beg; c < end; ++c) {
+      const long long g = seg0 + c;
+      const int jr = (int)(g / sd.nt), t = (int)(g % sd.nt);
+      const int aj = sd.atom0 + wk.j0 + jr;
+      const int tt = sd.t0 + t;
+      const double vx = (A.trans_cart[3 * tt] + A.atom_pos[3 * aj]) - rix;
+      const double vy = (A.trans_cart[3 * tt + 1] + A.atom_pos[3 * aj + 1]) - riy;
+      const double vz = (A.trans_cart[3 * tt + 2] + A.atom_pos[3 * aj + 2]) - riz;
+      const double r = sqrt(vx * vx + vy * vy + vz * vz);
+      if (r < A.cutoff && r > 0.0) s_list[pos++] = (jr << 20) | t;
+    }
+    __syncthreads();
+    const int total = s_total;
+    for (int b0 = 0; b0 < total; b0 += MAXPI) {
+      const int np = min(MAXPI, total - b0);
+      if (tid < 2 * np) {
+        const int p = tid >> 1, dir = tid & 1;
+        const int packed = s_list[b0 + p];
+        const int jr = packed >> 20, t = packed & 0xFFFFF;
+        const int aj = sd.atom0 + wk.j0 + jr;
+        const int tt = sd.t0 + t;
+        const double vx = (A.trans_cart[3 * tt] + A.atom_pos[3 * aj]) - rix;
+        const double vy = (A.trans_cart[3 * tt + 1] + A.atom_pos[3 * aj + 1]) - riy;
+        const double vz = (A.trans_cart[3 * tt + 2] + A.atom_pos[3 * aj + 2]) - riz;
+        const double r = sqrt(vx * vx + vy * vy + vz * vz);
+        const int sj = A.atom_spec[aj];
+        const int pid = dir == 0 ? A.tab.pair_index[si * A.tab.nspec + sj]
+                                 : A.tab.pair_index[sj * A.tab.nspec + si];
+        if (dir == 0) {
+          pi_j[p] = wk.j0 + jr;
+          pi_t[p] = t;
+          pi_u[p][0] = vx / r; pi_u[p][1] = vy / r; pi_u[p][2] = vz / r;
+        }
+        int base = -1;
+        double w[8];
+        if (pid >= 0) {
+          interp_weights(r, A.tab.pair_ngrid[pid], A.tab.pair_step[pid], A.tab.tail, &base, w);
+          if (base >= 0) base += A.tab.pair_row0[pid];
+        }
+        pi_base[p][dir] = base;
+        for (int m = 0; m < 8; ++m) pi_w[p][dir][m] = (base >= 0) ? w[m] : 0.0;
+      }
+      for (int q = tid; q < np * 2 * 81; q += BUILD_THREADS) (&G[0][0][0])[q] = 0.0;
+      for (int q = tid; q < np * 2 * 20; q += BUILD_THREADS) (&dV[0][0][0])[q] = 0.0;
+      __syncthreads();
+      // G[p][H/S][a*9+b] = Re sum_k Q(k)[nu][mu] exp(2 pi i k.T), over k chunks
+      for (int k0 = 0; k0 < wk.nkc; k0 += ADJ_KCH) {
+        const int nkc = min(ADJ_KCH, wk.nkc - k0);
+        for (int task = tid; task < np * ADJ_KCH; task += BUILD_THREADS) {
+          const int p = task / ADJ_KCH, kk = task % ADJ_KCH;
+          double2 v = make_double2(0.0, 0.0);
+          if (kk < nkc) {
+            const int kg = sd.k0 + wk.kc0 + k0 + kk;
+            const int tt = sd.t0 + pi_t[p];
+            const double arg = 2.0 * (A.kfrac[3 * kg] * A.trans_int[3 * tt] +
+                                      A.kfrac[3 * kg + 1] * A.trans_int[3 * tt + 1] +
+                                      A.kfrac[3 * kg + 2] * A.trans_int[3 * tt + 2]);
+            double sn, cs;
+            sincospi(arg, &sn, &cs);
+            v = make_double2(cs, sn);
+          }
+          ph[p][kk] = v;
+        }
+        __syncthreads();
+        for (int task = tid; task < np * 81; task += BUILD_THREADS) {
+          const int p = task / 81, ab = task % 81, a = ab / 9, b = ab % 9;
+          // local orbitals of slots a (atom i) and b (atom j), if present
+          int ml = -1, cl = -1;
+          for (int o = 0; o < norb_i; ++o)
+            if (A.tab.spec_slot[si * 9 + o] == a) ml = o;
+          const int j = pi_j[p];
+          for (int c = 0; c < wk.ncol; ++c)
+            if (col_atom[c] == j && col_slot[c] == b) cl = c;
+          if (ml < 0 || cl < 0) continue;
+          const size_t mu = (size_t)(oi + ml), nu = (size_t)col_orb[cl];
+          double gh = 0.0, gs = 0.0;
+          for (int kk = 0; kk < nkc; ++kk) {
+            const size_t off = (size_t)(sd.mat0 + wk.kc0 + k0 + kk) * mat_el + nu * sd.ld + mu;
+            const double2 f = ph[p][kk];
+            const double2 qh = QH[off], qs = QS[off];
+            gh += qh.x * f.x - qh.y * f.y;
+            gs += qs.x * f.x - qs.y * f.y;
+          }
+          G[p][0][ab] += gh;
+          G[p][1][ab] += gs;
+        }
+        __syncthreads();
+      }
+      // contract with the SK coefficients (unit-integral evaluations)
+      for (int task = tid; task < np * 81; task += BUILD_THREADS) {
+        const int p = task / 81, ab = task % 81, a = ab / 9, b = ab % 9;
+        const double gh = G[p][0][ab], gs = G[p][1][ab];
+        if (gh == 0.0 && gs == 0.0) continue;
+        const int la = c_slot_l[a], lb = c_slot_l[b];
+        const int j = pi_j[p];
+        const bool direct = (la < lb) || (la == lb && wk.i <= j);
+        const double ux = pi_u[p][0], uy = pi_u[p][1], uz = pi_u[p][2];
+        int l1, l2, m1, m2, dir;
+        double x, y, z;
+        if (direct) { l1 = la; l2 = lb; m1 = c_slot_m[a]; m2 = c_slot_m[b]; dir = 0; x = ux; y = uy; z = uz; }
+        else { l1 = lb; l2 = la; m1 = c_slot_m[b]; m2 = c_slot_m[a]; dir = 1; x = -ux; y = -uy; z = -uz; }
+        const int ib = lpair_base(l1, l2);
+        for (int q = 0; q <= l1; ++q) {
+          double unit[3] = {0.0, 0.0, 0.0};
+          unit[q] = 1.0;
+          const double cf = sk_elem(l1, l2, m1, m2, x, y, z, unit);
+          if (cf != 0.0) {
+            atomicAdd(&dV[p][dir][ib + q], gh * cf);
+            atomicAdd(&dV[p][dir][10 + ib + q], gs * cf);
+          }
+        }
+      }
+      __syncthreads();
+      // scatter onto the table rows through the interpolation weights
+      for (int task = tid; task < np * 2 * 20; task += BUILD_THREADS) {
+        const int p = task / 40, rem = task % 40, dir = rem / 20, ig = rem % 20;
+        const int base = pi_base[p][dir];
+        const double g = dV[p][dir][ig];
+        if (base < 0 || g == 0.0) continue;
+        for (int m = 0; m < 8; ++m) atomicAdd(&dtab[(size_t)(base + m) * 20 + ig], pi_w[p][dir][m] * g);
+      }
+      __syncthreads();
+    }
+  }
+}
+
+// Y[b][r][c] = C[b][r][c] * g[b][c] (complex128 C, real g with row stride gld).
+__global__ void scale_columns_kernel(const double2* __restrict__ Cm, const double* __restrict__ g, int n, int ld,
+                                     int gld, double2* __restrict__ Y, long long total) {
+  for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < total;
+       q += (long long)gridDim.x * blockDim.x) {
+    const long long b = q / ((long long)ld * ld);
+    const int rem = (int)(q % ((long long)ld * ld));
+    const int c = rem % ld;
+    const double s = c < n ? g[b * gld + c] : 0.0;
+    const double2 v = Cm[q];
+    Y[q] = make_double2(v.x * s, v.y * s);
+  }
+}
+
+extern "C" {
+
+int slk_adjoint_tables_launch(const SlkBuildArgs* args, const void* QH, const void* QS, double* dtab,
+                              hipStream_t stream) {
+  if (args->n_work <= 0) return 0;
+  hipLaunchKernelGGL(slk_adjoint_kernel, dim3(args->n_work), dim3(BUILD_THREADS), 0, stream, *args,
+                     reinterpret_cast<const double2*>(QH), reinterpret_cast<const double2*>(QS), dtab);
+  return (int)hipGetLastError();
+}
+
+int slk_scale_columns(const void* Cm, const double* g, int n, int ld, int batch, int gld, void* Y,
+                      hipStream_t stream) {
+  const long long total = (long long)batch * ld * ld;
+  if (total <= 0) return 0;
+  long long blocks = (total + 255) / 256;
+  if (blocks > 16384) blocks = 16384;
+  hipLaunchKernelGGL(scale_columns_kernel, dim3((unsigned)blocks), dim3(256), 0, stream,
+                     reinterpret_cast<const double2*>(Cm), g, n, ld, gld, reinterpret_cast<double2*>(Y), total);
+  return (int)hipGetLastError();
+}
+
+}  // extern "C"

@@ -216,6 +216,42 @@ __global__ __launch_bounds__(POST_THREADS) void dos_kernel(const double* __restr
   if (e < ne) dos[jb.out0 + e] = acc;
 }
 
+// Adjoint of dos_kernel: for every eigenvalue e_kb of a job,
+//   gW[k][b] = dL/de_kb = H2E * w_k * sum_E gdos(E) N exp(-t^2/2) t / sigma,
+//   t = (E - (e_kb - mu) * H2E) / sigma,
+// i.e. the derivative at fixed mu (the caller chains mu's own derivative).
+// One thread per eigenvalue, the grid and the incoming gradient staged in LDS.
+__global__ __launch_bounds__(POST_THREADS) void dos_adjoint_kernel(
+    const double* __restrict__ W, const double* __restrict__ KW, const SlkDosJob* jobs,
+    const double* __restrict__ grid, int ne, double sigma, const double* edges, const double* __restrict__ gdos,
+    double* __restrict__ gW) {
+  const SlkDosJob jb = jobs[blockIdx.y];
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  const int tot = jb.nk * jb.nb;
+  const double mu = jb.edge >= 0 ? edges[(size_t)jb.edge * SLK_EDGE_FIELDS + E_MU] : 0.0;
+  const double inv = 1.0 / sigma;
+  const double norm = 1.0 / (sigma * sqrt(2.0 * 3.141592653589793));
+  const int k = q < tot ? q / jb.nb : 0, b = q < tot ? q % jb.nb : 0;
+  const double x = q < tot ? (W[jb.w0 + (long long)k * jb.wld + b] - mu) * H2E_OUT : 0.0;
+  __shared__ double sE[DOS_CHUNK];
+  __shared__ double sg[DOS_CHUNK];
+  double acc = 0.0;
+  for (int c0 = 0; c0 < ne; c0 += DOS_CHUNK) {
+    const int cn = min(DOS_CHUNK, ne - c0);
+    __syncthreads();
+    for (int i = threadIdx.x; i < cn; i += blockDim.x) {
+      sE[i] = grid[c0 + i];
+      sg[i] = gdos[jb.out0 + c0 + i];
+    }
+    __syncthreads();
+    for (int i = 0; i < cn; ++i) {
+      const double t = (sE[i] - x) * inv;
+      acc += sg[i] * (exp(-0.5 * t * t) * t);
+    }
+  }
+  if (q < tot) gW[jb.w0 + (long long)k * jb.wld + b] = acc * norm * inv * H2E_OUT * KW[jb.kw0 + k];
+}
+
 // dos stats: [dos_at_fermi, gap_center, trapz_total]
 __global__ __launch_bounds__(POST_THREADS) void dos_stats_kernel(const SlkDosJob* jobs, const double* __restrict__ grid,
                                                                  int ne, const double* __restrict__ dos,
@@ -276,6 +312,15 @@ int slk_dos_launch(const double* W, const double* KW, const SlkDosJob* jobs, int
                      W, KW, jobs, grid, ne, sigma, edges, dos);
   if (stats)
     hipLaunchKernelGGL(dos_stats_kernel, dim3(n_jobs), dim3(POST_THREADS), 0, stream, jobs, grid, ne, dos, stats);
+  return (int)hipGetLastError();
+}
+
+int slk_dos_adjoint_launch(const double* W, const double* KW, const SlkDosJob* jobs, int n_jobs, int max_states,
+                           const double* grid, int ne, double sigma, const double* edges, const double* gdos,
+                           double* gW, hipStream_t stream) {
+  if (n_jobs <= 0 || max_states <= 0) return 0;
+  hipLaunchKernelGGL(dos_adjoint_kernel, dim3((max_states + POST_THREADS - 1) / POST_THREADS, n_jobs),
+                     dim3(POST_THREADS), 0, stream, W, KW, jobs, grid, ne, sigma, edges, gdos, gW);
   return (int)hipGetLastError();
 }
 

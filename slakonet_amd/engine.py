@@ -170,6 +170,16 @@ def plan_build(tables: TableSet, inputs: Sequence[StructureInput], ld: Optional[
     return BuildPlan(len(inputs), m0, ld, n_orb, nks, mat0, kw, nelec, dev, len(work))
 
 
+def build_args(tables: TableSet, plan: BuildPlan, H_ptr: int, S_ptr: int, out_f32: bool = False,
+               cutoff: float = HS_CUTOFF_BOHR, table_tensor: Optional[torch.Tensor] = None):
+    """SlkBuildArgs of a plan (shared by the build and its adjoint)."""
+    d = plan.dev
+    return _lib.SlkBuildArgs(tables.c_struct(table_tensor), d["structs"].data_ptr(), d["work"].data_ptr(),
+                             plan.n_work, d["atom_pos"].data_ptr(), d["atom_spec"].data_ptr(),
+                             d["atom_orb"].data_ptr(), d["trans_cart"].data_ptr(), d["trans_int"].data_ptr(),
+                             d["kfrac"].data_ptr(), float(cutoff), H_ptr, S_ptr, int(out_f32))
+
+
 def run_build(tables: TableSet, plan: BuildPlan, out_f32: bool = False, cutoff: float = HS_CUTOFF_BOHR,
               table_tensor: Optional[torch.Tensor] = None, stream=None):
     """H, S [n_mat, ld, ld] complex128 (or complex64)."""
@@ -178,10 +188,7 @@ def run_build(tables: TableSet, plan: BuildPlan, out_f32: bool = False, cutoff: 
     H = torch.empty(plan.n_mat, plan.ld, plan.ld, dtype=dt, device=tables.device)
     S = torch.empty_like(H)
     d = plan.dev
-    args = _lib.SlkBuildArgs(tables.c_struct(table_tensor), d["structs"].data_ptr(), d["work"].data_ptr(),
-                             plan.n_work, d["atom_pos"].data_ptr(), d["atom_spec"].data_ptr(),
-                             d["atom_orb"].data_ptr(), d["trans_cart"].data_ptr(), d["trans_int"].data_ptr(),
-                             d["kfrac"].data_ptr(), float(cutoff), H.data_ptr(), S.data_ptr(), int(out_f32))
+    args = build_args(tables, plan, H.data_ptr(), S.data_ptr(), out_f32, cutoff, table_tensor)
     sh = _lib.stream_handle(stream)
     _lib.check(lib.slk_build_hs_launch(C.byref(args), sh), "slk_build_hs_launch")
     for s in range(plan.n_struct):
@@ -260,6 +267,14 @@ def spectra(plan: BuildPlan, w_ld: int) -> np.ndarray:
     return sp
 
 
+def dos_jobs(plan: BuildPlan, w_ld: int, ne: int) -> np.ndarray:
+    """One DOS job per structure: its eigenvalue rows, Fermi shift from its edges row."""
+    jobs = np.zeros(plan.n_struct, dtype=DOS_DT)
+    for s in range(plan.n_struct):
+        jobs[s] = (plan.mat0[s] * w_ld, plan.nk[s], plan.n_orb[s], w_ld, plan.mat0[s], s, s * ne)
+    return jobs
+
+
 def post(plan: BuildPlan, w: torch.Tensor, fermi_mode: int = 0, kT: float = FERMI_KT_HA,
          dos_grid: Optional[torch.Tensor] = None, sigma: float = 0.1, stream=None):
     """Fermi level, band edges and (optionally) DOS for every structure of the plan.
@@ -282,10 +297,7 @@ def post(plan: BuildPlan, w: torch.Tensor, fermi_mode: int = 0, kT: float = FERM
     dos = stats = None
     if dos_grid is not None:
         ne = int(dos_grid.numel())
-        jobs = np.zeros(plan.n_struct, dtype=DOS_DT)
-        for s in range(plan.n_struct):
-            jobs[s] = (plan.mat0[s] * w.shape[1], plan.nk[s], plan.n_orb[s], w.shape[1], plan.mat0[s], s, s * ne)
-        jd = _upload(jobs, dev)
+        jd = _upload(dos_jobs(plan, w.shape[1], ne), dev)
         dos = torch.empty(plan.n_struct, ne, dtype=torch.float64, device=dev)
         stats = torch.empty(plan.n_struct, 3, dtype=torch.float64, device=dev)
         _lib.check(lib.slk_dos_launch(C.c_void_p(w.data_ptr()), C.c_void_p(kw.data_ptr()), C.c_void_p(jd.data_ptr()),

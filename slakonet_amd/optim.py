@@ -194,14 +194,15 @@ class MultiElementSkfParameterOptimizer(nn.Module):
         return properties, True
 
     # ------------------------------------------------------------------ I/O
-    def save_model(self, save_path, method="compact"):
+    def save_model(self, save_path, method="state_dict"):
+        """Save in the reference layouts: "state_dict", "compact" or "universal_params"."""
         from .model_io import save_model
         return save_model(self, save_path, method)
 
     @classmethod
-    def load_model(cls, load_path, method="compact", skf_directory=None):
+    def load_model(cls, load_path, method="state_dict", skf_directory=None):
         from .model_io import load_model
-        return load_model(cls, load_path, method)
+        return load_model(cls, load_path, method, skf_directory)
 
 
 def _load_universal(path):

@@ -115,3 +115,26 @@ def test_stream_structures_respect_min_distance():
             dd[a] = 99
             assert dd.min() >= 1.8 - 1e-9
         assert 2 <= st.n_atoms <= 256
+
+
+@pytest.mark.parametrize("method", ["state_dict", "compact", "universal_params"])
+def test_model_save_load_round_trip(tmp_path, method):
+    """Reference save/load layouts (slakonet/optim.py:204-625), read back with
+    weights_only loads; trained values and constraint originals survive."""
+    import torch
+    from slakonet_amd import synth
+    from slakonet_amd.optim import MultiElementSkfParameterOptimizer
+    model = MultiElementSkfParameterOptimizer(skf_dicts=synth.make_skf_set([6, 14], seed=2), device="cpu")
+    with torch.no_grad():
+        for p in model.parameters():
+            p.mul_(1.01)
+    model.save_model(tmp_path / "m", method=method)
+    back = MultiElementSkfParameterOptimizer.load_model(tmp_path / "m", method=method)
+    a, b = model.state_dict(), back.state_dict()
+    assert a.keys() == b.keys()
+    assert all(torch.equal(a[k], b[k]) for k in a)
+    if method == "state_dict":
+        for key, opt in model.skf_optimizers.items():
+            ob = back.skf_optimizers[key]
+            assert all(torch.equal(opt.original_h_params[k], ob.original_h_params[k]) for k in opt.original_h_params)
+    back.apply_constraints()

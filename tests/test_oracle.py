@@ -210,3 +210,52 @@ def test_gap_rules():
     assert abs(bs["minimum_direct_gap"] - 0.2 * 27.211) < 1e-12
     # metal rule: a level within 1e-3 eV of mu
     assert post.band_gap(np.array([[-0.1, 0.0]]), 0.0)["metal"]
+
+
+# ---- eigenvalue adjoints (oracle/adjoint.py) --------------------------------
+
+def _herm(rng, n, cplx):
+    a = rng.normal(size=(n, n)) + (1j * rng.normal(size=(n, n)) if cplx else 0.0)
+    return (a + np.conj(a.T)) / 2
+
+
+def _spd(rng, n, cplx):
+    b = rng.normal(size=(n, n)) + (1j * rng.normal(size=(n, n)) if cplx else 0.0)
+    return b @ np.conj(b.T) + n * np.eye(n)
+
+
+@pytest.mark.parametrize("cplx", [False, True])
+def test_eig_densities_match_central_differences(cplx):
+    """de = c^H (dH - e dS) c as densities vs central FD (step 1e-4)."""
+    from oracle import adjoint
+    rng = np.random.default_rng(11 + cplx)
+    n = 7
+    H, S = _herm(rng, n, cplx), _spd(rng, n, cplx)
+    g = rng.normal(size=n)
+    QH, QS, _ = adjoint.eig_densities(H, S, g)
+    DH, DS = _herm(rng, n, cplx), 0.1 * _herm(rng, n, cplx)
+    h = 1e-4
+    loss = lambda t: g @ scipy.linalg.eigh(H + t * DH, S + t * DS, eigvals_only=True)
+    fd = (loss(h) - loss(-h)) / (2 * h)
+    an = adjoint.directional(QH, DH) + adjoint.directional(QS, DS)
+    assert abs(an - fd) < 1e-6 * abs(fd)
+
+
+def test_reference_autograd_agrees_at_gamma_only():
+    """The reference's _SymEigB eigenvalue gradient (V diag(g) V^T, no conj)
+    equals the exact one for real matrices and not for complex ones."""
+    from oracle import adjoint
+    rng = np.random.default_rng(5)
+    n = 6
+    for cplx in (False, True):
+        H, S = _herm(rng, n, cplx), _spd(rng, n, cplx)
+        g = rng.normal(size=n)
+        QH, QS, _ = adjoint.eig_densities(H, S, g)
+        gh, gs = adjoint.reference_gradients(H, S, g)
+        DH, DS = _herm(rng, n, cplx), 0.1 * _herm(rng, n, cplx)
+        ours = adjoint.directional(QH, DH) + adjoint.directional(QS, DS)
+        ref = adjoint.torch_directional(gh, DH) + adjoint.torch_directional(gs, DS)
+        if cplx:
+            assert abs(ref - ours) > 1e-3 * abs(ours)
+        else:
+            assert abs(ref - ours) < 1e-10 * max(1.0, abs(ours))
