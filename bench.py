@@ -43,6 +43,12 @@ def parse():
     p.add_argument("--mode", choices=["serve", "plain", "fit"], default="serve",
                    help="fit: the fitting workload (32 cells of 8-64 atoms, 3 species, DOS + gap loss, Adam steps)")
     p.add_argument("--fit-steps", type=int, default=200, help="Adam steps timed in --mode fit")
+    p.add_argument("--sample-every", type=int, default=8,
+                   help="sample every n-th tridiagonalisation mat-vec launch for the roofline")
+    p.add_argument("--sample-capacity", type=int, default=8192,
+                   help="event-sampled launches at most (0: count launches and bytes only, no events)")
+    p.add_argument("--copy-mib", type=int, default=0,
+                   help="run only a plain device copy of this many MiB (counter calibration) and exit")
     p.add_argument("--cpu-seconds", type=float, default=20.0, help="CPU baseline sample budget (0 = skip)")
     p.add_argument("--no-roofline", action="store_true")
     p.add_argument("--precision", choices=["mixed", "fp64"], default="mixed",
@@ -108,35 +114,71 @@ def run_serve(tables, inputs, arrivals, grid):
     return res["wall_s"], res["latency_s"]
 
 
-def roofline(tables, inputs):
-    """Live HIP-event timing of the dominant kernel over a sample batch."""
-    import torch
-    from slakonet_amd import engine
-    # representative sample: the largest structures of the stream dominate time
-    order = np.argsort([-(len(i.z)) for i in inputs])[:4]
-    stage_tot = {}
-    flops = 0.0
-    build_bytes = 0.0
-    build_ms = 0.0
-    for idx in order:
-        inp = inputs[idx]
-        plan = engine.plan_build(tables, [inp])
-        s0 = torch.cuda.Event(enable_timing=True)
-        s1 = torch.cuda.Event(enable_timing=True)
-        s0.record()
-        H, S = engine.run_build(tables, plan)
-        s1.record()
-        torch.cuda.synchronize()
-        build_ms += s0.elapsed_time(s1)
-        n = plan.n_orb[0]
-        build_bytes += plan.n_mat * 2 * n * n * 16
-        ms = []
-        engine.eigh(H, S, stage_ms=ms, precision=PRECISION)
-        for k, v in ms[0].items():
-            stage_tot[k] = stage_tot.get(k, 0.0) + v
-        flops += plan.n_mat * (4.0 / 3.0) * 4 * n ** 3   # complex hetrd, LAPACK count
-    dom = max(stage_tot, key=stage_tot.get)
-    return dom, stage_tot, flops, build_bytes, build_ms
+HETRD_SAMPLE_EVERY = 8
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hetrd_step_traffic.json")
+
+
+def start_sampling(every, capacity):
+    """Live sampling over the timed region (inside the library): every launch
+    of the tridiagonalisation mat-vec and of the build is counted with its
+    algorithmic bytes; every ``every``-th launch of each records its execution
+    window (earliest workgroup start, latest workgroup end on the GPU's
+    constant-rate wall clock) and is bracketed by a HIP start/stop event pair
+    on its launch stream. ``avg_launch_us`` is the execution window;
+    ``avg_launch_us_events`` the event pair, which under four concurrent
+    streams also holds the wait for a dispatch slot."""
+    from slakonet_amd import _lib, engine
+    _lib.check(_lib.lib().slk_sampler_reset(1, every, capacity), "slk_sampler_reset")
+    engine.SAMPLING = True
+
+
+def stop_sampling():
+    import ctypes as C
+    from slakonet_amd import _lib, engine
+    lib = _lib.lib()
+    res = []
+    for kind in (0, 1):
+        buf = (C.c_double * 7)()
+        _lib.check(lib.slk_sampler_read(kind, buf), "slk_sampler_read")
+        res.append(tuple(buf))
+    _lib.check(lib.slk_sampler_reset(0, 0, 0), "slk_sampler_reset")
+    engine.SAMPLING = False
+    return res
+
+
+def roofline_fields(hetrd, build):
+    """roofline (dominant kernel: the tridiagonalisation mat-vec) and
+    build_roofline from the live samples; traffic = measured HBM bytes per
+    algorithmic byte (counter runs, profiles/hetrd_step_traffic.json) times
+    the algorithmic bytes per launch."""
+    out = {}
+    n, ms, nbytes, n_all, bytes_all, ev_ms, _ = hetrd
+    if n_all > 0:
+        ach = nbytes / (ms * 1e-3) / 1e9 if n > 0 and ms > 0 else None
+        per_launch = nbytes / n if n > 0 else bytes_all / n_all
+        traffic = None
+        if os.path.exists(TRAFFIC_FILE):
+            with open(TRAFFIC_FILE) as f:
+                ratio = json.load(f).get("hbm_bytes_per_algorithmic_byte")
+            if ratio:
+                traffic = ratio * per_launch
+        out["roofline"] = {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                           "frac": None if ach is None else ach / HBM_PEAK_GBS, "traffic": traffic,
+                           "kernel": "hetrd_step_kernel<float> (tridiagonalisation trailing mat-vec)",
+                           "avg_launch_us": 1000.0 * ms / n if n > 0 else None,
+                           "avg_launch_us_events": 1000.0 * ev_ms / n if n > 0 else None,
+                           "algorithmic_bytes_per_launch": per_launch, "launches_sampled": int(n),
+                           "sample_every": HETRD_SAMPLE_EVERY, "launches_total": int(n_all),
+                           "algorithmic_bytes_total": bytes_all}
+    bn, bms, bbytes, bn_all, bbytes_all, bev_ms, _ = build
+    if bn > 0 and bms > 0:
+        ach = bbytes / (bms * 1e-3) / 1e9
+        out["build_roofline"] = {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                 "frac": ach / HBM_PEAK_GBS, "kernel": "slk_build_kernel<complex128>",
+                                 "avg_launch_us": 1000.0 * bms / bn, "avg_launch_us_events": 1000.0 * bev_ms / bn,
+                                 "algorithmic_bytes_per_launch": bbytes / bn,
+                                 "launches_sampled": int(bn), "launches_total": int(bn_all)}
+    return out
 
 
 def run_fit(args):
@@ -219,6 +261,16 @@ def main():
     import torch
     from slakonet_amd import synth
     torch.cuda.set_device(0)
+    global HETRD_SAMPLE_EVERY
+    HETRD_SAMPLE_EVERY = max(1, args.sample_every)
+    if args.copy_mib > 0:
+        # plain copy of a known size: HBM counter calibration (reads and writes copy_mib MiB)
+        src = torch.ones(args.copy_mib << 18, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes": src.numel() * 4}))
+        return
     if args.mode == "fit":
         run_fit(args)
         return
@@ -238,6 +290,8 @@ def main():
         runner(tables, inputs[:m], np.zeros(m), grid)
         print("# warmup done", file=sys.stderr, flush=True)
     torch.cuda.synchronize()
+    if not args.no_roofline:
+        start_sampling(HETRD_SAMPLE_EVERY, args.sample_capacity)
     walls, lat = [], None
     t_start = time.perf_counter()
     for s in range(args.steps):
@@ -246,6 +300,7 @@ def main():
         print(f"# step {s}: {wall:.2f} s", file=sys.stderr, flush=True)
     torch.cuda.synchronize()
     total = time.perf_counter() - t_start
+    samples = stop_sampling() if not args.no_roofline else None
     value = args.requests * args.steps / total
     out = {
         "metric": METRIC, "value": value, "unit": "structures/s", "n_gpus": 1, "steps": args.steps,
@@ -257,19 +312,8 @@ def main():
                                f"Z<=65, Poisson {args.rate:g}/s, full spectrum + Fermi + gap + 2001-pt DOS",
                    "mode": args.mode, "requests": args.requests, "rate_per_s": args.rate},
     }
-    if not args.no_roofline:
-        dom, stages, flops, bbytes, bms = roofline(tables, inputs)
-        tot = stages[dom]
-        if dom == "hetrd":
-            ach = flops / (tot * 1e-3) / 1e12
-            out["roofline"] = {"bound": "mfma", "achieved": ach, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                               "frac": ach / FP64_PEAK_TFLOPS, "traffic": None, "kernel": "hetrd_kernel",
-                               "stage_ms": stages}
-        else:
-            out["roofline"] = {"bound": "mfma", "achieved": None, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                               "frac": None, "traffic": None, "kernel": dom, "stage_ms": stages}
-        out["build_roofline"] = {"bound": "hbm", "achieved": bbytes / (bms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS,
-                                 "unit": "GB/s", "frac": bbytes / (bms * 1e-3) / 1e9 / HBM_PEAK_GBS}
+    if samples is not None:
+        out.update(roofline_fields(*samples))
     if args.cpu_seconds > 0:
         out["cpu_baseline"] = cpu_baseline(structs, skfs, args.cpu_seconds)
     print(json.dumps(out))

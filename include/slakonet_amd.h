@@ -158,6 +158,24 @@ typedef struct SlkDosJob {
 int slk_dos_launch(const double* W, const double* KW, const SlkDosJob* jobs, int n_jobs, const double* grid, int ne,
                    double sigma, const double* edges, double* dos, double* stats, hipStream_t stream);
 
+/* ---- Measurement --------------------------------------------------------- */
+
+/* Live kernel sampling, no reference counterpart (the reference has no
+ * tracing, only time.time() prints). While enabled, every launch of a
+ * sampled kernel kind is counted with its algorithmic bytes, and every
+ * `every`-th one (at most `capacity`) is started with a HIP start/stop event
+ * pair that the runtime records at the kernel's own begin and end on its
+ * launch stream. Kinds: 0 = tridiagonalisation mat-vec (bytes = one triangle
+ * of the trailing Hermitian block per matrix), 1 = H/S build (bytes from
+ * slk_sampler_hint: the H and S elements written). enable = 0 disables and
+ * frees. Single host thread. */
+int slk_sampler_reset(int enable, int every, int capacity);
+int slk_sampler_hint(int kind, double bytes);
+/* out[0] = event samples, out[1] = their summed milliseconds, out[2] = their
+ * summed bytes, out[3] = launches counted, out[4] = bytes of all counted
+ * launches; waits for the sampled launches to complete. */
+int slk_sampler_read(int kind, double* out);
+
 /* ---- Differentiation form (hand-written adjoints) ----------------------- */
 
 /* Adjoint of slk_dos_launch at fixed Fermi level: gW[w0 + k*wld + b] =

@@ -80,6 +80,9 @@ SIGNATURES = {
     "slk_edge_fields": (C.c_int, []),
     "slk_dos_launch": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_double,
                                  C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "slk_sampler_reset": (C.c_int, [C.c_int, C.c_int, C.c_int]),
+    "slk_sampler_hint": (C.c_int, [C.c_int, C.c_double]),
+    "slk_sampler_read": (C.c_int, [C.c_int, C.c_void_p]),
     "slk_dos_adjoint_launch": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_int,
                                          C.c_double, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "slk_scale_columns": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_void_p,

@@ -18,6 +18,9 @@
 // Every output element is written exactly once; zero blocks included.
 #include "slk_common.h"
 #include "slakonet_amd.h"
+#include <hip/hip_ext.h>
+
+#include "slk_sampler.h"
 
 #define BUILD_THREADS 256
 #define KC 4            // k-points per workgroup
@@ -176,7 +179,8 @@ template <>
 __device__ __forceinline__ void store_c<float2>(float2* p, double2 v) { *p = make_float2((float)v.x, (float)v.y); }
 
 template <typename OUT>
-__global__ __launch_bounds__(BUILD_THREADS) void slk_build_kernel(SlkBuildArgs A) {
+__global__ __launch_bounds__(BUILD_THREADS) void slk_build_kernel(SlkBuildArgs A, unsigned long long* clk) {
+  sample_start(clk);
   __shared__ int s_list[SEG];            // compacted candidates (packed jrel<<20 | t)
   __shared__ int s_cnt[BUILD_THREADS];
   __shared__ int s_total;
@@ -398,6 +402,10 @@ __global__ __launch_bounds__(BUILD_THREADS) void slk_build_kernel(SlkBuildArgs A
       }
     }
   }
+  if (clk != nullptr) {
+    __syncthreads();
+    sample_end(clk);
+  }
 }
 
 // Padding rows/columns [n_orb, ld) of every k-matrix: zero off-diagonal,
@@ -429,10 +437,22 @@ extern "C" {
 
 int slk_build_hs_launch(const SlkBuildArgs* args, hipStream_t stream) {
   if (args->n_work <= 0) return 0;
-  if (args->out_f32)
-    hipLaunchKernelGGL(slk_build_kernel<float2>, dim3(args->n_work), dim3(BUILD_THREADS), 0, stream, *args);
-  else
-    hipLaunchKernelGGL(slk_build_kernel<double2>, dim3(args->n_work), dim3(BUILD_THREADS), 0, stream, *args);
+  SampleSlot sl;
+  unsigned long long* none = nullptr;
+  const bool sampled = sampler_take(SLK_SAMPLE_BUILD, -1.0, &sl);
+  if (args->out_f32) {
+    if (sampled)
+      hipExtLaunchKernelGGL(slk_build_kernel<float2>, dim3(args->n_work), dim3(BUILD_THREADS), 0, stream, sl.e0,
+                            sl.e1, 0, *args, sl.clk);
+    else
+      hipLaunchKernelGGL(slk_build_kernel<float2>, dim3(args->n_work), dim3(BUILD_THREADS), 0, stream, *args, none);
+  } else {
+    if (sampled)
+      hipExtLaunchKernelGGL(slk_build_kernel<double2>, dim3(args->n_work), dim3(BUILD_THREADS), 0, stream, sl.e0,
+                            sl.e1, 0, *args, sl.clk);
+    else
+      hipLaunchKernelGGL(slk_build_kernel<double2>, dim3(args->n_work), dim3(BUILD_THREADS), 0, stream, *args, none);
+  }
   return (int)hipGetLastError();
 }
 

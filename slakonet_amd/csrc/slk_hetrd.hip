@@ -4,24 +4,28 @@
 // always returned in FP64):
 //
 //   per panel:  latrd_start (first column reflector)
-//               per column j: hetrd_step = 2-D tiled trailing mat-vec
+//               per column j: hetrd_step = trailing Hermitian mat-vec
 //                   y = A22 v_j - V (W^H v_j) - W (V^H v_j)
-//                 writing per-tile partial products (enough tiles to cover
-//                 the chip for one matrix), then hetrd_tail (one workgroup
-//                 per matrix) sums them, forms w_j and the reflector of
-//                 column j+1;
-//               rank-2k update A22 -= V W^H + W V^H on the MFMA GEMM.
+//                 over 128x128 tiles of the lower triangle, each read once
+//                 and used for both A v and A^H v, writing per-tile partial
+//                 products; then hetrd_tail (one workgroup per matrix) sums
+//                 them, forms w_j and the reflector of column j+1;
+//               rank-2k update A22 -= V W^H + W V^H on the MFMA GEMM, lower
+//               triangle tiles only (the upper triangle is never read).
 // A kernel boundary orders the tail after the tiles; a device-scope fence
 // per tile flushes L2 on this multi-XCD part and costs far more.
 // Panel buffers are column-major: Vc[k*ld + r], Wc[k*ld + r].
 #include "slk_common.h"
 #include "slk_dense.h"
+#include <hip/hip_ext.h>
+
+#include "slk_sampler.h"
 
 #define NB SLK_NB
 #define HT_THREADS 512
 #define TAIL_THREADS 1024
-#define HT_ROWS 16     // rows per tile: 2 per wave
-#define HT_COLS 256    // columns per tile: 4 per lane
+#define HT_TS 128      // square tile of the trailing mat-vec: 2 columns per lane
+#define HT_RPW (HT_TS / (HT_THREADS / 64))   // tile rows per wave
 
 template <typename T> struct C2T;
 template <> struct C2T<double> { using t = double2; };
@@ -162,11 +166,23 @@ __global__ __launch_bounds__(TAIL_THREADS) void latrd_start_kernel(HetrdCtx<T> c
   latrd_column(c, blockIdx.x, j, 0, red);
 }
 
-// Trailing mat-vec tile for column j: partial y over HT_COLS columns.
+// Trailing Hermitian mat-vec for column j from the lower triangle only.
+// The trailing block (rows/columns >= j+1) is cut into HT_TS x HT_TS tiles;
+// workgroup (t, b) takes lower tile t = (bi, bj), bi >= bj, of matrix b and
+// reads it once: its rows get A v (row partial), its columns get A^H v from
+// the same elements (column partial, strictly lower elements only). Partial
+// (ch, r) of row r in block bi comes from exactly one tile: ch = bj of tile
+// (bi, bj) for bj <= bi, or ch = bi' of tile (bi', bi) for bi' > bi. The
+// diagonal tile of a row block also subtracts the panel correction
+// V (W^H v) + W (V^H v) for its rows.
 template <typename T>
-__global__ __launch_bounds__(HT_THREADS) void hetrd_step_kernel(HetrdCtx<T> c, int j, int jj) {
+__global__ __launch_bounds__(HT_THREADS) void hetrd_step_kernel(HetrdCtx<T> c, int j, int jj,
+                                                                unsigned long long* clk) {
   using C2 = typename C2T<T>::t;
-  const int b = blockIdx.z;
+  sample_start(clk);
+  __shared__ C2 s_row[HT_TS];
+  __shared__ C2 s_col[HT_THREADS / 64][HT_TS];
+  const int b = blockIdx.y;
   const int n = c.n, ld = c.ld;
   const C2* a = c.A + (size_t)b * ld * ld;
   const C2* vc = c.Vc + (size_t)b * NB * ld;
@@ -174,43 +190,74 @@ __global__ __launch_bounds__(HT_THREADS) void hetrd_step_kernel(HetrdCtx<T> c, i
   const C2* v = vc + (size_t)jj * ld;
   const C2* c12 = c.C12 + (size_t)b * 2 * NB;
   C2* yp = c.Yp + (size_t)b * c.ncmax * ld;
+  int bi = 0;
+  const int t = blockIdx.x;
+  while ((bi + 1) * (bi + 2) / 2 <= t) ++bi;
+  const int bj = t - bi * (bi + 1) / 2;
+  const bool diag = bi == bj;
+  const int base = j + 1;
+  const int row0 = base + bi * HT_TS, col0 = base + bj * HT_TS;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  const int r0 = j + 1 + blockIdx.x * HT_ROWS + wv * 2;
-  const int cbase = j + 1 + blockIdx.y * HT_COLS;
-  C2 acc0 = mk<T>(0, 0), acc1 = mk<T>(0, 0);
-#pragma unroll
-  for (int i = 0; i < HT_COLS / 64; ++i) {
-    const int col = cbase + lane + 64 * i;
-    if (col < n) {
-      const C2 vv = v[col];
-      if (r0 < n) {
-        const C2 x = a[(size_t)r0 * ld + col];
-        acc0.x += x.x * vv.x - x.y * vv.y;
-        acc0.y += x.x * vv.y + x.y * vv.x;
-      }
-      if (r0 + 1 < n) {
-        const C2 x = a[(size_t)(r0 + 1) * ld + col];
-        acc1.x += x.x * vv.x - x.y * vv.y;
-        acc1.y += x.x * vv.y + x.y * vv.x;
-      }
+  const int cA = col0 + 2 * lane, cB = cA + 1;
+  const C2 zero = mk<T>(0, 0);
+  const C2 vA = cA < n ? v[cA] : zero, vB = cB < n ? v[cB] : zero;
+  C2 colA = zero, colB = zero;
+#pragma unroll 4
+  for (int rr = 0; rr < HT_RPW; ++rr) {
+    const int rl = wv * HT_RPW + rr;
+    const int r = row0 + rl;
+    C2 xA = zero, xB = zero, vr = zero;
+    if (r < n) {
+      vr = v[r];
+      if (cA < n && (!diag || cA <= r)) xA = a[(size_t)r * ld + cA];
+      if (cB < n && (!diag || cB <= r)) xB = a[(size_t)r * ld + cB];
+    }
+    // row partial: A[r][c] v[c]
+    T px = xA.x * vA.x - xA.y * vA.y + xB.x * vB.x - xB.y * vB.y;
+    T py = xA.x * vA.y + xA.y * vA.x + xB.x * vB.y + xB.y * vB.x;
+    px = wave_sum(px);
+    py = wave_sum(py);
+    if (lane == 0) s_row[rl] = mk<T>(px, py);
+    // column partial: conj(A[r][c]) v[r], strictly lower elements
+    if (!diag || cA < r) {
+      colA.x += xA.x * vr.x + xA.y * vr.y;
+      colA.y += xA.x * vr.y - xA.y * vr.x;
+    }
+    if (!diag || cB < r) {
+      colB.x += xB.x * vr.x + xB.y * vr.y;
+      colB.y += xB.x * vr.y - xB.y * vr.x;
     }
   }
-  acc0.x = wave_sum(acc0.x); acc0.y = wave_sum(acc0.y);
-  acc1.x = wave_sum(acc1.x); acc1.y = wave_sum(acc1.y);
-  if (lane < 2) {
-    const int r = r0 + lane;
-    if (r < n) {
-      C2 s = lane == 0 ? acc0 : acc1;
-      if (blockIdx.y == 0) {
+  s_col[wv][2 * lane] = colA;
+  s_col[wv][2 * lane + 1] = colB;
+  __syncthreads();
+  const int tid = threadIdx.x;
+  if (tid < HT_TS) {
+    C2 cs = zero;
+#pragma unroll
+    for (int w = 0; w < HT_THREADS / 64; ++w) cs = c_add(cs, s_col[w][tid]);
+    if (diag) {
+      const int r = row0 + tid;
+      if (r < n) {
+        C2 s = c_add(s_row[tid], cs);
         for (int k = 0; k < jj; ++k) {
           const C2 t1 = c_mul(vc[(size_t)k * ld + r], c12[k]);
           const C2 t2 = c_mul(wc[(size_t)k * ld + r], c12[NB + k]);
           s.x -= t1.x + t2.x;
           s.y -= t1.y + t2.y;
         }
+        yp[(size_t)bi * ld + r] = s;
       }
-      yp[(size_t)blockIdx.y * ld + r] = s;
+    } else {
+      const int r = row0 + tid;
+      if (r < n) yp[(size_t)bj * ld + r] = s_row[tid];
+      const int col = col0 + tid;
+      if (col < n) yp[(size_t)bi * ld + col] = cs;
     }
+  }
+  if (clk != nullptr) {
+    __syncthreads();
+    sample_end(clk);
   }
 }
 
@@ -245,7 +292,7 @@ __global__ void hetrd_last_kernel(const typename C2T<T>::t* A, int n, int ld, do
   }
 }
 
-size_t hetrd_partials_elems(int ld) { return (size_t)((ld + HT_COLS - 1) / HT_COLS) * ld; }
+size_t hetrd_partials_elems(int ld) { return (size_t)((ld + HT_TS - 1) / HT_TS) * ld; }
 
 template <typename T>
 static void rank2k(hipStream_t st, int batch, int m, int k, typename C2T<T>::t* Vc, typename C2T<T>::t* Wc,
@@ -253,14 +300,14 @@ static void rank2k(hipStream_t st, int batch, int m, int k, typename C2T<T>::t* 
 template <>
 void rank2k<double>(hipStream_t st, int batch, int m, int k, double2* Vc, double2* Wc, double2* C, int ld,
                     long long sP, long long sM) {
-  launch_zgemm(st, batch, m, m, k, -1.0, Vc, ld, sP, OP_T, Wc, ld, sP, OP_CONJ, 1.0, C, ld, sM, 0, 0);
-  launch_zgemm(st, batch, m, m, k, -1.0, Wc, ld, sP, OP_T, Vc, ld, sP, OP_CONJ, 1.0, C, ld, sM, 0, 0);
+  launch_zgemm(st, batch, m, m, k, -1.0, Vc, ld, sP, OP_T, Wc, ld, sP, OP_CONJ, 1.0, C, ld, sM, 0, 1);
+  launch_zgemm(st, batch, m, m, k, -1.0, Wc, ld, sP, OP_T, Vc, ld, sP, OP_CONJ, 1.0, C, ld, sM, 0, 1);
 }
 template <>
 void rank2k<float>(hipStream_t st, int batch, int m, int k, float2* Vc, float2* Wc, float2* C, int ld,
                    long long sP, long long sM) {
-  launch_cgemm(st, batch, m, m, k, -1.0f, Vc, ld, sP, OP_T, Wc, ld, sP, OP_CONJ, 1.0f, C, ld, sM, 0, 0);
-  launch_cgemm(st, batch, m, m, k, -1.0f, Wc, ld, sP, OP_T, Vc, ld, sP, OP_CONJ, 1.0f, C, ld, sM, 0, 0);
+  launch_cgemm(st, batch, m, m, k, -1.0f, Vc, ld, sP, OP_T, Wc, ld, sP, OP_CONJ, 1.0f, C, ld, sM, 0, 1);
+  launch_cgemm(st, batch, m, m, k, -1.0f, Wc, ld, sP, OP_T, Vc, ld, sP, OP_CONJ, 1.0f, C, ld, sM, 0, 1);
 }
 
 template <typename T>
@@ -269,16 +316,24 @@ static void hetrd_impl(hipStream_t st, typename C2T<T>::t* C, int n, int ld, int
                        typename C2T<T>::t* Y, typename C2T<T>::t* Yp, typename C2T<T>::t* C12) {
   HetrdCtx<T> c;
   c.A = C; c.n = n; c.ld = ld; c.Vc = Vc; c.Wc = Wc; c.Y = Y; c.Yp = Yp; c.C12 = C12; c.TAU = TAU;
-  c.D = D; c.E = E; c.ncmax = (ld + HT_COLS - 1) / HT_COLS;
+  c.D = D; c.E = E; c.ncmax = (ld + HT_TS - 1) / HT_TS;
   const long long sP = (long long)NB * ld, sM = (long long)ld * ld;
   for (int j0 = 0; j0 < n - 1; j0 += NB) {
     const int jn = min(j0 + NB, n - 1);
     hipLaunchKernelGGL(latrd_start_kernel<T>, dim3(batch), dim3(TAIL_THREADS), 0, st, c, j0);
     for (int j = j0; j < jn; ++j) {
       const int m = n - j - 1;
-      dim3 g((m + HT_ROWS - 1) / HT_ROWS, (m + HT_COLS - 1) / HT_COLS, batch);
-      hipLaunchKernelGGL(hetrd_step_kernel<T>, g, dim3(HT_THREADS), 0, st, c, j, j - j0);
-      hipLaunchKernelGGL(hetrd_tail_kernel<T>, dim3(batch), dim3(TAIL_THREADS), 0, st, c, j, j - j0, jn, (int)g.y);
+      const int nbk = (m + HT_TS - 1) / HT_TS;
+      dim3 g(nbk * (nbk + 1) / 2, batch);
+      // algorithmic bytes: one triangle of the Hermitian trailing block per matrix
+      SampleSlot sl;
+      if (sampler_take(SLK_SAMPLE_HETRD_STEP, (double)batch * m * (m + 1) / 2 * sizeof(typename C2T<T>::t), &sl))
+        hipExtLaunchKernelGGL(hetrd_step_kernel<T>, g, dim3(HT_THREADS), 0, st, sl.e0, sl.e1, 0, c, j, j - j0,
+                              sl.clk);
+      else
+        hipLaunchKernelGGL(hetrd_step_kernel<T>, g, dim3(HT_THREADS), 0, st, c, j, j - j0,
+                           (unsigned long long*)nullptr);
+      hipLaunchKernelGGL(hetrd_tail_kernel<T>, dim3(batch), dim3(TAIL_THREADS), 0, st, c, j, j - j0, jn, nbk);
     }
     if (jn < n) rank2k<T>(st, batch, n - jn, jn - j0, Vc + jn, Wc + jn, C + (size_t)jn * ld + jn, ld, sP, sM);
   }

@@ -170,6 +170,10 @@ def plan_build(tables: TableSet, inputs: Sequence[StructureInput], ld: Optional[
     return BuildPlan(len(inputs), m0, ld, n_orb, nks, mat0, kw, nelec, dev, len(work))
 
 
+# set while the library's live kernel sampler is enabled (slk_sampler_reset)
+SAMPLING = False
+
+
 def build_args(tables: TableSet, plan: BuildPlan, H_ptr: int, S_ptr: int, out_f32: bool = False,
                cutoff: float = HS_CUTOFF_BOHR, table_tensor: Optional[torch.Tensor] = None):
     """SlkBuildArgs of a plan (shared by the build and its adjoint)."""
@@ -190,6 +194,10 @@ def run_build(tables: TableSet, plan: BuildPlan, out_f32: bool = False, cutoff: 
     d = plan.dev
     args = build_args(tables, plan, H.data_ptr(), S.data_ptr(), out_f32, cutoff, table_tensor)
     sh = _lib.stream_handle(stream)
+    if SAMPLING:
+        # algorithmic bytes of this launch for the live sampler: the H and S
+        # elements the build writes (padding excluded)
+        lib.slk_sampler_hint(1, float(sum(nk * 2 * n * n for nk, n in zip(plan.nk, plan.n_orb)) * H.element_size()))
     _lib.check(lib.slk_build_hs_launch(C.byref(args), sh), "slk_build_hs_launch")
     for s in range(plan.n_struct):
         if plan.n_orb[s] < plan.ld:
