@@ -138,3 +138,36 @@ def test_model_save_load_round_trip(tmp_path, method):
             ob = back.skf_optimizers[key]
             assert all(torch.equal(opt.original_h_params[k], ob.original_h_params[k]) for k in opt.original_h_params)
     back.apply_constraints()
+
+
+def test_skf_text_round_trip_and_repeat_notation(tmp_path):
+    """DFTB+ simple-format .skf: write synthetic tables, read back; the
+    "n*value" notation and comma separators expand."""
+    from slakonet_amd import skf_io
+    d = synth.make_skf(14, 14, seed=0)
+    path = tmp_path / "Si-Si.skf"
+    skf_io.write_skf(str(path), d)
+    back = skf_io.read_skf(str(path))
+    assert back["atom_pair"] == [14, 14]
+    for part in ("hamiltonian", "overlap"):
+        for key, v in d[part].items():
+            k = key if isinstance(key, str) else f"{key[0]}-{key[1]}"
+            assert np.allclose(np.asarray(back[part][k]), np.asarray(v), rtol=1e-14, atol=0)
+    assert np.allclose(back["grid"], d["grid"], rtol=1e-12)
+    assert np.allclose(back["atomic_data"]["on_sites"], d["atomic_data"]["on_sites"], rtol=1e-12)
+    # hand-written heteronuclear file: 2 grid points, repeat notation, commas
+    txt = "0.5, 2\n12.0 19*0.0\n" + "\n".join(
+        " ".join(str(10 * g + c) for c in range(20)) for g in (1, 2)) + "\n"
+    p2 = tmp_path / "C-Si.skf"
+    p2.write_text(txt)
+    h = skf_io.read_skf(str(p2))
+    assert h["atom_pair"] == [6, 14] and not h["atomic"]
+    assert h["grid"] == [0.5, 1.0]
+    # file column 9 is Hss0, 0 is Hdd0, 10+9 is Sss0
+    assert h["hamiltonian"]["0-0"][0] == [19.0, 29.0]
+    assert h["hamiltonian"]["2-2"][0] == [10.0, 20.0]
+    assert h["overlap"]["0-0"][0] == [29.0, 39.0]
+    assert h["hamiltonian"]["1-2"][1] == [14.0, 24.0]      # Hpd1
+    dd = skf_io.read_skf_directory(str(tmp_path), ["Si", "C"])
+    assert set(dd) == {(14, 14), (6, 14)}
+    assert skf_io._numbers("3*1.5, 2") .tolist() == [1.5, 1.5, 1.5, 2.0]
