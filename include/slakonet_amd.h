@@ -167,8 +167,8 @@ int slk_dos_launch(const double* W, const double* KW, const SlkDosJob* jobs, int
  * pair that the runtime records at the kernel's own begin and end on its
  * launch stream. Kinds: 0 = tridiagonalisation mat-vec (bytes = one triangle
  * of the trailing Hermitian block per matrix), 1 = H/S build (bytes from
- * slk_sampler_hint: the H and S elements written). enable = 0 disables and
- * frees. Single host thread. */
+ * slk_sampler_hint, per calling thread: the H and S elements written).
+ * enable = 0 disables and frees. Thread-safe. */
 int slk_sampler_reset(int enable, int every, int capacity);
 int slk_sampler_hint(int kind, double bytes);
 /* out[0] = event samples, out[1] = their summed milliseconds, out[2] = their

@@ -5,7 +5,8 @@
 // workgroup end (atomic max) of the constant-rate wall clock. The same
 // launch is also bracketed by a HIP start/stop event pair
 // (hipExtLaunchKernelGGL) on its launch stream. Read back per kind after the
-// work completes. Single host thread; off unless enabled.
+// work completes. Thread-safe (launch threads may call concurrently; the
+// byte hint is per thread); off unless enabled.
 #pragma once
 #include <hip/hip_runtime.h>
 

@@ -1,4 +1,5 @@
 // Live kernel sampling (see slk_sampler.h).
+#include <mutex>
 #include <vector>
 
 #include "slakonet_amd.h"
@@ -22,12 +23,14 @@ struct Sampler {
   int every = 0;
   long long launches[SLK_SAMPLE_KINDS] = {};
   double bytes[SLK_SAMPLE_KINDS] = {};
-  double hint[SLK_SAMPLE_KINDS] = {};
   std::vector<Sample> pool;
   unsigned long long* clk = nullptr;   // device [capacity][2]
   size_t used = 0;
 };
 Sampler g;
+std::mutex g_mu;
+// byte hint of the next launch of each kind, per launching thread
+thread_local double t_hint[SLK_SAMPLE_KINDS] = {};
 
 void free_pool() {
   for (auto& s : g.pool) {
@@ -43,7 +46,9 @@ void free_pool() {
 
 bool sampler_take(int kind, double bytes, SampleSlot* slot) {
   if (!g.enabled || kind < 0 || kind >= SLK_SAMPLE_KINDS) return false;
-  if (bytes < 0) bytes = g.hint[kind];
+  if (bytes < 0) bytes = t_hint[kind];
+  std::lock_guard<std::mutex> lock(g_mu);
+  if (!g.enabled) return false;
   const long long idx = g.launches[kind]++;
   g.bytes[kind] += bytes;
   if (g.every <= 0 || idx % g.every != 0 || g.used >= g.pool.size()) return false;
@@ -60,12 +65,12 @@ bool sampler_take(int kind, double bytes, SampleSlot* slot) {
 extern "C" {
 
 int slk_sampler_reset(int enable, int every, int capacity) {
+  std::lock_guard<std::mutex> lock(g_mu);
   (void)hipDeviceSynchronize();
   free_pool();
   for (int k = 0; k < SLK_SAMPLE_KINDS; ++k) {
     g.launches[k] = 0;
     g.bytes[k] = 0.0;
-    g.hint[k] = 0.0;
   }
   g.enabled = enable;
   g.every = every;
@@ -96,12 +101,13 @@ int slk_sampler_reset(int enable, int every, int capacity) {
 
 int slk_sampler_hint(int kind, double bytes) {
   if (kind < 0 || kind >= SLK_SAMPLE_KINDS) return 1;
-  g.hint[kind] = bytes;
+  t_hint[kind] = bytes;
   return 0;
 }
 
 int slk_sampler_read(int kind, double* out) {
   if (kind < 0 || kind >= SLK_SAMPLE_KINDS) return 1;
+  std::lock_guard<std::mutex> lock(g_mu);
   for (int i = 0; i < 7; ++i) out[i] = 0.0;
   out[3] = (double)g.launches[kind];
   out[4] = g.bytes[kind];

@@ -7,8 +7,13 @@ set of HIP streams whenever a stream slot and HBM budget are free; requests
 that arrive while others are on the device queue up and join the next batch
 of their bucket. Per batch: one fused build launch, one batched eigensolve,
 Fermi / band-edge / DOS reductions, and one asynchronous copy of the scalar
-results to pinned host memory; completion is observed with events, so the
-host never blocks on the device while work is pending.
+results to pinned host memory; completion is observed with events.
+
+Each stream has its own launch thread: a batch's kernel sequence (thousands
+of launches for a large matrix) is enqueued by that thread, so a stream whose
+hardware queue is full blocks only its own thread while the admission loop
+and the other streams keep going (ctypes releases the GIL in the library
+calls).
 
 SK tables stay in HBM for the lifetime of the engine.
 """
@@ -16,6 +21,7 @@ from __future__ import annotations
 
 import time
 from collections import deque
+from concurrent.futures import Future, ThreadPoolExecutor
 from dataclasses import dataclass, field
 from typing import Dict, List, Optional, Sequence
 
@@ -67,8 +73,12 @@ class ServingEngine:
                  fermi_mode: int = 0, keep_outputs: bool = False, precision: str = "mixed"):
         self.tables = tables
         self.dev = tables.device
+        self.dev_index = self.dev.index if self.dev.index is not None else torch.cuda.current_device()
         self.grid = dos_grid
         self.streams = [torch.cuda.Stream(device=self.dev) for _ in range(n_streams)]
+        # one launch thread per stream keeps each stream's launch order
+        self.launchers = [ThreadPoolExecutor(max_workers=1) for _ in range(n_streams)]
+        self._pending = [0] * n_streams
         self.max_inflight = max_inflight
         self.budget = hbm_budget_bytes
         self.max_batch = max_batch_matrices
@@ -88,9 +98,21 @@ class ServingEngine:
         # H, S + eigensolver workspace (3 matrices + panels) + eigenvalues
         return int(nmat * ld * ld * 16 * 5.3) + nmat * ld * 8
 
-    def _launch(self, reqs: List[Request], ld: int) -> Batch:
-        stream = self.streams[self._rr % len(self.streams)]
+    def _submit(self, reqs: List[Request], ld: int) -> Future:
+        """Queue a batch on the least-loaded stream's launch thread."""
+        i = min(range(len(self.streams)), key=lambda q: (self._pending[q], (q - self._rr) % len(self.streams)))
         self._rr += 1
+        self._pending[i] += 1
+        fut = self.launchers[i].submit(self._launch, reqs, ld, self.streams[i])
+        fut.stream_index = i
+        return fut
+
+    def close(self):
+        for ex in self.launchers:
+            ex.shutdown(wait=True)
+
+    def _launch(self, reqs: List[Request], ld: int, stream) -> Batch:
+        torch.cuda.set_device(self.dev_index)
         nmat = sum(len(r.inp.kpoints) for r in reqs)
         with torch.cuda.stream(stream):
             plan = engine.plan_build(self.tables, [r.inp for r in reqs], ld=ld)
@@ -147,7 +169,7 @@ class ServingEngine:
         order = np.argsort(np.asarray(arrivals), kind="stable")
         pending = deque(int(i) for i in order)
         queues: Dict[int, deque] = {}
-        inflight: List[Batch] = []
+        inflight: List[list] = []      # [future, nbytes, t_submit]
         used = 0
         results: Dict[int, dict] = {}
         done_t: Dict[int, float] = {}
@@ -162,11 +184,17 @@ class ServingEngine:
                 ld = bucket_dim(no)
                 queues.setdefault(ld, deque()).append(Request(i, inp, no, float(arrivals[i]), ld))
             progressed = False
-            for b in list(inflight):
+            for item in list(inflight):
+                fut, nbytes, t_sub = item
+                if not fut.done():
+                    continue
+                b = fut.result()          # re-raises a launch failure
                 if b.event.query():
+                    self._pending[fut.stream_index] -= 1
+                    b.t_launch = t_sub
                     self._finish(b, results, done_t, time.perf_counter() - t0)
-                    inflight.remove(b)
-                    used -= b.nbytes
+                    inflight.remove(item)
+                    used -= nbytes
                     progressed = True
             # launch: oldest bucket first while slots and budget allow
             while len(inflight) < self.max_inflight:
@@ -186,10 +214,9 @@ class ServingEngine:
                     nmat += nk
                 if not take:
                     break
-                b = self._launch(take, ld)
-                b.t_launch = time.perf_counter() - t0
-                used += b.nbytes
-                inflight.append(b)
+                nbytes = self.batch_bytes(ld, nmat)
+                inflight.append([self._submit(take, ld), nbytes, time.perf_counter() - t0])
+                used += nbytes
                 progressed = True
             if not progressed:
                 if inflight:
