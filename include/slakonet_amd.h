@@ -158,6 +158,22 @@ typedef struct SlkDosJob {
 int slk_dos_launch(const double* W, const double* KW, const SlkDosJob* jobs, int n_jobs, const double* grid, int ne,
                    double sigma, const double* edges, double* dos, double* stats, hipStream_t stream);
 
+/* Precision form, band-edge refinement (after slk_eigh_mixed, slk_fermi_launch
+ * and slk_edges_launch on the same batch): for each structure's VBM and CBM
+ * state, an FP64 Rayleigh quotient z^H H z / z^H S z with the FP64 H, S, where
+ * z is the state's eigenvector from inverse iteration on the FP64 tridiagonal
+ * T carried back through the FP32 reflectors and W = L^-1 (read from the
+ * mixed solve's workspace `ws`, untouched since). Writes the refined energies
+ * into w and into the edge rows (VBM/CBM, shifted and Hartree edges).
+ * scratch: slk_refine_scratch_bytes(ld, n_spec). Returns 0 without work for
+ * n <= 32 (solved in FP64). Replaces nothing in the reference, whose solve
+ * is FP64 throughout (slakonet/utils.py:740-783); this is how the FP32 bulk
+ * keeps band edges at FP64 accuracy. */
+size_t slk_refine_scratch_bytes(int ld, int n_spec);
+int slk_refine_edges_mixed(const void* H, const void* S, int n, int ld, int batch, const void* ws, size_t ws_bytes,
+                           double* w, int wld, const SlkSpectrum* spec, int n_spec, double* edges, double* scratch,
+                           size_t scratch_bytes, hipStream_t stream);
+
 /* ---- Measurement --------------------------------------------------------- */
 
 /* Live kernel sampling, no reference counterpart (the reference has no

@@ -80,6 +80,10 @@ SIGNATURES = {
     "slk_edge_fields": (C.c_int, []),
     "slk_dos_launch": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_double,
                                  C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "slk_refine_scratch_bytes": (C.c_size_t, [C.c_int, C.c_int]),
+    "slk_refine_edges_mixed": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_size_t,
+                                         C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p,
+                                         C.c_size_t, C.c_void_p]),
     "slk_sampler_reset": (C.c_int, [C.c_int, C.c_int, C.c_int]),
     "slk_sampler_hint": (C.c_int, [C.c_int, C.c_double]),
     "slk_sampler_read": (C.c_int, [C.c_int, C.c_void_p]),

@@ -14,6 +14,7 @@
 #include "slk_common.h"
 #include "slakonet_amd.h"
 #include "slk_dense.h"
+#include "slk_edges.h"
 
 #define EIG_THREADS 256
 
@@ -304,6 +305,226 @@ __global__ __launch_bounds__(EIG_THREADS) void apply_q_kernel(const double2* __r
   }
 }
 
+// ---------------------------------------------------------------------------
+// Band-edge refinement of the precision form. For each structure, the VBM
+// and CBM states picked by the edge kernel get an FP64 Rayleigh quotient
+// rho = z^H H z / z^H S z with the FP64 H and S of the build, where z is the
+// state's eigenvector carried back from the FP32 reduction:
+//   y: inverse iteration on the FP64 tridiagonal T (LU with partial
+//      pivoting, LAPACK dgttrf/dgttrs, three solves);
+//   x = Q y: the FP32 Householder reflectors, applied in FP64 arithmetic;
+//   z = W^H x with W = L^-1 in FP64.
+// z carries the FP32 reduction error (~1e-7 relative); the quotient's error
+// is second order in it. One workgroup per (structure, edge).
+#define RF_THREADS 256
+#define RF_MAXC 16          // columns per thread in z = W^H x (m <= 4096)
+#define RF_SCR 10           // scratch doubles per ld per (structure, edge)
+
+__global__ __launch_bounds__(RF_THREADS) void refine_edges_kernel(
+    const double2* __restrict__ H, const double2* __restrict__ S, const double2* __restrict__ Wm,
+    const float2* __restrict__ R, const float2* __restrict__ TAU, const double* __restrict__ D,
+    const double* __restrict__ Ed, int ld, const double* __restrict__ w, const SlkSpectrum* spec,
+    const double* __restrict__ edges, double* scratch, double* rho_out) {
+  const int s = blockIdx.x, edge = blockIdx.y;
+  const SlkSpectrum sp = spec[s];
+  const double* row = edges + (size_t)s * SLK_EDGE_FIELDS;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  __shared__ double red[RF_THREADS / 64];
+  __shared__ int s_band;
+  double* rho_slot = rho_out + 2 * ((size_t)s * 2 + edge);
+  if (row[E_NVAL] <= 0.0 || row[E_NCON] <= 0.0) {
+    if (tid == 0) { rho_slot[0] = 0.0; rho_slot[1] = -1.0; }
+    return;
+  }
+  const double mu = row[E_MU];
+  const int k = (int)row[edge == 0 ? E_VBMK : E_CBMK];
+  const int m = sp.nb;
+  const double* wk = w + sp.w0 + (size_t)k * sp.wld;
+  if (tid == 0) {
+    int c = 0;
+    for (int b = 0; b < m; ++b)
+      if (wk[b] * H2E_OUT < mu * H2E_OUT) ++c;
+    s_band = edge == 0 ? c - 1 : c;
+  }
+  __syncthreads();
+  const int band = s_band;
+  if (band < 0 || band >= m) {
+    if (tid == 0) { rho_slot[0] = 0.0; rho_slot[1] = -1.0; }
+    return;
+  }
+  const double lam = wk[band];
+  const size_t mat = (size_t)sp.kw0 + k;
+  const double* d = D + mat * ld;
+  const double* e = Ed + mat * ld;
+  double* base = scratch + ((size_t)s * 2 + edge) * RF_SCR * ld;
+  double *dl = base, *dd = base + ld, *du = base + 2 * (size_t)ld, *du2 = base + 3 * (size_t)ld;
+  double* piv = base + 4 * (size_t)ld;
+  double* y = base + 5 * (size_t)ld;
+  double2* x = reinterpret_cast<double2*>(base + 6 * (size_t)ld);
+  double2* z = reinterpret_cast<double2*>(base + 8 * (size_t)ld);
+
+  // (1) inverse iteration on T[0:m) - lam I (one lane)
+  if (tid == 0) {
+    double tn = 0.0;
+    for (int i = 0; i < m; ++i) {
+      const double a = fabs(d[i]) + (i > 0 ? fabs(e[i - 1]) : 0.0) + (i + 1 < m ? fabs(e[i]) : 0.0);
+      tn = fmax(tn, a);
+    }
+    const double tiny = 2.2e-16 * fmax(tn, 1e-300);
+    for (int i = 0; i < m; ++i) {
+      dd[i] = d[i] - lam;
+      du2[i] = 0.0;
+      if (i + 1 < m) { dl[i] = e[i]; du[i] = e[i]; }
+    }
+    for (int i = 0; i + 1 < m; ++i) {
+      if (fabs(dd[i]) >= fabs(dl[i])) {
+        if (dd[i] == 0.0) dd[i] = tiny;
+        const double f = dl[i] / dd[i];
+        dl[i] = f;
+        dd[i + 1] -= f * du[i];
+        piv[i] = 0.0;
+      } else {
+        const double f = dd[i] / dl[i];
+        dd[i] = dl[i];
+        dl[i] = f;
+        const double t = du[i];
+        du[i] = dd[i + 1];
+        dd[i + 1] = t - f * dd[i + 1];
+        if (i + 2 < m) {
+          du2[i] = du[i + 1];
+          du[i + 1] = -f * du[i + 1];
+        }
+        piv[i] = 1.0;
+      }
+    }
+    if (dd[m - 1] == 0.0) dd[m - 1] = tiny;
+    for (int i = 0; i < m; ++i) y[i] = 1.0 + 0.37 * sin(0.7 * i + 0.1);
+    for (int it = 0; it < 3; ++it) {
+      for (int i = 0; i + 1 < m; ++i) {
+        if (piv[i] == 0.0) {
+          y[i + 1] -= dl[i] * y[i];
+        } else {
+          const double t = y[i];
+          y[i] = y[i + 1];
+          y[i + 1] = t - dl[i] * y[i];
+        }
+      }
+      y[m - 1] /= dd[m - 1];
+      if (m > 1) y[m - 2] = (y[m - 2] - du[m - 2] * y[m - 1]) / dd[m - 2];
+      for (int i = m - 3; i >= 0; --i) y[i] = (y[i] - du[i] * y[i + 1] - du2[i] * y[i + 2]) / dd[i];
+      double nr = 0.0;
+      for (int i = 0; i < m; ++i) nr += y[i] * y[i];
+      nr = 1.0 / sqrt(nr);
+      for (int i = 0; i < m; ++i) y[i] *= nr;
+    }
+  }
+  __syncthreads();
+  // (2) x = Q y, reflectors last to first: x -= v tau (v^H x)
+  for (int r = tid; r < m; r += RF_THREADS) x[r] = make_double2(y[r], 0.0);
+  __syncthreads();
+  const float2* a = R + mat * ld * ld;
+  const float2* tau = TAU + mat * ld;
+  for (int j = m - 2; j >= 0; --j) {
+    const float2 tf = tau[j];
+    if (tf.x == 0.0f && tf.y == 0.0f) continue;
+    double px = 0.0, py = 0.0;
+    for (int r = j + 1 + tid; r < m; r += RF_THREADS) {
+      const float2 vf = r == j + 1 ? make_float2(1.0f, 0.0f) : a[(size_t)r * ld + j];
+      const double2 xv = x[r];
+      px += (double)vf.x * xv.x + (double)vf.y * xv.y;
+      py += (double)vf.x * xv.y - (double)vf.y * xv.x;
+    }
+    px = block_sum(px, red);
+    py = block_sum(py, red);
+    const double sx = (double)tf.x * px - (double)tf.y * py, sy = (double)tf.x * py + (double)tf.y * px;
+    for (int r = j + 1 + tid; r < m; r += RF_THREADS) {
+      const float2 vf = r == j + 1 ? make_float2(1.0f, 0.0f) : a[(size_t)r * ld + j];
+      double2 xv = x[r];
+      xv.x -= (double)vf.x * sx - (double)vf.y * sy;
+      xv.y -= (double)vf.x * sy + (double)vf.y * sx;
+      x[r] = xv;
+    }
+    __syncthreads();
+  }
+  // (3) z = W^H x: z[c] = sum_{r >= c} conj(W[r][c]) x[r]
+  {
+    const double2* wm = Wm + mat * ld * ld;
+    double2 acc[RF_MAXC];
+#pragma unroll
+    for (int q = 0; q < RF_MAXC; ++q) acc[q] = make_double2(0.0, 0.0);
+    for (int r = 0; r < m; ++r) {
+      const double2 xr = x[r];
+#pragma unroll
+      for (int q = 0; q < RF_MAXC; ++q) {
+        const int c = tid + q * RF_THREADS;
+        if (c <= r) {
+          const double2 wv2 = wm[(size_t)r * ld + c];
+          acc[q].x += wv2.x * xr.x + wv2.y * xr.y;
+          acc[q].y += wv2.x * xr.y - wv2.y * xr.x;
+        }
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < RF_MAXC; ++q) {
+      const int c = tid + q * RF_THREADS;
+      if (c < m) z[c] = acc[q];
+    }
+  }
+  __syncthreads();
+  // (4) rho = z^H H z / z^H S z
+  const double2* hm = H + mat * ld * ld;
+  const double2* sm = S + mat * ld * ld;
+  double num = 0.0, den = 0.0;
+  for (int r = wv; r < m; r += RF_THREADS / 64) {
+    double hx = 0.0, hy = 0.0, sx2 = 0.0, sy2 = 0.0;
+    for (int c = lane; c < m; c += 64) {
+      const double2 zc = z[c], hv = hm[(size_t)r * ld + c], sv = sm[(size_t)r * ld + c];
+      hx += hv.x * zc.x - hv.y * zc.y;
+      hy += hv.x * zc.y + hv.y * zc.x;
+      sx2 += sv.x * zc.x - sv.y * zc.y;
+      sy2 += sv.x * zc.y + sv.y * zc.x;
+    }
+    hx = wave_sum(hx); hy = wave_sum(hy); sx2 = wave_sum(sx2); sy2 = wave_sum(sy2);
+    const double2 zr = z[r];
+    num += zr.x * hx + zr.y * hy;     // Re(conj(z_r) (Hz)_r)
+    den += zr.x * sx2 + zr.y * sy2;
+  }
+  if (lane != 0) { num = 0.0; den = 0.0; }
+  num = block_sum(num, red);
+  den = block_sum(den, red);
+  if (tid == 0) {
+    rho_slot[0] = num / den;
+    rho_slot[1] = (double)band;
+  }
+}
+
+// Writes the refined edge energies into the eigenvalue array and the edge row.
+__global__ void refine_commit_kernel(double* w, const SlkSpectrum* spec, double* edges, const double* rho_out,
+                                     int n_spec) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n_spec) return;
+  const SlkSpectrum sp = spec[s];
+  double* row = edges + (size_t)s * SLK_EDGE_FIELDS;
+  const double mu = row[E_MU];
+  for (int edge = 0; edge < 2; ++edge) {
+    const double* slot = rho_out + 2 * ((size_t)s * 2 + edge);
+    if (slot[1] < 0.0) continue;
+    const double rho = slot[0];
+    const int band = (int)slot[1];
+    const int k = (int)row[edge == 0 ? E_VBMK : E_CBMK];
+    w[sp.w0 + (size_t)k * sp.wld + band] = rho;
+    if (edge == 0) {
+      row[E_VBM] = rho * H2E_OUT;
+      row[E_VB2] = rho;
+      row[E_SVBM] = (rho - mu) * H2E_OUT;
+    } else {
+      row[E_CBM] = rho * H2E_OUT;
+      row[E_CB2] = rho;
+      row[E_SCBM] = (rho - mu) * H2E_OUT;
+    }
+  }
+}
+
 extern "C" {
 
 // Workspace (per matrix). FP64 path: L/C, W, W*H (and Y for vectors), the
@@ -463,6 +684,29 @@ int slk_eigh_f64_timed(const void* H, const void* S, int n, int ld, int batch, d
 int slk_eigh_mixed(const void* H, const void* S, int n, int ld, int batch, double* w, int wld, void* ws,
                    size_t ws_bytes, int* info, hipStream_t stream) {
   return eigh_impl(H, S, n, ld, batch, w, wld, nullptr, ws, ws_bytes, info, stream, nullptr, 1);
+}
+
+size_t slk_refine_scratch_bytes(int ld, int n_spec) {
+  return ((size_t)n_spec * 2 * RF_SCR * ld + (size_t)n_spec * 4) * sizeof(double);
+}
+
+int slk_refine_edges_mixed(const void* H, const void* S, int n, int ld, int batch, const void* ws, size_t ws_bytes,
+                           double* w, int wld, const SlkSpectrum* spec, int n_spec, double* edges, double* scratch,
+                           size_t scratch_bytes, hipStream_t stream) {
+  (void)wld;
+  if (n_spec <= 0 || n <= 32) return 0;      // n <= 32 was solved in FP64 by the LDS kernel
+  if (n > RF_MAXC * RF_THREADS) return 3;
+  if (ws_bytes < slk_eigh_mixed_workspace_bytes(n, ld, batch)) return 2;
+  if (scratch_bytes < slk_refine_scratch_bytes(ld, n_spec)) return 2;
+  EighWs s;
+  carve(reinterpret_cast<char*>(const_cast<void*>(ws)), ld, batch, 0, 1, &s);
+  double* rho = scratch + (size_t)n_spec * 2 * RF_SCR * ld;
+  hipLaunchKernelGGL(refine_edges_kernel, dim3(n_spec, 2), dim3(RF_THREADS), 0, stream,
+                     reinterpret_cast<const double2*>(H), reinterpret_cast<const double2*>(S), s.B1, s.F1, s.fTAU,
+                     s.D, s.E, ld, w, spec, edges, scratch, rho);
+  hipLaunchKernelGGL(refine_commit_kernel, dim3((n_spec + 63) / 64), dim3(64), 0, stream, w, spec, edges, rho,
+                     n_spec);
+  return (int)hipGetLastError();
 }
 
 int slk_eigh_mixed_timed(const void* H, const void* S, int n, int ld, int batch, double* w, int wld, void* ws,

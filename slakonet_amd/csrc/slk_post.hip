@@ -12,14 +12,9 @@
 #include "slk_common.h"
 #include "slakonet_amd.h"
 
-#define POST_THREADS 256
-#define H2E_OUT 27.211
+#include "slk_edges.h"
 
-// edges[s*SLK_EDGE_FIELDS + f]
-enum {
-  E_MU = 0, E_METAL, E_VBM, E_CBM, E_VBMK, E_CBMK, E_SVBM, E_VBW, E_SCBM, E_CBW, E_MINDG, E_VB2, E_CB2,
-  E_NITER, E_NVAL, E_NCON, SLK_EDGE_FIELDS
-};
+#define POST_THREADS 256
 
 __device__ __forceinline__ double fd(double e, double mu, double kT) { return 1.0 / (exp((e - mu) / kT) + 1.0); }
 

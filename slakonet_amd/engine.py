@@ -218,13 +218,15 @@ PRECISIONS = ("fp64", "mixed")
 
 
 def eigh(H: torch.Tensor, S: torch.Tensor, vectors: bool = False, stream=None, check: bool = True,
-         stage_ms: Optional[list] = None, precision: str = "fp64"):
+         stage_ms: Optional[list] = None, precision: str = "fp64", workspace: Optional[list] = None):
     """Batched generalized eigensolve. Returns w [batch, ld], V or None, info.
 
     precision "fp64": every stage in FP64. "mixed": FP64 Cholesky and L^-1,
     FP32 reduction and tridiagonalisation, FP64 bisection (values only).
     With ``stage_ms`` (a list) the per-stage milliseconds measured with HIP
-    events on the launch stream are appended (one synchronisation)."""
+    events on the launch stream are appended (one synchronisation). With
+    ``workspace`` (a list) the solver workspace is appended (the band-edge
+    refinement of the mixed path reads it)."""
     if precision not in PRECISIONS:
         raise ValueError(f"precision must be one of {PRECISIONS}")
     mixed = precision == "mixed"
@@ -236,6 +238,8 @@ def eigh(H: torch.Tensor, S: torch.Tensor, vectors: bool = False, stream=None, c
     ws_bytes = int(lib.slk_eigh_mixed_workspace_bytes(n, ld, batch) if mixed else
                    lib.slk_eigh_workspace_bytes(n, ld, batch, int(vectors)))
     ws = torch.empty(ws_bytes, dtype=torch.uint8, device=H.device)
+    if workspace is not None:
+        workspace.append(ws)
     w = torch.empty(batch, ld, dtype=torch.float64, device=H.device)
     info = torch.zeros(2 * batch, dtype=torch.int32, device=H.device)
     V = torch.empty_like(H) if vectors else None
@@ -283,10 +287,30 @@ def dos_jobs(plan: BuildPlan, w_ld: int, ne: int) -> np.ndarray:
     return jobs
 
 
+def refine_edges(H: torch.Tensor, S: torch.Tensor, ws: torch.Tensor, w: torch.Tensor, plan: BuildPlan,
+                 spd: torch.Tensor, edges: torch.Tensor, stream=None):
+    """FP64 Rayleigh-quotient refinement of every structure's VBM and CBM after
+    a mixed-precision solve (see slk_refine_edges_mixed); updates w and edges."""
+    lib = _lib.lib()
+    batch, ld, _ = H.shape
+    nbytes = int(lib.slk_refine_scratch_bytes(ld, plan.n_struct))
+    scratch = torch.empty(nbytes // 8 + 1, dtype=torch.float64, device=H.device)
+    rc = lib.slk_refine_edges_mixed(C.c_void_p(H.data_ptr()), C.c_void_p(S.data_ptr()), ld, ld, batch,
+                                    C.c_void_p(ws.data_ptr()), ws.numel(), C.c_void_p(w.data_ptr()), w.shape[1],
+                                    C.c_void_p(spd.data_ptr()), plan.n_struct, C.c_void_p(edges.data_ptr()),
+                                    C.c_void_p(scratch.data_ptr()), scratch.numel() * 8, _lib.stream_handle(stream))
+    if rc == 3:
+        raise NotImplementedError(f"band-edge refinement supports n <= 4096 (got {ld})")
+    _lib.check(rc, "slk_refine_edges_mixed")
+    return scratch
+
+
 def post(plan: BuildPlan, w: torch.Tensor, fermi_mode: int = 0, kT: float = FERMI_KT_HA,
-         dos_grid: Optional[torch.Tensor] = None, sigma: float = 0.1, stream=None):
+         dos_grid: Optional[torch.Tensor] = None, sigma: float = 0.1, stream=None, refine=None):
     """Fermi level, band edges and (optionally) DOS for every structure of the plan.
 
+    ``refine`` = (H, S, workspace) of a mixed-precision solve: the VBM and CBM
+    of every structure are refined in FP64 before the DOS is formed.
     Returns edges [n_struct, F] (device), dos [n_struct, ne] or None, stats [n_struct, 3] or None.
     """
     lib = _lib.lib()
@@ -302,6 +326,9 @@ def post(plan: BuildPlan, w: torch.Tensor, fermi_mode: int = 0, kT: float = FERM
                                     C.c_void_p(edges.data_ptr()), sh), "slk_fermi_launch")
     _lib.check(lib.slk_edges_launch(C.c_void_p(w.data_ptr()), C.c_void_p(spd.data_ptr()), plan.n_struct,
                                     METAL_THRESHOLD_EV, C.c_void_p(edges.data_ptr()), sh), "slk_edges_launch")
+    if refine is not None:
+        H, S, ws = refine
+        refine_edges(H, S, ws, w, plan, spd, edges, stream)
     dos = stats = None
     if dos_grid is not None:
         ne = int(dos_grid.numel())

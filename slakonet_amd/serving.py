@@ -70,7 +70,8 @@ class Batch:
 class ServingEngine:
     def __init__(self, tables: TableSet, dos_grid: Optional[torch.Tensor] = None, n_streams: int = 4,
                  max_inflight: int = 8, hbm_budget_bytes: float = 64e9, max_batch_matrices: int = 512,
-                 fermi_mode: int = 0, keep_outputs: bool = False, precision: str = "mixed"):
+                 fermi_mode: int = 0, keep_outputs: bool = False, precision: str = "mixed",
+                 refine_edges: bool = True):
         self.tables = tables
         self.dev = tables.device
         self.dev_index = self.dev.index if self.dev.index is not None else torch.cuda.current_device()
@@ -85,6 +86,7 @@ class ServingEngine:
         self.fermi_mode = fermi_mode
         self.keep_outputs = keep_outputs
         self.precision = precision
+        self.refine_edges = refine_edges
         self.nf = len(engine.EDGE_FIELDS)
         self._rr = 0
 
@@ -117,8 +119,10 @@ class ServingEngine:
         with torch.cuda.stream(stream):
             plan = engine.plan_build(self.tables, [r.inp for r in reqs], ld=ld)
             H, S = engine.run_build(self.tables, plan)
-            w, _, info = engine.eigh(H, S, check=False, precision=self.precision)
-            edges, dos, stats = engine.post(plan, w, fermi_mode=self.fermi_mode, dos_grid=self.grid)
+            wsl = []
+            w, _, info = engine.eigh(H, S, check=False, precision=self.precision, workspace=wsl)
+            refine = (H, S, wsl[0]) if (self.refine_edges and self.precision == "mixed") else None
+            edges, dos, stats = engine.post(plan, w, fermi_mode=self.fermi_mode, dos_grid=self.grid, refine=refine)
             parts = [edges.reshape(-1), info.to(torch.float64)]
             if stats is not None:
                 parts.append(stats.reshape(-1))
@@ -127,7 +131,7 @@ class ServingEngine:
             host.copy_(flat, non_blocking=True)
             ev = torch.cuda.Event()
             ev.record(stream)
-        b = Batch(reqs, ld, stream, ev, host, self.batch_bytes(ld, nmat), keep=[H, S, w, edges, info, flat])
+        b = Batch(reqs, ld, stream, ev, host, self.batch_bytes(ld, nmat), keep=[H, S, w, edges, info, flat, wsl])
         if self.keep_outputs:
             b.dos, b.w, b.plan = dos, w, plan
         return b

@@ -75,3 +75,57 @@ def test_mixed_precision_stream_within_1mev():
         err = float((res["eigenvalues"] - w[:, :n]).abs().max()) * 27.211
         worst = max(worst, err)
     assert worst < MIXED_TOL_EV, worst
+
+
+REFINED_TOL_EV = 1e-6   # band edges of the precision form after FP64 Rayleigh refinement
+
+
+def test_mixed_refined_band_edges_match_fp64():
+    """Precision form: VBM/CBM after the FP64 Rayleigh-quotient refinement
+    agree with the FP64 path to 1e-6 eV, alone and batched with padding."""
+    tab, inputs = _stream(12, seed=43, n_max=128)
+    worst_raw = worst_ref = 0.0
+    refs = []
+    for inp in inputs:
+        plan = engine.plan_build(tab, [inp])
+        H, S = engine.run_build(tab, plan)
+        w64, _, _ = engine.eigh(H, S, precision="fp64")
+        e64 = engine.edges_dict(engine.post(plan, w64)[0][0].cpu().numpy())
+        refs.append(e64)
+        wsl = []
+        wm, _, _ = engine.eigh(H, S, precision="mixed", workspace=wsl)
+        raw = engine.edges_dict(engine.post(plan, wm.clone())[0][0].cpu().numpy())
+        ref = engine.edges_dict(engine.post(plan, wm, refine=(H, S, wsl[0]))[0][0].cpu().numpy())
+        if e64["n_val"] > 0 and e64["n_con"] > 0:
+            for key in ("vbm", "cbm"):
+                worst_raw = max(worst_raw, abs(raw[key] - e64[key]))
+                worst_ref = max(worst_ref, abs(ref[key] - e64[key]))
+                assert abs(ref[key] - e64[key]) < REFINED_TOL_EV, (key, ref[key], e64[key])
+    assert worst_ref <= worst_raw
+    # batched with padding, through the serving engine
+    eng = ServingEngine(tab, keep_outputs=True, precision="mixed", refine_edges=True)
+    out = eng.replay(inputs, np.zeros(len(inputs)))
+    for i, e64 in enumerate(refs):
+        if e64["n_val"] > 0 and e64["n_con"] > 0:
+            for key in ("vbm", "cbm"):
+                assert abs(out["results"][i][key] - e64[key]) < REFINED_TOL_EV, (i, key)
+
+
+def test_rocksalt_576_refined_edges():
+    """64-atom rocksalt (576 orbitals), 8 of its k-points: refined edges within 1e-6 eV of FP64."""
+    st = synth.rocksalt_64()
+    skfs = synth.make_skf_set([12, 8], seed=1)
+    tab = TableSet(skfs, synth.shell_dict_spd([8, 12]), device="cuda")
+    k = st.kpoints[::8]
+    inp = engine.StructureInput(st.z, st.positions * AA_BOHR, st.cell * AA_BOHR, k, np.full(len(k), 1 / len(k)))
+    plan = engine.plan_build(tab, [inp])
+    H, S = engine.run_build(tab, plan)
+    w64, _, _ = engine.eigh(H, S, precision="fp64")
+    e64 = engine.edges_dict(engine.post(plan, w64)[0][0].cpu().numpy())
+    wsl = []
+    wm, _, _ = engine.eigh(H, S, precision="mixed", workspace=wsl)
+    raw_err = float((wm - w64).abs().max()) * 27.211
+    ref = engine.edges_dict(engine.post(plan, wm, refine=(H, S, wsl[0]))[0][0].cpu().numpy())
+    assert raw_err < 1e-3
+    for key in ("vbm", "cbm"):
+        assert abs(ref[key] - e64[key]) < REFINED_TOL_EV, (key, ref[key], e64[key])
