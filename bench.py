@@ -40,8 +40,9 @@ def parse():
     p.add_argument("--requests", type=int, default=2000)
     p.add_argument("--rate", type=float, default=2000.0,
                    help="Poisson arrival rate, requests/s (default saturates the device: value = capacity)")
-    p.add_argument("--mode", choices=["serve", "plain", "fit"], default="serve",
-                   help="fit: the fitting workload (32 cells of 8-64 atoms, 3 species, DOS + gap loss, Adam steps)")
+    p.add_argument("--mode", choices=["serve", "plain", "fit", "large"], default="serve",
+                   help="fit: the fitting workload (32 cells of 8-64 atoms, 3 species, DOS + gap loss, Adam "
+                        "steps); large: one 1024-atom cubic cell (9216 orbitals) at Gamma + 4 complex k")
     p.add_argument("--fit-steps", type=int, default=200, help="Adam steps timed in --mode fit")
     p.add_argument("--sample-every", type=int, default=8,
                    help="sample every n-th tridiagonalisation mat-vec launch for the roofline")
@@ -222,6 +223,73 @@ def run_fit(args):
     print(json.dumps(out))
 
 
+def run_large(args):
+    """1024-atom cubic cell (L = 22 A, 3 species, d_min 1.9 A): 9216x9216 at
+    Gamma and four complex k. Time to solution of the precision form (FP32
+    bulk + FP64 Cholesky / tridiagonal / edge refinement) and its errors
+    against the all-FP64 device path on the same matrices."""
+    import torch
+    from slakonet_amd import engine, synth
+    from slakonet_amd.tables import TableSet
+    from slakonet_amd.units import AA_BOHR
+    dev = torch.device("cuda:0")
+    t0 = time.perf_counter()
+    st = synth.large_cubic()
+    print(f"# structure generated in {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
+    species = sorted({int(z) for z in st.z})
+    tab = TableSet(synth.make_skf_set(species, seed=TABLE_SEED), synth.shell_dict_spd(species), device=dev)
+    inp = engine.StructureInput(st.z, st.positions * AA_BOHR, st.cell * AA_BOHR, st.kpoints, st.k_weights)
+    grid = torch.linspace(-20.0, 10.0, 2001, dtype=torch.float64, device=dev)
+    res = {}
+    for prec in ("mixed", "fp64"):
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        plan = engine.plan_build(tab, [inp])
+        H, S = engine.run_build(tab, plan)
+        torch.cuda.synchronize()
+        t_build = time.perf_counter() - t1
+        stage = []
+        wsl = []
+        w, _, _ = engine.eigh(H, S, precision=prec, stage_ms=stage, workspace=wsl)
+        refine = (H, S, wsl[0]) if prec == "mixed" else None
+        edges, dos, _ = engine.post(plan, w, dos_grid=grid, refine=refine)
+        torch.cuda.synchronize()
+        t_all = time.perf_counter() - t1
+        res[prec] = dict(w=w[:, :plan.n_orb[0]].cpu().numpy(), edges=engine.edges_dict(edges[0].cpu().numpy()),
+                         dos=dos[0].cpu().numpy(), t=t_all, t_build=t_build, stages=stage[0])
+        print(f"# {prec}: {t_all:.2f} s (build {t_build:.2f} s; stages ms {stage[0]})", file=sys.stderr,
+              flush=True)
+        del H, S, wsl, w
+        torch.cuda.empty_cache()
+    n = 9 * len(st.z)
+    m, f = res["mixed"], res["fp64"]
+    eig_err = float(np.abs(m["w"] - f["w"]).max()) * 27.211
+    e = grid.cpu().numpy()
+    l1 = float(np.trapezoid(np.abs(m["dos"] - f["dos"]), e) / np.trapezoid(f["dos"], e))
+    nk = len(st.kpoints)
+    # LAPACK counts, complex: potrf n^3/3, trinv n^3/3 (FP64); reduce 2n^3, hetrd 4n^3/3 (FP32 in the
+    # precision form); x4 real flops per complex multiply-add
+    fl = {"potrf": 4 * n ** 3 / 3, "trinv": 4 * n ** 3 / 3, "reduce": 8 * n ** 3, "hetrd": 16 * n ** 3 / 3}
+    mfma = {k: nk * fl[k] / (m["stages"][k] * 1e-3) / 1e12 for k in fl}
+    out = {
+        "metric": "time to solution, 1024-atom cell (9216 orbitals), Gamma + 4 k, eigenvalues + gap + DOS",
+        "value": m["t"], "unit": "s", "n_gpus": 1, "steps": 1, "warmup": 0, "ms_per_step": 1000.0 * m["t"],
+        "higher_is_better": False, "scaling": "strong", "vs_baseline": None, "dtype": "f32+f64",
+        "data": "synthetic",
+        "fp64_path_s": f["t"],
+        "errors_vs_fp64_path": {"max_eig_err_eV": eig_err,
+                                "vbm_err_eV": abs(m["edges"]["vbm"] - f["edges"]["vbm"]),
+                                "cbm_err_eV": abs(m["edges"]["cbm"] - f["edges"]["cbm"]),
+                                "dos_l1_rel": l1, "metal": bool(f["edges"]["metal"])},
+        "stage_ms": m["stages"], "stage_tflops": mfma,
+        "mfma_utilisation": {"potrf": mfma["potrf"] / FP64_PEAK_TFLOPS, "trinv": mfma["trinv"] / FP64_PEAK_TFLOPS,
+                             "reduce": mfma["reduce"] / 157.3},
+        "config": {"workload": f"cubic L=22 A, {len(st.z)} atoms (Z 6/8/14), d_min 1.9 A, {nk} k-points, "
+                               f"n={n}, 2001-pt DOS"},
+    }
+    print(json.dumps(out))
+
+
 def cpu_baseline(structs, skfs, budget_s):
     """FP64 oracle (numpy/LAPACK) over the stream in order until the budget is spent."""
     from oracle import build as obuild, post as opost, solve as osolve
@@ -273,6 +341,9 @@ def main():
         return
     if args.mode == "fit":
         run_fit(args)
+        return
+    if args.mode == "large":
+        run_large(args)
         return
     dev = torch.device("cuda:0")
     t_gen = time.perf_counter()

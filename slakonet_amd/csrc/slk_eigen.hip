@@ -17,6 +17,8 @@
 #include "slk_edges.h"
 
 #define EIG_THREADS 256
+#define BISECT_LDS_MAX 4096
+#define EIGH_MAX_N 16384
 
 // ---------------------------------------------------------------------------
 // Eigenvalues of the real symmetric tridiagonal (d, e) by bisection /
@@ -50,14 +52,17 @@ __device__ __forceinline__ int sturm_count(const double* d, const double* e2, in
 // sub-interval where the count crosses the eigenvalue index ((G+1)x shrink per
 // step). G = 1 is plain bisection with 64 eigenvalues per wave; larger G
 // trades total work for parallelism when few eigenvalues are in flight.
+// LDS = true: d and e^2 staged in LDS (2n doubles, n <= 4096); false: read
+// from global memory (d in place, e^2 from E2g written by square_e_kernel).
+template <bool LDS>
 __global__ __launch_bounds__(EIG_THREADS) void bisect_kernel(const double* D, const double* Ed, int n, int ld,
-                                                             double* Wout, int wld, int G) {
+                                                             double* Wout, int wld, int G, const double* E2g) {
   const int b = blockIdx.y;
   const double* d = D + (size_t)b * ld;
   const double* e = Ed + (size_t)b * ld;
   extern __shared__ double sm[];
-  double* sd = sm;
-  double* se = sm + n;
+  double* sd = LDS ? sm : const_cast<double*>(d);
+  double* se = LDS ? sm + n : const_cast<double*>(E2g + (size_t)b * ld);
   __shared__ double s_red[3][EIG_THREADS / 64];
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   double lo = 1e300, hi = -1e300, emax = 0.0;
@@ -65,8 +70,10 @@ __global__ __launch_bounds__(EIG_THREADS) void bisect_kernel(const double* D, co
     const double di = d[i];
     const double ei = i < n - 1 ? e[i] : 0.0;
     const double ep = i > 0 ? e[i - 1] : 0.0;
-    sd[i] = di;
-    se[i] = ei * ei;
+    if (LDS) {
+      sd[i] = di;
+      se[i] = ei * ei;
+    }
     const double r = fabs(ep) + fabs(ei);
     lo = fmin(lo, di - r);
     hi = fmax(hi, di + r);
@@ -111,6 +118,15 @@ __global__ __launch_bounds__(EIG_THREADS) void bisect_kernel(const double* D, co
     }
   }
   if (sub == 0 && idx < n) Wout[(size_t)b * wld + idx] = 0.5 * (lo + hi);
+}
+
+// e2[b][i] = e[b][i]^2 (i < n-1), 0 at n-1: global Sturm-count input for n > 4096.
+__global__ void square_e_kernel(const double* Ed, int n, int ld, double* E2) {
+  const int b = blockIdx.y;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const double ei = i < n - 1 ? Ed[(size_t)b * ld + i] : 0.0;
+    E2[(size_t)b * ld + i] = ei * ei;
+  }
 }
 
 // lanes per eigenvalue so that about 2^17 lanes are busy
@@ -315,55 +331,76 @@ __global__ __launch_bounds__(EIG_THREADS) void apply_q_kernel(const double2* __r
 //   x = Q y: the FP32 Householder reflectors, applied in FP64 arithmetic;
 //   z = W^H x with W = L^-1 in FP64.
 // z carries the FP32 reduction error (~1e-7 relative); the quotient's error
-// is second order in it. One workgroup per (structure, edge).
+// is second order in it. Four launches per batch: refine_tri (one
+// workgroup per (structure, edge): T solve and reflectors), refine_wh
+// (z = W^H x over 64-column blocks), refine_rq (partial z^H H z, z^H S z over
+// 64-row blocks), refine_commit (sums, writes w and the edge rows).
 #define RF_THREADS 256
-#define RF_MAXC 16          // columns per thread in z = W^H x (m <= 4096)
+#define RF_BLK 64           // columns (refine_wh) / rows (refine_rq) per workgroup
 #define RF_SCR 10           // scratch doubles per ld per (structure, edge)
 
-__global__ __launch_bounds__(RF_THREADS) void refine_edges_kernel(
-    const double2* __restrict__ H, const double2* __restrict__ S, const double2* __restrict__ Wm,
-    const float2* __restrict__ R, const float2* __restrict__ TAU, const double* __restrict__ D,
-    const double* __restrict__ Ed, int ld, const double* __restrict__ w, const SlkSpectrum* spec,
-    const double* __restrict__ edges, double* scratch, double* rho_out) {
-  const int s = blockIdx.x, edge = blockIdx.y;
-  const SlkSpectrum sp = spec[s];
-  const double* row = edges + (size_t)s * SLK_EDGE_FIELDS;
-  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+struct RefineCtx {
+  const double2* H;
+  const double2* S;
+  const double2* Wm;        // W = L^-1, FP64
+  const float2* R;          // FP32 reflectors (strict lower part)
+  const float2* TAU;
+  const double* D;
+  const double* E;
+  int ld;
+  const double* w;
+  const SlkSpectrum* spec;
+  const double* edges;
+  double* scratch;          // [n_spec * 2][RF_SCR * ld]
+  int* meta;                // [n_spec * 2][2]: band (-1: skip), matrix index
+  double* part;             // [n_spec * 2][nblk][2]: partial num, den
+  int nblk;
+};
+
+__device__ __forceinline__ double* rf_base(const RefineCtx& c, int v) {
+  return c.scratch + (size_t)v * RF_SCR * c.ld;
+}
+
+__global__ __launch_bounds__(RF_THREADS) void refine_tri_kernel(RefineCtx c) {
+  const int s = blockIdx.x, edge = blockIdx.y, v = s * 2 + edge;
+  const SlkSpectrum sp = c.spec[s];
+  const double* row = c.edges + (size_t)s * SLK_EDGE_FIELDS;
+  const int tid = threadIdx.x;
+  const int ld = c.ld;
   __shared__ double red[RF_THREADS / 64];
   __shared__ int s_band;
-  double* rho_slot = rho_out + 2 * ((size_t)s * 2 + edge);
   if (row[E_NVAL] <= 0.0 || row[E_NCON] <= 0.0) {
-    if (tid == 0) { rho_slot[0] = 0.0; rho_slot[1] = -1.0; }
+    if (tid == 0) c.meta[2 * v] = -1;
     return;
   }
   const double mu = row[E_MU];
   const int k = (int)row[edge == 0 ? E_VBMK : E_CBMK];
   const int m = sp.nb;
-  const double* wk = w + sp.w0 + (size_t)k * sp.wld;
+  const double* wk = c.w + sp.w0 + (size_t)k * sp.wld;
   if (tid == 0) {
-    int c = 0;
+    int cnt = 0;
     for (int b = 0; b < m; ++b)
-      if (wk[b] * H2E_OUT < mu * H2E_OUT) ++c;
-    s_band = edge == 0 ? c - 1 : c;
+      if (wk[b] * H2E_OUT < mu * H2E_OUT) ++cnt;
+    s_band = edge == 0 ? cnt - 1 : cnt;
   }
   __syncthreads();
   const int band = s_band;
-  if (band < 0 || band >= m) {
-    if (tid == 0) { rho_slot[0] = 0.0; rho_slot[1] = -1.0; }
-    return;
-  }
-  const double lam = wk[band];
   const size_t mat = (size_t)sp.kw0 + k;
-  const double* d = D + mat * ld;
-  const double* e = Ed + mat * ld;
-  double* base = scratch + ((size_t)s * 2 + edge) * RF_SCR * ld;
+  if (tid == 0) {
+    c.meta[2 * v] = (band < 0 || band >= m) ? -1 : band;
+    c.meta[2 * v + 1] = (int)mat;
+  }
+  if (band < 0 || band >= m) return;
+  const double lam = wk[band];
+  const double* d = c.D + mat * ld;
+  const double* e = c.E + mat * ld;
+  double* base = rf_base(c, v);
   double *dl = base, *dd = base + ld, *du = base + 2 * (size_t)ld, *du2 = base + 3 * (size_t)ld;
   double* piv = base + 4 * (size_t)ld;
   double* y = base + 5 * (size_t)ld;
   double2* x = reinterpret_cast<double2*>(base + 6 * (size_t)ld);
-  double2* z = reinterpret_cast<double2*>(base + 8 * (size_t)ld);
 
-  // (1) inverse iteration on T[0:m) - lam I (one lane)
+  // inverse iteration on T[0:m) - lam I (one lane)
   if (tid == 0) {
     double tn = 0.0;
     for (int i = 0; i < m; ++i) {
@@ -419,11 +456,11 @@ __global__ __launch_bounds__(RF_THREADS) void refine_edges_kernel(
     }
   }
   __syncthreads();
-  // (2) x = Q y, reflectors last to first: x -= v tau (v^H x)
-  for (int r = tid; r < m; r += RF_THREADS) x[r] = make_double2(y[r], 0.0);
+  // x = Q y, reflectors last to first: x -= v tau (v^H x)
+  for (int r = tid; r < ld; r += RF_THREADS) x[r] = make_double2(r < m ? y[r] : 0.0, 0.0);
   __syncthreads();
-  const float2* a = R + mat * ld * ld;
-  const float2* tau = TAU + mat * ld;
+  const float2* a = c.R + mat * ld * ld;
+  const float2* tau = c.TAU + mat * ld;
   for (int j = m - 2; j >= 0; --j) {
     const float2 tf = tau[j];
     if (tf.x == 0.0f && tf.y == 0.0f) continue;
@@ -446,71 +483,103 @@ __global__ __launch_bounds__(RF_THREADS) void refine_edges_kernel(
     }
     __syncthreads();
   }
-  // (3) z = W^H x: z[c] = sum_{r >= c} conj(W[r][c]) x[r]
-  {
-    const double2* wm = Wm + mat * ld * ld;
-    double2 acc[RF_MAXC];
-#pragma unroll
-    for (int q = 0; q < RF_MAXC; ++q) acc[q] = make_double2(0.0, 0.0);
-    for (int r = 0; r < m; ++r) {
-      const double2 xr = x[r];
-#pragma unroll
-      for (int q = 0; q < RF_MAXC; ++q) {
-        const int c = tid + q * RF_THREADS;
-        if (c <= r) {
-          const double2 wv2 = wm[(size_t)r * ld + c];
-          acc[q].x += wv2.x * xr.x + wv2.y * xr.y;
-          acc[q].y += wv2.x * xr.y - wv2.y * xr.x;
-        }
-      }
-    }
-#pragma unroll
-    for (int q = 0; q < RF_MAXC; ++q) {
-      const int c = tid + q * RF_THREADS;
-      if (c < m) z[c] = acc[q];
+}
+
+// z[c] = sum_{r >= c} conj(W[r][c]) x[r] for the RF_BLK columns of this block
+// (coalesced row segments; waves split the rows, lanes own the columns).
+__global__ __launch_bounds__(RF_THREADS) void refine_wh_kernel(RefineCtx c) {
+  const int v = blockIdx.y;
+  const int band = c.meta[2 * v];
+  if (band < 0) return;
+  const int s = v / 2;
+  const int m = c.spec[s].nb;
+  const int c0 = blockIdx.x * RF_BLK;
+  if (c0 >= m) return;
+  const int ld = c.ld;
+  const size_t mat = (size_t)c.meta[2 * v + 1];
+  const double2* wm = c.Wm + mat * ld * ld;
+  double* base = rf_base(c, v);
+  const double2* x = reinterpret_cast<const double2*>(base + 6 * (size_t)ld);
+  double2* z = reinterpret_cast<double2*>(base + 8 * (size_t)ld);
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int col = c0 + lane;
+  double ax = 0.0, ay = 0.0;
+  if (col < m) {
+    for (int r = c0 + wv; r < m; r += RF_THREADS / 64) {
+      if (r < col) continue;
+      const double2 wr = wm[(size_t)r * ld + col], xr = x[r];
+      ax += wr.x * xr.x + wr.y * xr.y;
+      ay += wr.x * xr.y - wr.y * xr.x;
     }
   }
+  __shared__ double2 acc[RF_THREADS / 64][RF_BLK];
+  acc[wv][lane] = make_double2(ax, ay);
   __syncthreads();
-  // (4) rho = z^H H z / z^H S z
-  const double2* hm = H + mat * ld * ld;
-  const double2* sm = S + mat * ld * ld;
+  if (wv == 0 && col < m) {
+    double2 t = acc[0][lane];
+    for (int q = 1; q < RF_THREADS / 64; ++q) t = c_add(t, acc[q][lane]);
+    z[col] = t;
+  }
+}
+
+// Partial Re(z^H H z), Re(z^H S z) over the RF_BLK rows of this block.
+__global__ __launch_bounds__(RF_THREADS) void refine_rq_kernel(RefineCtx c) {
+  const int v = blockIdx.y;
+  const int band = c.meta[2 * v];
+  if (band < 0) return;
+  const int s = v / 2;
+  const int m = c.spec[s].nb;
+  const int r0 = blockIdx.x * RF_BLK;
+  double* part = c.part + ((size_t)v * c.nblk + blockIdx.x) * 2;
+  const int ld = c.ld;
+  const size_t mat = (size_t)c.meta[2 * v + 1];
+  const double2* hm = c.H + mat * ld * ld;
+  const double2* sm = c.S + mat * ld * ld;
+  const double2* z = reinterpret_cast<const double2*>(rf_base(c, v) + 8 * (size_t)ld);
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  __shared__ double red[RF_THREADS / 64];
   double num = 0.0, den = 0.0;
-  for (int r = wv; r < m; r += RF_THREADS / 64) {
-    double hx = 0.0, hy = 0.0, sx2 = 0.0, sy2 = 0.0;
-    for (int c = lane; c < m; c += 64) {
-      const double2 zc = z[c], hv = hm[(size_t)r * ld + c], sv = sm[(size_t)r * ld + c];
+  for (int r = r0 + wv; r < min(r0 + RF_BLK, m); r += RF_THREADS / 64) {
+    double hx = 0.0, hy = 0.0, sx = 0.0, sy = 0.0;
+    for (int q = lane; q < m; q += 64) {
+      const double2 zc = z[q], hv = hm[(size_t)r * ld + q], sv = sm[(size_t)r * ld + q];
       hx += hv.x * zc.x - hv.y * zc.y;
       hy += hv.x * zc.y + hv.y * zc.x;
-      sx2 += sv.x * zc.x - sv.y * zc.y;
-      sy2 += sv.x * zc.y + sv.y * zc.x;
+      sx += sv.x * zc.x - sv.y * zc.y;
+      sy += sv.x * zc.y + sv.y * zc.x;
     }
-    hx = wave_sum(hx); hy = wave_sum(hy); sx2 = wave_sum(sx2); sy2 = wave_sum(sy2);
+    hx = wave_sum(hx); hy = wave_sum(hy); sx = wave_sum(sx); sy = wave_sum(sy);
     const double2 zr = z[r];
-    num += zr.x * hx + zr.y * hy;     // Re(conj(z_r) (Hz)_r)
-    den += zr.x * sx2 + zr.y * sy2;
+    num += zr.x * hx + zr.y * hy;
+    den += zr.x * sx + zr.y * sy;
   }
   if (lane != 0) { num = 0.0; den = 0.0; }
   num = block_sum(num, red);
   den = block_sum(den, red);
   if (tid == 0) {
-    rho_slot[0] = num / den;
-    rho_slot[1] = (double)band;
+    part[0] = r0 < m ? num : 0.0;
+    part[1] = r0 < m ? den : 0.0;
   }
 }
 
-// Writes the refined edge energies into the eigenvalue array and the edge row.
-__global__ void refine_commit_kernel(double* w, const SlkSpectrum* spec, double* edges, const double* rho_out,
-                                     int n_spec) {
+// rho per (structure, edge) from the partials; writes w and the edge row.
+__global__ void refine_commit_kernel(RefineCtx c, double* w, double* edges, int n_spec) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= n_spec) return;
-  const SlkSpectrum sp = spec[s];
+  const SlkSpectrum sp = c.spec[s];
   double* row = edges + (size_t)s * SLK_EDGE_FIELDS;
   const double mu = row[E_MU];
   for (int edge = 0; edge < 2; ++edge) {
-    const double* slot = rho_out + 2 * ((size_t)s * 2 + edge);
-    if (slot[1] < 0.0) continue;
-    const double rho = slot[0];
-    const int band = (int)slot[1];
+    const int v = s * 2 + edge;
+    const int band = c.meta[2 * v];
+    if (band < 0) continue;
+    const int nb = (sp.nb + RF_BLK - 1) / RF_BLK;
+    double num = 0.0, den = 0.0;
+    for (int b = 0; b < nb; ++b) {
+      num += c.part[((size_t)v * c.nblk + b) * 2];
+      den += c.part[((size_t)v * c.nblk + b) * 2 + 1];
+    }
+    const double rho = num / den;
     const int k = (int)row[edge == 0 ? E_VBMK : E_CBMK];
     w[sp.w0 + (size_t)k * sp.wld + band] = rho;
     if (edge == 0) {
@@ -598,8 +667,9 @@ static int eigh_impl(const void* H, const void* S, int n, int ld, int batch, dou
   const size_t need = mixed ? slk_eigh_mixed_workspace_bytes(n, ld, batch)
                             : slk_eigh_workspace_bytes(n, ld, batch, V != nullptr);
   if (ws_bytes < need) return 2;
-  // LDS-resident tridiagonal data: values need 2n doubles, vectors 4n doubles (<= 64 KiB)
-  if (n > 4096 || (V != nullptr && n > 2048)) return 3;
+  // values: any n up to EIGH_MAX_N (LDS-staged Sturm counts up to 4096, global beyond);
+  // vectors: the QL sweep keeps 4n doubles in LDS (n <= 2048)
+  if (n > EIGH_MAX_N || (V != nullptr && n > 2048)) return 3;
   if (V == nullptr && n <= 32) {
     // whole solve in LDS, one launch (stages collapse into the first interval)
     if (ev) (void)hipEventRecord(ev[0], stream);
@@ -640,8 +710,17 @@ static int eigh_impl(const void* H, const void* S, int n, int ld, int batch, dou
   if (V == nullptr) {
     const int G = bisect_group(n, batch);
     const int eig_per_block = (EIG_THREADS / 64) * (64 / G);
-    hipLaunchKernelGGL(bisect_kernel, dim3((n + eig_per_block - 1) / eig_per_block, batch), dim3(EIG_THREADS),
-                       2 * n * sizeof(double), stream, s.D, s.E, n, ld, w, wld, G);
+    const dim3 grid((n + eig_per_block - 1) / eig_per_block, batch);
+    if (n <= BISECT_LDS_MAX) {
+      hipLaunchKernelGGL(bisect_kernel<true>, grid, dim3(EIG_THREADS), 2 * n * sizeof(double), stream, s.D, s.E, n,
+                         ld, w, wld, G, (const double*)nullptr);
+    } else {
+      // e^2 into the (now free) mat-vec vector buffer, ld doubles per matrix
+      double* e2 = mixed ? reinterpret_cast<double*>(s.fY) : reinterpret_cast<double*>(s.Y);
+      hipLaunchKernelGGL(square_e_kernel, dim3((n + 255) / 256, batch), dim3(256), 0, stream, s.E, n, ld, e2);
+      hipLaunchKernelGGL(bisect_kernel<false>, grid, dim3(EIG_THREADS), 0, stream, s.D, s.E, n, ld, w, wld, G,
+                         (const double*)e2);
+    }
     if (ev) (void)hipEventRecord(ev[5], stream);
   } else {
     hipLaunchKernelGGL(tql2_kernel, dim3(batch), dim3(EIG_THREADS), 4 * n * sizeof(double), stream, s.D, s.E, n, ld,
@@ -687,7 +766,8 @@ int slk_eigh_mixed(const void* H, const void* S, int n, int ld, int batch, doubl
 }
 
 size_t slk_refine_scratch_bytes(int ld, int n_spec) {
-  return ((size_t)n_spec * 2 * RF_SCR * ld + (size_t)n_spec * 4) * sizeof(double);
+  const size_t nblk = (size_t)(ld + RF_BLK - 1) / RF_BLK;
+  return ((size_t)n_spec * 2 * RF_SCR * ld + (size_t)n_spec * 2 * nblk * 2 + (size_t)n_spec * 2) * sizeof(double);
 }
 
 int slk_refine_edges_mixed(const void* H, const void* S, int n, int ld, int batch, const void* ws, size_t ws_bytes,
@@ -695,17 +775,31 @@ int slk_refine_edges_mixed(const void* H, const void* S, int n, int ld, int batc
                            size_t scratch_bytes, hipStream_t stream) {
   (void)wld;
   if (n_spec <= 0 || n <= 32) return 0;      // n <= 32 was solved in FP64 by the LDS kernel
-  if (n > RF_MAXC * RF_THREADS) return 3;
+  if (n > EIGH_MAX_N) return 3;
   if (ws_bytes < slk_eigh_mixed_workspace_bytes(n, ld, batch)) return 2;
   if (scratch_bytes < slk_refine_scratch_bytes(ld, n_spec)) return 2;
   EighWs s;
   carve(reinterpret_cast<char*>(const_cast<void*>(ws)), ld, batch, 0, 1, &s);
-  double* rho = scratch + (size_t)n_spec * 2 * RF_SCR * ld;
-  hipLaunchKernelGGL(refine_edges_kernel, dim3(n_spec, 2), dim3(RF_THREADS), 0, stream,
-                     reinterpret_cast<const double2*>(H), reinterpret_cast<const double2*>(S), s.B1, s.F1, s.fTAU,
-                     s.D, s.E, ld, w, spec, edges, scratch, rho);
-  hipLaunchKernelGGL(refine_commit_kernel, dim3((n_spec + 63) / 64), dim3(64), 0, stream, w, spec, edges, rho,
-                     n_spec);
+  RefineCtx c;
+  c.H = reinterpret_cast<const double2*>(H);
+  c.S = reinterpret_cast<const double2*>(S);
+  c.Wm = s.B1;
+  c.R = s.F1;
+  c.TAU = s.fTAU;
+  c.D = s.D;
+  c.E = s.E;
+  c.ld = ld;
+  c.w = w;
+  c.spec = spec;
+  c.edges = edges;
+  c.scratch = scratch;
+  c.nblk = (ld + RF_BLK - 1) / RF_BLK;
+  c.part = scratch + (size_t)n_spec * 2 * RF_SCR * ld;
+  c.meta = reinterpret_cast<int*>(c.part + (size_t)n_spec * 2 * c.nblk * 2);
+  hipLaunchKernelGGL(refine_tri_kernel, dim3(n_spec, 2), dim3(RF_THREADS), 0, stream, c);
+  hipLaunchKernelGGL(refine_wh_kernel, dim3(c.nblk, n_spec * 2), dim3(RF_THREADS), 0, stream, c);
+  hipLaunchKernelGGL(refine_rq_kernel, dim3(c.nblk, n_spec * 2), dim3(RF_THREADS), 0, stream, c);
+  hipLaunchKernelGGL(refine_commit_kernel, dim3((n_spec + 63) / 64), dim3(64), 0, stream, c, w, edges, n_spec);
   return (int)hipGetLastError();
 }
 

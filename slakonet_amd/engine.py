@@ -253,7 +253,8 @@ def eigh(H: torch.Tensor, S: torch.Tensor, vectors: bool = False, stream=None, c
         rc = (lib.slk_eigh_mixed_timed if mixed else lib.slk_eigh_f64_timed)(*args, ms)
         stage_ms.append(dict(zip(EIGH_STAGES, list(ms))))
     if rc == 3:
-        raise NotImplementedError(f"matrix dimension {n} exceeds the solver's LDS-resident tridiagonal limit")
+        raise NotImplementedError(f"matrix dimension {n} is beyond the solver's limits (values: n <= 16384, "
+                                  "vectors: n <= 2048)")
     _lib.check(rc, "slk_eigh_f64")
     if check:
         check_info(info, batch)
@@ -300,7 +301,7 @@ def refine_edges(H: torch.Tensor, S: torch.Tensor, ws: torch.Tensor, w: torch.Te
                                     C.c_void_p(spd.data_ptr()), plan.n_struct, C.c_void_p(edges.data_ptr()),
                                     C.c_void_p(scratch.data_ptr()), scratch.numel() * 8, _lib.stream_handle(stream))
     if rc == 3:
-        raise NotImplementedError(f"band-edge refinement supports n <= 4096 (got {ld})")
+        raise NotImplementedError(f"band-edge refinement supports n <= 16384 (got {ld})")
     _lib.check(rc, "slk_refine_edges_mixed")
     return scratch
 

@@ -212,3 +212,27 @@ def test_nonpositive_overlap_raises_with_k_index():
     assert bad_k, "test input must make S indefinite"
     with pytest.raises(engine.EigenFailure, match=f"k-point index {bad_k[0]}"):
         engine.eigh(H, S)
+
+
+def test_values_beyond_lds_limit_padded():
+    """ld = 4224 > 4096 (Sturm counts from global memory): a structure padded
+    to that dimension keeps its eigenvalues (FP64 and refined mixed edges)."""
+    st = synth.diamond_si()
+    skfs = synth.make_skf_set([14], seed=0)
+    tab = _tables(skfs, [14])
+    inp = _inputs(st, skfs, kmax=3)
+    plan = engine.plan_build(tab, [inp])
+    H, S = engine.run_build(tab, plan)
+    w_ref, _, _ = engine.eigh(H, S)
+    big = engine.plan_build(tab, [inp], ld=4224)
+    Hb, Sb = engine.run_build(tab, big)
+    w_big, _, _ = engine.eigh(Hb, Sb)
+    n = plan.n_orb[0]
+    assert torch.allclose(w_big[:, :n], w_ref[:, :n], atol=EIG_ATOL, rtol=0)
+    assert torch.allclose(w_big[:, n:], torch.full_like(w_big[:, n:], engine.PAD_H), atol=1e-9, rtol=0)
+    e_ref = engine.edges_dict(engine.post(plan, w_ref)[0][0].cpu().numpy())
+    wsl = []
+    w_mix, _, _ = engine.eigh(Hb, Sb, precision="mixed", workspace=wsl)
+    e_mix = engine.edges_dict(engine.post(big, w_mix, refine=(Hb, Sb, wsl[0]))[0][0].cpu().numpy())
+    for key in ("vbm", "cbm"):
+        assert abs(e_mix[key] - e_ref[key]) < 1e-6, (key, e_mix[key], e_ref[key])
