@@ -67,6 +67,40 @@ typedef struct SlkBuildArgs {
   int out_f32;              /* 0: complex128, 1: complex64 */
 } SlkBuildArgs;
 
+/* Host-side batch planner (no GPU needed). Replaces the per-structure image
+ * and index bookkeeping of the reference (Periodic / get_cell_translations,
+ * slakonet/atoms.py:48-243, and the basis index matrices of
+ * slakonet/basis.py:66-153) with one pass over a batch: lattice translations
+ * per structure (|T_a| <= floor(cutoff/d_a + spread_a) + 1, x slowest),
+ * column tiles of <= 64 orbitals, build work items, all packed into one blob
+ * for a single host-to-device copy. */
+#define SLK_PLAN_SECTIONS 9   /* structs, work, atom_pos, atom_spec, atom_orb, trans_cart, trans_int, kfrac, kw */
+typedef struct SlkPlanIn {
+  int n_struct;
+  const int* n_atoms;       /* [n_struct] */
+  const int* spec;          /* [sum atoms] species index into spec_norb */
+  const double* pos;        /* [sum atoms][3] Bohr */
+  const double* cell;       /* [n_struct][3][3] Bohr, rows are lattice vectors */
+  const int* nk;            /* [n_struct] */
+  const double* kfrac;      /* [sum nk][3] */
+  const double* kw;         /* [sum nk] */
+  const int* spec_norb;     /* [nspec] */
+  int nspec;
+  double cutoff;            /* Bohr */
+  int ld;                   /* padded dimension, 0: the largest n_orb */
+} SlkPlanIn;
+typedef struct SlkPlanOut {
+  long long off[SLK_PLAN_SECTIONS];    /* byte offset of each section (256-aligned) */
+  long long bytes[SLK_PLAN_SECTIONS];  /* used bytes of each section */
+  long long total;
+  int n_work, n_trans, n_mat, ld;
+} SlkPlanOut;
+/* 0 on success, 1 bad input (empty structure, unknown species, singular
+ * cell), 2 ld smaller than the largest structure. */
+int slk_plan_size(const SlkPlanIn* in, SlkPlanOut* out);
+/* Fills blob (out->total bytes); n_orb [n_struct], mat0 [n_struct]. */
+int slk_plan_fill(const SlkPlanIn* in, const SlkPlanOut* out, void* blob, int* n_orb, long long* mat0);
+
 /* Fused SK interpolation + rotation + Bloch sum + on-site into H(k), S(k).
  * Replaces hs_matrix(geometry, basis, sk_feed, cutoff=10.0)
  * (slakonet/slaterkoster.py:103-441), SkfFeed.off_site/on_site

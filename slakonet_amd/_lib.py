@@ -50,8 +50,25 @@ class SlkDosJob(C.Structure):
                 ("edge", C.c_int), ("out0", C.c_longlong)]
 
 
+PLAN_SECTIONS = 9
+
+
+class SlkPlanIn(C.Structure):
+    _fields_ = [("n_struct", C.c_int), ("n_atoms", C.c_void_p), ("spec", C.c_void_p), ("pos", C.c_void_p),
+                ("cell", C.c_void_p), ("nk", C.c_void_p), ("kfrac", C.c_void_p), ("kw", C.c_void_p),
+                ("spec_norb", C.c_void_p), ("nspec", C.c_int), ("cutoff", C.c_double), ("ld", C.c_int)]
+
+
+class SlkPlanOut(C.Structure):
+    _fields_ = [("off", C.c_longlong * PLAN_SECTIONS), ("bytes", C.c_longlong * PLAN_SECTIONS),
+                ("total", C.c_longlong), ("n_work", C.c_int), ("n_trans", C.c_int), ("n_mat", C.c_int),
+                ("ld", C.c_int)]
+
+
 # name -> (restype, argtypes)
 SIGNATURES = {
+    "slk_plan_size": (C.c_int, [C.POINTER(SlkPlanIn), C.POINTER(SlkPlanOut)]),
+    "slk_plan_fill": (C.c_int, [C.POINTER(SlkPlanIn), C.POINTER(SlkPlanOut), C.c_void_p, C.c_void_p, C.c_void_p]),
     "slk_build_hs_launch": (C.c_int, [C.POINTER(SlkBuildArgs), C.c_void_p]),
     "slk_pad_launch": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p,
                                  C.c_double, C.c_int, C.c_void_p]),
@@ -109,6 +126,17 @@ def load_library(path: str = LIB_PATH):
         fn.restype = res
         fn.argtypes = args
     return lib
+
+
+def host_lib():
+    """The library for its host-only entry points (no GPU needed)."""
+    global _HOST
+    if _HOST is None:
+        _HOST = load_library()
+    return _HOST
+
+
+_HOST = None
 
 
 def lib():

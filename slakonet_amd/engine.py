@@ -90,84 +90,58 @@ def column_tiles(norb_atoms: np.ndarray):
 
 def plan_build(tables: TableSet, inputs: Sequence[StructureInput], ld: Optional[int] = None,
                cutoff: float = HS_CUTOFF_BOHR) -> BuildPlan:
+    """Plan one batch: the native planner (slk_plan_size / slk_plan_fill)
+    computes translations, column tiles and work items and packs everything
+    into one pinned blob, uploaded with a single stream-ordered copy."""
     device = tables.device
-    structs = np.zeros(len(inputs), dtype=STRUCT_DT)
-    pos_l, spec_l, orb_l, tc_l, ti_l, k_l, kw_l, work_l = [], [], [], [], [], [], [], []
-    n_orb, nks, mat0, nelec = [], [], [], []
-    atom0 = t0 = k0 = m0 = 0
-    for s, inp in enumerate(inputs):
-        z = np.asarray(inp.z, dtype=np.int64)
+    zs = [np.asarray(inp.z, dtype=np.int64).reshape(-1) for inp in inputs]
+    for z in zs:
         tables.check_pairs(z)
-        sp = tables.species_index(z)
-        norb_atoms = tables.norb[sp]
-        orb = np.concatenate([[0], np.cumsum(norb_atoms)[:-1]]).astype(np.int32)
-        n = int(norb_atoms.sum())
-        pos = np.asarray(inp.positions, dtype=np.float64)
-        cell = np.asarray(inp.cell, dtype=np.float64)
-        ti, tcart = lattice_translations(cell, pos, cutoff)
-        kp = np.asarray(inp.kpoints, dtype=np.float64).reshape(-1, 3)
-        nk = len(kp)
-        na = len(z)
-        structs[s] = (atom0, na, n, 0, m0, k0, nk, t0, len(ti))
-        tiles = column_tiles(norb_atoms)
-        nkc = (nk + KC - 1) // KC
-        ii, tt, kk = np.meshgrid(np.arange(na), np.arange(len(tiles)), np.arange(nkc), indexing="ij")
-        ii, tt, kk = ii.ravel(), tt.ravel(), kk.ravel()
-        w = np.zeros(len(ii), dtype=WORK_DT)
-        w["s"] = s
-        w["i"] = ii
-        w["j0"] = tiles[tt, 0]
-        w["j1"] = tiles[tt, 1]
-        w["c0"] = tiles[tt, 2]
-        w["ncol"] = tiles[tt, 3]
-        w["kc0"] = kk * KC
-        w["nkc"] = np.minimum(KC, nk - kk * KC)
-        work_l.append(w)
-        pos_l.append(pos)
-        spec_l.append(sp)
-        orb_l.append(orb)
-        tc_l.append(tcart)
-        ti_l.append(ti)
-        k_l.append(kp)
-        kw_l.append(np.asarray(inp.k_weights, dtype=np.float64).reshape(-1))
-        n_orb.append(n)
-        nks.append(nk)
-        mat0.append(m0)
-        nelec.append(inp.nelec if inp.nelec is not None else tables.n_electrons(z))
-        atom0 += na
-        t0 += len(ti)
-        k0 += nk
-        m0 += nk
-    ld = int(max(n_orb)) if ld is None else int(ld)
-    if ld < max(n_orb):
+    spec = tables.species_index(np.concatenate(zs)) if zs else np.zeros(0, np.int32)
+    n_atoms = np.array([len(z) for z in zs], dtype=np.int32)
+    pos = np.ascontiguousarray(np.concatenate([np.asarray(i.positions, np.float64).reshape(-1, 3) for i in inputs]))
+    cell = np.ascontiguousarray(np.stack([np.asarray(i.cell, np.float64).reshape(3, 3) for i in inputs]))
+    kps = [np.asarray(i.kpoints, np.float64).reshape(-1, 3) for i in inputs]
+    nk = np.array([len(k) for k in kps], dtype=np.int32)
+    kfrac = np.ascontiguousarray(np.concatenate(kps))
+    kw = np.ascontiguousarray(np.concatenate([np.asarray(i.k_weights, np.float64).reshape(-1) for i in inputs]))
+    if len(kw) != len(kfrac):
+        raise ValueError("k_weights and kpoints differ in length")
+    spec = np.ascontiguousarray(spec, dtype=np.int32)
+    norb = np.ascontiguousarray(tables.norb, dtype=np.int32)
+    host = _lib.host_lib()
+    pin = _lib.SlkPlanIn(len(inputs), n_atoms.ctypes.data, spec.ctypes.data, pos.ctypes.data, cell.ctypes.data,
+                         nk.ctypes.data, kfrac.ctypes.data, kw.ctypes.data, norb.ctypes.data, len(norb),
+                         float(cutoff), int(ld or 0))
+    out = _lib.SlkPlanOut()
+    rc = host.slk_plan_size(C.byref(pin), C.byref(out))
+    if rc == 2:
         raise ValueError("ld smaller than the largest structure")
-    structs["ld"] = ld
-    work = np.concatenate(work_l)
-    # one pinned staging buffer, one H2D copy, device views per array
-    arrays = {
-        "structs": np.ascontiguousarray(structs).view(np.uint8).reshape(-1),
-        "work": np.ascontiguousarray(work).view(np.uint8).reshape(-1),
-        "atom_pos": np.concatenate(pos_l).reshape(-1).view(np.uint8),
-        "atom_spec": np.concatenate(spec_l).astype(np.int32).view(np.uint8),
-        "atom_orb": np.concatenate(orb_l).astype(np.int32).view(np.uint8),
-        "trans_cart": np.ascontiguousarray(np.concatenate(tc_l).reshape(-1)).view(np.uint8),
-        "trans_int": np.concatenate(ti_l).astype(np.int32).reshape(-1).view(np.uint8),
-        "kfrac": np.ascontiguousarray(np.concatenate(k_l).reshape(-1)).view(np.uint8),
-        "kw": np.ascontiguousarray(np.concatenate(kw_l)).view(np.uint8),
-    }
-    offs, tot = {}, 0
-    for k, a in arrays.items():
-        offs[k] = tot
-        tot += (a.nbytes + 255) // 256 * 256
-    stage = np.zeros(tot, dtype=np.uint8)
-    for k, a in arrays.items():
-        stage[offs[k]:offs[k] + a.nbytes] = a
-    blob = _pinned(stage).to(device, non_blocking=True)
-    views = {k: blob[offs[k]:offs[k] + arrays[k].nbytes] for k in arrays}
+    if rc != 0:
+        raise ValueError("invalid structure in the batch (no atoms, no k-points or a singular cell)")
+    stage = _pinned_bytes(int(out.total))
+    n_orb = np.zeros(len(inputs), dtype=np.int32)
+    mat0 = np.zeros(len(inputs), dtype=np.int64)
+    rc = host.slk_plan_fill(C.byref(pin), C.byref(out), C.c_void_p(stage.data_ptr()), n_orb.ctypes.data,
+                            mat0.ctypes.data)
+    if rc != 0:
+        raise RuntimeError("slk_plan_fill failed")
+    blob = stage.to(device, non_blocking=True)
+    names = ["structs", "work", "atom_pos", "atom_spec", "atom_orb", "trans_cart", "trans_int", "kfrac", "kw"]
+    views = {k: blob[out.off[q]:out.off[q] + out.bytes[q]] for q, k in enumerate(names)}
     dev = {k: v for k, v in views.items() if k != "kw"}
     dev["blob"] = blob
-    kw = views["kw"].view(torch.float64)
-    return BuildPlan(len(inputs), m0, ld, n_orb, nks, mat0, kw, nelec, dev, len(work))
+    dev["stage"] = stage
+    kwd = views["kw"].view(torch.float64)
+    nelec = [inp.nelec if inp.nelec is not None else tables.n_electrons(z) for inp, z in zip(inputs, zs)]
+    return BuildPlan(len(inputs), int(out.n_mat), int(out.ld), n_orb.tolist(), nk.tolist(), mat0.tolist(), kwd,
+                     nelec, dev, int(out.n_work))
+
+
+def _pinned_bytes(n: int) -> torch.Tensor:
+    if torch.cuda.is_available():
+        return torch.empty(max(n, 1), dtype=torch.uint8, pin_memory=True)
+    return torch.empty(max(n, 1), dtype=torch.uint8)
 
 
 # set while the library's live kernel sampler is enabled (slk_sampler_reset)

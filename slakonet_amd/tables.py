@@ -68,6 +68,9 @@ class TableSet:
         zs = sorted({z for pair in self.skfs for z in pair})
         self.species = zs
         self.index = {z: i for i, z in enumerate(zs)}
+        self._lut = -np.ones(max(zs + [0]) + 1, dtype=np.int32)
+        for z, i in self.index.items():
+            self._lut[z] = i
         self.tail = float(tail)
         ns = len(zs)
         self.pairs = sorted(self.skfs)
@@ -150,10 +153,14 @@ class TableSet:
         return self.tables
 
     def species_index(self, z: Iterable[int]) -> np.ndarray:
-        try:
-            return np.array([self.index[int(v)] for v in z], dtype=np.int32)
-        except KeyError as e:
-            raise KeyError(f"no SK tables for species Z={e.args[0]}") from None
+        z = np.asarray(z, dtype=np.int64).reshape(-1)
+        if len(z) and (z.min() < 0 or z.max() >= len(self._lut)):
+            bad = z[(z < 0) | (z >= len(self._lut))][0]
+            raise KeyError(f"no SK tables for species Z={int(bad)}")
+        idx = self._lut[z]
+        if len(idx) and idx.min() < 0:
+            raise KeyError(f"no SK tables for species Z={int(z[np.argmin(idx)])}")
+        return idx
 
     def check_pairs(self, z: Iterable[int]):
         present = sorted({int(v) for v in z})

@@ -171,3 +171,39 @@ def test_skf_text_round_trip_and_repeat_notation(tmp_path):
     dd = skf_io.read_skf_directory(str(tmp_path), ["Si", "C"])
     assert set(dd) == {(14, 14), (6, 14)}
     assert skf_io._numbers("3*1.5, 2") .tolist() == [1.5, 1.5, 1.5, 2.0]
+
+
+def test_native_planner_matches_python_bookkeeping():
+    """slk_plan_size / slk_plan_fill (host C++) against the numpy statement of
+    the same bookkeeping: translations (x slowest), column tiles, work items."""
+    from slakonet_amd.tables import TableSet
+    sts = [synth.stream_structure(77, i, n_max=40) for i in range(6)]
+    species = sorted({int(z) for st in sts for z in st.z})
+    tab = TableSet(synth.make_skf_set(species, seed=1), synth.shell_dict_spd(species), device="cpu")
+    inputs = [engine.StructureInput(st.z, st.positions * AA_BOHR, st.cell * AA_BOHR, st.kpoints, st.k_weights)
+              for st in sts]
+    plan = engine.plan_build(tab, inputs)
+    d = plan.dev
+    structs = np.frombuffer(d["structs"].numpy().tobytes(), dtype=engine.STRUCT_DT)
+    work = np.frombuffer(d["work"].numpy().tobytes(), dtype=engine.WORK_DT)
+    tint = np.frombuffer(d["trans_int"].numpy().tobytes(), dtype=np.int32).reshape(-1, 3)
+    tcart = np.frombuffer(d["trans_cart"].numpy().tobytes(), dtype=np.float64).reshape(-1, 3)
+    w0 = 0
+    for s, inp in enumerate(inputs):
+        ti, tc = lattice_translations(inp.cell, inp.positions, 10.0)
+        sd = structs[s]
+        assert sd["nt"] == len(ti) and sd["nk"] == len(inp.kpoints)
+        assert np.array_equal(tint[sd["t0"]:sd["t0"] + sd["nt"]], ti)
+        assert np.allclose(tcart[sd["t0"]:sd["t0"] + sd["nt"]], tc, rtol=0, atol=1e-12)
+        norb_atoms = tab.norb[tab.species_index(inp.z)]
+        assert sd["n_orb"] == norb_atoms.sum() == plan.n_orb[s]
+        tiles = engine.column_tiles(norb_atoms)
+        nkc = (len(inp.kpoints) + engine.KC - 1) // engine.KC
+        ii, tt, kk = np.meshgrid(np.arange(len(inp.z)), np.arange(len(tiles)), np.arange(nkc), indexing="ij")
+        ws = work[w0:w0 + ii.size]
+        assert np.array_equal(ws["i"], ii.ravel()) and np.array_equal(ws["kc0"], kk.ravel() * engine.KC)
+        assert np.array_equal(ws["j0"], tiles[tt.ravel(), 0]) and np.array_equal(ws["ncol"], tiles[tt.ravel(), 3])
+        w0 += ii.size
+    assert w0 == plan.n_work == len(work)
+    with pytest.raises(ValueError):
+        engine.plan_build(tab, inputs, ld=8)
