@@ -32,11 +32,14 @@ void launch_trinv_blocked(hipStream_t st, const double2* L, double2* W, double2*
 // Blocked Householder tridiagonalisation of the Hermitian C (in place): d, e
 // (real tridiagonal), tau, reflectors in the strict lower part of C.
 // Workspace: Vc, Wc [NB][ld], Y [ld], Yp [hetrd_partials_elems(ld)], C12
-// [2 NB] per matrix. D, E are FP64 for both precisions.
+// [2 NB] per matrix. D, E are FP64 for both precisions. VT (FP32, nullable):
+// reflector j is also written as row j of VT (contiguous, for single-vector
+// back-transforms).
 void launch_hetrd_blocked(hipStream_t st, double2* C, int n, int ld, int batch, double* D, double* E, double2* TAU,
                           double2* Vc, double2* Wc, double2* Y, double2* Yp, double2* C12);
 void launch_hetrd_blocked_f32(hipStream_t st, float2* C, int n, int ld, int batch, double* D, double* E,
-                              float2* TAU, float2* Vc, float2* Wc, float2* Y, float2* Yp, float2* C12);
+                              float2* TAU, float2* Vc, float2* Wc, float2* Y, float2* Yp, float2* C12,
+                              float2* VT);
 size_t hetrd_partials_elems(int ld);
 
 // complex128 -> complex64 copy of `count` elements.

@@ -343,7 +343,7 @@ struct RefineCtx {
   const double2* H;
   const double2* S;
   const double2* Wm;        // W = L^-1, FP64
-  const float2* R;          // FP32 reflectors (strict lower part)
+  const float2* R;          // FP32 reflectors as rows (row j = reflector j, entries r > j)
   const float2* TAU;
   const double* D;
   const double* E;
@@ -355,38 +355,41 @@ struct RefineCtx {
   int* meta;                // [n_spec * 2][2]: band (-1: skip), matrix index
   double* part;             // [n_spec * 2][nblk][2]: partial num, den
   int nblk;
+  int lds_x;                // 1: refine_tri was launched with ld complex doubles of LDS for x
 };
 
 __device__ __forceinline__ double* rf_base(const RefineCtx& c, int v) {
   return c.scratch + (size_t)v * RF_SCR * c.ld;
 }
 
-__global__ __launch_bounds__(RF_THREADS) void refine_tri_kernel(RefineCtx c) {
+// One wave per (structure, edge). Lane 0 runs the O(m) tridiagonal solves
+// with loads issued RF_CH at a time; the wave then applies the reflectors
+// (rows of VT, contiguous) with wave reductions only. x lives in LDS when
+// the padded dimension ld <= RF_LDS_X, else in the scratch slot.
+#define RF_CH 8
+#define RF_LDS_X 2048
+
+__global__ __launch_bounds__(64) void refine_tri_kernel(RefineCtx c) {
   const int s = blockIdx.x, edge = blockIdx.y, v = s * 2 + edge;
   const SlkSpectrum sp = c.spec[s];
   const double* row = c.edges + (size_t)s * SLK_EDGE_FIELDS;
-  const int tid = threadIdx.x;
+  const int lane = threadIdx.x;
   const int ld = c.ld;
-  __shared__ double red[RF_THREADS / 64];
-  __shared__ int s_band;
+  extern __shared__ double2 s_x[];
   if (row[E_NVAL] <= 0.0 || row[E_NCON] <= 0.0) {
-    if (tid == 0) c.meta[2 * v] = -1;
+    if (lane == 0) c.meta[2 * v] = -1;
     return;
   }
   const double mu = row[E_MU];
   const int k = (int)row[edge == 0 ? E_VBMK : E_CBMK];
   const int m = sp.nb;
   const double* wk = c.w + sp.w0 + (size_t)k * sp.wld;
-  if (tid == 0) {
-    int cnt = 0;
-    for (int b = 0; b < m; ++b)
-      if (wk[b] * H2E_OUT < mu * H2E_OUT) ++cnt;
-    s_band = edge == 0 ? cnt - 1 : cnt;
-  }
-  __syncthreads();
-  const int band = s_band;
+  int cnt = 0;
+  for (int b = lane; b < m; b += 64) cnt += (wk[b] * H2E_OUT < mu * H2E_OUT) ? 1 : 0;
+  cnt = wave_sum(cnt);
+  const int band = edge == 0 ? cnt - 1 : cnt;
   const size_t mat = (size_t)sp.kw0 + k;
-  if (tid == 0) {
+  if (lane == 0) {
     c.meta[2 * v] = (band < 0 || band >= m) ? -1 : band;
     c.meta[2 * v + 1] = (int)mat;
   }
@@ -398,90 +401,161 @@ __global__ __launch_bounds__(RF_THREADS) void refine_tri_kernel(RefineCtx c) {
   double *dl = base, *dd = base + ld, *du = base + 2 * (size_t)ld, *du2 = base + 3 * (size_t)ld;
   double* piv = base + 4 * (size_t)ld;
   double* y = base + 5 * (size_t)ld;
-  double2* x = reinterpret_cast<double2*>(base + 6 * (size_t)ld);
+  double2* x = c.lds_x ? s_x : reinterpret_cast<double2*>(base + 6 * (size_t)ld);
 
-  // inverse iteration on T[0:m) - lam I (one lane)
-  if (tid == 0) {
-    double tn = 0.0;
-    for (int i = 0; i < m; ++i) {
-      const double a = fabs(d[i]) + (i > 0 ? fabs(e[i - 1]) : 0.0) + (i + 1 < m ? fabs(e[i]) : 0.0);
-      tn = fmax(tn, a);
-    }
-    const double tiny = 2.2e-16 * fmax(tn, 1e-300);
-    for (int i = 0; i < m; ++i) {
-      dd[i] = d[i] - lam;
-      du2[i] = 0.0;
-      if (i + 1 < m) { dl[i] = e[i]; du[i] = e[i]; }
-    }
-    for (int i = 0; i + 1 < m; ++i) {
-      if (fabs(dd[i]) >= fabs(dl[i])) {
-        if (dd[i] == 0.0) dd[i] = tiny;
-        const double f = dl[i] / dd[i];
-        dl[i] = f;
-        dd[i + 1] -= f * du[i];
-        piv[i] = 0.0;
-      } else {
-        const double f = dd[i] / dl[i];
-        dd[i] = dl[i];
-        dl[i] = f;
-        const double t = du[i];
-        du[i] = dd[i + 1];
-        dd[i + 1] = t - f * dd[i + 1];
-        if (i + 2 < m) {
-          du2[i] = du[i + 1];
-          du[i + 1] = -f * du[i + 1];
-        }
-        piv[i] = 1.0;
-      }
-    }
-    if (dd[m - 1] == 0.0) dd[m - 1] = tiny;
-    for (int i = 0; i < m; ++i) y[i] = 1.0 + 0.37 * sin(0.7 * i + 0.1);
-    for (int it = 0; it < 3; ++it) {
-      for (int i = 0; i + 1 < m; ++i) {
-        if (piv[i] == 0.0) {
-          y[i + 1] -= dl[i] * y[i];
-        } else {
-          const double t = y[i];
-          y[i] = y[i + 1];
-          y[i + 1] = t - dl[i] * y[i];
-        }
-      }
-      y[m - 1] /= dd[m - 1];
-      if (m > 1) y[m - 2] = (y[m - 2] - du[m - 2] * y[m - 1]) / dd[m - 2];
-      for (int i = m - 3; i >= 0; --i) y[i] = (y[i] - du[i] * y[i + 1] - du2[i] * y[i + 2]) / dd[i];
-      double nr = 0.0;
-      for (int i = 0; i < m; ++i) nr += y[i] * y[i];
-      nr = 1.0 / sqrt(nr);
-      for (int i = 0; i < m; ++i) y[i] *= nr;
-    }
+  // copies with the whole wave: dd = d - lam, dl = du = e, du2 = 0, tn = max row sum
+  double tn = 0.0;
+  for (int i = lane; i < m; i += 64) {
+    const double di = d[i];
+    const double ei = i + 1 < m ? e[i] : 0.0;
+    const double ep = i > 0 ? e[i - 1] : 0.0;
+    tn = fmax(tn, fabs(di) + fabs(ei) + fabs(ep));
+    dd[i] = di - lam;
+    dl[i] = ei;
+    du[i] = ei;
+    du2[i] = 0.0;
+    y[i] = 1.0 + 0.37 * sin(0.7 * i + 0.1);
   }
-  __syncthreads();
-  // x = Q y, reflectors last to first: x -= v tau (v^H x)
-  for (int r = tid; r < ld; r += RF_THREADS) x[r] = make_double2(r < m ? y[r] : 0.0, 0.0);
-  __syncthreads();
-  const float2* a = c.R + mat * ld * ld;
+  tn = wave_max(tn);
+  __threadfence_block();
+  if (lane == 0) {
+    const double tiny = 2.2e-16 * fmax(tn, 1e-300);
+    // LU with partial pivoting (dgttrf); the diagonal element carried in a register
+    double dcur = dd[0];
+    for (int i0 = 0; i0 + 1 < m; i0 += RF_CH) {
+      double rdl[RF_CH], rdu[RF_CH], rdu1[RF_CH], rdn[RF_CH];
+#pragma unroll
+      for (int q = 0; q < RF_CH; ++q) {
+        const int i = i0 + q;
+        rdl[q] = i + 1 < m ? dl[i] : 0.0;
+        rdu[q] = i + 1 < m ? du[i] : 0.0;
+        rdu1[q] = i + 2 < m ? du[i + 1] : 0.0;
+        rdn[q] = i + 1 < m ? dd[i + 1] : 0.0;
+      }
+#pragma unroll
+      for (int q = 0; q < RF_CH; ++q) {
+        const int i = i0 + q;
+        if (i + 1 >= m) break;
+        double dnext = rdn[q];
+        const double dui1 = q + 1 < RF_CH ? rdu[q + 1] : rdu1[q];
+        if (fabs(dcur) >= fabs(rdl[q])) {
+          if (dcur == 0.0) dcur = tiny;
+          const double f = rdl[q] / dcur;
+          dl[i] = f;
+          dd[i] = dcur;
+          dnext -= f * rdu[q];
+          piv[i] = 0.0;
+        } else {
+          const double f = dcur / rdl[q];
+          dd[i] = rdl[q];
+          dl[i] = f;
+          du[i] = dnext;
+          dnext = rdu[q] - f * dnext;
+          if (i + 2 < m) {
+            du2[i] = dui1;
+            du[i + 1] = -f * dui1;
+            if (q + 1 < RF_CH) rdu[q + 1] = -f * dui1; else rdu1[q] = -f * dui1;
+          }
+          piv[i] = 1.0;
+        }
+        dcur = dnext;
+      }
+    }
+    if (dcur == 0.0) dcur = tiny;
+    dd[m - 1] = dcur;
+  }
+  __threadfence_block();
+  for (int it = 0; it < 2; ++it) {
+    if (lane == 0) {
+      // forward: apply the row interchanges and multipliers
+      double ycur = y[0];
+      for (int i0 = 0; i0 + 1 < m; i0 += RF_CH) {
+        double rl[RF_CH], rp[RF_CH], ry[RF_CH];
+#pragma unroll
+        for (int q = 0; q < RF_CH; ++q) {
+          const int i = i0 + q;
+          rl[q] = i + 1 < m ? dl[i] : 0.0;
+          rp[q] = i + 1 < m ? piv[i] : 0.0;
+          ry[q] = i + 1 < m ? y[i + 1] : 0.0;
+        }
+#pragma unroll
+        for (int q = 0; q < RF_CH; ++q) {
+          const int i = i0 + q;
+          if (i + 1 >= m) break;
+          double ynext = ry[q];
+          if (rp[q] == 0.0) {
+            ynext -= rl[q] * ycur;
+            y[i] = ycur;
+          } else {
+            y[i] = ynext;
+            ynext = ycur - rl[q] * ynext;
+          }
+          ycur = ynext;
+        }
+      }
+      y[m - 1] = ycur;
+      // back substitution with U (diagonal dd, super du, du2)
+      double y1 = 0.0, y2 = 0.0;   // y[i+1], y[i+2]
+      for (int i0 = m - 1; i0 >= 0; i0 -= RF_CH) {
+        double rd[RF_CH], ru[RF_CH], ru2[RF_CH], ry[RF_CH];
+#pragma unroll
+        for (int q = 0; q < RF_CH; ++q) {
+          const int i = i0 - q;
+          rd[q] = i >= 0 ? dd[i] : 1.0;
+          ru[q] = (i >= 0 && i + 1 < m) ? du[i] : 0.0;
+          ru2[q] = (i >= 0 && i + 2 < m) ? du2[i] : 0.0;
+          ry[q] = i >= 0 ? y[i] : 0.0;
+        }
+#pragma unroll
+        for (int q = 0; q < RF_CH; ++q) {
+          const int i = i0 - q;
+          if (i < 0) break;
+          const double yi = (ry[q] - ru[q] * y1 - ru2[q] * y2) / rd[q];
+          y[i] = yi;
+          y2 = y1;
+          y1 = yi;
+        }
+      }
+    }
+    __threadfence_block();
+    double nr = 0.0;
+    for (int i = lane; i < m; i += 64) nr += y[i] * y[i];
+    nr = 1.0 / sqrt(wave_sum(nr));
+    for (int i = lane; i < m; i += 64) y[i] *= nr;
+    __threadfence_block();
+  }
+  // x = Q y, reflectors last to first: x -= v tau (v^H x), v = row j of VT
+  for (int r = lane; r < m; r += 64) x[r] = make_double2(y[r], 0.0);
+  __threadfence_block();
+  const float2* vt = c.R + mat * ld * ld;
   const float2* tau = c.TAU + mat * ld;
   for (int j = m - 2; j >= 0; --j) {
     const float2 tf = tau[j];
     if (tf.x == 0.0f && tf.y == 0.0f) continue;
+    const float2* vr = vt + (size_t)j * ld;
     double px = 0.0, py = 0.0;
-    for (int r = j + 1 + tid; r < m; r += RF_THREADS) {
-      const float2 vf = r == j + 1 ? make_float2(1.0f, 0.0f) : a[(size_t)r * ld + j];
+    for (int r = j + 1 + lane; r < m; r += 64) {
+      const float2 vf = vr[r];
       const double2 xv = x[r];
       px += (double)vf.x * xv.x + (double)vf.y * xv.y;
       py += (double)vf.x * xv.y - (double)vf.y * xv.x;
     }
-    px = block_sum(px, red);
-    py = block_sum(py, red);
+    px = wave_sum(px);
+    py = wave_sum(py);
     const double sx = (double)tf.x * px - (double)tf.y * py, sy = (double)tf.x * py + (double)tf.y * px;
-    for (int r = j + 1 + tid; r < m; r += RF_THREADS) {
-      const float2 vf = r == j + 1 ? make_float2(1.0f, 0.0f) : a[(size_t)r * ld + j];
+    for (int r = j + 1 + lane; r < m; r += 64) {
+      const float2 vf = vr[r];
       double2 xv = x[r];
       xv.x -= (double)vf.x * sx - (double)vf.y * sy;
       xv.y -= (double)vf.x * sy + (double)vf.y * sx;
       x[r] = xv;
     }
-    __syncthreads();
+    __threadfence_block();   // x[r] written by one lane is read by another for the next reflector
+  }
+  if (c.lds_x) {
+    // hand x to the next kernel through the scratch slot
+    double2* xg = reinterpret_cast<double2*>(base + 6 * (size_t)ld);
+    for (int r = lane; r < m; r += 64) xg[r] = x[r];
   }
 }
 
@@ -704,7 +778,9 @@ static int eigh_impl(const void* H, const void* S, int n, int ld, int batch, dou
     launch_cgemm(stream, batch, n, n, n, 1.0f, s.F2, ld, sM, OP_N, s.F0, ld, sM, OP_C, 0.0f, s.F1, ld, sM,
                  KLIM_B_UPPER, 0);
     if (ev) (void)hipEventRecord(ev[3], stream);
-    launch_hetrd_blocked_f32(stream, s.F1, n, ld, batch, s.D, s.E, s.fTAU, s.fVc, s.fWc, s.fY, s.fYp, s.fC12);
+    // F0 (W in FP32) is free after the reduction: it receives the reflector rows
+    launch_hetrd_blocked_f32(stream, s.F1, n, ld, batch, s.D, s.E, s.fTAU, s.fVc, s.fWc, s.fY, s.fYp, s.fC12,
+                             s.F0);
   }
   if (ev) (void)hipEventRecord(ev[4], stream);
   if (V == nullptr) {
@@ -784,7 +860,7 @@ int slk_refine_edges_mixed(const void* H, const void* S, int n, int ld, int batc
   c.H = reinterpret_cast<const double2*>(H);
   c.S = reinterpret_cast<const double2*>(S);
   c.Wm = s.B1;
-  c.R = s.F1;
+  c.R = s.F0;
   c.TAU = s.fTAU;
   c.D = s.D;
   c.E = s.E;
@@ -796,7 +872,9 @@ int slk_refine_edges_mixed(const void* H, const void* S, int n, int ld, int batc
   c.nblk = (ld + RF_BLK - 1) / RF_BLK;
   c.part = scratch + (size_t)n_spec * 2 * RF_SCR * ld;
   c.meta = reinterpret_cast<int*>(c.part + (size_t)n_spec * 2 * c.nblk * 2);
-  hipLaunchKernelGGL(refine_tri_kernel, dim3(n_spec, 2), dim3(RF_THREADS), 0, stream, c);
+  // x in LDS when every structure fits (m <= ld <= RF_LDS_X), else in the scratch slot
+  c.lds_x = ld <= RF_LDS_X ? 1 : 0;
+  hipLaunchKernelGGL(refine_tri_kernel, dim3(n_spec, 2), dim3(64), c.lds_x ? ld * sizeof(double2) : 0, stream, c);
   hipLaunchKernelGGL(refine_wh_kernel, dim3(c.nblk, n_spec * 2), dim3(RF_THREADS), 0, stream, c);
   hipLaunchKernelGGL(refine_rq_kernel, dim3(c.nblk, n_spec * 2), dim3(RF_THREADS), 0, stream, c);
   hipLaunchKernelGGL(refine_commit_kernel, dim3((n_spec + 63) / 64), dim3(64), 0, stream, c, w, edges, n_spec);

@@ -45,6 +45,7 @@ struct HetrdCtx {
   C2* A;
   int n, ld;
   C2 *Vc, *Wc, *Y, *Yp, *C12, *TAU;
+  C2* VT;      // optional: reflector j also stored as row j (VT[j*ld + r], r > j)
   double *D, *E;
   int ncmax;   // partial-sum chunks per matrix (>= ceil(ld / HT_COLS))
 };
@@ -141,6 +142,7 @@ __device__ void latrd_column(const HetrdCtx<T>& c, int b, int j, int jj, T* red)
     else if (r > j + 1) v = s_skip ? mk<T>(0, 0) : c_mul(a[(size_t)r * ld + j], scale);
     vc[(size_t)jj * ld + r] = v;
     if (r > j) a[(size_t)r * ld + j] = v;
+    if (c.VT != nullptr && r > j) c.VT[(size_t)b * ld * ld + (size_t)j * ld + r] = v;
   }
   __syncthreads();
   const int lane = tid & 63, wv = tid >> 6, nw = blockDim.x >> 6;
@@ -313,9 +315,10 @@ void rank2k<float>(hipStream_t st, int batch, int m, int k, float2* Vc, float2* 
 template <typename T>
 static void hetrd_impl(hipStream_t st, typename C2T<T>::t* C, int n, int ld, int batch, double* D, double* E,
                        typename C2T<T>::t* TAU, typename C2T<T>::t* Vc, typename C2T<T>::t* Wc,
-                       typename C2T<T>::t* Y, typename C2T<T>::t* Yp, typename C2T<T>::t* C12) {
+                       typename C2T<T>::t* Y, typename C2T<T>::t* Yp, typename C2T<T>::t* C12,
+                       typename C2T<T>::t* VT) {
   HetrdCtx<T> c;
-  c.A = C; c.n = n; c.ld = ld; c.Vc = Vc; c.Wc = Wc; c.Y = Y; c.Yp = Yp; c.C12 = C12; c.TAU = TAU;
+  c.A = C; c.n = n; c.ld = ld; c.Vc = Vc; c.Wc = Wc; c.Y = Y; c.Yp = Yp; c.C12 = C12; c.TAU = TAU; c.VT = VT;
   c.D = D; c.E = E; c.ncmax = (ld + HT_TS - 1) / HT_TS;
   const long long sP = (long long)NB * ld, sM = (long long)ld * ld;
   for (int j0 = 0; j0 < n - 1; j0 += NB) {
@@ -342,10 +345,11 @@ static void hetrd_impl(hipStream_t st, typename C2T<T>::t* C, int n, int ld, int
 
 void launch_hetrd_blocked(hipStream_t st, double2* C, int n, int ld, int batch, double* D, double* E, double2* TAU,
                           double2* Vc, double2* Wc, double2* Y, double2* Yp, double2* C12) {
-  hetrd_impl<double>(st, C, n, ld, batch, D, E, TAU, Vc, Wc, Y, Yp, C12);
+  hetrd_impl<double>(st, C, n, ld, batch, D, E, TAU, Vc, Wc, Y, Yp, C12, nullptr);
 }
 
 void launch_hetrd_blocked_f32(hipStream_t st, float2* C, int n, int ld, int batch, double* D, double* E,
-                              float2* TAU, float2* Vc, float2* Wc, float2* Y, float2* Yp, float2* C12) {
-  hetrd_impl<float>(st, C, n, ld, batch, D, E, TAU, Vc, Wc, Y, Yp, C12);
+                              float2* TAU, float2* Vc, float2* Wc, float2* Y, float2* Yp, float2* C12,
+                              float2* VT) {
+  hetrd_impl<float>(st, C, n, ld, batch, D, E, TAU, Vc, Wc, Y, Yp, C12, VT);
 }

@@ -71,7 +71,7 @@ class ServingEngine:
     def __init__(self, tables: TableSet, dos_grid: Optional[torch.Tensor] = None, n_streams: int = 4,
                  max_inflight: int = 8, hbm_budget_bytes: float = 64e9, max_batch_matrices: int = 512,
                  fermi_mode: int = 0, keep_outputs: bool = False, precision: str = "mixed",
-                 refine_edges: bool = True):
+                 refine_edges: bool = True, bucket=bucket_dim):
         self.tables = tables
         self.dev = tables.device
         self.dev_index = self.dev.index if self.dev.index is not None else torch.cuda.current_device()
@@ -87,6 +87,7 @@ class ServingEngine:
         self.keep_outputs = keep_outputs
         self.precision = precision
         self.refine_edges = refine_edges
+        self.bucket = bucket
         self.nf = len(engine.EDGE_FIELDS)
         self._rr = 0
 
@@ -185,7 +186,7 @@ class ServingEngine:
                 i = pending.popleft()
                 inp = inputs[i]
                 no = self._orbitals(inp)
-                ld = bucket_dim(no)
+                ld = self.bucket(no)
                 queues.setdefault(ld, deque()).append(Request(i, inp, no, float(arrivals[i]), ld))
             progressed = False
             for item in list(inflight):
