@@ -31,8 +31,8 @@ void launch_trinv_blocked(hipStream_t st, const double2* L, double2* W, double2*
 
 // Blocked Householder tridiagonalisation of the Hermitian C (in place): d, e
 // (real tridiagonal), tau, reflectors in the strict lower part of C.
-// Workspace: Vc, Wc [NB][ld], Y [ld], Yp [hetrd_partials_elems(ld)], C12
-// [2 NB] per matrix. D, E are FP64 for both precisions. VT (FP32, nullable):
+// Workspace: Vc, Wc [NB][ld]; Y [batch][2 ld] (y, then the next column);
+// Yp [hetrd_partials_elems(ld)] and C12 [hetrd_c12_elems(ld)] per matrix. D, E are FP64 for both precisions. VT (FP32, nullable):
 // reflector j is also written as row j of VT (contiguous, for single-vector
 // back-transforms).
 void launch_hetrd_blocked(hipStream_t st, double2* C, int n, int ld, int batch, double* D, double* E, double2* TAU,
@@ -41,6 +41,7 @@ void launch_hetrd_blocked_f32(hipStream_t st, float2* C, int n, int ld, int batc
                               float2* TAU, float2* Vc, float2* Wc, float2* Y, float2* Yp, float2* C12,
                               float2* VT);
 size_t hetrd_partials_elems(int ld);
+size_t hetrd_c12_elems(int ld);
 
 // complex128 -> complex64 copy of `count` elements.
 void launch_to_c64(hipStream_t st, const double2* src, float2* dst, size_t count);

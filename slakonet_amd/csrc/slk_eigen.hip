@@ -699,9 +699,9 @@ static size_t carve(char* base, int ld, int batch, int vectors, int mixed, EighW
     w->B2 = reinterpret_cast<double2*>(take(matel * batch * sizeof(double2)));
     w->Vc = reinterpret_cast<double2*>(take((size_t)SLK_NB * ld * batch * sizeof(double2)));
     w->Wc = reinterpret_cast<double2*>(take((size_t)SLK_NB * ld * batch * sizeof(double2)));
-    w->Y = reinterpret_cast<double2*>(take((size_t)ld * batch * sizeof(double2)));
+    w->Y = reinterpret_cast<double2*>(take((size_t)2 * ld * batch * sizeof(double2)));
     w->Yp = reinterpret_cast<double2*>(take(hetrd_partials_elems(ld) * batch * sizeof(double2)));
-    w->C12 = reinterpret_cast<double2*>(take((size_t)2 * SLK_NB * batch * sizeof(double2)));
+    w->C12 = reinterpret_cast<double2*>(take(hetrd_c12_elems(ld) * batch * sizeof(double2)));
     w->TAU = reinterpret_cast<double2*>(take((size_t)ld * batch * sizeof(double2)));
     w->Z = vectors ? reinterpret_cast<double*>(take(matel * batch * sizeof(double))) : nullptr;
   } else {
@@ -710,9 +710,9 @@ static size_t carve(char* base, int ld, int batch, int vectors, int mixed, EighW
     w->F2 = reinterpret_cast<float2*>(take(matel * batch * sizeof(float2)));
     w->fVc = reinterpret_cast<float2*>(take((size_t)SLK_NB * ld * batch * sizeof(float2)));
     w->fWc = reinterpret_cast<float2*>(take((size_t)SLK_NB * ld * batch * sizeof(float2)));
-    w->fY = reinterpret_cast<float2*>(take((size_t)ld * batch * sizeof(float2)));
+    w->fY = reinterpret_cast<float2*>(take((size_t)2 * ld * batch * sizeof(float2)));
     w->fYp = reinterpret_cast<float2*>(take(hetrd_partials_elems(ld) * batch * sizeof(float2)));
-    w->fC12 = reinterpret_cast<float2*>(take((size_t)2 * SLK_NB * batch * sizeof(float2)));
+    w->fC12 = reinterpret_cast<float2*>(take(hetrd_c12_elems(ld) * batch * sizeof(float2)));
     w->fTAU = reinterpret_cast<float2*>(take((size_t)ld * batch * sizeof(float2)));
   }
   return off;

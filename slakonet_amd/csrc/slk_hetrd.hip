@@ -4,12 +4,15 @@
 // always returned in FP64):
 //
 //   per panel:  latrd_start (first column reflector)
-//               per column j: hetrd_step = trailing Hermitian mat-vec
-//                   y = A22 v_j - V (W^H v_j) - W (V^H v_j)
+//               per column j: hetrd_step = trailing Hermitian mat-vec A22 v_j
 //                 over 128x128 tiles of the lower triangle, each read once
 //                 and used for both A v and A^H v, writing per-tile partial
-//                 products; then hetrd_tail (one workgroup per matrix) sums
-//                 them, forms w_j and the reflector of column j+1;
+//                 products (diagonal tiles also partial W^H v_j, V^H v_j);
+//                 hetrd_sum (many workgroups) forms
+//                 y = A22 v_j - V (W^H v_j) - W (V^H v_j) and column j+1 with
+//                 the panel's older pairs applied; hetrd_tail (one workgroup
+//                 per matrix, O(n) work) forms w_j, applies the newest pair
+//                 to column j+1 and generates its reflector;
 //               rank-2k update A22 -= V W^H + W V^H on the MFMA GEMM, lower
 //               triangle tiles only (the upper triangle is never read).
 // A kernel boundary orders the tail after the tiles; a device-scope fence
@@ -46,6 +49,8 @@ struct HetrdCtx {
   int n, ld;
   C2 *Vc, *Wc, *Y, *Yp, *C12, *TAU;
   C2* VT;      // optional: reflector j also stored as row j (VT[j*ld + r], r > j)
+  C2* COL;     // next column with the older panel pairs applied ([batch][ld])
+  C2* C12p;    // per row block partials of W^H v, V^H v ([batch][ncmax][2 NB])
   double *D, *E;
   int ncmax;   // partial-sum chunks per matrix (>= ceil(ld / HT_COLS))
 };
@@ -162,6 +167,69 @@ __device__ void latrd_column(const HetrdCtx<T>& c, int b, int j, int jj, T* red)
   __syncthreads();
 }
 
+// Column j (panel position jj >= 1) from COL (older pairs already applied):
+// subtract the newest pair (v_{jj-1}, w_{jj-1}), generate the reflector and
+// store it (panel buffer, strict lower column j, VT row). W^H v and V^H v of
+// the new reflector are formed by the next mat-vec's diagonal tiles.
+template <typename T>
+__device__ void latrd_next(const HetrdCtx<T>& c, int b, int j, int jj, T* red) {
+  using C2 = typename C2T<T>::t;
+  const int n = c.n, ld = c.ld;
+  C2* a = c.A + (size_t)b * ld * ld;
+  C2* vc = c.Vc + (size_t)b * NB * ld;
+  const C2* wc = c.Wc + (size_t)b * NB * ld;
+  const C2* col = c.COL + (size_t)b * ld;
+  __shared__ C2 s_tau, s_scale;
+  __shared__ T s_beta;
+  __shared__ int s_skip;
+  const int tid = threadIdx.x;
+  const int kp = jj - 1;
+  const C2 vj = vc[(size_t)kp * ld + j], wj = wc[(size_t)kp * ld + j];
+  T part = 0;
+  for (int r = j + tid; r < n; r += blockDim.x) {
+    C2 acc = col[r];
+    const C2 t1 = c_mulc(vc[(size_t)kp * ld + r], wj);
+    const C2 t3 = c_mulc(wc[(size_t)kp * ld + r], vj);
+    acc.x -= t1.x + t3.x;
+    acc.y -= t1.y + t3.y;
+    a[(size_t)r * ld + j] = acc;
+    if (r >= j + 2) part += acc.x * acc.x + acc.y * acc.y;
+  }
+  const T xn2 = block_sum(part, red);
+  if (tid == 0) {
+    const C2 alpha = a[(size_t)(j + 1) * ld + j];
+    if (xn2 == T(0) && alpha.y == T(0)) {
+      s_skip = 1;
+      s_tau = mk<T>(0, 0);
+      s_beta = alpha.x;
+      s_scale = mk<T>(0, 0);
+    } else {
+      s_skip = 0;
+      const T nrm = sqrt(alpha.x * alpha.x + alpha.y * alpha.y + xn2);
+      const T beta = alpha.x >= T(0) ? -nrm : nrm;
+      s_beta = beta;
+      s_tau = mk<T>((beta - alpha.x) / beta, -alpha.y / beta);
+      const C2 den = mk<T>(alpha.x - beta, alpha.y);
+      const T dd = den.x * den.x + den.y * den.y;
+      s_scale = mk<T>(den.x / dd, -den.y / dd);
+    }
+    c.D[(size_t)b * ld + j] = (double)a[(size_t)j * ld + j].x;
+    c.E[(size_t)b * ld + j] = (double)s_beta;
+    c.TAU[(size_t)b * ld + j] = s_tau;
+  }
+  __syncthreads();
+  const C2 scale = s_scale;
+  for (int r = tid; r < n; r += blockDim.x) {
+    C2 v = mk<T>(0, 0);
+    if (r == j + 1) v = mk<T>(1, 0);
+    else if (r > j + 1) v = s_skip ? mk<T>(0, 0) : c_mul(a[(size_t)r * ld + j], scale);
+    vc[(size_t)jj * ld + r] = v;
+    if (r > j) a[(size_t)r * ld + j] = v;
+    if (c.VT != nullptr && r > j) c.VT[(size_t)b * ld * ld + (size_t)j * ld + r] = v;
+  }
+  __syncthreads();
+}
+
 template <typename T>
 __global__ __launch_bounds__(TAIL_THREADS) void latrd_start_kernel(HetrdCtx<T> c, int j) {
   __shared__ T red[TAIL_THREADS / 64];
@@ -190,7 +258,6 @@ __global__ __launch_bounds__(HT_THREADS) void hetrd_step_kernel(HetrdCtx<T> c, i
   const C2* vc = c.Vc + (size_t)b * NB * ld;
   const C2* wc = c.Wc + (size_t)b * NB * ld;
   const C2* v = vc + (size_t)jj * ld;
-  const C2* c12 = c.C12 + (size_t)b * 2 * NB;
   C2* yp = c.Yp + (size_t)b * c.ncmax * ld;
   int bi = 0;
   const int t = blockIdx.x;
@@ -240,21 +307,32 @@ __global__ __launch_bounds__(HT_THREADS) void hetrd_step_kernel(HetrdCtx<T> c, i
     for (int w = 0; w < HT_THREADS / 64; ++w) cs = c_add(cs, s_col[w][tid]);
     if (diag) {
       const int r = row0 + tid;
-      if (r < n) {
-        C2 s = c_add(s_row[tid], cs);
-        for (int k = 0; k < jj; ++k) {
-          const C2 t1 = c_mul(vc[(size_t)k * ld + r], c12[k]);
-          const C2 t2 = c_mul(wc[(size_t)k * ld + r], c12[NB + k]);
-          s.x -= t1.x + t2.x;
-          s.y -= t1.y + t2.y;
-        }
-        yp[(size_t)bi * ld + r] = s;
-      }
+      if (r < n) yp[(size_t)bi * ld + r] = c_add(s_row[tid], cs);
     } else {
       const int r = row0 + tid;
       if (r < n) yp[(size_t)bj * ld + r] = s_row[tid];
       const int col = col0 + tid;
       if (col < n) yp[(size_t)bi * ld + col] = cs;
+    }
+  }
+  if (diag && jj > 0) {
+    // partial W^H v and V^H v over this row block (reduced by hetrd_sum)
+    C2* cp = c.C12p + ((size_t)b * c.ncmax + bi) * 2 * NB;
+    for (int k = wv; k < jj; k += HT_THREADS / 64) {
+      C2 s1 = zero, s2 = zero;
+      for (int rl = lane; rl < HT_TS; rl += 64) {
+        const int r = row0 + rl;
+        if (r < n) {
+          const C2 vv = v[r];
+          const C2 x1 = c_cmul(wc[(size_t)k * ld + r], vv);
+          const C2 x2 = c_cmul(vc[(size_t)k * ld + r], vv);
+          s1.x += x1.x; s1.y += x1.y;
+          s2.x += x2.x; s2.y += x2.y;
+        }
+      }
+      s1.x = wave_sum(s1.x); s1.y = wave_sum(s1.y);
+      s2.x = wave_sum(s2.x); s2.y = wave_sum(s2.y);
+      if (lane == 0) { cp[k] = s1; cp[NB + k] = s2; }
     }
   }
   if (clk != nullptr) {
@@ -263,24 +341,68 @@ __global__ __launch_bounds__(HT_THREADS) void hetrd_step_kernel(HetrdCtx<T> c, i
   }
 }
 
+// Column sum (many workgroups, HS_ROWS rows each): y = sum of the partial
+// products - V (W^H v) - W (V^H v), and, inside a panel, the next column with
+// the panel's older pairs applied, COL = A[:, j+1] - V_<jj W_<jj[j+1]^H -
+// W_<jj V_<jj[j+1]^H (the tail adds the newest pair).
+#define HS_ROWS 256
+template <typename T>
+__global__ __launch_bounds__(HS_ROWS) void hetrd_sum_kernel(HetrdCtx<T> c, int j, int jj, int jn, int nch) {
+  using C2 = typename C2T<T>::t;
+  const int b = blockIdx.y;
+  const int n = c.n, ld = c.ld;
+  const C2* yp = c.Yp + (size_t)b * c.ncmax * ld;
+  const C2* vc = c.Vc + (size_t)b * NB * ld;
+  const C2* wc = c.Wc + (size_t)b * NB * ld;
+  const C2* a = c.A + (size_t)b * ld * ld;
+  C2* y = c.Y + (size_t)b * ld;
+  C2* col = c.COL + (size_t)b * ld;
+  __shared__ C2 s_c12[2 * NB];
+  __shared__ C2 s_vj[NB], s_wj[NB];
+  const int tid = threadIdx.x;
+  if (tid < 2 * jj) {
+    const int k = tid < jj ? tid : NB + tid - jj;
+    const C2* cp = c.C12p + (size_t)b * c.ncmax * 2 * NB;
+    C2 acc = mk<T>(0, 0);
+    for (int ch = 0; ch < nch; ++ch) acc = c_add(acc, cp[(size_t)ch * 2 * NB + k]);
+    s_c12[k] = acc;
+  }
+  const bool next = j + 1 < jn;
+  if (next && tid < jj) {
+    s_vj[tid] = vc[(size_t)tid * ld + j + 1];
+    s_wj[tid] = wc[(size_t)tid * ld + j + 1];
+  }
+  __syncthreads();
+  const int r = j + 1 + blockIdx.x * HS_ROWS + tid;
+  if (r >= n) return;
+  C2 s = mk<T>(0, 0);
+  for (int ch = 0; ch < nch; ++ch) s = c_add(s, yp[(size_t)ch * ld + r]);
+  C2 cl = next ? a[(size_t)r * ld + j + 1] : mk<T>(0, 0);
+  for (int k = 0; k < jj; ++k) {
+    const C2 vk = vc[(size_t)k * ld + r], wk = wc[(size_t)k * ld + r];
+    const C2 t1 = c_mul(vk, s_c12[k]);
+    const C2 t2 = c_mul(wk, s_c12[NB + k]);
+    s.x -= t1.x + t2.x;
+    s.y -= t1.y + t2.y;
+    if (next) {
+      const C2 u1 = c_mulc(vk, s_wj[k]);
+      const C2 u2 = c_mulc(wk, s_vj[k]);
+      cl.x -= u1.x + u2.x;
+      cl.y -= u1.y + u2.y;
+    }
+  }
+  y[r] = s;
+  if (next) col[r] = cl;
+}
+
 // Column tail (one workgroup per matrix): y = sum of the nch partial
 // products, w_j from y, then the reflector of column j+1 if it is in the panel.
 template <typename T>
-__global__ __launch_bounds__(TAIL_THREADS) void hetrd_tail_kernel(HetrdCtx<T> c, int j, int jj, int jn, int nch) {
-  using C2 = typename C2T<T>::t;
+__global__ __launch_bounds__(TAIL_THREADS) void hetrd_tail_kernel(HetrdCtx<T> c, int j, int jj, int jn) {
   __shared__ T red[TAIL_THREADS / 64];
   const int b = blockIdx.x;
-  const int n = c.n, ld = c.ld;
-  const C2* yp = c.Yp + (size_t)b * c.ncmax * ld;
-  C2* y = c.Y + (size_t)b * ld;
-  for (int r = j + 1 + threadIdx.x; r < n; r += blockDim.x) {
-    C2 s = mk<T>(0, 0);
-    for (int ch = 0; ch < nch; ++ch) s = c_add(s, yp[(size_t)ch * ld + r]);
-    y[r] = s;
-  }
-  __syncthreads();
   latrd_finalize(c, b, j + 1, jj + 1, red);
-  if (j + 1 < jn) latrd_column(c, b, j + 1, jj + 1, red);
+  if (j + 1 < jn) latrd_next(c, b, j + 1, jj + 1, red);
 }
 
 template <typename T>
@@ -295,6 +417,7 @@ __global__ void hetrd_last_kernel(const typename C2T<T>::t* A, int n, int ld, do
 }
 
 size_t hetrd_partials_elems(int ld) { return (size_t)((ld + HT_TS - 1) / HT_TS) * ld; }
+size_t hetrd_c12_elems(int ld) { return (size_t)((ld + HT_TS - 1) / HT_TS) * 2 * NB; }
 
 template <typename T>
 static void rank2k(hipStream_t st, int batch, int m, int k, typename C2T<T>::t* Vc, typename C2T<T>::t* Wc,
@@ -319,6 +442,8 @@ static void hetrd_impl(hipStream_t st, typename C2T<T>::t* C, int n, int ld, int
                        typename C2T<T>::t* VT) {
   HetrdCtx<T> c;
   c.A = C; c.n = n; c.ld = ld; c.Vc = Vc; c.Wc = Wc; c.Y = Y; c.Yp = Yp; c.C12 = C12; c.TAU = TAU; c.VT = VT;
+  c.COL = Y + (size_t)batch * ld;   // second half of the Y buffer
+  c.C12p = C12;
   c.D = D; c.E = E; c.ncmax = (ld + HT_TS - 1) / HT_TS;
   const long long sP = (long long)NB * ld, sM = (long long)ld * ld;
   for (int j0 = 0; j0 < n - 1; j0 += NB) {
@@ -336,7 +461,9 @@ static void hetrd_impl(hipStream_t st, typename C2T<T>::t* C, int n, int ld, int
       else
         hipLaunchKernelGGL(hetrd_step_kernel<T>, g, dim3(HT_THREADS), 0, st, c, j, j - j0,
                            (unsigned long long*)nullptr);
-      hipLaunchKernelGGL(hetrd_tail_kernel<T>, dim3(batch), dim3(TAIL_THREADS), 0, st, c, j, j - j0, jn, nbk);
+      hipLaunchKernelGGL(hetrd_sum_kernel<T>, dim3((m + HS_ROWS - 1) / HS_ROWS, batch), dim3(HS_ROWS), 0, st, c, j,
+                         j - j0, jn, nbk);
+      hipLaunchKernelGGL(hetrd_tail_kernel<T>, dim3(batch), dim3(TAIL_THREADS), 0, st, c, j, j - j0, jn);
     }
     if (jn < n) rank2k<T>(st, batch, n - jn, jn - j0, Vc + jn, Wc + jn, C + (size_t)jn * ld + jn, ld, sP, sM);
   }
