@@ -367,8 +367,11 @@ __device__ __forceinline__ double* rf_base(const RefineCtx& c, int v) {
 // (rows of VT, contiguous) with wave reductions only. x lives in LDS when
 // the padded dimension ld <= RF_LDS_X, else in the scratch slot.
 #define RF_CH 8
-#define RF_LDS_X 2048
+#define RF_LDS_X 2560
 
+// NQ > 0: reflector elements per lane in registers (ld <= 64 NQ, x in LDS);
+// NQ = 0: generic path (x in LDS or scratch, reflectors re-read).
+template <int NQ>
 __global__ __launch_bounds__(64) void refine_tri_kernel(RefineCtx c) {
   const int s = blockIdx.x, edge = blockIdx.y, v = s * 2 + edge;
   const SlkSpectrum sp = c.spec[s];
@@ -529,28 +532,74 @@ __global__ __launch_bounds__(64) void refine_tri_kernel(RefineCtx c) {
   __threadfence_block();
   const float2* vt = c.R + mat * ld * ld;
   const float2* tau = c.TAU + mat * ld;
-  for (int j = m - 2; j >= 0; --j) {
-    const float2 tf = tau[j];
-    if (tf.x == 0.0f && tf.y == 0.0f) continue;
-    const float2* vr = vt + (size_t)j * ld;
-    double px = 0.0, py = 0.0;
-    for (int r = j + 1 + lane; r < m; r += 64) {
-      const float2 vf = vr[r];
-      const double2 xv = x[r];
-      px += (double)vf.x * xv.x + (double)vf.y * xv.y;
-      py += (double)vf.x * xv.y - (double)vf.y * xv.x;
+  if (NQ > 0) {
+    // each lane keeps its <= NQ elements of the current reflector in registers
+    // and prefetches the next reflector while this one is applied (x in LDS)
+    float2 vn[NQ > 0 ? NQ : 1];
+    auto load = [&](int j, float2* dst) {
+      const float2* vr = vt + (size_t)j * ld;
+#pragma unroll
+      for (int q = 0; q < (NQ > 0 ? NQ : 1); ++q) {
+        const int r = j + 1 + lane + 64 * q;
+        dst[q] = r < m ? vr[r] : make_float2(0.0f, 0.0f);
+      }
+    };
+    if (m >= 2) load(m - 2, vn);
+    for (int j = m - 2; j >= 0; --j) {
+      float2 vc[NQ > 0 ? NQ : 1];
+#pragma unroll
+      for (int q = 0; q < (NQ > 0 ? NQ : 1); ++q) vc[q] = vn[q];
+      if (j > 0) load(j - 1, vn);
+      const float2 tf = tau[j];
+      if (tf.x == 0.0f && tf.y == 0.0f) continue;
+      double px = 0.0, py = 0.0;
+#pragma unroll
+      for (int q = 0; q < (NQ > 0 ? NQ : 1); ++q) {
+        const int r = j + 1 + lane + 64 * q;
+        if (r < m) {
+          const double2 xv = x[r];
+          px += (double)vc[q].x * xv.x + (double)vc[q].y * xv.y;
+          py += (double)vc[q].x * xv.y - (double)vc[q].y * xv.x;
+        }
+      }
+      px = wave_sum(px);
+      py = wave_sum(py);
+      const double sx = (double)tf.x * px - (double)tf.y * py, sy = (double)tf.x * py + (double)tf.y * px;
+#pragma unroll
+      for (int q = 0; q < (NQ > 0 ? NQ : 1); ++q) {
+        const int r = j + 1 + lane + 64 * q;
+        if (r < m) {
+          double2 xv = x[r];
+          xv.x -= (double)vc[q].x * sx - (double)vc[q].y * sy;
+          xv.y -= (double)vc[q].x * sy + (double)vc[q].y * sx;
+          x[r] = xv;
+        }
+      }
     }
-    px = wave_sum(px);
-    py = wave_sum(py);
-    const double sx = (double)tf.x * px - (double)tf.y * py, sy = (double)tf.x * py + (double)tf.y * px;
-    for (int r = j + 1 + lane; r < m; r += 64) {
-      const float2 vf = vr[r];
-      double2 xv = x[r];
-      xv.x -= (double)vf.x * sx - (double)vf.y * sy;
-      xv.y -= (double)vf.x * sy + (double)vf.y * sx;
-      x[r] = xv;
+  } else {
+    for (int j = m - 2; j >= 0; --j) {
+      const float2 tf = tau[j];
+      if (tf.x == 0.0f && tf.y == 0.0f) continue;
+      const float2* vr = vt + (size_t)j * ld;
+      double px = 0.0, py = 0.0;
+      for (int r = j + 1 + lane; r < m; r += 64) {
+        const float2 vf = vr[r];
+        const double2 xv = x[r];
+        px += (double)vf.x * xv.x + (double)vf.y * xv.y;
+        py += (double)vf.x * xv.y - (double)vf.y * xv.x;
+      }
+      px = wave_sum(px);
+      py = wave_sum(py);
+      const double sx = (double)tf.x * px - (double)tf.y * py, sy = (double)tf.x * py + (double)tf.y * px;
+      for (int r = j + 1 + lane; r < m; r += 64) {
+        const float2 vf = vr[r];
+        double2 xv = x[r];
+        xv.x -= (double)vf.x * sx - (double)vf.y * sy;
+        xv.y -= (double)vf.x * sy + (double)vf.y * sx;
+        x[r] = xv;
+      }
+      __threadfence_block();   // x[r] written by one lane is read by another for the next reflector
     }
-    __threadfence_block();   // x[r] written by one lane is read by another for the next reflector
   }
   if (c.lds_x) {
     // hand x to the next kernel through the scratch slot
@@ -874,7 +923,18 @@ int slk_refine_edges_mixed(const void* H, const void* S, int n, int ld, int batc
   c.meta = reinterpret_cast<int*>(c.part + (size_t)n_spec * 2 * c.nblk * 2);
   // x in LDS when every structure fits (m <= ld <= RF_LDS_X), else in the scratch slot
   c.lds_x = ld <= RF_LDS_X ? 1 : 0;
-  hipLaunchKernelGGL(refine_tri_kernel, dim3(n_spec, 2), dim3(64), c.lds_x ? ld * sizeof(double2) : 0, stream, c);
+  const size_t lds = c.lds_x ? ld * sizeof(double2) : 0;
+  const dim3 g(n_spec, 2);
+  if (ld <= 64 * 4)
+    hipLaunchKernelGGL(refine_tri_kernel<4>, g, dim3(64), lds, stream, c);
+  else if (ld <= 64 * 8)
+    hipLaunchKernelGGL(refine_tri_kernel<8>, g, dim3(64), lds, stream, c);
+  else if (ld <= 64 * 16)
+    hipLaunchKernelGGL(refine_tri_kernel<16>, g, dim3(64), lds, stream, c);
+  else if (ld <= 64 * 40)
+    hipLaunchKernelGGL(refine_tri_kernel<40>, g, dim3(64), lds, stream, c);
+  else
+    hipLaunchKernelGGL(refine_tri_kernel<0>, g, dim3(64), lds, stream, c);
   hipLaunchKernelGGL(refine_wh_kernel, dim3(c.nblk, n_spec * 2), dim3(RF_THREADS), 0, stream, c);
   hipLaunchKernelGGL(refine_rq_kernel, dim3(c.nblk, n_spec * 2), dim3(RF_THREADS), 0, stream, c);
   hipLaunchKernelGGL(refine_commit_kernel, dim3((n_spec + 63) / 64), dim3(64), 0, stream, c, w, edges, n_spec);
