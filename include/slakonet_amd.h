@@ -226,6 +226,19 @@ int slk_sampler_hint(int kind, double bytes);
  * launches; waits for the sampled launches to complete. */
 int slk_sampler_read(int kind, double* out);
 
+/* Atom-type projected DOS from the eigenvectors of one structure (V: the
+ * [nk][ld][ld] complex128 output of slk_eigh_f64, eigenvectors in columns;
+ * W: eigenvalues [nk][wld], Hartree). Weight of type t for state (k, b):
+ * sum of |V[k][mu][b]|^2 over orbitals mu with orb_type[mu] == t (no
+ * overlap); out[t][e] = scale * sum_kb weight * Gaussian(grid[e] + shift_ev -
+ * eps_kb * 27.211, sigma). n_types <= 8; wts: nk * n * n_types doubles of
+ * scratch. Replaces compute_atom_projected_dos
+ * (slakonet/predict_slakonet.py:119-210; calculate_projected_dos,
+ * slakonet/main.py:799-840, is a placeholder in the reference). */
+int slk_pdos_launch(const void* V, const double* W, int n, int ld, int nk, int wld, const int* orb_type, int n_types,
+                    const double* grid, int ne, double sigma, double shift_ev, double scale, double* wts, double* out,
+                    hipStream_t stream);
+
 /* ---- Differentiation form (hand-written adjoints) ----------------------- */
 
 /* Adjoint of slk_dos_launch at fixed Fermi level: gW[w0 + k*wld + b] =

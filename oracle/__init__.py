@@ -30,7 +30,8 @@ How each part is pinned (tests/test_oracle_*.py):
   hand-evaluated dimer block.
 * solve: scipy.linalg.eigh(H, S) (LAPACK zhegvd) and residuals.
 * post: DOS closed form / integral, bisection root of N(mu), hand-made
-  eigenvalue sets for the gap rules. The exact stopped iterate of the
+  eigenvalue sets for the gap rules; atom-projected DOS summed over types
+  equals the total DOS for orthonormal eigenvectors. The exact stopped iterate of the
   reference's half-step Newton Fermi search is pinned by its invariants only
   (first iterate meeting |dL/dmu| < tol, start at mean eigenvalue): parity
   for fermi_energy_eV is therefore "partial" (see module docstring).

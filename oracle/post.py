@@ -155,3 +155,23 @@ def fermi_bisect(eig, n_electrons, k_weights, kT=0.01):
         else:
             lo = mid
     return 0.5 * (lo + hi)
+
+
+def atom_projected_dos(eig, vecs, orb_type, n_types, mu, sigma=0.1, energy_range=(-8.0, 6.0), n_points=1000):
+    """compute_atom_projected_dos (slakonet/predict_slakonet.py:119-210): per
+    state the weight of atom type t is sum |psi_mu|^2 over the type's orbitals
+    (S-orthonormal eigenvector, no overlap), Gaussians on grid + mu (eV),
+    divided by the number of k-points. eig [nk, nb] Hartree; vecs [nk, n, nb]."""
+    eig = np.asarray(eig, dtype=np.float64)
+    grid = np.linspace(energy_range[0], energy_range[1], n_points)
+    grid_ev = grid + mu * H2E
+    orb_type = np.asarray(orb_type)
+    norm = 1.0 / (sigma * np.sqrt(2.0 * np.pi))
+    out = np.zeros((n_types, n_points))
+    nk, nb = eig.shape
+    for k in range(nk):
+        p = np.abs(np.asarray(vecs[k])) ** 2                       # [n, nb]
+        w = np.stack([p[orb_type == t].sum(0) for t in range(n_types)])   # [T, nb]
+        g = norm * np.exp(-0.5 * ((grid_ev[None, :] - eig[k][:, None] * H2E) / sigma) ** 2)   # [nb, E]
+        out += w @ g
+    return grid, out / nk

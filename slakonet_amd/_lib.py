@@ -101,6 +101,9 @@ SIGNATURES = {
     "slk_refine_edges_mixed": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_size_t,
                                          C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p,
                                          C.c_size_t, C.c_void_p]),
+    "slk_pdos_launch": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_int,
+                                  C.c_void_p, C.c_int, C.c_double, C.c_double, C.c_double, C.c_void_p, C.c_void_p,
+                                  C.c_void_p]),
     "slk_sampler_reset": (C.c_int, [C.c_int, C.c_int, C.c_int]),
     "slk_sampler_hint": (C.c_int, [C.c_int, C.c_double]),
     "slk_sampler_read": (C.c_int, [C.c_int, C.c_void_p]),

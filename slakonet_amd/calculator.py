@@ -181,6 +181,41 @@ class SimpleDftb:
         self._dos_stats = stats
         return grid, dos[0]
 
+    def calculate_atom_projected_dos(self, sigma=0.1, energy_range=(-8, 6), num_points=1000):
+        """Atom-type projected DOS on the device (compute_atom_projected_dos,
+        slakonet/predict_slakonet.py:119-210): per state, the weight of an atom
+        type is the sum of |psi_mu|^2 over its atoms' orbitals; Gaussians of
+        width ``sigma`` eV on a grid relative to the Fermi level; averaged over
+        k-points (1/nk). Orbitals are attributed to atoms by their actual
+        counts (the reference divides n_orb evenly, equal for uniform shells).
+        Returns (energy grid [num_points] numpy, {symbol: pdos numpy}, symbols)."""
+        import ctypes as C
+        from . import _lib
+        from .units import CHEMICAL_SYMBOLS
+        if getattr(self, "_V", None) is None:
+            raise RuntimeError("projected DOS needs eigenvectors: construct with with_eigenvectors=True")
+        z = np.asarray(self.geometry.atomic_numbers_np)
+        symbols = [CHEMICAL_SYMBOLS[int(v)] for v in z]
+        unique = list(dict.fromkeys(symbols))
+        sp = self.tables.species_index(z)
+        norb_atoms = self.tables.norb[sp]
+        orb_type = np.repeat(np.array([unique.index(sym) for sym in symbols], dtype=np.int32), norb_atoms)
+        n = int(norb_atoms.sum())
+        nk = self._w.shape[0]
+        ed = self._edge_values()
+        dev = self.device
+        grid = torch.linspace(energy_range[0], energy_range[1], num_points, dtype=torch.float64, device=dev)
+        ot = torch.from_numpy(orb_type).to(dev)
+        wts = torch.empty(nk * n * len(unique), dtype=torch.float64, device=dev)
+        out = torch.empty(len(unique), num_points, dtype=torch.float64, device=dev)
+        lib = _lib.lib()
+        _lib.check(lib.slk_pdos_launch(_lib.ptr(self._V), _lib.ptr(self._w), n, self._V.shape[1], nk,
+                                       self._w.shape[1], _lib.ptr(ot), len(unique), _lib.ptr(grid), num_points,
+                                       float(sigma), ed["mu"] * H2E, 1.0 / nk, _lib.ptr(wts), _lib.ptr(out),
+                                       _lib.stream_handle()), "slk_pdos_launch")
+        o = out.cpu().numpy()
+        return grid.cpu().numpy(), {sym: o[i] for i, sym in enumerate(unique)}, unique
+
     def _calculate_electronic_energy(self):
         mu = self.get_fermi_energy()
         kT = 0.025 / H2E
@@ -232,3 +267,9 @@ class SimpleDftb:
                 "dos_values_tensor": dos,
             })
         return props
+
+
+def compute_atom_projected_dos(properties, geometry=None, sigma=0.1, energy_range=(-8, 6), filename=None):
+    """Drop-in for slakonet/predict_slakonet.py:119 (plotting excluded): the
+    properties dict must hold "calc" (get_fermi=True) solved with eigenvectors."""
+    return properties["calc"].calculate_atom_projected_dos(sigma=sigma, energy_range=energy_range)

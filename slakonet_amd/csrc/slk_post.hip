@@ -247,6 +247,67 @@ __global__ __launch_bounds__(POST_THREADS) void dos_adjoint_kernel(
   if (q < tot) gW[jb.w0 + (long long)k * jb.wld + b] = acc * norm * inv * H2E_OUT * KW[jb.kw0 + k];
 }
 
+// Atom-type projected DOS (after predict_slakonet.py:119-210): per state the
+// weight of type t is sum over its orbitals of |psi_mu|^2 (coefficients of the
+// S-orthonormal eigenvector, no overlap), Gaussians on E + shift, averaged
+// over k with `scale` (1/nk).
+#define PDOS_MAXT 8
+__global__ __launch_bounds__(POST_THREADS) void pdos_weights_kernel(const double2* __restrict__ V, int n, int ld,
+                                                                    const int* __restrict__ orb_type, int n_types,
+                                                                    double* __restrict__ wts) {
+  const int k = blockIdx.y;
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= n) return;
+  double acc[PDOS_MAXT];
+#pragma unroll
+  for (int t = 0; t < PDOS_MAXT; ++t) acc[t] = 0.0;
+  const double2* vk = V + (size_t)k * ld * ld;
+  for (int mu = 0; mu < n; ++mu) {
+    const double2 c = vk[(size_t)mu * ld + b];
+    const double p = c.x * c.x + c.y * c.y;
+    const int t = orb_type[mu];
+#pragma unroll
+    for (int q = 0; q < PDOS_MAXT; ++q)
+      if (q == t) acc[q] += p;
+  }
+  for (int t = 0; t < n_types; ++t) wts[((size_t)k * n + b) * n_types + t] = acc[t];
+}
+
+__global__ __launch_bounds__(POST_THREADS) void pdos_kernel(const double* __restrict__ W, int n, int wld, int nk,
+                                                            const double* __restrict__ wts, int n_types,
+                                                            const double* __restrict__ grid, int ne, double sigma,
+                                                            double shift_ev, double scale, double* __restrict__ out) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  const double E = (e < ne ? grid[e] : 0.0) + shift_ev;
+  const double inv = 1.0 / sigma;
+  const double norm = scale / (sigma * sqrt(2.0 * 3.141592653589793));
+  __shared__ double se[DOS_CHUNK / 4];
+  __shared__ double sw[DOS_CHUNK / 4][PDOS_MAXT];
+  double acc[PDOS_MAXT];
+#pragma unroll
+  for (int t = 0; t < PDOS_MAXT; ++t) acc[t] = 0.0;
+  const int tot = nk * n;
+  for (int c0 = 0; c0 < tot; c0 += DOS_CHUNK / 4) {
+    const int cn = min(DOS_CHUNK / 4, tot - c0);
+    __syncthreads();
+    for (int q = threadIdx.x; q < cn; q += blockDim.x) {
+      const int g = c0 + q, k = g / n, b = g % n;
+      se[q] = W[(size_t)k * wld + b] * H2E_OUT;
+      for (int t = 0; t < n_types; ++t) sw[q][t] = wts[(size_t)g * n_types + t];
+    }
+    __syncthreads();
+    for (int q = 0; q < cn; ++q) {
+      const double x = (E - se[q]) * inv;
+      const double gss = exp(-0.5 * x * x) * norm;
+#pragma unroll
+      for (int t = 0; t < PDOS_MAXT; ++t)
+        if (t < n_types) acc[t] += sw[q][t] * gss;
+    }
+  }
+  if (e < ne)
+    for (int t = 0; t < n_types; ++t) out[(size_t)t * ne + e] = acc[t];
+}
+
 // dos stats: [dos_at_fermi, gap_center, trapz_total]
 __global__ __launch_bounds__(POST_THREADS) void dos_stats_kernel(const SlkDosJob* jobs, const double* __restrict__ grid,
                                                                  int ne, const double* __restrict__ dos,
@@ -316,6 +377,18 @@ int slk_dos_adjoint_launch(const double* W, const double* KW, const SlkDosJob* j
   if (n_jobs <= 0 || max_states <= 0) return 0;
   hipLaunchKernelGGL(dos_adjoint_kernel, dim3((max_states + POST_THREADS - 1) / POST_THREADS, n_jobs),
                      dim3(POST_THREADS), 0, stream, W, KW, jobs, grid, ne, sigma, edges, gdos, gW);
+  return (int)hipGetLastError();
+}
+
+int slk_pdos_launch(const void* V, const double* W, int n, int ld, int nk, int wld, const int* orb_type, int n_types,
+                    const double* grid, int ne, double sigma, double shift_ev, double scale, double* wts, double* out,
+                    hipStream_t stream) {
+  if (n_types < 1 || n_types > PDOS_MAXT) return 1;
+  if (n <= 0 || nk <= 0 || ne <= 0) return 0;
+  hipLaunchKernelGGL(pdos_weights_kernel, dim3((n + POST_THREADS - 1) / POST_THREADS, nk), dim3(POST_THREADS), 0,
+                     stream, reinterpret_cast<const double2*>(V), n, ld, orb_type, n_types, wts);
+  hipLaunchKernelGGL(pdos_kernel, dim3((ne + POST_THREADS - 1) / POST_THREADS), dim3(POST_THREADS), 0, stream, W, n,
+                     wld, nk, wts, n_types, grid, ne, sigma, shift_ev, scale, out);
   return (int)hipGetLastError();
 }
 

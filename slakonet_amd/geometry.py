@@ -158,6 +158,14 @@ def klines_points(klines):
     return k, w
 
 
+def kpts_to_klines(kpts, default_points=10):
+    """Pairs (k[0], k[1]), (k[2], k[3]), ... of a k-point list as k-line rows
+    [k1, k2, default_points] (slakonet/optim.py:74-100)."""
+    k = _as_np(kpts, np.float64).reshape(-1, 3)
+    rows = [np.concatenate([k[i], k[i + 1], [float(default_points)]]) for i in range(0, len(k) - 1, 2)]
+    return torch.as_tensor(np.stack(rows) if rows else np.zeros((0, 7)), dtype=torch.float64)
+
+
 def lattice_translations(cell: np.ndarray, positions: np.ndarray, cutoff: float):
     """Integer cell vectors covering every pair closer than ``cutoff`` (Bohr).
 

@@ -236,3 +236,39 @@ def test_values_beyond_lds_limit_padded():
     e_mix = engine.edges_dict(engine.post(big, w_mix, refine=(Hb, Sb, wsl[0]))[0][0].cpu().numpy())
     for key in ("vbm", "cbm"):
         assert abs(e_mix[key] - e_ref[key]) < 1e-6, (key, e_mix[key], e_ref[key])
+
+
+@pytest.mark.parametrize("case", ["diamond", "random3"])
+def test_atom_projected_dos_matches_oracle(case):
+    """Device atom-type PDOS (eigenvectors of the FP64 solve) vs the oracle on
+    scipy eigenvectors; degenerate subspaces contribute through their projector,
+    so the result does not depend on the basis chosen inside them."""
+    from slakonet_amd.calculator import SimpleDftb
+    from slakonet_amd.geometry import Geometry
+    if case == "diamond":
+        st = synth.diamond_si()
+        z, pos, cell, species = st.z, st.positions, st.cell, [14]
+        k = st.kpoints[:6]
+    else:
+        rng = np.random.default_rng(12)
+        species = [1, 22, 31]
+        z, pos, cell = synth.random_cell(rng, 7, species, 1.8)
+        k = np.array([[0.1, -0.2, 0.3], [0.0, 0.0, 0.0], [0.5, 0.25, 0.0]])
+    skfs = synth.make_skf_set(species, seed=5)
+    tab = _tables(skfs, species)
+    geom = Geometry(z, pos, cell=cell, units="angstrom")
+    calc = SimpleDftb(geom, shell_dict=synth.shell_dict_spd(species), tables=tab, kpoints_frac=k,
+                      with_eigenvectors=True)
+    calc()
+    grid, pd, uniq = calc.calculate_atom_projected_dos(sigma=0.1, energy_range=(-8, 6), num_points=700)
+    ref = obuild.build_hs(z, pos * AA_BOHR, cell * AA_BOHR, synth.shell_dict_spd(species), skfs, k)
+    w_ref, v_ref = osolve.solve_all(ref["H"], ref["S"], vectors=True)
+    mu = calc._edge_values()["mu"]
+    sp = tab.species_index(z)
+    from slakonet_amd.units import CHEMICAL_SYMBOLS
+    syms = [CHEMICAL_SYMBOLS[int(v)] for v in z]
+    types = np.repeat([uniq.index(s) for s in syms], tab.norb[sp])
+    _, pref = opost.atom_projected_dos(w_ref, v_ref, types, len(uniq), mu, sigma=0.1, energy_range=(-8, 6),
+                                       n_points=700)
+    for i, s in enumerate(uniq):
+        assert np.allclose(pd[s], pref[i], rtol=1e-7, atol=1e-9 * pref.max()), s

@@ -207,3 +207,14 @@ def test_native_planner_matches_python_bookkeeping():
     assert w0 == plan.n_work == len(work)
     with pytest.raises(ValueError):
         engine.plan_build(tab, inputs, ld=8)
+
+
+def test_kpts_to_klines_pairs():
+    """kpts_to_klines (slakonet/optim.py:74-100): consecutive pairs become rows."""
+    from slakonet_amd.geometry import kpts_to_klines
+    kp = [[0, 0, 0], [0.5, 0, 0], [0.5, 0.5, 0], [0, 0, 0], [0.1, 0.1, 0.1]]
+    kl = kpts_to_klines(kp, default_points=7).numpy()
+    assert kl.shape == (2, 7)
+    assert np.array_equal(kl[0], [0, 0, 0, 0.5, 0, 0, 7]) and np.array_equal(kl[1], [0.5, 0.5, 0, 0, 0, 0, 7])
+    k, w = klines_points(kl)
+    assert len(k) == 14 and np.allclose(k[6], [0.5, 0, 0]) and np.allclose(w, 1 / 14)

@@ -259,3 +259,27 @@ def test_reference_autograd_agrees_at_gamma_only():
             assert abs(ref - ours) > 1e-3 * abs(ours)
         else:
             assert abs(ref - ours) < 1e-10 * max(1.0, abs(ours))
+
+
+def test_atom_projected_dos_sums_to_total_for_orthonormal_vectors():
+    """With S = I the type weights of every state sum to 1, so the projected
+    DOS summed over types is the total DOS at uniform k-weights."""
+    rng = np.random.default_rng(9)
+    nk, n = 3, 10
+    eig, vecs = [], []
+    for _ in range(nk):
+        a = rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n))
+        h = (a + a.conj().T) / 20
+        w, v = np.linalg.eigh(h)
+        eig.append(w)
+        vecs.append(v)
+    eig = np.array(eig)
+    types = np.array([0, 0, 1, 1, 1, 2, 2, 0, 1, 2])
+    mu = 0.01
+    grid, pd = post.atom_projected_dos(eig, vecs, types, 3, mu, sigma=0.1, energy_range=(-8, 6), n_points=500)
+    _, tot = post.dos(eig, np.full(nk, 1.0 / nk), mu, energy_range=(-8, 6), num_points=500, sigma=0.1)
+    assert np.allclose(pd.sum(0), tot, rtol=1e-12, atol=1e-12)
+    # one state on a single type-1 orbital -> only type 1 gets its Gaussian
+    vec = np.zeros((1, n, 1)); vec[0, 3, 0] = 1.0
+    _, one = post.atom_projected_dos(np.array([[0.0]]), vec, types, 3, 0.0, n_points=11, energy_range=(-1, 1))
+    assert one[0].max() == 0.0 and one[2].max() == 0.0 and abs(one[1][5] - 1 / (0.1 * np.sqrt(2 * np.pi))) < 1e-12
