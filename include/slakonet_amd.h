@@ -269,6 +269,15 @@ int slk_scale_columns(const void* C, const double* g, int n, int ld, int batch, 
 int slk_adjoint_tables_launch(const SlkBuildArgs* args, const void* QH, const void* QS, double* dtab,
                               hipStream_t stream);
 
+/* Same contraction with respect to the atom positions: dR[atom][c] +=
+ * dL/dR_atom,c (Bohr), exact derivatives of the interpolated integrals and
+ * the direction cosines (dual numbers). With QH, QS from the energy weights
+ * dE/de = w_k (f + e f') this is the gradient of the band energy; forces are
+ * its negative. dR [atoms][3] zeroed by the caller; FP64 atomics. Replaces the
+ * autograd force path of SimpleDftb.get_forces (slakonet/main.py:400-585) and
+ * its finite-difference fallback (slakonet/main.py:198-281). */
+int slk_forces_launch(const SlkBuildArgs* args, const void* QH, const void* QS, double* dR, hipStream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -113,6 +113,7 @@ SIGNATURES = {
                                     C.c_void_p]),
     "slk_adjoint_tables_launch": (C.c_int, [C.POINTER(SlkBuildArgs), C.c_void_p, C.c_void_p, C.c_void_p,
                                             C.c_void_p]),
+    "slk_forces_launch": (C.c_int, [C.POINTER(SlkBuildArgs), C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
 }
 
 _LIB = None

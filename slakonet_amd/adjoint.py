@@ -74,6 +74,29 @@ def table_gradient_from_densities(tables: TableSet, plan: engine.BuildPlan, QH: 
     return dtab
 
 
+def position_gradient_from_densities(tables: TableSet, plan: engine.BuildPlan, QH: torch.Tensor, QS: torch.Tensor,
+                                     cutoff: float = HS_CUTOFF_BOHR, stream=None) -> torch.Tensor:
+    """dL/dR [atoms of the plan, 3] (Bohr) given QH, QS laid out as the plan's H/S."""
+    lib = _lib.lib()
+    n_atoms = int(plan.dev["atom_spec"].numel() // 4)
+    dR = torch.zeros(n_atoms, 3, dtype=torch.float64, device=tables.device)
+    args = engine.build_args(tables, plan, 0, 0, False, cutoff)
+    _lib.check(lib.slk_forces_launch(C.byref(args), _lib.ptr(QH), _lib.ptr(QS), _lib.ptr(dR),
+                                     _lib.stream_handle(stream)), "slk_forces_launch")
+    return dR
+
+
+def band_energy_weights(w: torch.Tensor, k_weights: torch.Tensor, mu: float, kT: float, n: int) -> torch.Tensor:
+    """dE/de for E = sum_k w_k sum_b f(e_kb) e_kb at fixed mu (Fermi-Dirac f)."""
+    e = w[:, :n]
+    f = 1.0 / (1.0 + torch.exp((e - mu) / kT))
+    f = torch.where(e == 0, torch.zeros_like(f), f)
+    fp = -f * (1.0 - f) / kT
+    g = torch.zeros_like(w)
+    g[:, :n] = k_weights.reshape(-1, 1) * (f + e * fp)
+    return g
+
+
 def table_gradient(tables: TableSet, plan: engine.BuildPlan, V: torch.Tensor, w: torch.Tensor, g: torch.Tensor,
                    table_tensor: Optional[torch.Tensor] = None, stream=None) -> torch.Tensor:
     """dL/d(table rows) for eigenvalue gradient g [n_mat, ld] of the plan's solve."""

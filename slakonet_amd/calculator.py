@@ -216,6 +216,24 @@ class SimpleDftb:
         o = out.cpu().numpy()
         return grid.cpu().numpy(), {sym: o[i] for i, sym in enumerate(unique)}, unique
 
+    def get_forces(self):
+        """Forces [1, N, 3] (Hartree/Bohr) = -dE/dR of the band energy of
+        _calculate_electronic_energy (Fermi-Dirac occupations at kT = 0.025 eV,
+        Fermi level held fixed as in the reference), from the hand-written
+        position adjoint of the build (slakonet/main.py:400-585 differentiates
+        the same energy with autograd)."""
+        from . import adjoint
+        from .units import H2E
+        if getattr(self, "_V", None) is None:
+            raise RuntimeError("forces need eigenvectors: construct with with_eigenvectors=True")
+        mu = self._edge_values()["mu"]
+        n = self._plan.n_orb[0]
+        g = adjoint.band_energy_weights(self._w, self._plan.k_weights, mu, 0.025 / H2E, n)
+        QH, QS = adjoint.weighted_densities(self._V, self._w, g)
+        dR = adjoint.position_gradient_from_densities(self.tables, self._plan, QH, QS)
+        self._forces = (-dR).unsqueeze(0)
+        return self._forces
+
     def _calculate_electronic_energy(self):
         mu = self.get_fermi_energy()
         kT = 0.025 / H2E

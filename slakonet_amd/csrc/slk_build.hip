@@ -38,18 +38,84 @@ __device__ __forceinline__ int lpair_base(int l1, int l2) {
 
 #define SQ3 1.7320508075688772
 
+// Forward-mode dual number with N partial derivatives: exact derivatives of
+// the polynomial Slater-Koster table and of the interpolation weights.
+template <int N>
+struct Dual {
+  double v;
+  double g[N];
+  __device__ Dual() : v(0.0) {
+#pragma unroll
+    for (int i = 0; i < N; ++i) g[i] = 0.0;
+  }
+  __device__ Dual(double x) : v(x) {
+#pragma unroll
+    for (int i = 0; i < N; ++i) g[i] = 0.0;
+  }
+};
+template <int N>
+__device__ __forceinline__ Dual<N> operator+(const Dual<N>& a, const Dual<N>& b) {
+  Dual<N> r(a.v + b.v);
+#pragma unroll
+  for (int i = 0; i < N; ++i) r.g[i] = a.g[i] + b.g[i];
+  return r;
+}
+template <int N>
+__device__ __forceinline__ Dual<N> operator-(const Dual<N>& a, const Dual<N>& b) {
+  Dual<N> r(a.v - b.v);
+#pragma unroll
+  for (int i = 0; i < N; ++i) r.g[i] = a.g[i] - b.g[i];
+  return r;
+}
+template <int N>
+__device__ __forceinline__ Dual<N> operator-(const Dual<N>& a) {
+  Dual<N> r(-a.v);
+#pragma unroll
+  for (int i = 0; i < N; ++i) r.g[i] = -a.g[i];
+  return r;
+}
+template <int N>
+__device__ __forceinline__ Dual<N> operator*(const Dual<N>& a, const Dual<N>& b) {
+  Dual<N> r(a.v * b.v);
+#pragma unroll
+  for (int i = 0; i < N; ++i) r.g[i] = a.g[i] * b.v + a.v * b.g[i];
+  return r;
+}
+template <int N>
+__device__ __forceinline__ Dual<N> operator*(double s, const Dual<N>& a) {
+  Dual<N> r(s * a.v);
+#pragma unroll
+  for (int i = 0; i < N; ++i) r.g[i] = s * a.g[i];
+  return r;
+}
+template <int N>
+__device__ __forceinline__ Dual<N> operator*(const Dual<N>& a, double s) { return s * a; }
+template <int N>
+__device__ __forceinline__ Dual<N> operator+(const Dual<N>& a, double s) { Dual<N> r = a; r.v += s; return r; }
+template <int N>
+__device__ __forceinline__ Dual<N> operator+(double s, const Dual<N>& a) { return a + s; }
+template <int N>
+__device__ __forceinline__ Dual<N> operator-(const Dual<N>& a, double s) { Dual<N> r = a; r.v -= s; return r; }
+template <int N>
+__device__ __forceinline__ Dual<N> operator-(double s, const Dual<N>& a) { return (-a) + s; }
+template <int N>
+__device__ __forceinline__ Dual<N> operator/(const Dual<N>& a, double s) { return a * (1.0 / s); }
+__device__ __forceinline__ double dval(double x) { return x; }
+template <int N>
+__device__ __forceinline__ double dval(const Dual<N>& x) { return x.v; }
+
 // Two-centre Slater-Koster element for l1 <= l2: orbital a of shell l1 on the
 // row atom, orbital b of shell l2 on the column atom, direction cosines
 // (x, y, z) from row to column atom, V = {sigma, pi, delta}.
-__device__ double sk_elem(int l1, int l2, int a, int b, double x, double y, double z,
-                          const double* V) {
+template <typename R>
+__device__ R sk_elem_t(int l1, int l2, int a, int b, R x, R y, R z, const R* V) {
   if (l1 == 0 && l2 == 0) return V[0];
   if (l1 == 0 && l2 == 1) {
-    const double c[3] = {y, z, x};
+    const R c[3] = {y, z, x};
     return c[b] * V[0];
   }
   if (l1 == 0 && l2 == 2) {
-    double g;
+    R g;
     switch (b) {
       case 0: g = SQ3 * x * y; break;
       case 1: g = SQ3 * y * z; break;
@@ -60,12 +126,12 @@ __device__ double sk_elem(int l1, int l2, int a, int b, double x, double y, doub
     return g * V[0];
   }
   if (l1 == 1 && l2 == 1) {
-    const double c[3] = {y, z, x};
-    return c[a] * c[b] * (V[0] - V[1]) + (a == b ? V[1] : 0.0);
+    const R c[3] = {y, z, x};
+    return c[a] * c[b] * (V[0] - V[1]) + (a == b ? V[1] : R(0.0));
   }
-  const double l = x, m = y, n = z;
-  const double l2s = l * l, m2s = m * m, n2s = n * n;
-  const double s = V[0], p = V[1];
+  const R l = x, m = y, n = z;
+  const R l2s = l * l, m2s = m * m, n2s = n * n;
+  const R s = V[0], p = V[1];
   if (l1 == 1 && l2 == 2) {
     // a: 0=y 1=z 2=x ; b: 0=xy 1=yz 2=z2 3=xz 4=x2-y2
     const int pa = (a == 2) ? 0 : (a == 0 ? 1 : 2);  // 0=x 1=y 2=z
@@ -88,11 +154,11 @@ __device__ double sk_elem(int l1, int l2, int a, int b, double x, double y, doub
     }
   }
   // d-d, symmetric in (a, b)
-  const double d = V[2];
+  const R d = V[2];
   int ia = a < b ? a : b, ib = a < b ? b : a;
-  const double lm = l * m, mn = m * n, nl = n * l;
-  const double q = l2s - m2s;                 // l^2 - m^2
-  const double r = n2s - 0.5 * (l2s + m2s);   // n^2 - (l^2+m^2)/2
+  const R lm = l * m, mn = m * n, nl = n * l;
+  const R q = l2s - m2s;                 // l^2 - m^2
+  const R r = n2s - 0.5 * (l2s + m2s);   // n^2 - (l^2+m^2)/2
   switch (ia * 5 + ib) {
     case 0: return 3.0 * l2s * m2s * s + (l2s + m2s - 4.0 * l2s * m2s) * p + (n2s + l2s * m2s) * d;
     case 1: return 3.0 * l * m2s * n * s + nl * (1.0 - 4.0 * m2s) * p + nl * (m2s - 1.0) * d;
@@ -112,22 +178,28 @@ __device__ double sk_elem(int l1, int l2, int a, int b, double x, double y, doub
   }
 }
 
+__device__ __forceinline__ double sk_elem(int l1, int l2, int a, int b, double x, double y, double z,
+                                          const double* V) {
+  return sk_elem_t<double>(l1, l2, a, b, x, y, z, V);
+}
+
 // Interpolation weights: value = sum_m w[m] * table[base + m]; base = -1 -> 0.
 // Follows slakonet/interpolation.py:107-238 (8-point stencil, clamp, tail).
-__device__ void interp_weights(double r, int ngrid, double step, double tail, int* base, double* w) {
+template <typename R>
+__device__ void interp_weights_t(R r, int ngrid, double step, double tail, int* base, R* w) {
   const int NI = 8;
-  long ind = (long)floor(r / step);
+  long ind = (long)floor(dval(r) / step);
   if (ind < ngrid) {
     long il = ind + 5;
     if (il > ngrid) il = ngrid;
     if (il < NI + 1) il = NI + 1;
     *base = (int)(il - NI - 1);
-    const double t = r / step - (double)(il - NI);   // position in stencil units
-    double left[NI], right[NI];
-    left[0] = 1.0;
-    for (int m = 1; m < NI; ++m) left[m] = left[m - 1] * (t - (m - 1));
-    right[NI - 1] = 1.0;
-    for (int m = NI - 2; m >= 0; --m) right[m] = right[m + 1] * (t - (m + 1));
+    const R t = r / step - (double)(il - NI);   // position in stencil units
+    R left[NI], right[NI];
+    left[0] = R(1.0);
+    for (int m = 1; m < NI; ++m) left[m] = left[m - 1] * (t - (double)(m - 1));
+    right[NI - 1] = R(1.0);
+    for (int m = NI - 2; m >= 0; --m) right[m] = right[m + 1] * (t - (double)(m + 1));
     // denominators prod_{j != m} (m - j) = (-1)^(7-m) m! (7-m)!
     const double fact[8] = {1.0, 1.0, 2.0, 6.0, 24.0, 120.0, 720.0, 5040.0};
     for (int m = 0; m < NI; ++m) {
@@ -143,11 +215,11 @@ __device__ void interp_weights(double r, int ngrid, double step, double tail, in
   const double delta = 1e-5;
   const double rmax = (ngrid - 1) * step + tail;
   const double dx = -tail;
-  const double xr = (r - rmax) / dx;
-  const double x3 = xr * xr * xr;
-  const double A = ((6.0 * xr - 15.0) * xr + 10.0) * x3;
-  const double B = ((-3.0 * xr + 7.0) * xr - 4.0) * x3;
-  const double C = ((0.5 * xr - 1.0) * xr + 0.5) * x3;
+  const R xr = (r - rmax) / dx;
+  const R x3 = xr * xr * xr;
+  const R A = ((6.0 * xr - 15.0) * xr + 10.0) * x3;
+  const R B = ((-3.0 * xr + 7.0) * xr - 4.0) * x3;
+  const R C = ((0.5 * xr - 1.0) * xr + 0.5) * x3;
   double w0[NI], w2[NI];
   for (int pass = 0; pass < 2; ++pass) {
     const double t = (NI - 1) + (pass == 0 ? -delta : delta) / step;
@@ -167,8 +239,12 @@ __device__ void interp_weights(double r, int ngrid, double step, double tail, in
     const double e7 = (m == NI - 1) ? 1.0 : 0.0;
     const double yp = (w2[m] - w0[m]) / (2.0 * delta);
     const double ypp = (w2[m] + w0[m] - 2.0 * e7) / (delta * delta);
-    w[m] = A * e7 + B * dx * yp + C * dx * dx * ypp;
+    w[m] = A * e7 + B * (dx * yp) + C * (dx * dx * ypp);
   }
+}
+
+__device__ __forceinline__ void interp_weights(double r, int ngrid, double step, double tail, int* base, double* w) {
+  interp_weights_t<double>(r, ngrid, step, tail, base, w);
 }
 
 template <typename OUT>
@@ -489,6 +565,12 @@ int slk_pad_launch(const SlkStructDesc* structs_dev, int s, int nk, int n_orb, i
 // ---------------------------------------------------------------------------
 #define ADJ_KCH 8
 
+// MODE 0: gradient with respect to the table rows (out = dtab [rows][20]).
+// MODE 1: gradient with respect to the atom positions (out = [atoms][3]):
+// each pair-image block E(d) depends on d = R_j + T - R_i through the
+// direction cosines and the interpolated integrals; dual numbers give
+// dE/dd exactly, and -dE/dd / +dE/dd go to atoms i / j.
+template <int MODE>
 __global__ __launch_bounds__(BUILD_THREADS) void slk_adjoint_kernel(SlkBuildArgs A, const double2* __restrict__ QH,
                                                                     const double2* __restrict__ QS,
                                                                     double* __restrict__ dtab) {
@@ -503,6 +585,9 @@ __global__ __launch_bounds__(BUILD_THREADS) void slk_adjoint_kernel(SlkBuildArgs
   __shared__ double G[MAXPI][2][81];
   __shared__ double dV[MAXPI][2][20];
   __shared__ double2 ph[MAXPI][ADJ_KCH];
+  __shared__ double pi_r[MAXPI];
+  __shared__ double pi_dw[MAXPI][2][8];
+  __shared__ double gd[MAXPI][3];
   __shared__ int col_atom[MAXCOL], col_slot[MAXCOL], col_orb[MAXCOL];
 
   const SlkBuildWork wk = A.work[blockIdx.x];
@@ -590,15 +675,22 @@ __global__ __launch_bounds__(BUILD_THREADS) void slk_adjoint_kernel(SlkBuildArgs
           pi_u[p][0] = vx / r; pi_u[p][1] = vy / r; pi_u[p][2] = vz / r;
         }
         int base = -1;
-        double w[8];
+        Dual<1> w[8];
         if (pid >= 0) {
-          interp_weights(r, A.tab.pair_ngrid[pid], A.tab.pair_step[pid], A.tab.tail, &base, w);
+          Dual<1> rd(r);
+          rd.g[0] = 1.0;
+          interp_weights_t<Dual<1>>(rd, A.tab.pair_ngrid[pid], A.tab.pair_step[pid], A.tab.tail, &base, w);
           if (base >= 0) base += A.tab.pair_row0[pid];
         }
         pi_base[p][dir] = base;
-        for (int m = 0; m < 8; ++m) pi_w[p][dir][m] = (base >= 0) ? w[m] : 0.0;
+        if (dir == 0) pi_r[p] = r;
+        for (int m = 0; m < 8; ++m) {
+          pi_w[p][dir][m] = (base >= 0) ? w[m].v : 0.0;
+          pi_dw[p][dir][m] = (base >= 0) ? w[m].g[0] : 0.0;
+        }
       }
       for (int q = tid; q < np * 2 * 81; q += BUILD_THREADS) (&G[0][0][0])[q] = 0.0;
+      for (int q = tid; q < np * 3; q += BUILD_THREADS) (&gd[0][0])[q] = 0.0;
       for (int q = tid; q < np * 2 * 20; q += BUILD_THREADS) (&dV[0][0][0])[q] = 0.0;
       __syncthreads();
       // G[p][H/S][a*9+b] = Re sum_k Q(k)[nu][mu] exp(2 pi i k.T), over k chunks
@@ -644,6 +736,7 @@ __global__ __launch_bounds__(BUILD_THREADS) void slk_adjoint_kernel(SlkBuildArgs
         }
         __syncthreads();
       }
+      if (MODE == 0) {
       // contract with the SK coefficients (unit-integral evaluations)
       for (int task = tid; task < np * 81; task += BUILD_THREADS) {
         const int p = task / 81, ab = task % 81, a = ab / 9, b = ab % 9;
@@ -677,6 +770,67 @@ __global__ __launch_bounds__(BUILD_THREADS) void slk_adjoint_kernel(SlkBuildArgs
         if (base < 0 || g == 0.0) continue;
         for (int m = 0; m < 8; ++m) atomicAdd(&dtab[(size_t)(base + m) * 20 + ig], pi_w[p][dir][m] * g);
       }
+      } else {
+        // dE/dd for every (pair-image, a, b) with dual numbers over d
+        for (int task = tid; task < np * 81; task += BUILD_THREADS) {
+          const int p = task / 81, ab = task % 81, a = ab / 9, b = ab % 9;
+          const double gh = G[p][0][ab], gs = G[p][1][ab];
+          if (gh == 0.0 && gs == 0.0) continue;
+          const int la = c_slot_l[a], lb = c_slot_l[b];
+          const int j = pi_j[p];
+          const bool direct = (la < lb) || (la == lb && wk.i <= j);
+          const int dir = direct ? 0 : 1;
+          const int base = pi_base[p][dir];
+          if (base < 0) continue;
+          const double r = pi_r[p];
+          const double u[3] = {pi_u[p][0], pi_u[p][1], pi_u[p][2]};
+          Dual<3> ud[3];
+#pragma unroll
+          for (int c0 = 0; c0 < 3; ++c0) {
+            ud[c0] = Dual<3>(direct ? u[c0] : -u[c0]);
+#pragma unroll
+            for (int e2 = 0; e2 < 3; ++e2) {
+              const double du = ((c0 == e2 ? 1.0 : 0.0) - u[c0] * u[e2]) / r;
+              ud[c0].g[e2] = direct ? du : -du;
+            }
+          }
+          int l1, l2, m1, m2;
+          if (direct) { l1 = la; l2 = lb; m1 = c_slot_m[a]; m2 = c_slot_m[b]; }
+          else { l1 = lb; l2 = la; m1 = c_slot_m[b]; m2 = c_slot_m[a]; }
+          const int ib = lpair_base(l1, l2);
+          Dual<3> vh[3], vs[3];
+          for (int q = 0; q <= l1; ++q) {
+            double h0 = 0.0, h1 = 0.0, s0 = 0.0, s1 = 0.0;
+            for (int m = 0; m < 8; ++m) {
+              const double* tb = A.tab.tables + (size_t)(base + m) * 20;
+              h0 += pi_w[p][dir][m] * tb[ib + q];
+              h1 += pi_dw[p][dir][m] * tb[ib + q];
+              s0 += pi_w[p][dir][m] * tb[10 + ib + q];
+              s1 += pi_dw[p][dir][m] * tb[10 + ib + q];
+            }
+            vh[q] = Dual<3>(h0);
+            vs[q] = Dual<3>(s0);
+#pragma unroll
+            for (int e2 = 0; e2 < 3; ++e2) {
+              vh[q].g[e2] = h1 * u[e2];     // dr/dd_e = u_e
+              vs[q].g[e2] = s1 * u[e2];
+            }
+          }
+          const Dual<3> eh = sk_elem_t<Dual<3>>(l1, l2, m1, m2, ud[0], ud[1], ud[2], vh);
+          const Dual<3> es = sk_elem_t<Dual<3>>(l1, l2, m1, m2, ud[0], ud[1], ud[2], vs);
+#pragma unroll
+          for (int e2 = 0; e2 < 3; ++e2) atomicAdd(&gd[p][e2], gh * eh.g[e2] + gs * es.g[e2]);
+        }
+        __syncthreads();
+        for (int task = tid; task < np * 3; task += BUILD_THREADS) {
+          const int p = task / 3, e2 = task % 3;
+          const double g = gd[p][e2];
+          if (g == 0.0) continue;
+          const int gi = sd.atom0 + wk.i, gj = sd.atom0 + pi_j[p];
+          atomicAdd(&dtab[(size_t)gi * 3 + e2], -g);
+          atomicAdd(&dtab[(size_t)gj * 3 + e2], g);
+        }
+      }
       __syncthreads();
     }
   }
@@ -701,8 +855,15 @@ extern "C" {
 int slk_adjoint_tables_launch(const SlkBuildArgs* args, const void* QH, const void* QS, double* dtab,
                               hipStream_t stream) {
   if (args->n_work <= 0) return 0;
-  hipLaunchKernelGGL(slk_adjoint_kernel, dim3(args->n_work), dim3(BUILD_THREADS), 0, stream, *args,
+  hipLaunchKernelGGL(slk_adjoint_kernel<0>, dim3(args->n_work), dim3(BUILD_THREADS), 0, stream, *args,
                      reinterpret_cast<const double2*>(QH), reinterpret_cast<const double2*>(QS), dtab);
+  return (int)hipGetLastError();
+}
+
+int slk_forces_launch(const SlkBuildArgs* args, const void* QH, const void* QS, double* dR, hipStream_t stream) {
+  if (args->n_work <= 0) return 0;
+  hipLaunchKernelGGL(slk_adjoint_kernel<1>, dim3(args->n_work), dim3(BUILD_THREADS), 0, stream, *args,
+                     reinterpret_cast<const double2*>(QH), reinterpret_cast<const double2*>(QS), dR);
   return (int)hipGetLastError();
 }
 

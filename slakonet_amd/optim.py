@@ -9,7 +9,9 @@ Mirrors slakonet/optim.py:
 * MultiElementSkfParameterOptimizer (:139-1352) with
   compute_multi_element_properties(geometry, shell_dict, kpoints, klines,
   phonons, get_fermi, get_energy, get_forces, get_bulk_mod, device,
-  with_eigenvectors) -> (properties, True) (:949-1074).
+  with_eigenvectors) -> (properties, True) (:949-1074); get_forces uses the
+  hand-written position adjoint (phonons and the bulk modulus are not on
+  this path).
 The device tables are packed once from the parameters on every call (no
 per-call rebuild of interpolator objects) and gradients flow through the
 hand-written adjoint when the parameters require grad.
@@ -154,8 +156,8 @@ class MultiElementSkfParameterOptimizer(nn.Module):
                                          get_bulk_mod=False, device=None, with_eigenvectors=False,
                                          fermi_method: str = "bisection", kpoints_frac=None, k_weights=None):
         """Properties of one structure (slakonet/optim.py:949-1074); returns (dict, True)."""
-        if phonons or get_forces or get_bulk_mod:
-            raise NotImplementedError("phonons, forces and bulk modulus are outside the hot path")
+        if phonons or get_bulk_mod:
+            raise NotImplementedError("phonons and bulk modulus are outside the hot path")
         if device is None:
             device = "cuda"
         dev = torch.device(device)
@@ -165,7 +167,8 @@ class MultiElementSkfParameterOptimizer(nn.Module):
         tables = self.table_set(shell_dict, dev)
         grad = torch.is_grad_enabled() and any(p.requires_grad for p in self.parameters())
         calc = SimpleDftb(geometry, shell_dict=shell_dict, tables=tables, kpoints=kpoints, klines=klines,
-                          device=dev, with_eigenvectors=with_eigenvectors or grad, kpoints_frac=kpoints_frac,
+                          device=dev, with_eigenvectors=with_eigenvectors or grad or get_forces,
+                          kpoints_frac=kpoints_frac,
                           k_weights=k_weights, fermi_method=fermi_method,
                           table_tensor=tables.tables.detach())
         eigenvalues = calc()
@@ -190,6 +193,8 @@ class MultiElementSkfParameterOptimizer(nn.Module):
             properties["calc"] = calc
         if get_energy:
             properties["total_energy"] = calc._calculate_electronic_energy()
+        if get_forces:
+            properties["forces"] = calc.get_forces()
         properties["eigenvalues"] = eigenvalues
         return properties, True
 
