@@ -405,6 +405,132 @@ __global__ __launch_bounds__(TAIL_THREADS) void hetrd_tail_kernel(HetrdCtx<T> c,
   if (j + 1 < jn) latrd_next(c, b, j + 1, jj + 1, red);
 }
 
+// Finisher: unblocked Householder on the trailing block (rows/columns >= j0,
+// at most HF_MAX of them) of each matrix, one workgroup per matrix, every
+// column in one launch. The block's lower triangle is current (all panel
+// updates applied); per column: reflector from column j, y = A22 v from the
+// lower triangle (waves own rows; the A^H part goes to per-wave column
+// accumulators), w = tau y - |tau|^2/2 (v^H y) v, A22 -= v w^H + w v^H on
+// the lower triangle. Outputs (d, e, tau, reflectors in column j and VT)
+// match the blocked path's.
+#define HF_MAX 256
+#define HF_THREADS 512
+#define HF_WAVES (HF_THREADS / 64)
+template <typename T>
+__global__ __launch_bounds__(HF_THREADS) void hetrd_finish_kernel(HetrdCtx<T> c, int j0) {
+  using C2 = typename C2T<T>::t;
+  __shared__ T red[HF_WAVES];
+  __shared__ C2 s_v[HF_MAX], s_y[HF_MAX], s_w[HF_MAX];
+  __shared__ C2 s_col[HF_WAVES][HF_MAX];
+  __shared__ C2 s_tau, s_scale;
+  __shared__ T s_beta;
+  __shared__ int s_skip;
+  const int b = blockIdx.x;
+  const int n = c.n, ld = c.ld;
+  C2* a = c.A + (size_t)b * ld * ld;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const C2 zero = mk<T>(0, 0);
+  for (int j = j0; j < n - 1; ++j) {
+    const int base = j + 1;           // first row / column of A22
+    const int m = n - base;
+    // (1) reflector from column j (rows >= j+1)
+    T part = 0;
+    for (int r = j + 2 + tid; r < n; r += HF_THREADS) {
+      const C2 x = a[(size_t)r * ld + j];
+      part += x.x * x.x + x.y * x.y;
+    }
+    const T xn2 = block_sum(part, red);
+    if (tid == 0) {
+      const C2 alpha = a[(size_t)base * ld + j];
+      if (xn2 == T(0) && alpha.y == T(0)) {
+        s_skip = 1;
+        s_tau = zero;
+        s_beta = alpha.x;
+        s_scale = zero;
+      } else {
+        s_skip = 0;
+        const T nrm = sqrt(alpha.x * alpha.x + alpha.y * alpha.y + xn2);
+        const T beta = alpha.x >= T(0) ? -nrm : nrm;
+        s_beta = beta;
+        s_tau = mk<T>((beta - alpha.x) / beta, -alpha.y / beta);
+        const C2 den = mk<T>(alpha.x - beta, alpha.y);
+        const T dd = den.x * den.x + den.y * den.y;
+        s_scale = mk<T>(den.x / dd, -den.y / dd);
+      }
+      c.D[(size_t)b * ld + j] = (double)a[(size_t)j * ld + j].x;
+      c.E[(size_t)b * ld + j] = (double)s_beta;
+      c.TAU[(size_t)b * ld + j] = s_tau;
+    }
+    __syncthreads();
+    const C2 scale = s_scale, tau = s_tau;
+    const int skip = s_skip;
+    for (int i = tid; i < m; i += HF_THREADS) {
+      const int r = base + i;
+      C2 vv = i == 0 ? mk<T>(1, 0) : (skip ? zero : c_mul(a[(size_t)r * ld + j], scale));
+      s_v[i] = vv;
+      a[(size_t)r * ld + j] = vv;
+      if (c.VT != nullptr) c.VT[(size_t)b * ld * ld + (size_t)j * ld + r] = vv;
+    }
+    for (int q = tid; q < HF_WAVES * HF_MAX; q += HF_THREADS) (&s_col[0][0])[q] = zero;
+    __syncthreads();
+    if (tau.x == T(0) && tau.y == T(0)) continue;   // H_j = I: nothing to apply
+    // (2) y = A22 v from the lower triangle
+    for (int i = wv; i < m; i += HF_WAVES) {
+      const int r = base + i;
+      const C2 vr = s_v[i];
+      T px = 0, py = 0;
+      for (int q = lane; q <= i; q += 64) {
+        const C2 x = a[(size_t)r * ld + base + q];
+        const C2 vq = s_v[q];
+        px += x.x * vq.x - x.y * vq.y;
+        py += x.x * vq.y + x.y * vq.x;
+        if (q < i) {   // conj(A[r][q]) v[r] -> y[q]
+          C2 cc = s_col[wv][q];
+          cc.x += x.x * vr.x + x.y * vr.y;
+          cc.y += x.x * vr.y - x.y * vr.x;
+          s_col[wv][q] = cc;
+        }
+      }
+      px = wave_sum(px);
+      py = wave_sum(py);
+      if (lane == 0) s_y[i] = mk<T>(px, py);
+    }
+    __syncthreads();
+    T vhy_part = 0;
+    for (int i = tid; i < m; i += HF_THREADS) {
+      C2 yy = s_y[i];
+#pragma unroll
+      for (int w2 = 0; w2 < HF_WAVES; ++w2) yy = c_add(yy, s_col[w2][i]);
+      s_y[i] = yy;
+      const C2 vv = s_v[i];
+      vhy_part += vv.x * yy.x + vv.y * yy.y;
+    }
+    // (3) w = tau y - |tau|^2/2 (v^H y) v
+    const T vhy = block_sum(vhy_part, red);
+    const T t2 = T(0.5) * (tau.x * tau.x + tau.y * tau.y) * vhy;
+    for (int i = tid; i < m; i += HF_THREADS) {
+      const C2 p = c_mul(tau, s_y[i]);
+      const C2 vv = s_v[i];
+      s_w[i] = mk<T>(p.x - t2 * vv.x, p.y - t2 * vv.y);
+    }
+    __syncthreads();
+    // (4) A22 -= v w^H + w v^H (lower triangle)
+    for (int i = wv; i < m; i += HF_WAVES) {
+      const int r = base + i;
+      const C2 vr = s_v[i], wr = s_w[i];
+      for (int q = lane; q <= i; q += 64) {
+        C2 x = a[(size_t)r * ld + base + q];
+        const C2 t1 = c_mulc(vr, s_w[q]);
+        const C2 t3 = c_mulc(wr, s_v[q]);
+        x.x -= t1.x + t3.x;
+        x.y -= t1.y + t3.y;
+        a[(size_t)r * ld + base + q] = x;
+      }
+    }
+    __syncthreads();
+  }
+}
+
 template <typename T>
 __global__ void hetrd_last_kernel(const typename C2T<T>::t* A, int n, int ld, double* D, double* E,
                                   typename C2T<T>::t* TAU) {
@@ -446,7 +572,11 @@ static void hetrd_impl(hipStream_t st, typename C2T<T>::t* C, int n, int ld, int
   c.C12p = C12;
   c.D = D; c.E = E; c.ncmax = (ld + HT_TS - 1) / HT_TS;
   const long long sP = (long long)NB * ld, sM = (long long)ld * ld;
-  for (int j0 = 0; j0 < n - 1; j0 += NB) {
+  // blocked panels until the trailing block fits the one-workgroup finisher
+  // (a panel boundary jf with n - jf <= HF_MAX), then the finisher
+  int jf = n - HF_MAX;
+  jf = jf <= 0 ? 0 : ((jf + NB - 1) / NB) * NB;
+  for (int j0 = 0; j0 < jf && j0 < n - 1; j0 += NB) {
     const int jn = min(j0 + NB, n - 1);
     hipLaunchKernelGGL(latrd_start_kernel<T>, dim3(batch), dim3(TAIL_THREADS), 0, st, c, j0);
     for (int j = j0; j < jn; ++j) {
@@ -467,6 +597,7 @@ static void hetrd_impl(hipStream_t st, typename C2T<T>::t* C, int n, int ld, int
     }
     if (jn < n) rank2k<T>(st, batch, n - jn, jn - j0, Vc + jn, Wc + jn, C + (size_t)jn * ld + jn, ld, sP, sM);
   }
+  if (jf < n - 1) hipLaunchKernelGGL(hetrd_finish_kernel<T>, dim3(batch), dim3(HF_THREADS), 0, st, c, jf);
   hipLaunchKernelGGL(hetrd_last_kernel<T>, dim3(batch), dim3(64), 0, st, C, n, ld, D, E, TAU);
 }
 
