@@ -132,6 +132,15 @@ int slk_eigh_mixed(const void* H, const void* S, int n, int ld, int batch, doubl
                    size_t ws_bytes, int* info, hipStream_t stream);
 int slk_eigh_mixed_timed(const void* H, const void* S, int n, int ld, int batch, double* w, int wld, void* ws,
                          size_t ws_bytes, int* info, hipStream_t stream, float* stage_ms);
+/* Tridiagonalisation used by the eigenvalue-only solves (slk_eigh_f64 with
+ * V null, slk_eigh_mixed, and the matching slk_refine_edges_mixed): 0
+ * automatic (two-stage: dense -> band of width 16 -> tridiagonal by bulge
+ * chasing), 1 blocked one-stage Householder, 2 two-stage. Process-wide; set it
+ * before solving, not while solves are in flight. Replaces the
+ * tridiagonalisation inside torch.linalg.eigh called by eighb
+ * (slakonet/utils.py:585-816); the reference exposes no such switch.
+ * Returns 1 for an unknown mode. */
+int slk_set_tridiag_mode(int mode);
 /* Eigenvalues only for n <= 32: the whole solve of one matrix in LDS of one
  * wave64 workgroup, one launch per batch (slk_eigh_f64 dispatches here for
  * values-only solves of that size). info: [batch]. */

@@ -80,6 +80,7 @@ SIGNATURES = {
                                  C.c_void_p, C.c_size_t, C.c_void_p, C.c_void_p]),
     "slk_eigh_mixed_timed": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_int,
                                        C.c_void_p, C.c_size_t, C.c_void_p, C.c_void_p, C.POINTER(C.c_float)]),
+    "slk_set_tridiag_mode": (C.c_int, [C.c_int]),
     "slk_eigh_small_f64": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_int,
                                      C.c_void_p, C.c_void_p]),
     "slk_eigh_f64_timed": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_int,

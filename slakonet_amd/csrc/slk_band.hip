@@ -1,0 +1,791 @@
+// Two-stage reduction of a batch of Hermitian matrices to real tridiagonal
+// form for the eigenvalue path (FP64 or FP32 arithmetic; d, e always FP64).
+//
+// Stage 1, dense -> band of width BW = 16, per 16-column panel j0
+// (rows r0 = j0 + BW .. n-1, m = n - r0; LAPACK zhetrd_he2hb structure):
+//   panel_qr   one workgroup per matrix: Householder QR of the m x BW panel
+//              held in registers (thread = (row group, column)), one block
+//              reduction for the column norm and one for the 15 inner products
+//              per column; the block reflector factor T (Q = I - V T V^H) is
+//              built alongside. R goes into the band, the reflectors below it.
+//   band_av    Y = A22 V on MFMA, 64 rows x 256-column chunks (split K).
+//   band_x     X = Y T, per 64-row block partial Z = V^H X.
+//   band_w     W = X - V (T^H Z) / 2.
+//   rank-2k    A22 -= V W^H + W V^H on the MFMA GEMM (full tiles, so A22
+//              stays fully stored for the next panel's product).
+// Stage 2, band -> tridiagonal: band_chase, one workgroup (16 waves) per
+// matrix on a compact copy of the band (column j holds A[j..j+2BW-1][j]).
+// Sweep s annihilates column s below the subdiagonal with one reflector and
+// chases the bulge down in steps of BW rows; each step is one wave: reflector
+// from the bulge column, left application to the bulge block, two-sided
+// application to the diagonal block, right application to the block below.
+// Sweep s+1 may run step c together with sweep s's step c+3 (their regions
+// are disjoint), so 16 sweeps run at once, in lock-step rounds of one step.
+#include "slk_common.h"
+#include "slk_dense.h"
+
+#define BW SLK_BW
+#define AV_KT 64                      // k per LDS tile of band_av
+#define AV_KCH 128                    // k per workgroup of band_av (split K)
+#define CH_THREADS 1024
+#define CH_WAVES (CH_THREADS / 64)
+#define BND_LD (2 * BW)               // compact band: entries per column
+
+template <typename T>
+struct BandCtx {
+  using C2 = typename C2T<T>::t;
+  C2* A;
+  int n, ld;
+  C2 *Vc, *Wc;   // [batch][2 BW][ld]: [V | W] and [W | V] (column k at Vc[k*ld + r]); X goes to Wc first
+  C2* Yp;        // [batch][nch][ld][BW]: split-K partials of A22 V
+  C2* Zp;        // [batch][nrb][BW][BW]: per row block partials of V^H X
+  C2* TT;        // [batch][BW][BW]: block reflector factor, row-major
+  C2* TAU;       // [batch][ld]: stage-1 tau of column j
+  C2* VT;        // nullable [batch][ld][ld]: stage-1 reflector of column j as row j
+  C2* BND;       // [batch][ld][BND_LD]: compact band for the chase
+  int* STARTS;   // [batch][ld]: first round of each sweep
+  double *D, *E;
+  int nch, nrb;  // chunk / row-block counts used for the strides of Yp, Zp
+};
+
+template <typename T> struct MF;
+template <> struct MF<float> {
+  using v4 = __attribute__((ext_vector_type(4))) float;
+  static __device__ __forceinline__ v4 mfma(float a, float b, v4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+  // output row of accumulator element v in lane group g (column = lane % 16)
+  static __device__ __forceinline__ int row(int g, int v) { return 4 * g + v; }
+};
+template <> struct MF<double> {
+  using v4 = __attribute__((ext_vector_type(4))) double;
+  static __device__ __forceinline__ v4 mfma(double a, double b, v4 c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ int row(int g, int v) { return g + 4 * v; }
+};
+
+template <typename C2>
+__device__ __forceinline__ C2 shfl_c(C2 v, int src) {
+  v.x = __shfl(v.x, src, 64);
+  v.y = __shfl(v.y, src, 64);
+  return v;
+}
+template <typename C2>
+__device__ __forceinline__ C2 shfl_xor_c(C2 v, int o) {
+  v.x = __shfl_xor(v.x, o, 64);
+  v.y = __shfl_xor(v.y, o, 64);
+  return v;
+}
+// DPP row operations (within rows of 16 lanes): quad_perm [1,0,3,2] (0xB1),
+// quad_perm [2,3,0,1] (0x4E), row_half_mirror (0x141), row_mirror (0x140).
+template <int CTRL>
+__device__ __forceinline__ float dpp(float v) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), CTRL, 0xF, 0xF, false));
+}
+template <int CTRL>
+__device__ __forceinline__ double dpp(double v) {
+  const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(u & 0xffffffffull), CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(u >> 32), CTRL, 0xF, 0xF, false);
+  return __builtin_bit_cast(double, ((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
+}
+// sum over the 16 lanes of a lane group (lanes 16g .. 16g+15), every lane
+// gets the total: pair sums, quad sums, then the mirrored quad and half row
+// (whose values are already uniform) complete the butterfly
+template <typename T>
+__device__ __forceinline__ T sum16(T v) {
+  v += dpp<0xB1>(v);
+  v += dpp<0x4E>(v);
+  v += dpp<0x141>(v);
+  v += dpp<0x140>(v);
+  return v;
+}
+template <typename C2>
+__device__ __forceinline__ C2 sum16c(C2 v) {
+  v.x = sum16(v.x);
+  v.y = sum16(v.y);
+  return v;
+}
+// sum over the 4 lane groups of a wave (same lane % 16)
+template <typename C2>
+__device__ __forceinline__ C2 sum4g(C2 v) {
+  v = c_add(v, shfl_xor_c(v, 16));
+  return c_add(v, shfl_xor_c(v, 32));
+}
+
+// Householder reflector of x = (alpha, rest) with |rest|^2 = xn2 (LAPACK
+// zlarfg convention H^H x = beta e0, H = I - tau v v^H, v = (1, rest * scale)).
+template <typename T>
+struct House {
+  typename C2T<T>::t tau, scale;
+  T beta;
+  bool skip;
+};
+template <typename T>
+__device__ __forceinline__ House<T> house(typename C2T<T>::t alpha, T xn2) {
+  House<T> h;
+  if (xn2 == T(0) && alpha.y == T(0)) {
+    h.skip = true;
+    h.beta = alpha.x;
+    h.tau = mk<T>(0, 0);
+    h.scale = mk<T>(0, 0);
+    return h;
+  }
+  h.skip = false;
+  const T nrm = sqrt(alpha.x * alpha.x + alpha.y * alpha.y + xn2);
+  const T bt = alpha.x >= T(0) ? -nrm : nrm;
+  h.beta = bt;
+  h.tau = mk<T>((bt - alpha.x) / bt, -alpha.y / bt);
+  const T dx = alpha.x - bt, dy = alpha.y, dd = dx * dx + dy * dy;
+  h.scale = mk<T>(dx / dd, -dy / dd);
+  return h;
+}
+
+// ---------------------------------------------------------------------------
+// Stage 1, panel QR. Thread t owns column j = t % BW of rows r = t / BW +
+// PQ_GROUPS q of the panel (registers for RPT > 0, global memory for RPT = 0).
+// The 16 columns of a row group are 16 consecutive lanes, so a reflector
+// element reaches its row group by one shuffle.
+template <typename T, int RPT, int NT>
+__global__ __launch_bounds__(NT) void panel_qr_kernel(BandCtx<T> c, int j0) {
+  constexpr int NG = NT / BW;   // row groups
+  using C2 = typename C2T<T>::t;
+  const int b = blockIdx.x;
+  const int n = c.n, ld = c.ld;
+  const int r0 = j0 + BW, m = n - r0;
+  const int kb = m < BW ? m : BW;
+  C2* a = c.A + (size_t)b * ld * ld;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int j = tid & (BW - 1), rg = tid >> 4;
+  const int gsrc = lane & ~(BW - 1);
+  __shared__ T s_nrm[NT / 64];
+  __shared__ C2 s_u[NT / 64][BW];
+  __shared__ C2 s_T[BW][BW];
+  __shared__ C2 s_G[BW][BW];   // s_G[k][l] = V[:,l]^H v_k, l < k
+  __shared__ C2 s_alpha;
+  __shared__ C2 s_tau[BW];
+  __shared__ C2 s_v[RPT > 0 ? NG * RPT : 1];   // current reflector (register variant)
+  const int nq = RPT > 0 ? RPT : (m + NG - 1) / NG;
+  constexpr int QN = RPT > 0 ? RPT : 1;   // rows per fully unrolled inner loop
+  C2 p[RPT > 0 ? RPT : 1];
+  const C2 zero = mk<T>(0, 0);
+  auto gaddr = [&](int q) -> C2* { return a + (size_t)(r0 + rg + NG * q) * ld + j0 + j; };
+  auto get = [&](int q) -> C2 {
+    if constexpr (RPT > 0) return p[q];
+    else return (rg + NG * q < m) ? *gaddr(q) : mk<T>(0, 0);
+  };
+  auto put = [&](int q, C2 v) {
+    if constexpr (RPT > 0) p[q] = v;
+    else if (rg + NG * q < m) *gaddr(q) = v;
+  };
+  if constexpr (RPT > 0) {
+    const C2* pl = a + (size_t)(r0 + rg) * ld + j0 + j;
+#pragma unroll
+    for (int q = 0; q < RPT; ++q) {
+      const int r = rg + NG * q;
+      p[q] = r < m ? pl[(size_t)NG * q * ld] : zero;
+    }
+  }
+  if (tid < BW * BW) s_T[tid / BW][tid % BW] = zero;
+  // column 0: partial norm below row 0 and alpha
+  T nrm_part = 0;
+  for (int q0 = 0; q0 < nq; q0 += QN)
+#pragma unroll
+    for (int qq = 0; qq < QN; ++qq) {
+    const int q = q0 + qq;
+    const int r = rg + NG * q;
+    const C2 e = get(q);
+    if (j == 0 && r > 0 && r < m) nrm_part += e.x * e.x + e.y * e.y;
+    if (j == 0 && r == 0) s_alpha = e;
+  }
+  for (int k = 0; k < kb; ++k) {
+    // (1) norm of column k below row k
+    T v1 = nrm_part;
+    v1 += __shfl_xor(v1, 16, 64);
+    v1 += __shfl_xor(v1, 32, 64);
+    if (lane == k) s_nrm[wv] = v1;
+    __syncthreads();
+    T xn2 = 0;
+#pragma unroll
+    for (int w = 0; w < NT / 64; ++w) xn2 += s_nrm[w];
+    const House<T> h = house<T>(s_alpha, xn2);
+    const T beta = h.beta;
+    const C2 tau = h.tau, scale = h.scale;
+    const bool skip = h.skip;
+    // (2) the reflector replaces column k (rows > k), beta goes to row k;
+    // (3) partial u_j = sum_r conj(v_r) A[r][j] over rows r >= k
+    C2 u = zero;
+    if constexpr (RPT > 0) {
+      // the owner lane of each row publishes v_r in LDS; the other 15 lanes of
+      // the row group read it back (same wave, LDS operations in order)
+#pragma unroll
+      for (int q = 0; q < RPT; ++q) {
+        const int r = rg + NG * q;
+        if (j == k) {
+          C2 e = p[q];
+          C2 vv = zero;
+          if (r == k) {
+            e = mk<T>(beta, 0);
+            vv = mk<T>(1, 0);
+          } else if (r > k && r < m) {
+            e = skip ? zero : c_mul(e, scale);
+            vv = e;
+          }
+          p[q] = e;
+          s_v[r] = vv;
+        }
+      }
+      __builtin_amdgcn_wave_barrier();
+#pragma unroll
+      for (int q = 0; q < RPT; ++q) {
+        const C2 t = c_cmul(s_v[rg + NG * q], p[q]);
+        u.x += t.x;
+        u.y += t.y;
+      }
+    } else {
+      for (int q = 0; q < nq; ++q) {
+        const int r = rg + NG * q;
+        C2 e = get(q);
+        if (j == k) {
+          if (r == k) e = mk<T>(beta, 0);
+          else if (r > k && r < m) e = skip ? zero : c_mul(e, scale);
+          put(q, e);
+        }
+        C2 vr = shfl_c(e, gsrc | k);
+        if (r == k) vr = mk<T>(1, 0);
+        else if (r < k || r >= m) vr = zero;
+        const C2 t = c_cmul(vr, e);
+        u.x += t.x;
+        u.y += t.y;
+      }
+    }
+    u = c_add(u, shfl_xor_c(u, 16));
+    u = c_add(u, shfl_xor_c(u, 32));
+    if (lane < BW) s_u[wv][lane] = u;
+    if (tid == 0) s_tau[k] = tau;
+    __syncthreads();
+    C2 uj = zero;
+#pragma unroll
+    for (int w = 0; w < NT / 64; ++w) uj = c_add(uj, s_u[w][j]);
+    // T[0:k, k] = -tau_k T[0:k, 0:k] g with g_l = V[:,l]^H v_k = conj(u_l):
+    // lanes l < k of wave 0 publish g_l, lane i < k updates row i of T (rows
+    // are private to their lane; same wave, LDS operations in order)
+    if (tid < k) s_G[k][tid] = c_conj(uj);
+    if (tid < BW) {
+      __builtin_amdgcn_wave_barrier();
+      C2 acc = zero;
+#pragma unroll
+      for (int l = 0; l < BW; ++l)
+        if (l >= tid && l < k) acc = c_add(acc, c_mul(s_T[tid][l], s_G[k][l]));
+      const C2 t = c_mul(tau, acc);
+      s_T[tid][k] = tid < k ? mk<T>(-t.x, -t.y) : (tid == k ? tau : zero);
+    }
+    // (4) columns j > k: A[r][j] -= conj(tau) v_r u_j; partial norm of column k+1
+    const C2 cu = c_mul(c_conj(tau), uj);
+    nrm_part = 0;
+    for (int q0 = 0; q0 < nq; q0 += QN)
+#pragma unroll
+      for (int qq = 0; qq < QN; ++qq) {
+      const int q = q0 + qq;
+      const int r = rg + NG * q;
+      C2 e = get(q);
+      C2 vr;
+      if constexpr (RPT > 0) {
+        vr = s_v[r];
+      } else {
+        vr = shfl_c(e, gsrc | k);
+        if (r == k) vr = mk<T>(1, 0);
+        else if (r < k || r >= m) vr = zero;
+      }
+      if (j > k && r >= k && r < m) {
+        const C2 t = c_mul(vr, cu);
+        e.x -= t.x;
+        e.y -= t.y;
+        put(q, e);
+      }
+      if (j == k + 1) {
+        if (r > k + 1 && r < m) nrm_part += e.x * e.x + e.y * e.y;
+        if (r == k + 1) s_alpha = e;
+      }
+    }
+  }
+  __syncthreads();
+  // outputs: panel (R in the band, reflectors below), Vc, VT rows, TAU, T.
+  // The row index is passed through an empty asm statement so that the
+  // addresses are recomputed here instead of being held since the loads.
+  int rgo = rg;
+  asm volatile("" : "+v"(rgo));
+  C2* vc = c.Vc + (size_t)b * 2 * BW * ld;
+  C2* wcv = c.Wc + (size_t)b * 2 * BW * ld + (size_t)BW * ld;
+  C2* pa = a + (size_t)(r0 + rgo) * ld + j0 + j;
+  for (int q0 = 0; q0 < nq; q0 += QN)
+#pragma unroll
+    for (int qq = 0; qq < QN; ++qq) {
+    const int q = q0 + qq;
+    const int r = rgo + NG * q;
+    if (r >= m) continue;
+    const C2 e = get(q);
+    if constexpr (RPT > 0) pa[(size_t)NG * q * ld] = e;
+    C2 v = zero;
+    if (j < kb) v = r > j ? e : (r == j ? mk<T>(1, 0) : zero);
+    vc[(size_t)j * ld + r0 + r] = v;
+    wcv[(size_t)j * ld + r0 + r] = v;
+    if (c.VT != nullptr && j < kb && r >= j) c.VT[(size_t)b * ld * ld + (size_t)(j0 + j) * ld + r0 + r] = v;
+  }
+  if (tid < BW) c.TAU[(size_t)b * ld + j0 + tid] = tid < kb ? s_tau[tid] : zero;
+  if (tid < BW * BW) c.TT[(size_t)b * BW * BW + tid] = s_T[tid / BW][tid % BW];
+}
+
+// ---------------------------------------------------------------------------
+// Stage 1, Y = A22 V: workgroup (row block, k chunk, matrix), 4 waves of 16
+// rows x 16 columns, complex product as 4 real 16x16x4 MFMAs per k-step.
+// The next 64-column tile is fetched into registers while the current one is
+// multiplied from LDS.
+template <typename T>
+__global__ __launch_bounds__(256) void band_av_kernel(BandCtx<T> c, int r0) {
+  using C2 = typename C2T<T>::t;
+  using V4 = typename MF<T>::v4;
+  const int n = c.n, ld = c.ld;
+  const int b = blockIdx.z;
+  const int row0 = r0 + blockIdx.x * 64;
+  const int kbeg = r0 + blockIdx.y * AV_KCH;
+  const int kend = min(kbeg + AV_KCH, n);
+  const C2* a = c.A + (size_t)b * ld * ld;
+  const C2* vc = c.Vc + (size_t)b * 2 * BW * ld;
+  __shared__ T Ar[64][AV_KT + 1], Ai[64][AV_KT + 1];
+  __shared__ T Br[AV_KT][BW + 1], Bi[AV_KT][BW + 1];
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int li = lane & 15, lk = lane >> 4;
+  constexpr int NA = 64 * AV_KT / 256, NBT = AV_KT * BW / 256;
+  C2 ra[NA], rb[NBT];
+  auto fetch = [&](int k0) {
+#pragma unroll
+    for (int i = 0; i < NA; ++i) {
+      const int e = tid + 256 * i;
+      const int gr = row0 + e / AV_KT, gk = k0 + e % AV_KT;
+      ra[i] = (gr < n && gk < kend) ? a[(size_t)gr * ld + gk] : mk<T>(0, 0);
+    }
+#pragma unroll
+    for (int i = 0; i < NBT; ++i) {
+      const int e = tid + 256 * i;
+      const int gk = k0 + e % AV_KT;
+      rb[i] = gk < kend ? vc[(size_t)(e / AV_KT) * ld + gk] : mk<T>(0, 0);
+    }
+  };
+  V4 accR = V4{0, 0, 0, 0}, accI = V4{0, 0, 0, 0};
+  fetch(kbeg);
+  for (int k0 = kbeg; k0 < kend; k0 += AV_KT) {
+#pragma unroll
+    for (int i = 0; i < NA; ++i) {
+      const int e = tid + 256 * i;
+      Ar[e / AV_KT][e % AV_KT] = ra[i].x;
+      Ai[e / AV_KT][e % AV_KT] = ra[i].y;
+    }
+#pragma unroll
+    for (int i = 0; i < NBT; ++i) {
+      const int e = tid + 256 * i;
+      Br[e % AV_KT][e / AV_KT] = rb[i].x;
+      Bi[e % AV_KT][e / AV_KT] = rb[i].y;
+    }
+    __syncthreads();
+    if (k0 + AV_KT < kend) fetch(k0 + AV_KT);
+#pragma unroll
+    for (int k4 = 0; k4 < AV_KT; k4 += 4) {
+      const T ar = Ar[16 * wv + li][k4 + lk], ai = Ai[16 * wv + li][k4 + lk];
+      const T br = Br[k4 + lk][li], bi = Bi[k4 + lk][li];
+      accR = MF<T>::mfma(ar, br, accR);
+      accR = MF<T>::mfma(-ai, bi, accR);
+      accI = MF<T>::mfma(ar, bi, accI);
+      accI = MF<T>::mfma(ai, br, accI);
+    }
+    __syncthreads();
+  }
+  C2* yp = c.Yp + ((size_t)b * c.nch + blockIdx.y) * ld * BW;
+#pragma unroll
+  for (int v = 0; v < 4; ++v) {
+    const int gr = row0 + 16 * wv + MF<T>::row(lk, v);
+    if (gr < n) yp[(size_t)gr * BW + li] = mk<T>(accR[v], accI[v]);
+  }
+}
+
+// Stage 1, X = Y T for a block of 64 rows, and the block's partial V^H X.
+template <typename T>
+__global__ __launch_bounds__(256) void band_x_kernel(BandCtx<T> c, int r0, int nchk) {
+  using C2 = typename C2T<T>::t;
+  const int n = c.n, ld = c.ld;
+  const int b = blockIdx.y;
+  const int row0 = r0 + blockIdx.x * 64;
+  __shared__ C2 s_T[BW][BW];
+  __shared__ C2 s_y[64][BW + 1], s_v[64][BW + 1], s_x[64][BW + 1];
+  const int tid = threadIdx.x;
+  s_T[tid / BW][tid % BW] = c.TT[(size_t)b * BW * BW + tid];
+  const int rl = tid >> 2, c0 = (tid & 3) * 4;
+  const int gr = row0 + rl;
+  const C2* vc = c.Vc + (size_t)b * 2 * BW * ld;
+  C2* wc = c.Wc + (size_t)b * 2 * BW * ld;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int cc = c0 + q;
+    C2 y = mk<T>(0, 0), v = mk<T>(0, 0);
+    if (gr < n) {
+      for (int ch = 0; ch < nchk; ++ch) y = c_add(y, c.Yp[(((size_t)b * c.nch + ch) * ld + gr) * BW + cc]);
+      v = vc[(size_t)cc * ld + gr];
+    }
+    s_y[rl][cc] = y;
+    s_v[rl][cc] = v;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int cc = c0 + q;
+    C2 x = mk<T>(0, 0);
+#pragma unroll
+    for (int i = 0; i < BW; ++i) {
+      const C2 t = c_mul(s_y[rl][i], s_T[i][cc]);
+      x.x += t.x;
+      x.y += t.y;
+    }
+    s_x[rl][cc] = x;
+    if (gr < n) wc[(size_t)cc * ld + gr] = x;
+  }
+  __syncthreads();
+  const int i = tid / BW, kk = tid % BW;
+  C2 z = mk<T>(0, 0);
+  for (int r = 0; r < 64; ++r) {
+    const C2 t = c_cmul(s_v[r][i], s_x[r][kk]);
+    z.x += t.x;
+    z.y += t.y;
+  }
+  c.Zp[((size_t)b * c.nrb + blockIdx.x) * BW * BW + tid] = z;
+}
+
+// Stage 1, W = X - V M / 2 with M = T^H (V^H X) (Hermitian), 64 rows per block.
+template <typename T>
+__global__ __launch_bounds__(256) void band_w_kernel(BandCtx<T> c, int r0, int nrbk) {
+  using C2 = typename C2T<T>::t;
+  const int n = c.n, ld = c.ld;
+  const int b = blockIdx.y;
+  const int row0 = r0 + blockIdx.x * 64;
+  __shared__ C2 s_T[BW][BW], s_Z[BW][BW], s_M[BW][BW];
+  __shared__ C2 s_v[64][BW + 1];
+  const int tid = threadIdx.x;
+  const int i = tid / BW, kk = tid % BW;
+  C2 z = mk<T>(0, 0);
+  for (int q = 0; q < nrbk; ++q) z = c_add(z, c.Zp[((size_t)b * c.nrb + q) * BW * BW + tid]);
+  s_Z[i][kk] = z;
+  s_T[i][kk] = c.TT[(size_t)b * BW * BW + tid];
+  const int rl = tid >> 2, c0 = (tid & 3) * 4;
+  const int gr = row0 + rl;
+  const C2* vc = c.Vc + (size_t)b * 2 * BW * ld;
+  C2* wc = c.Wc + (size_t)b * 2 * BW * ld;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) s_v[rl][c0 + q] = gr < n ? vc[(size_t)(c0 + q) * ld + gr] : mk<T>(0, 0);
+  __syncthreads();
+  C2 mm = mk<T>(0, 0);
+#pragma unroll
+  for (int l = 0; l < BW; ++l) {
+    const C2 t = c_cmul(s_T[l][i], s_Z[l][kk]);
+    mm.x += t.x;
+    mm.y += t.y;
+  }
+  s_M[i][kk] = mm;
+  __syncthreads();
+  if (gr >= n) return;
+  C2* vw = c.Vc + (size_t)b * 2 * BW * ld + (size_t)BW * ld;   // W also goes after V
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int cc = c0 + q;
+    C2 x = wc[(size_t)cc * ld + gr];
+#pragma unroll
+    for (int l = 0; l < BW; ++l) {
+      const C2 t = c_mul(s_v[rl][l], s_M[l][cc]);
+      x.x -= T(0.5) * t.x;
+      x.y -= T(0.5) * t.y;
+    }
+    wc[(size_t)cc * ld + gr] = x;
+    vw[(size_t)cc * ld + gr] = x;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Stage 2, bulge chasing on the compact band (element (r, col) of the lower
+// band at bnd[col * BND_LD + r - col], r - col < BND_LD).
+//
+// One step (one wave; lane = (li, lg), li = lane % 16, lg = lane / 16), with
+// reflector rows q1..q2 (L = q2 - q1 + 1 <= BW) from column c1:
+//   (a) reflector of A[q1..q2][c1]; beta stays, the rest becomes 0;
+//   (b) left H^H on the bulge block rows q1..q2, columns c1+1 .. q1-1;
+//   (c) two-sided H^H D H on the diagonal block rows/columns q1..q2;
+//   (d) right H on rows q2+1 .. q2+BW, columns q1..q2 (the next bulge).
+template <typename T>
+__device__ __forceinline__ T chase_step(typename C2T<T>::t* bnd, typename C2T<T>::t* v2row, int n, int c1, int q1,
+                                        int q2, int lane) {
+  using C2 = typename C2T<T>::t;
+  const int li = lane & 15, lg = lane >> 4, gb = lane & 48;
+  const int L = q2 - q1 + 1;
+  const int Le = min(BW, n - 1 - q2);
+  const C2 zero = mk<T>(0, 0);
+  // issue every load of the step first
+  const C2 x = li < L ? bnd[(size_t)c1 * BND_LD + q1 - c1 + li] : zero;
+  C2 eb[4], ed[4], ee[4];
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {
+    const int cc = c1 + 1 + lg + 4 * t;
+    eb[t] = (cc < q1 && li < L) ? bnd[(size_t)cc * BND_LD + q1 + li - cc] : zero;
+    const int dc = lg + 4 * t;
+    if (li < L && dc < L) {
+      if (dc <= li) ed[t] = bnd[(size_t)(q1 + dc) * BND_LD + li - dc];
+      else ed[t] = c_conj(bnd[(size_t)(q1 + li) * BND_LD + dc - li]);
+    } else {
+      ed[t] = zero;
+    }
+    ee[t] = (li < Le && dc < L) ? bnd[(size_t)(q1 + dc) * BND_LD + q2 + 1 + li - q1 - dc] : zero;
+  }
+  // (a)
+  const T xn2 = sum16(li >= 1 && li < L ? x.x * x.x + x.y * x.y : T(0));
+  const C2 alpha = shfl_c(x, gb);
+  const House<T> h = house<T>(alpha, xn2);
+  const T beta = h.beta;
+  const C2 tau = h.tau, scale = h.scale;
+  const bool skip = h.skip;
+  C2 v = li == 0 ? mk<T>(1, 0) : ((li < L && !skip) ? c_mul(x, scale) : zero);
+  if (lg == 0 && li < L) {
+    bnd[(size_t)c1 * BND_LD + q1 - c1 + li] = li == 0 ? mk<T>(beta, 0) : zero;
+    if (v2row != nullptr) v2row[q1 + li] = li == 0 ? tau : v;
+  }
+  const C2 ctau = c_conj(tau);
+  // (b)
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {
+    const int cc = c1 + 1 + lg + 4 * t;
+    const C2 d = sum16c(c_cmul(v, eb[t]));
+    const C2 f = c_mul(ctau, d);
+    const C2 g = c_mul(v, f);
+    if (cc < q1 && li < L) bnd[(size_t)cc * BND_LD + q1 + li - cc] = c_sub(eb[t], g);
+  }
+  // (c) y = D v, w = tau y - |tau|^2/2 (v^H y) v, D -= v w^H + w v^H
+  C2 y = zero;
+  C2 vc[4];
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {
+    vc[t] = shfl_c(v, gb | (lg + 4 * t));
+    y = c_add(y, c_mul(ed[t], vc[t]));
+  }
+  y = sum4g(y);
+  const T vhy = sum16(li < L ? v.x * y.x + v.y * y.y : T(0));
+  const T t2 = T(0.5) * (tau.x * tau.x + tau.y * tau.y) * vhy;
+  const C2 ty = c_mul(tau, y);
+  const C2 w = li < L ? mk<T>(ty.x - t2 * v.x, ty.y - t2 * v.y) : zero;
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {
+    const int dc = lg + 4 * t;
+    const C2 wc = shfl_c(w, gb | dc);
+    if (li < L && dc <= li) {
+      const C2 t1 = c_mulc(v, wc), t3 = c_mulc(w, vc[t]);
+      C2 e = ed[t];
+      e.x -= t1.x + t3.x;
+      e.y -= t1.y + t3.y;
+      if (dc == li) e.y = T(0);
+      bnd[(size_t)(q1 + dc) * BND_LD + li - dc] = e;
+    }
+  }
+  // (d) E -= tau (E v) v^H
+  C2 z = zero;
+#pragma unroll
+  for (int t = 0; t < 4; ++t) z = c_add(z, c_mul(ee[t], vc[t]));
+  z = sum4g(z);
+  const C2 tz = c_mul(tau, z);
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {
+    const int dc = lg + 4 * t;
+    if (li < Le && dc < L) {
+      const C2 g = c_mulc(tz, vc[t]);
+      bnd[(size_t)(q1 + dc) * BND_LD + q2 + 1 + li - q1 - dc] = c_sub(ee[t], g);
+    }
+  }
+  return beta;
+}
+
+// Steps of sweep s: step 0, then one per BW rows while at least two rows remain.
+__device__ __forceinline__ int chase_nsteps(int n, int s) {
+  const int r = n - 3 - s;
+  return 1 + (r > 0 ? r / BW : 0);
+}
+
+// Schedule: sweep s runs on wave s % CH_WAVES from round start(s), one step
+// per round, with start(s) = max(start(s-1) + 3, end(s - CH_WAVES)): three
+// rounds behind the previous sweep (their regions are then disjoint) and after
+// the wave's previous sweep has ended. Rounds are separated by a workgroup
+// barrier. LDSB: the compact band lives in LDS (dynamic shared memory) for
+// the whole kernel; otherwise in global memory.
+template <typename T, bool LDSB>
+__global__ __launch_bounds__(CH_THREADS) void band_chase_kernel(BandCtx<T> c) {
+  using C2 = typename C2T<T>::t;
+  extern __shared__ double ch_smem[];
+  __shared__ int s_total;
+  const int b = blockIdx.x;
+  const int n = c.n, ld = c.ld;
+  C2* a = c.A + (size_t)b * ld * ld;
+  C2* bnd = LDSB ? reinterpret_cast<C2*>(ch_smem) : c.BND + (size_t)b * ld * BND_LD;
+  int* starts = LDSB ? reinterpret_cast<int*>(reinterpret_cast<C2*>(ch_smem) + (size_t)n * BND_LD)
+                     : c.STARTS + (size_t)b * ld;
+  double* D = c.D + (size_t)b * ld;
+  double* E = c.E + (size_t)b * ld;
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int ns = n - 1;   // sweeps 0 .. n-2
+  if (tid == 0) {
+    int prev = -3, total = 0;
+    for (int s = 0; s < ns; ++s) {
+      int st = prev + 3;
+      if (s >= CH_WAVES) st = max(st, starts[s - CH_WAVES] + chase_nsteps(n, s - CH_WAVES));
+      starts[s] = st;
+      prev = st;
+      total = max(total, st + chase_nsteps(n, s));
+    }
+    s_total = total;
+  }
+  for (int e = tid; e < n * BND_LD; e += CH_THREADS) {
+    const int col = e / BND_LD, d = e % BND_LD, r = col + d;
+    bnd[e] = (d <= BW && r < n) ? a[(size_t)r * ld + col] : mk<T>(0, 0);
+  }
+  __syncthreads();
+  const int total = s_total;
+  int s = w;
+  int start = s < ns ? starts[s] : total;
+  int nst = s < ns ? chase_nsteps(n, s) : 0;
+  int c1 = 0, q1 = 0, q2 = 0;
+  for (int round = 0; round < total; ++round) {
+    if (s < ns && round >= start) {
+      const int cs = round - start;
+      if (cs == 0) {
+        c1 = s;
+        q1 = s + 1;
+        q2 = min(s + BW, n - 1);
+      } else {
+        c1 = q1;
+        q1 = q2 + 1;
+        q2 = min(q2 + BW, n - 1);
+      }
+      if (cs == 0 && lane == 0) D[s] = (double)bnd[(size_t)s * BND_LD].x;
+      const T beta = chase_step<T>(bnd, a + (size_t)s * ld, n, c1, q1, q2, lane);
+      if (cs == 0 && lane == 0) E[s] = (double)beta;
+      if (cs + 1 == nst) {
+        s += CH_WAVES;
+        start = s < ns ? starts[s] : total;
+        nst = s < ns ? chase_nsteps(n, s) : 0;
+      }
+    }
+    __syncthreads();
+  }
+  if (tid == 0) {
+    D[n - 1] = (double)bnd[(size_t)(n - 1) * BND_LD].x;
+    E[n - 1] = 0.0;
+  }
+}
+
+size_t band_partials_elems(int ld) { return (size_t)((ld + AV_KCH - 1) / AV_KCH) * ld * BW; }
+// compact band, Zp, T
+size_t band_scratch_elems(int ld) {
+  return (size_t)ld * BND_LD + (size_t)((ld + 63) / 64) * BW * BW + (size_t)BW * BW + (size_t)ld;
+}
+
+// A22 -= [V | W] [W | V]^H = V W^H + W V^H, one GEMM with K = 2 BW (full tiles)
+template <typename T>
+static void rank2k_full(hipStream_t st, int batch, int m, typename C2T<T>::t* Vc, typename C2T<T>::t* Wc,
+                        typename C2T<T>::t* C, int ld, long long sP, long long sM);
+template <>
+void rank2k_full<double>(hipStream_t st, int batch, int m, double2* Vc, double2* Wc, double2* C, int ld,
+                         long long sP, long long sM) {
+  launch_zgemm(st, batch, m, m, 2 * BW, -1.0, Vc, ld, sP, OP_T, Wc, ld, sP, OP_CONJ, 1.0, C, ld, sM, 0, 0);
+}
+template <>
+void rank2k_full<float>(hipStream_t st, int batch, int m, float2* Vc, float2* Wc, float2* C, int ld, long long sP,
+                        long long sM) {
+  launch_cgemm(st, batch, m, m, 2 * BW, -1.0f, Vc, ld, sP, OP_T, Wc, ld, sP, OP_CONJ, 1.0f, C, ld, sM, 0, 0);
+}
+
+#define CH_LDS_MAX (150 * 1024)
+// Dynamic LDS beyond the default limit has to be allowed per kernel once.
+template <typename T>
+static bool chase_lds_ok() {
+  static const bool ok = hipFuncSetAttribute(reinterpret_cast<const void*>(&band_chase_kernel<T, true>),
+                                             hipFuncAttributeMaxDynamicSharedMemorySize, CH_LDS_MAX) == hipSuccess;
+  return ok;
+}
+
+template <typename T>
+static void launch_panel_qr(hipStream_t st, const BandCtx<T>& c, int batch, int j0, int m) {
+  const dim3 g(batch);
+  // rows per thread in registers: a complex value is 2 (FP32) or 4 (FP64) VGPRs
+  const int nq = (m + 63) / 64;   // rows per thread with 1024 threads
+  if constexpr (sizeof(T) == 4) {
+    if (m <= 256) hipLaunchKernelGGL((panel_qr_kernel<T, 16, 256>), g, dim3(256), 0, st, c, j0);
+    else if (nq <= 8) hipLaunchKernelGGL((panel_qr_kernel<T, 8, 1024>), g, dim3(1024), 0, st, c, j0);
+    else if (nq <= 16) hipLaunchKernelGGL((panel_qr_kernel<T, 16, 1024>), g, dim3(1024), 0, st, c, j0);
+    else if (nq <= 24) hipLaunchKernelGGL((panel_qr_kernel<T, 24, 1024>), g, dim3(1024), 0, st, c, j0);
+    else if (nq <= 32) hipLaunchKernelGGL((panel_qr_kernel<T, 32, 1024>), g, dim3(1024), 0, st, c, j0);
+    else hipLaunchKernelGGL((panel_qr_kernel<T, 0, 1024>), g, dim3(1024), 0, st, c, j0);
+  } else {
+    if (m <= 128) hipLaunchKernelGGL((panel_qr_kernel<T, 8, 256>), g, dim3(256), 0, st, c, j0);
+    else if (nq <= 4) hipLaunchKernelGGL((panel_qr_kernel<T, 4, 1024>), g, dim3(1024), 0, st, c, j0);
+    else hipLaunchKernelGGL((panel_qr_kernel<T, 0, 1024>), g, dim3(1024), 0, st, c, j0);
+  }
+}
+
+template <typename T>
+static void band_impl(hipStream_t st, typename C2T<T>::t* A, int n, int ld, int batch, double* D, double* E,
+                      typename C2T<T>::t* TAU, typename C2T<T>::t* Vc, typename C2T<T>::t* Wc,
+                      typename C2T<T>::t* Yp, typename C2T<T>::t* scratch, typename C2T<T>::t* VT) {
+  using C2 = typename C2T<T>::t;
+  BandCtx<T> c;
+  c.A = A;
+  c.n = n;
+  c.ld = ld;
+  c.Vc = Vc;
+  c.Wc = Wc;
+  c.Yp = Yp;
+  c.TAU = TAU;
+  c.VT = VT;
+  c.D = D;
+  c.E = E;
+  c.nch = (ld + AV_KCH - 1) / AV_KCH;
+  c.nrb = (ld + 63) / 64;
+  // scratch per matrix: band, then Zp, then T (strided by band_scratch_elems)
+  const size_t per = band_scratch_elems(ld);
+  (void)per;
+  c.BND = scratch;
+  c.Zp = scratch + (size_t)batch * ld * BND_LD;
+  c.TT = c.Zp + (size_t)batch * c.nrb * BW * BW;
+  c.STARTS = reinterpret_cast<int*>(c.TT + (size_t)batch * BW * BW);
+  const long long sP = (long long)2 * BW * ld, sM = (long long)ld * ld;
+  for (int j0 = 0;; j0 += BW) {
+    const int r0 = j0 + BW, m = n - r0;
+    // m = 1 gives a one-element (phase) reflector: it leaves the band as it is
+    // and defines tau for every column whose reflector starts inside the matrix
+    if (m < 1) break;
+    launch_panel_qr<T>(st, c, batch, j0, m);
+    const int nrbk = (m + 63) / 64, nchk = (m + AV_KCH - 1) / AV_KCH;
+    hipLaunchKernelGGL(band_av_kernel<T>, dim3(nrbk, nchk, batch), dim3(256), 0, st, c, r0);
+    hipLaunchKernelGGL(band_x_kernel<T>, dim3(nrbk, batch), dim3(256), 0, st, c, r0, nchk);
+    hipLaunchKernelGGL(band_w_kernel<T>, dim3(nrbk, batch), dim3(256), 0, st, c, r0, nrbk);
+    rank2k_full<T>(st, batch, m, Vc + r0, Wc + r0, A + (size_t)r0 * ld + r0, ld, sP, sM);
+  }
+  // the band (and the schedule) in LDS when it fits, else in global memory
+  const size_t lds = (size_t)n * BND_LD * sizeof(C2) + (size_t)n * sizeof(int);
+  if (lds <= CH_LDS_MAX && chase_lds_ok<T>())
+    hipLaunchKernelGGL((band_chase_kernel<T, true>), dim3(batch), dim3(CH_THREADS), lds, st, c);
+  else
+    hipLaunchKernelGGL((band_chase_kernel<T, false>), dim3(batch), dim3(CH_THREADS), 0, st, c);
+  (void)sizeof(C2);
+}
+
+void launch_band_tridiag(hipStream_t st, double2* C, int n, int ld, int batch, double* D, double* E, double2* TAU,
+                         double2* Vc, double2* Wc, double2* Yp, double2* scratch, double2* VT) {
+  band_impl<double>(st, C, n, ld, batch, D, E, TAU, Vc, Wc, Yp, scratch, VT);
+}
+
+void launch_band_tridiag_f32(hipStream_t st, float2* C, int n, int ld, int batch, double* D, double* E, float2* TAU,
+                             float2* Vc, float2* Wc, float2* Yp, float2* scratch, float2* VT) {
+  band_impl<float>(st, C, n, ld, batch, D, E, TAU, Vc, Wc, Yp, scratch, VT);
+}

@@ -5,6 +5,19 @@
 
 #define SLK_WAVE 64
 
+// complex type of a real type, and its constructor
+template <typename T> struct C2T;
+template <> struct C2T<double> { using t = double2; };
+template <> struct C2T<float> { using t = float2; };
+
+template <typename T>
+__device__ __forceinline__ typename C2T<T>::t mk(T x, T y) {
+  typename C2T<T>::t r;
+  r.x = x;
+  r.y = y;
+  return r;
+}
+
 __device__ __forceinline__ double2 c_mul(double2 a, double2 b) {
   return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }

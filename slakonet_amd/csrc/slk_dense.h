@@ -47,3 +47,20 @@ size_t hetrd_c12_elems(int ld);
 void launch_to_c64(hipStream_t st, const double2* src, float2* dst, size_t count);
 
 constexpr int SLK_NB = 32;   // panel width
+constexpr int SLK_BW = 16;   // band width of the two-stage reduction
+
+// Two-stage reduction of the Hermitian C (in place) for the eigenvalue path:
+// dense -> band of width SLK_BW (16-column panels, block reflectors, trailing
+// updates on the MFMA GEMM), then band -> real tridiagonal d, e by bulge
+// chasing (one workgroup per matrix). Stage-1 reflectors: tau in TAU[j],
+// reflector of column j as row j of VT (entries r >= j + SLK_BW, explicit 1
+// at j + SLK_BW; VT nullable). Chase reflectors: row s of C's strict upper
+// triangle holds sweep s's reflectors, tau in place of each leading 1.
+// Workspace: Vc, Wc [SLK_BW][ld] and Yp [band_partials_elems(ld)] per
+// matrix, band scratch [band_scratch_elems(ld)] per matrix.
+void launch_band_tridiag(hipStream_t st, double2* C, int n, int ld, int batch, double* D, double* E, double2* TAU,
+                         double2* Vc, double2* Wc, double2* Yp, double2* scratch, double2* VT);
+void launch_band_tridiag_f32(hipStream_t st, float2* C, int n, int ld, int batch, double* D, double* E, float2* TAU,
+                             float2* Vc, float2* Wc, float2* Yp, float2* scratch, float2* VT);
+size_t band_partials_elems(int ld);
+size_t band_scratch_elems(int ld);

@@ -343,8 +343,11 @@ struct RefineCtx {
   const double2* H;
   const double2* S;
   const double2* Wm;        // W = L^-1, FP64
-  const float2* R;          // FP32 reflectors as rows (row j = reflector j, entries r > j)
+  const float2* R;          // FP32 reflectors as rows (row j = reflector j, entries r >= j + off)
   const float2* TAU;
+  const float2* V2;         // nullable: band-chase reflectors (row s = sweep s, tau in place of each leading 1)
+  int off;                  // first row of reflector j is j + off (1: one-stage, SLK_BW: two-stage)
+  int n;                    // dimension of the reduced matrices (chase block partition)
   const double* D;
   const double* E;
   int ld;
@@ -527,11 +530,50 @@ __global__ __launch_bounds__(64) void refine_tri_kernel(RefineCtx c) {
     for (int i = lane; i < m; i += 64) y[i] *= nr;
     __threadfence_block();
   }
-  // x = Q y, reflectors last to first: x -= v tau (v^H x), v = row j of VT
   for (int r = lane; r < m; r += 64) x[r] = make_double2(y[r], 0.0);
   __threadfence_block();
+  if (c.V2 != nullptr) {
+    // band chase: x = Q2 x, sweeps last to first; the blocks of one sweep
+    // (rows s+1+BW t .. s+BW(t+1)) are disjoint, one per 16-lane group
+    const float2* v2 = c.V2 + mat * ld * ld;
+    const int li = lane & 15, lg = lane >> 4;
+    const int nn = c.n;
+    for (int s2 = m - 2; s2 >= 0; --s2) {
+      const int rr = nn - 3 - s2;
+      const int nst = 1 + (rr > 0 ? rr / SLK_BW : 0);
+      const float2* row = v2 + (size_t)s2 * ld;
+      for (int t0 = 0; t0 < nst; t0 += 4) {
+        const int t = t0 + lg;
+        const int q1 = s2 + 1 + SLK_BW * t;
+        const bool act = t < nst && q1 < m;
+        const int r = q1 + li;
+        const bool in = act && li < SLK_BW && r < m && r < nn;
+        const float2 tf = act ? row[q1] : make_float2(0.0f, 0.0f);
+        float2 vf = make_float2(0.0f, 0.0f);
+        if (in) vf = li == 0 ? make_float2(1.0f, 0.0f) : row[r];
+        const double2 xv = in ? x[r] : make_double2(0.0, 0.0);
+        double px = (double)vf.x * xv.x + (double)vf.y * xv.y;
+        double py = (double)vf.x * xv.y - (double)vf.y * xv.x;
+#pragma unroll
+        for (int o = 1; o < 16; o <<= 1) {
+          px += __shfl_xor(px, o, 64);
+          py += __shfl_xor(py, o, 64);
+        }
+        const double sx = (double)tf.x * px - (double)tf.y * py, sy = (double)tf.x * py + (double)tf.y * px;
+        if (in) {
+          double2 o = xv;
+          o.x -= (double)vf.x * sx - (double)vf.y * sy;
+          o.y -= (double)vf.x * sy + (double)vf.y * sx;
+          x[r] = o;
+        }
+      }
+      __threadfence_block();
+    }
+  }
+  // x = Q1 x, reflectors last to first: x -= v tau (v^H x), v = row j of VT
   const float2* vt = c.R + mat * ld * ld;
   const float2* tau = c.TAU + mat * ld;
+  const int off = c.off;
   if (NQ > 0) {
     // each lane keeps its <= NQ elements of the current reflector in registers
     // and prefetches the next reflector while this one is applied (x in LDS)
@@ -540,12 +582,12 @@ __global__ __launch_bounds__(64) void refine_tri_kernel(RefineCtx c) {
       const float2* vr = vt + (size_t)j * ld;
 #pragma unroll
       for (int q = 0; q < (NQ > 0 ? NQ : 1); ++q) {
-        const int r = j + 1 + lane + 64 * q;
+        const int r = j + off + lane + 64 * q;
         dst[q] = r < m ? vr[r] : make_float2(0.0f, 0.0f);
       }
     };
-    if (m >= 2) load(m - 2, vn);
-    for (int j = m - 2; j >= 0; --j) {
+    if (m - 1 - off >= 0) load(m - 1 - off, vn);
+    for (int j = m - 1 - off; j >= 0; --j) {
       float2 vc[NQ > 0 ? NQ : 1];
 #pragma unroll
       for (int q = 0; q < (NQ > 0 ? NQ : 1); ++q) vc[q] = vn[q];
@@ -555,7 +597,7 @@ __global__ __launch_bounds__(64) void refine_tri_kernel(RefineCtx c) {
       double px = 0.0, py = 0.0;
 #pragma unroll
       for (int q = 0; q < (NQ > 0 ? NQ : 1); ++q) {
-        const int r = j + 1 + lane + 64 * q;
+        const int r = j + off + lane + 64 * q;
         if (r < m) {
           const double2 xv = x[r];
           px += (double)vc[q].x * xv.x + (double)vc[q].y * xv.y;
@@ -567,7 +609,7 @@ __global__ __launch_bounds__(64) void refine_tri_kernel(RefineCtx c) {
       const double sx = (double)tf.x * px - (double)tf.y * py, sy = (double)tf.x * py + (double)tf.y * px;
 #pragma unroll
       for (int q = 0; q < (NQ > 0 ? NQ : 1); ++q) {
-        const int r = j + 1 + lane + 64 * q;
+        const int r = j + off + lane + 64 * q;
         if (r < m) {
           double2 xv = x[r];
           xv.x -= (double)vc[q].x * sx - (double)vc[q].y * sy;
@@ -577,12 +619,12 @@ __global__ __launch_bounds__(64) void refine_tri_kernel(RefineCtx c) {
       }
     }
   } else {
-    for (int j = m - 2; j >= 0; --j) {
+    for (int j = m - 1 - off; j >= 0; --j) {
       const float2 tf = tau[j];
       if (tf.x == 0.0f && tf.y == 0.0f) continue;
       const float2* vr = vt + (size_t)j * ld;
       double px = 0.0, py = 0.0;
-      for (int r = j + 1 + lane; r < m; r += 64) {
+      for (int r = j + off + lane; r < m; r += 64) {
         const float2 vf = vr[r];
         const double2 xv = x[r];
         px += (double)vf.x * xv.x + (double)vf.y * xv.y;
@@ -591,7 +633,7 @@ __global__ __launch_bounds__(64) void refine_tri_kernel(RefineCtx c) {
       px = wave_sum(px);
       py = wave_sum(py);
       const double sx = (double)tf.x * px - (double)tf.y * py, sy = (double)tf.x * py + (double)tf.y * px;
-      for (int r = j + 1 + lane; r < m; r += 64) {
+      for (int r = j + off + lane; r < m; r += 64) {
         const float2 vf = vr[r];
         double2 xv = x[r];
         xv.x -= (double)vf.x * sx - (double)vf.y * sy;
@@ -724,8 +766,8 @@ extern "C" {
 // E, and the real Z for the vectors path. Mixed path: L and W (FP64), T, then
 // FP32 copies W32, H32 (reused for C), W32*H32 and FP32 panels.
 struct EighWs {
-  double2 *B0, *B1, *B2, *T, *Vc, *Wc, *Y, *Yp, *C12, *TAU;
-  float2 *F0, *F1, *F2, *fVc, *fWc, *fY, *fYp, *fC12, *fTAU;
+  double2 *B0, *B1, *B2, *T, *Vc, *Wc, *Y, *Yp, *C12, *TAU, *Band;
+  float2 *F0, *F1, *F2, *fVc, *fWc, *fY, *fYp, *fC12, *fTAU, *fBand;
   double *D, *E, *Z;
 };
 
@@ -752,6 +794,7 @@ static size_t carve(char* base, int ld, int batch, int vectors, int mixed, EighW
     w->Yp = reinterpret_cast<double2*>(take(hetrd_partials_elems(ld) * batch * sizeof(double2)));
     w->C12 = reinterpret_cast<double2*>(take(hetrd_c12_elems(ld) * batch * sizeof(double2)));
     w->TAU = reinterpret_cast<double2*>(take((size_t)ld * batch * sizeof(double2)));
+    w->Band = reinterpret_cast<double2*>(take(band_scratch_elems(ld) * batch * sizeof(double2)));
     w->Z = vectors ? reinterpret_cast<double*>(take(matel * batch * sizeof(double))) : nullptr;
   } else {
     w->F0 = reinterpret_cast<float2*>(take(matel * batch * sizeof(float2)));
@@ -763,6 +806,7 @@ static size_t carve(char* base, int ld, int batch, int vectors, int mixed, EighW
     w->fYp = reinterpret_cast<float2*>(take(hetrd_partials_elems(ld) * batch * sizeof(float2)));
     w->fC12 = reinterpret_cast<float2*>(take(hetrd_c12_elems(ld) * batch * sizeof(float2)));
     w->fTAU = reinterpret_cast<float2*>(take((size_t)ld * batch * sizeof(float2)));
+    w->fBand = reinterpret_cast<float2*>(take(band_scratch_elems(ld) * batch * sizeof(float2)));
   }
   return off;
 }
@@ -777,6 +821,19 @@ size_t slk_eigh_mixed_workspace_bytes(int n, int ld, int batch) {
   (void)n;
   EighWs w;
   return carve(nullptr, ld, batch, 0, 1, &w);
+}
+
+// Tridiagonalisation of the eigenvalue path: 0 automatic (two-stage band
+// reduction for values up to BAND_AUTO_MAX, blocked one-stage above and for
+// vectors), 1 one-stage, 2 two-stage. The two-stage chase runs on one
+// workgroup per matrix; beyond BAND_AUTO_MAX rows its single-CU work outweighs
+// the one-stage path's launch chain.
+#define BAND_AUTO_MAX 1280
+static int g_tridiag_mode = 0;
+
+static bool use_band(int n, int vectors) {
+  if (vectors || n <= 32 || g_tridiag_mode == 1) return false;
+  return g_tridiag_mode == 2 || n <= BAND_AUTO_MAX;
 }
 
 // Stage boundaries for optional timing: potrf | trinv | reduce (2 GEMMs) |
@@ -818,7 +875,11 @@ static int eigh_impl(const void* H, const void* S, int n, int ld, int batch, dou
     launch_zgemm(stream, batch, n, n, n, 1.0, s.B2, ld, sM, OP_N, s.B1, ld, sM, OP_C, 0.0, s.B0, ld, sM,
                  KLIM_B_UPPER, 0);
     if (ev) (void)hipEventRecord(ev[3], stream);
-    launch_hetrd_blocked(stream, s.B0, n, ld, batch, s.D, s.E, s.TAU, s.Vc, s.Wc, s.Y, s.Yp, s.C12);
+    if (use_band(n, V != nullptr))
+      // B2 (W H) is free after the reduction: it holds the split-K partials
+      launch_band_tridiag(stream, s.B0, n, ld, batch, s.D, s.E, s.TAU, s.Vc, s.Wc, s.B2, s.Band, nullptr);
+    else
+      launch_hetrd_blocked(stream, s.B0, n, ld, batch, s.D, s.E, s.TAU, s.Vc, s.Wc, s.Y, s.Yp, s.C12);
   } else {
     launch_to_c64(stream, s.B1, s.F0, (size_t)batch * ld * ld);
     launch_to_c64(stream, h, s.F1, (size_t)batch * ld * ld);
@@ -827,9 +888,13 @@ static int eigh_impl(const void* H, const void* S, int n, int ld, int batch, dou
     launch_cgemm(stream, batch, n, n, n, 1.0f, s.F2, ld, sM, OP_N, s.F0, ld, sM, OP_C, 0.0f, s.F1, ld, sM,
                  KLIM_B_UPPER, 0);
     if (ev) (void)hipEventRecord(ev[3], stream);
-    // F0 (W in FP32) is free after the reduction: it receives the reflector rows
-    launch_hetrd_blocked_f32(stream, s.F1, n, ld, batch, s.D, s.E, s.fTAU, s.fVc, s.fWc, s.fY, s.fYp, s.fC12,
-                             s.F0);
+    // F0 (W in FP32) is free after the reduction: it receives the reflector
+    // rows; F2 (W32 H32) holds the split-K partials of the band reduction
+    if (use_band(n, 0))
+      launch_band_tridiag_f32(stream, s.F1, n, ld, batch, s.D, s.E, s.fTAU, s.fVc, s.fWc, s.F2, s.fBand, s.F0);
+    else
+      launch_hetrd_blocked_f32(stream, s.F1, n, ld, batch, s.D, s.E, s.fTAU, s.fVc, s.fWc, s.fY, s.fYp, s.fC12,
+                               s.F0);
   }
   if (ev) (void)hipEventRecord(ev[4], stream);
   if (V == nullptr) {
@@ -875,6 +940,12 @@ static int eigh_timed(const void* H, const void* S, int n, int ld, int batch, do
   return rc;
 }
 
+int slk_set_tridiag_mode(int mode) {
+  if (mode < 0 || mode > 2) return 1;
+  g_tridiag_mode = mode;
+  return 0;
+}
+
 int slk_eigh_f64(const void* H, const void* S, int n, int ld, int batch, double* w, int wld, void* V,
                  void* ws, size_t ws_bytes, int* info, hipStream_t stream) {
   return eigh_impl(H, S, n, ld, batch, w, wld, V, ws, ws_bytes, info, stream, nullptr, 0);
@@ -911,6 +982,10 @@ int slk_refine_edges_mixed(const void* H, const void* S, int n, int ld, int batc
   c.Wm = s.B1;
   c.R = s.F0;
   c.TAU = s.fTAU;
+  const bool band = use_band(n, 0);
+  c.V2 = band ? s.F1 : nullptr;
+  c.off = band ? SLK_BW : 1;
+  c.n = n;
   c.D = s.D;
   c.E = s.E;
   c.ld = ld;

@@ -30,18 +30,6 @@
 #define HT_TS 128      // square tile of the trailing mat-vec: 2 columns per lane
 #define HT_RPW (HT_TS / (HT_THREADS / 64))   // tile rows per wave
 
-template <typename T> struct C2T;
-template <> struct C2T<double> { using t = double2; };
-template <> struct C2T<float> { using t = float2; };
-
-template <typename T>
-__device__ __forceinline__ typename C2T<T>::t mk(T x, T y) {
-  typename C2T<T>::t r;
-  r.x = x;
-  r.y = y;
-  return r;
-}
-
 template <typename T>
 struct HetrdCtx {
   using C2 = typename C2T<T>::t;
