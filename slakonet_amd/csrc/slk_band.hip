@@ -43,7 +43,6 @@ struct BandCtx {
   C2* TAU;       // [batch][ld]: stage-1 tau of column j
   C2* VT;        // nullable [batch][ld][ld]: stage-1 reflector of column j as row j
   C2* BND;       // [batch][ld][BND_LD]: compact band for the chase
-  int* STARTS;   // [batch][ld]: first round of each sweep
   double *D, *E;
   int nch, nrb;  // chunk / row-block counts used for the strides of Yp, Zp
 };
@@ -143,196 +142,192 @@ __device__ __forceinline__ House<T> house(typename C2T<T>::t alpha, T xn2) {
 }
 
 // ---------------------------------------------------------------------------
-// Stage 1, panel QR. Thread t owns column j = t % BW of rows r = t / BW +
-// PQ_GROUPS q of the panel (registers for RPT > 0, global memory for RPT = 0).
-// The 16 columns of a row group are 16 consecutive lanes, so a reflector
-// element reaches its row group by one shuffle.
-template <typename T, int RPT, int NT>
-__global__ __launch_bounds__(NT) void panel_qr_kernel(BandCtx<T> c, int j0) {
-  constexpr int NG = NT / BW;   // row groups
+// Stage 1, panel QR: 16 waves, wave w owns column w of the m x BW panel,
+// lane l owns rows l + 64 q (registers for RPT > 0; RPT = 0 re-reads the
+// panel from global memory). Column k: wave k reduces its norm, forms the
+// reflector and publishes v_k (LDS, or column k of Vc for RPT = 0); after one
+// barrier, waves j > k apply it to their columns (inner product and update
+// within the wave), wave k+1 also forms the next norm, and waves l < k form
+// g_l = v_l^H v_k for the block reflector factor T.
+template <typename C2>
+__device__ __forceinline__ C2 wave_sum_c(C2 v) {
+  v.x = sum16(v.x);
+  v.y = sum16(v.y);
+  v.x += __shfl_xor(v.x, 16, 64);
+  v.y += __shfl_xor(v.y, 16, 64);
+  v.x += __shfl_xor(v.x, 32, 64);
+  v.y += __shfl_xor(v.y, 32, 64);
+  return v;
+}
+template <typename T>
+__device__ __forceinline__ T wave_sum_r(T v) {
+  v = sum16(v);
+  v += __shfl_xor(v, 16, 64);
+  v += __shfl_xor(v, 32, 64);
+  return v;
+}
+
+// Calls f(q) for the rows q < nq of a lane: fully unrolled in chunks of 8 with
+// a scheduling barrier between chunks for register-resident rows (bounds the
+// LDS loads the compiler hoists), a plain loop otherwise.
+template <int RPT, typename F>
+__device__ __forceinline__ void for_rows(int nq, F f) {
+  if constexpr (RPT > 0) {
+#pragma unroll
+    for (int q0 = 0; q0 < RPT; q0 += 8) {
+#pragma unroll
+      for (int qq = 0; qq < 8; ++qq)
+        if (q0 + qq < RPT) f(q0 + qq);
+      __builtin_amdgcn_sched_barrier(0);
+    }
+  } else {
+    for (int q = 0; q < nq; ++q) f(q);
+  }
+}
+
+template <typename T, int RPT>
+__global__ __launch_bounds__(1024) void panel_qr_kernel(BandCtx<T> c, int j0) {
   using C2 = typename C2T<T>::t;
+  constexpr int NR = RPT > 0 ? RPT : 1;
   const int b = blockIdx.x;
   const int n = c.n, ld = c.ld;
   const int r0 = j0 + BW, m = n - r0;
   const int kb = m < BW ? m : BW;
   C2* a = c.A + (size_t)b * ld * ld;
-  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-  const int j = tid & (BW - 1), rg = tid >> 4;
-  const int gsrc = lane & ~(BW - 1);
-  __shared__ T s_nrm[NT / 64];
-  __shared__ C2 s_u[NT / 64][BW];
-  __shared__ C2 s_T[BW][BW];
-  __shared__ C2 s_G[BW][BW];   // s_G[k][l] = V[:,l]^H v_k, l < k
-  __shared__ C2 s_alpha;
+  C2* vcb = c.Vc + (size_t)b * 2 * BW * ld;
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  // published reflector, double-buffered by column parity: wave k+1 writes
+  // v_{k+1} while other waves may still read v_k (one barrier per column)
+  __shared__ C2 s_vb[2][RPT > 0 ? 64 * RPT : 1];
   __shared__ C2 s_tau[BW];
-  __shared__ C2 s_v[RPT > 0 ? NG * RPT : 1];   // current reflector (register variant)
-  const int nq = RPT > 0 ? RPT : (m + NG - 1) / NG;
-  constexpr int QN = RPT > 0 ? RPT : 1;   // rows per fully unrolled inner loop
-  C2 p[RPT > 0 ? RPT : 1];
+  __shared__ C2 s_G[BW][BW];   // s_G[k][l] = v_l^H v_k, l < k
+  __shared__ C2 s_T[BW][BW];
   const C2 zero = mk<T>(0, 0);
-  auto gaddr = [&](int q) -> C2* { return a + (size_t)(r0 + rg + NG * q) * ld + j0 + j; };
+  const int nq = RPT > 0 ? RPT : (m + 63) / 64;
+  C2* col = a + (size_t)(r0 + lane) * ld + j0 + w;   // this lane's rows of column w
+  C2 p[NR];
+  if constexpr (RPT > 0) {
+#pragma unroll
+    for (int q = 0; q < RPT; ++q) p[q] = lane + 64 * q < m ? col[(size_t)64 * q * ld] : zero;
+  }
   auto get = [&](int q) -> C2 {
     if constexpr (RPT > 0) return p[q];
-    else return (rg + NG * q < m) ? *gaddr(q) : mk<T>(0, 0);
+    else return lane + 64 * q < m ? col[(size_t)64 * q * ld] : mk<T>(0, 0);
   };
   auto put = [&](int q, C2 v) {
     if constexpr (RPT > 0) p[q] = v;
-    else if (rg + NG * q < m) *gaddr(q) = v;
+    else if (lane + 64 * q < m) col[(size_t)64 * q * ld] = v;
   };
-  if constexpr (RPT > 0) {
-    const C2* pl = a + (size_t)(r0 + rg) * ld + j0 + j;
-#pragma unroll
-    for (int q = 0; q < RPT; ++q) {
-      const int r = rg + NG * q;
-      p[q] = r < m ? pl[(size_t)NG * q * ld] : zero;
-    }
-  }
+  // T (Q = I - V T V^H): T[k][k] = tau_k, T[i][k] = -tau_k sum_{l=i}^{k-1} T[i][l] g_l^(k);
+  // lane i of wave 0 owns row i
   if (tid < BW * BW) s_T[tid / BW][tid % BW] = zero;
-  // column 0: partial norm below row 0 and alpha
-  T nrm_part = 0;
-  for (int q0 = 0; q0 < nq; q0 += QN)
+  auto t_column = [&](int k) {
+    if (k < 0 || lane > k) return;
+    if (lane == k) {
+      s_T[k][k] = s_tau[k];
+      return;
+    }
+    C2 acc = zero;
 #pragma unroll
-    for (int qq = 0; qq < QN; ++qq) {
-    const int q = q0 + qq;
-    const int r = rg + NG * q;
-    const C2 e = get(q);
-    if (j == 0 && r > 0 && r < m) nrm_part += e.x * e.x + e.y * e.y;
-    if (j == 0 && r == 0) s_alpha = e;
-  }
+    for (int l = 0; l < BW; ++l)
+      if (l >= lane && l < k) acc = c_add(acc, c_mul(s_T[lane][l], s_G[k][l]));
+    const C2 t = c_mul(s_tau[k], acc);
+    s_T[lane][k] = mk<T>(-t.x, -t.y);
+  };
+  T nrm = 0;
+  if (w == 0)
+    for_rows<RPT>(nq, [&](int q) {
+        const int r = lane + 64 * q;
+        const C2 e = get(q);
+        if (r > 0 && r < m) nrm += e.x * e.x + e.y * e.y;
+      });
   for (int k = 0; k < kb; ++k) {
-    // (1) norm of column k below row k
-    T v1 = nrm_part;
-    v1 += __shfl_xor(v1, 16, 64);
-    v1 += __shfl_xor(v1, 32, 64);
-    if (lane == k) s_nrm[wv] = v1;
-    __syncthreads();
-    T xn2 = 0;
-#pragma unroll
-    for (int w = 0; w < NT / 64; ++w) xn2 += s_nrm[w];
-    const House<T> h = house<T>(s_alpha, xn2);
-    const T beta = h.beta;
-    const C2 tau = h.tau, scale = h.scale;
-    const bool skip = h.skip;
-    // (2) the reflector replaces column k (rows > k), beta goes to row k;
-    // (3) partial u_j = sum_r conj(v_r) A[r][j] over rows r >= k
-    C2 u = zero;
-    if constexpr (RPT > 0) {
-      // the owner lane of each row publishes v_r in LDS; the other 15 lanes of
-      // the row group read it back (same wave, LDS operations in order)
-#pragma unroll
-      for (int q = 0; q < RPT; ++q) {
-        const int r = rg + NG * q;
-        if (j == k) {
-          C2 e = p[q];
+    const C2* vk = vcb + (size_t)k * ld + r0;   // global copy of v_k (RPT = 0)
+    C2* s_v = s_vb[k & 1];
+    if (w == k) {
+      const T xn2 = wave_sum_r(nrm);
+      C2 a0 = get(0);   // row k lives in lane k, q = 0 (k < BW <= 64)
+      const C2 alpha = mk<T>(__shfl(a0.x, k, 64), __shfl(a0.y, k, 64));
+      const House<T> h = house<T>(alpha, xn2);
+      for_rows<RPT>(nq, [&](int q) {
+          const int r = lane + 64 * q;
+          if (r >= m) return;
+          C2 e = get(q);
           C2 vv = zero;
           if (r == k) {
-            e = mk<T>(beta, 0);
             vv = mk<T>(1, 0);
-          } else if (r > k && r < m) {
-            e = skip ? zero : c_mul(e, scale);
-            vv = e;
+            put(q, mk<T>(h.beta, 0));
+          } else if (r > k) {
+            vv = h.skip ? zero : c_mul(e, h.scale);
+            put(q, vv);
           }
-          p[q] = e;
-          s_v[r] = vv;
-        }
-      }
-      __builtin_amdgcn_wave_barrier();
-#pragma unroll
-      for (int q = 0; q < RPT; ++q) {
-        const C2 t = c_cmul(s_v[rg + NG * q], p[q]);
-        u.x += t.x;
-        u.y += t.y;
-      }
-    } else {
-      for (int q = 0; q < nq; ++q) {
-        const int r = rg + NG * q;
-        C2 e = get(q);
-        if (j == k) {
-          if (r == k) e = mk<T>(beta, 0);
-          else if (r > k && r < m) e = skip ? zero : c_mul(e, scale);
-          put(q, e);
-        }
-        C2 vr = shfl_c(e, gsrc | k);
-        if (r == k) vr = mk<T>(1, 0);
-        else if (r < k || r >= m) vr = zero;
-        const C2 t = c_cmul(vr, e);
-        u.x += t.x;
-        u.y += t.y;
-      }
+          if constexpr (RPT > 0) s_v[r] = vv;
+          else const_cast<C2*>(vk)[r] = vv;
+        });
+      if (lane == 0) s_tau[k] = h.tau;
     }
-    u = c_add(u, shfl_xor_c(u, 16));
-    u = c_add(u, shfl_xor_c(u, 32));
-    if (lane < BW) s_u[wv][lane] = u;
-    if (tid == 0) s_tau[k] = tau;
     __syncthreads();
-    C2 uj = zero;
-#pragma unroll
-    for (int w = 0; w < NT / 64; ++w) uj = c_add(uj, s_u[w][j]);
-    // T[0:k, k] = -tau_k T[0:k, 0:k] g with g_l = V[:,l]^H v_k = conj(u_l):
-    // lanes l < k of wave 0 publish g_l, lane i < k updates row i of T (rows
-    // are private to their lane; same wave, LDS operations in order)
-    if (tid < k) s_G[k][tid] = c_conj(uj);
-    if (tid < BW) {
-      __builtin_amdgcn_wave_barrier();
-      C2 acc = zero;
-#pragma unroll
-      for (int l = 0; l < BW; ++l)
-        if (l >= tid && l < k) acc = c_add(acc, c_mul(s_T[tid][l], s_G[k][l]));
-      const C2 t = c_mul(tau, acc);
-      s_T[tid][k] = tid < k ? mk<T>(-t.x, -t.y) : (tid == k ? tau : zero);
-    }
-    // (4) columns j > k: A[r][j] -= conj(tau) v_r u_j; partial norm of column k+1
-    const C2 cu = c_mul(c_conj(tau), uj);
-    nrm_part = 0;
-    for (int q0 = 0; q0 < nq; q0 += QN)
-#pragma unroll
-      for (int qq = 0; qq < QN; ++qq) {
-      const int q = q0 + qq;
-      const int r = rg + NG * q;
-      C2 e = get(q);
-      C2 vr;
-      if constexpr (RPT > 0) {
-        vr = s_v[r];
-      } else {
-        vr = shfl_c(e, gsrc | k);
-        if (r == k) vr = mk<T>(1, 0);
-        else if (r < k || r >= m) vr = zero;
-      }
-      if (j > k && r >= k && r < m) {
-        const C2 t = c_mul(vr, cu);
-        e.x -= t.x;
-        e.y -= t.y;
-        put(q, e);
-      }
-      if (j == k + 1) {
-        if (r > k + 1 && r < m) nrm_part += e.x * e.x + e.y * e.y;
-        if (r == k + 1) s_alpha = e;
-      }
+    if (w > k) {
+      const C2 tau = s_tau[k];
+      C2 u = zero;
+      for_rows<RPT>(nq, [&](int q) {
+          const int r = lane + 64 * q;
+          if (r >= k && r < m) {
+            const C2 vr = RPT > 0 ? s_v[r] : vk[r];
+            const C2 t = c_cmul(vr, get(q));
+            u.x += t.x;
+            u.y += t.y;
+          }
+        });
+      u = wave_sum_c(u);
+      const C2 cu = c_mul(c_conj(tau), u);
+      nrm = 0;
+      for_rows<RPT>(nq, [&](int q) {
+          const int r = lane + 64 * q;
+          if (r >= k && r < m) {
+            const C2 vr = RPT > 0 ? s_v[r] : vk[r];
+            C2 e = get(q);
+            const C2 t = c_mul(vr, cu);
+            e.x -= t.x;
+            e.y -= t.y;
+            put(q, e);
+            if (r > k + 1) nrm += e.x * e.x + e.y * e.y;
+          }
+        });
+    } else if (w < k) {
+      // g_w = v_w^H v_k over rows >= k (column w holds v_w below row w)
+      C2 g = zero;
+      for_rows<RPT>(nq, [&](int q) {
+          const int r = lane + 64 * q;
+          if (r >= k && r < m) {
+            const C2 vr = RPT > 0 ? s_v[r] : vk[r];
+            const C2 t = c_cmul(get(q), vr);
+            g.x += t.x;
+            g.y += t.y;
+          }
+        });
+      g = wave_sum_c(g);
+      if (lane == 0) s_G[k][w] = g;
+      // wave 0 meanwhile builds column k-1 of T (its g values are complete)
+      if (w == 0) t_column(k - 1);
     }
   }
   __syncthreads();
-  // outputs: panel (R in the band, reflectors below), Vc, VT rows, TAU, T.
-  // The row index is passed through an empty asm statement so that the
-  // addresses are recomputed here instead of being held since the loads.
-  int rgo = rg;
-  asm volatile("" : "+v"(rgo));
-  C2* vc = c.Vc + (size_t)b * 2 * BW * ld;
+  if (w == 0) t_column(kb - 1);
+  __syncthreads();
+  // outputs: panel column w (R in the band, v_w below), V in Vc[w] and Wc[BW + w], VT row j0 + w
   C2* wcv = c.Wc + (size_t)b * 2 * BW * ld + (size_t)BW * ld;
-  C2* pa = a + (size_t)(r0 + rgo) * ld + j0 + j;
-  for (int q0 = 0; q0 < nq; q0 += QN)
-#pragma unroll
-    for (int qq = 0; qq < QN; ++qq) {
-    const int q = q0 + qq;
-    const int r = rgo + NG * q;
-    if (r >= m) continue;
-    const C2 e = get(q);
-    if constexpr (RPT > 0) pa[(size_t)NG * q * ld] = e;
-    C2 v = zero;
-    if (j < kb) v = r > j ? e : (r == j ? mk<T>(1, 0) : zero);
-    vc[(size_t)j * ld + r0 + r] = v;
-    wcv[(size_t)j * ld + r0 + r] = v;
-    if (c.VT != nullptr && j < kb && r >= j) c.VT[(size_t)b * ld * ld + (size_t)(j0 + j) * ld + r0 + r] = v;
-  }
+  for_rows<RPT>(nq, [&](int q) {
+      const int r = lane + 64 * q;
+      if (r >= m) return;
+      const C2 e = get(q);
+      if constexpr (RPT > 0) col[(size_t)64 * q * ld] = e;
+      C2 v = zero;
+      if (w < kb) v = r > w ? e : (r == w ? mk<T>(1, 0) : zero);
+      vcb[(size_t)w * ld + r0 + r] = v;
+      wcv[(size_t)w * ld + r0 + r] = v;
+      if (c.VT != nullptr && w < kb && r >= w) c.VT[(size_t)b * ld * ld + (size_t)(j0 + w) * ld + r0 + r] = v;
+    });
   if (tid < BW) c.TAU[(size_t)b * ld + j0 + tid] = tid < kb ? s_tau[tid] : zero;
   if (tid < BW * BW) c.TT[(size_t)b * BW * BW + tid] = s_T[tid / BW][tid % BW];
 }
@@ -515,96 +510,99 @@ __global__ __launch_bounds__(256) void band_w_kernel(BandCtx<T> c, int r0, int n
 // One step (one wave; lane = (li, lg), li = lane % 16, lg = lane / 16), with
 // reflector rows q1..q2 (L = q2 - q1 + 1 <= BW) from column c1:
 //   (a) reflector of A[q1..q2][c1]; beta stays, the rest becomes 0;
-//   (b) left H^H on the bulge block rows q1..q2, columns c1+1 .. q1-1;
-//   (c) two-sided H^H D H on the diagonal block rows/columns q1..q2;
-//   (d) right H on rows q2+1 .. q2+BW, columns q1..q2 (the next bulge).
-template <typename T>
-__device__ __forceinline__ T chase_step(typename C2T<T>::t* bnd, typename C2T<T>::t* v2row, int n, int c1, int q1,
-                                        int q2, int lane) {
+//   (b) left H^H on the bulge block rows q1..q2, columns c1+1 .. q1-1
+//       (lane: column c1+1+li, rows lg+4t);
+//   (c) two-sided H^H D H on the diagonal block rows/columns q1..q2
+//       (lane: row li, columns lg+4t);
+//   (d) right H on rows q2+1 .. q2+BW, columns q1..q2 (the next bulge)
+//       (lane: row q2+1+li, columns lg+4t).
+// v and w reach the lanes that need them by index lg+4t through a per-wave
+// LDS slot sv[2 BW]. FULL: L = BW, BW-1 bulge columns and BW rows below
+// (compile-time bounds).
+template <typename T, bool FULL>
+__device__ __forceinline__ T chase_step(typename C2T<T>::t* bnd, typename C2T<T>::t* v2row,
+                                        typename C2T<T>::t* sv, int n, int c1, int q1, int q2, int lane) {
   using C2 = typename C2T<T>::t;
-  const int li = lane & 15, lg = lane >> 4, gb = lane & 48;
-  const int L = q2 - q1 + 1;
-  const int Le = min(BW, n - 1 - q2);
+  const int li = lane & 15, lg = lane >> 4;
+  const int L = FULL ? BW : q2 - q1 + 1;
+  const int Le = FULL ? BW : min(BW, n - 1 - q2);
+  const int nbc = FULL ? BW - 1 : q1 - c1 - 1;
   const C2 zero = mk<T>(0, 0);
-  // issue every load of the step first
-  const C2 x = li < L ? bnd[(size_t)c1 * BND_LD + q1 - c1 + li] : zero;
+  // every load of the step first
+  const C2 x = (FULL || li < L) ? bnd[c1 * BND_LD + q1 - c1 + li] : zero;
   C2 eb[4], ed[4], ee[4];
+  const int cb = c1 + 1 + li;   // this lane's bulge column
 #pragma unroll
   for (int t = 0; t < 4; ++t) {
-    const int cc = c1 + 1 + lg + 4 * t;
-    eb[t] = (cc < q1 && li < L) ? bnd[(size_t)cc * BND_LD + q1 + li - cc] : zero;
-    const int dc = lg + 4 * t;
-    if (li < L && dc < L) {
-      if (dc <= li) ed[t] = bnd[(size_t)(q1 + dc) * BND_LD + li - dc];
-      else ed[t] = c_conj(bnd[(size_t)(q1 + li) * BND_LD + dc - li]);
-    } else {
-      ed[t] = zero;
-    }
-    ee[t] = (li < Le && dc < L) ? bnd[(size_t)(q1 + dc) * BND_LD + q2 + 1 + li - q1 - dc] : zero;
+    const int r = lg + 4 * t;   // row (b) / column (c), (d) index
+    eb[t] = ((FULL || r < L) && li < nbc) ? bnd[cb * BND_LD + q1 + r - cb] : zero;
+    const bool lo = r <= li;
+    const int od = lo ? (q1 + r) * BND_LD + li - r : (q1 + li) * BND_LD + r - li;
+    C2 d = (FULL || (li < L && r < L)) ? bnd[od] : zero;
+    if (!lo) d.y = -d.y;
+    ed[t] = d;
+    ee[t] = ((FULL || r < L) && (FULL || li < Le)) ? bnd[(q1 + r) * BND_LD + q2 + 1 + li - q1 - r] : zero;
   }
   // (a)
-  const T xn2 = sum16(li >= 1 && li < L ? x.x * x.x + x.y * x.y : T(0));
-  const C2 alpha = shfl_c(x, gb);
+  const T xn2 = sum16((li >= 1 && (FULL || li < L)) ? x.x * x.x + x.y * x.y : T(0));
+  const C2 alpha = mk<T>(__shfl(x.x, 0, 64), __shfl(x.y, 0, 64));
   const House<T> h = house<T>(alpha, xn2);
-  const T beta = h.beta;
-  const C2 tau = h.tau, scale = h.scale;
-  const bool skip = h.skip;
-  C2 v = li == 0 ? mk<T>(1, 0) : ((li < L && !skip) ? c_mul(x, scale) : zero);
-  if (lg == 0 && li < L) {
-    bnd[(size_t)c1 * BND_LD + q1 - c1 + li] = li == 0 ? mk<T>(beta, 0) : zero;
-    if (v2row != nullptr) v2row[q1 + li] = li == 0 ? tau : v;
+  const C2 tau = h.tau;
+  const C2 v = li == 0 ? mk<T>(1, 0) : (((FULL || li < L) && !h.skip) ? c_mul(x, h.scale) : zero);
+  if (lg == 0) {
+    if (FULL || li < L) {
+      bnd[c1 * BND_LD + q1 - c1 + li] = li == 0 ? mk<T>(h.beta, 0) : zero;
+      if (v2row != nullptr) v2row[q1 + li] = li == 0 ? tau : v;
+    }
+    sv[li] = v;
   }
-  const C2 ctau = c_conj(tau);
-  // (b)
+  __builtin_amdgcn_wave_barrier();
+  C2 vr[4];
+#pragma unroll
+  for (int t = 0; t < 4; ++t) vr[t] = sv[lg + 4 * t];
+  // (b) d_cb = sum_r conj(v_r) B[r][cb]; B[r][cb] -= conj(tau) v_r d_cb
+  C2 db = zero;
+#pragma unroll
+  for (int t = 0; t < 4; ++t) db = c_add(db, c_cmul(vr[t], eb[t]));
+  // (c) y_li = sum_c D[li][c] v_c;  (d) z_li = sum_c E[li][c] v_c
+  C2 y = zero, z = zero;
 #pragma unroll
   for (int t = 0; t < 4; ++t) {
-    const int cc = c1 + 1 + lg + 4 * t;
-    const C2 d = sum16c(c_cmul(v, eb[t]));
-    const C2 f = c_mul(ctau, d);
-    const C2 g = c_mul(v, f);
-    if (cc < q1 && li < L) bnd[(size_t)cc * BND_LD + q1 + li - cc] = c_sub(eb[t], g);
+    y = c_add(y, c_mul(ed[t], vr[t]));
+    z = c_add(z, c_mul(ee[t], vr[t]));
   }
-  // (c) y = D v, w = tau y - |tau|^2/2 (v^H y) v, D -= v w^H + w v^H
-  C2 y = zero;
-  C2 vc[4];
-#pragma unroll
-  for (int t = 0; t < 4; ++t) {
-    vc[t] = shfl_c(v, gb | (lg + 4 * t));
-    y = c_add(y, c_mul(ed[t], vc[t]));
-  }
+  db = sum4g(db);
   y = sum4g(y);
-  const T vhy = sum16(li < L ? v.x * y.x + v.y * y.y : T(0));
+  z = sum4g(z);
+  const C2 fb = c_mul(c_conj(tau), db);
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {
+    const int r = lg + 4 * t;
+    if ((FULL || r < L) && li < nbc) bnd[cb * BND_LD + q1 + r - cb] = c_sub(eb[t], c_mul(vr[t], fb));
+  }
+  const T vhy = sum16((FULL || li < L) ? v.x * y.x + v.y * y.y : T(0));
   const T t2 = T(0.5) * (tau.x * tau.x + tau.y * tau.y) * vhy;
   const C2 ty = c_mul(tau, y);
-  const C2 w = li < L ? mk<T>(ty.x - t2 * v.x, ty.y - t2 * v.y) : zero;
-#pragma unroll
-  for (int t = 0; t < 4; ++t) {
-    const int dc = lg + 4 * t;
-    const C2 wc = shfl_c(w, gb | dc);
-    if (li < L && dc <= li) {
-      const C2 t1 = c_mulc(v, wc), t3 = c_mulc(w, vc[t]);
-      C2 e = ed[t];
-      e.x -= t1.x + t3.x;
-      e.y -= t1.y + t3.y;
-      if (dc == li) e.y = T(0);
-      bnd[(size_t)(q1 + dc) * BND_LD + li - dc] = e;
-    }
-  }
-  // (d) E -= tau (E v) v^H
-  C2 z = zero;
-#pragma unroll
-  for (int t = 0; t < 4; ++t) z = c_add(z, c_mul(ee[t], vc[t]));
-  z = sum4g(z);
+  const C2 w = (FULL || li < L) ? mk<T>(ty.x - t2 * v.x, ty.y - t2 * v.y) : zero;
+  if (lg == 0) sv[BW + li] = w;
+  __builtin_amdgcn_wave_barrier();
   const C2 tz = c_mul(tau, z);
 #pragma unroll
   for (int t = 0; t < 4; ++t) {
-    const int dc = lg + 4 * t;
-    if (li < Le && dc < L) {
-      const C2 g = c_mulc(tz, vc[t]);
-      bnd[(size_t)(q1 + dc) * BND_LD + q2 + 1 + li - q1 - dc] = c_sub(ee[t], g);
+    const int r = lg + 4 * t;
+    const C2 wr = sv[BW + r];
+    if ((FULL || li < L) && r <= li) {
+      const C2 t1 = c_mulc(v, wr), t3 = c_mulc(w, vr[t]);
+      C2 e = ed[t];
+      e.x -= t1.x + t3.x;
+      e.y -= t1.y + t3.y;
+      if (r == li) e.y = T(0);
+      bnd[(q1 + r) * BND_LD + li - r] = e;
     }
+    if ((FULL || r < L) && (FULL || li < Le))
+      bnd[(q1 + r) * BND_LD + q2 + 1 + li - q1 - r] = c_sub(ee[t], c_mulc(tz, vr[t]));
   }
-  return beta;
+  return h.beta;
 }
 
 // Steps of sweep s: step 0, then one per BW rows while at least two rows remain.
@@ -613,51 +611,55 @@ __device__ __forceinline__ int chase_nsteps(int n, int s) {
   return 1 + (r > 0 ? r / BW : 0);
 }
 
-// Schedule: sweep s runs on wave s % CH_WAVES from round start(s), one step
-// per round, with start(s) = max(start(s-1) + 3, end(s - CH_WAVES)): three
-// rounds behind the previous sweep (their regions are then disjoint) and after
-// the wave's previous sweep has ended. Rounds are separated by a workgroup
-// barrier. LDSB: the compact band lives in LDS (dynamic shared memory) for
-// the whole kernel; otherwise in global memory.
+// Schedule (dataflow): wave w runs sweeps w, w + CH_WAVES, ... in order.
+// Step c of sweep s may run once sweep s-1 has completed step c+2 (the
+// regions of step c of sweep s and step c+3 of sweep s-1 are disjoint), so
+// each wave publishes its progress (sweep << 12 | completed steps, 4095 when
+// the sweep is done) in LDS with release semantics and waits on its
+// predecessor's counter with acquire semantics. A wait is bounded: past
+// CH_SPIN_MAX polls the kernel stops waiting and marks the result NaN.
+// LDSB: the compact band lives in LDS (dynamic shared memory); otherwise in
+// global memory.
+#define CH_SPIN_MAX (1 << 22)
 template <typename T, bool LDSB>
 __global__ __launch_bounds__(CH_THREADS) void band_chase_kernel(BandCtx<T> c) {
   using C2 = typename C2T<T>::t;
   extern __shared__ double ch_smem[];
-  __shared__ int s_total;
+  __shared__ int s_prog[CH_WAVES];
+  __shared__ int s_fail;
+  __shared__ C2 s_vw[CH_WAVES][2 * BW];   // per-wave v and w of the current step
   const int b = blockIdx.x;
   const int n = c.n, ld = c.ld;
   C2* a = c.A + (size_t)b * ld * ld;
   C2* bnd = LDSB ? reinterpret_cast<C2*>(ch_smem) : c.BND + (size_t)b * ld * BND_LD;
-  int* starts = LDSB ? reinterpret_cast<int*>(reinterpret_cast<C2*>(ch_smem) + (size_t)n * BND_LD)
-                     : c.STARTS + (size_t)b * ld;
   double* D = c.D + (size_t)b * ld;
   double* E = c.E + (size_t)b * ld;
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-  const int ns = n - 1;   // sweeps 0 .. n-2
-  if (tid == 0) {
-    int prev = -3, total = 0;
-    for (int s = 0; s < ns; ++s) {
-      int st = prev + 3;
-      if (s >= CH_WAVES) st = max(st, starts[s - CH_WAVES] + chase_nsteps(n, s - CH_WAVES));
-      starts[s] = st;
-      prev = st;
-      total = max(total, st + chase_nsteps(n, s));
-    }
-    s_total = total;
-  }
+  if (tid < CH_WAVES) s_prog[tid] = -1;
+  if (tid == 0) s_fail = 0;
   for (int e = tid; e < n * BND_LD; e += CH_THREADS) {
     const int col = e / BND_LD, d = e % BND_LD, r = col + d;
     bnd[e] = (d <= BW && r < n) ? a[(size_t)r * ld + col] : mk<T>(0, 0);
   }
   __syncthreads();
-  const int total = s_total;
-  int s = w;
-  int start = s < ns ? starts[s] : total;
-  int nst = s < ns ? chase_nsteps(n, s) : 0;
-  int c1 = 0, q1 = 0, q2 = 0;
-  for (int round = 0; round < total; ++round) {
-    if (s < ns && round >= start) {
-      const int cs = round - start;
+  const int ns = n - 1;   // sweeps 0 .. n-2
+  const int pw = (w + CH_WAVES - 1) % CH_WAVES;   // wave of the predecessor sweep
+  bool ok = true;
+  for (int s = w; s < ns; s += CH_WAVES) {
+    const int nst = chase_nsteps(n, s);
+    int c1 = 0, q1 = 0, q2 = 0;
+    for (int cs = 0; cs < nst; ++cs) {
+      if (s > 0 && ok) {
+        const int need = ((s - 1) << 12) + min(cs + 3, 4095);
+        int spins = 0;
+        while (__hip_atomic_load(&s_prog[pw], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP) < need) {
+          __builtin_amdgcn_s_sleep(1);
+          if (++spins > CH_SPIN_MAX) {
+            ok = false;
+            break;
+          }
+        }
+      }
       if (cs == 0) {
         c1 = s;
         q1 = s + 1;
@@ -668,26 +670,27 @@ __global__ __launch_bounds__(CH_THREADS) void band_chase_kernel(BandCtx<T> c) {
         q2 = min(q2 + BW, n - 1);
       }
       if (cs == 0 && lane == 0) D[s] = (double)bnd[(size_t)s * BND_LD].x;
-      const T beta = chase_step<T>(bnd, a + (size_t)s * ld, n, c1, q1, q2, lane);
+      const bool full = cs > 0 && q2 - q1 + 1 == BW && q1 - c1 == BW && q2 + BW <= n - 1;
+      const T beta = full ? chase_step<T, true>(bnd, a + (size_t)s * ld, s_vw[w], n, c1, q1, q2, lane)
+                          : chase_step<T, false>(bnd, a + (size_t)s * ld, s_vw[w], n, c1, q1, q2, lane);
       if (cs == 0 && lane == 0) E[s] = (double)beta;
-      if (cs + 1 == nst) {
-        s += CH_WAVES;
-        start = s < ns ? starts[s] : total;
-        nst = s < ns ? chase_nsteps(n, s) : 0;
-      }
+      __hip_atomic_store(&s_prog[w], (s << 12) + (cs + 1 == nst ? 4095 : cs + 1), __ATOMIC_RELEASE,
+                         __HIP_MEMORY_SCOPE_WORKGROUP);
     }
-    __syncthreads();
   }
+  if (!ok && lane == 0) s_fail = 1;
+  __syncthreads();
   if (tid == 0) {
     D[n - 1] = (double)bnd[(size_t)(n - 1) * BND_LD].x;
     E[n - 1] = 0.0;
+    if (s_fail) D[0] = __builtin_nan("");
   }
 }
 
 size_t band_partials_elems(int ld) { return (size_t)((ld + AV_KCH - 1) / AV_KCH) * ld * BW; }
 // compact band, Zp, T
 size_t band_scratch_elems(int ld) {
-  return (size_t)ld * BND_LD + (size_t)((ld + 63) / 64) * BW * BW + (size_t)BW * BW + (size_t)ld;
+  return (size_t)ld * BND_LD + (size_t)((ld + 63) / 64) * BW * BW + (size_t)BW * BW;
 }
 
 // A22 -= [V | W] [W | V]^H = V W^H + W V^H, one GEMM with K = 2 BW (full tiles)
@@ -716,21 +719,16 @@ static bool chase_lds_ok() {
 
 template <typename T>
 static void launch_panel_qr(hipStream_t st, const BandCtx<T>& c, int batch, int j0, int m) {
-  const dim3 g(batch);
-  // rows per thread in registers: a complex value is 2 (FP32) or 4 (FP64) VGPRs
-  const int nq = (m + 63) / 64;   // rows per thread with 1024 threads
-  if constexpr (sizeof(T) == 4) {
-    if (m <= 256) hipLaunchKernelGGL((panel_qr_kernel<T, 16, 256>), g, dim3(256), 0, st, c, j0);
-    else if (nq <= 8) hipLaunchKernelGGL((panel_qr_kernel<T, 8, 1024>), g, dim3(1024), 0, st, c, j0);
-    else if (nq <= 16) hipLaunchKernelGGL((panel_qr_kernel<T, 16, 1024>), g, dim3(1024), 0, st, c, j0);
-    else if (nq <= 24) hipLaunchKernelGGL((panel_qr_kernel<T, 24, 1024>), g, dim3(1024), 0, st, c, j0);
-    else if (nq <= 32) hipLaunchKernelGGL((panel_qr_kernel<T, 32, 1024>), g, dim3(1024), 0, st, c, j0);
-    else hipLaunchKernelGGL((panel_qr_kernel<T, 0, 1024>), g, dim3(1024), 0, st, c, j0);
-  } else {
-    if (m <= 128) hipLaunchKernelGGL((panel_qr_kernel<T, 8, 256>), g, dim3(256), 0, st, c, j0);
-    else if (nq <= 4) hipLaunchKernelGGL((panel_qr_kernel<T, 4, 1024>), g, dim3(1024), 0, st, c, j0);
-    else hipLaunchKernelGGL((panel_qr_kernel<T, 0, 1024>), g, dim3(1024), 0, st, c, j0);
-  }
+  const dim3 g(batch), blk(1024);
+  // rows per lane in registers: a complex value is 2 (FP32) or 4 (FP64) VGPRs
+  const int nq = (m + 63) / 64;
+  if (nq <= 2) hipLaunchKernelGGL((panel_qr_kernel<T, 2>), g, blk, 0, st, c, j0);
+  else if (nq <= 4) hipLaunchKernelGGL((panel_qr_kernel<T, 4>), g, blk, 0, st, c, j0);
+  else if (nq <= 8) hipLaunchKernelGGL((panel_qr_kernel<T, 8>), g, blk, 0, st, c, j0);
+  else if (nq <= 16) hipLaunchKernelGGL((panel_qr_kernel<T, 16>), g, blk, 0, st, c, j0);
+  else if (nq <= 24 && sizeof(T) == 4) hipLaunchKernelGGL((panel_qr_kernel<T, sizeof(T) == 4 ? 24 : 16>), g, blk, 0, st, c, j0);
+  else if (nq <= 32 && sizeof(T) == 4) hipLaunchKernelGGL((panel_qr_kernel<T, sizeof(T) == 4 ? 32 : 16>), g, blk, 0, st, c, j0);
+  else hipLaunchKernelGGL((panel_qr_kernel<T, 0>), g, blk, 0, st, c, j0);
 }
 
 template <typename T>
@@ -757,7 +755,6 @@ static void band_impl(hipStream_t st, typename C2T<T>::t* A, int n, int ld, int 
   c.BND = scratch;
   c.Zp = scratch + (size_t)batch * ld * BND_LD;
   c.TT = c.Zp + (size_t)batch * c.nrb * BW * BW;
-  c.STARTS = reinterpret_cast<int*>(c.TT + (size_t)batch * BW * BW);
   const long long sP = (long long)2 * BW * ld, sM = (long long)ld * ld;
   for (int j0 = 0;; j0 += BW) {
     const int r0 = j0 + BW, m = n - r0;
@@ -772,7 +769,7 @@ static void band_impl(hipStream_t st, typename C2T<T>::t* A, int n, int ld, int 
     rank2k_full<T>(st, batch, m, Vc + r0, Wc + r0, A + (size_t)r0 * ld + r0, ld, sP, sM);
   }
   // the band (and the schedule) in LDS when it fits, else in global memory
-  const size_t lds = (size_t)n * BND_LD * sizeof(C2) + (size_t)n * sizeof(int);
+  const size_t lds = (size_t)n * BND_LD * sizeof(C2);
   if (lds <= CH_LDS_MAX && chase_lds_ok<T>())
     hipLaunchKernelGGL((band_chase_kernel<T, true>), dim3(batch), dim3(CH_THREADS), lds, st, c);
   else

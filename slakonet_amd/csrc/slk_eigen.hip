@@ -828,7 +828,7 @@ size_t slk_eigh_mixed_workspace_bytes(int n, int ld, int batch) {
 // vectors), 1 one-stage, 2 two-stage. The two-stage chase runs on one
 // workgroup per matrix; beyond BAND_AUTO_MAX rows its single-CU work outweighs
 // the one-stage path's launch chain.
-#define BAND_AUTO_MAX 1280
+#define BAND_AUTO_MAX 2304
 static int g_tridiag_mode = 0;
 
 static bool use_band(int n, int vectors) {
