@@ -76,30 +76,6 @@ __device__ __forceinline__ C2 shfl_xor_c(C2 v, int o) {
   v.y = __shfl_xor(v.y, o, 64);
   return v;
 }
-// DPP row operations (within rows of 16 lanes): quad_perm [1,0,3,2] (0xB1),
-// quad_perm [2,3,0,1] (0x4E), row_half_mirror (0x141), row_mirror (0x140).
-template <int CTRL>
-__device__ __forceinline__ float dpp(float v) {
-  return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), CTRL, 0xF, 0xF, false));
-}
-template <int CTRL>
-__device__ __forceinline__ double dpp(double v) {
-  const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
-  const int lo = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(u & 0xffffffffull), CTRL, 0xF, 0xF, false);
-  const int hi = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(u >> 32), CTRL, 0xF, 0xF, false);
-  return __builtin_bit_cast(double, ((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
-}
-// sum over the 16 lanes of a lane group (lanes 16g .. 16g+15), every lane
-// gets the total: pair sums, quad sums, then the mirrored quad and half row
-// (whose values are already uniform) complete the butterfly
-template <typename T>
-__device__ __forceinline__ T sum16(T v) {
-  v += dpp<0xB1>(v);
-  v += dpp<0x4E>(v);
-  v += dpp<0x141>(v);
-  v += dpp<0x140>(v);
-  return v;
-}
 template <typename C2>
 __device__ __forceinline__ C2 sum16c(C2 v) {
   v.x = sum16(v.x);

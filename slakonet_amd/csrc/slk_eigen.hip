@@ -534,31 +534,49 @@ __global__ __launch_bounds__(64) void refine_tri_kernel(RefineCtx c) {
   __threadfence_block();
   if (c.V2 != nullptr) {
     // band chase: x = Q2 x, sweeps last to first; the blocks of one sweep
-    // (rows s+1+BW t .. s+BW(t+1)) are disjoint, one per 16-lane group
+    // (rows s+1+BW t .. s+BW(t+1)) are disjoint, one per 16-lane group.
+    // NQ > 0 (x in LDS): row s of the reflectors is staged in LDS, and row s-1
+    // is loaded into registers while sweep s is applied.
     const float2* v2 = c.V2 + mat * ld * ld;
     const int li = lane & 15, lg = lane >> 4;
     const int nn = c.n;
+    float2* s_row = reinterpret_cast<float2*>(s_x + ld);
+    constexpr int NP = NQ > 0 ? NQ : 1;
+    float2 pre[NP];
+    auto fetch = [&](int s2) {
+#pragma unroll
+      for (int q = 0; q < NP; ++q) {
+        const int r = s2 + 1 + lane + 64 * q;
+        pre[q] = r < m ? v2[(size_t)s2 * ld + r] : make_float2(0.0f, 0.0f);
+      }
+    };
+    if (NQ > 0 && m >= 2) fetch(m - 2);
     for (int s2 = m - 2; s2 >= 0; --s2) {
       const int rr = nn - 3 - s2;
       const int nst = 1 + (rr > 0 ? rr / SLK_BW : 0);
       const float2* row = v2 + (size_t)s2 * ld;
+      if (NQ > 0) {
+#pragma unroll
+        for (int q = 0; q < NP; ++q) {
+          const int r = s2 + 1 + lane + 64 * q;
+          if (r < m) s_row[r] = pre[q];
+        }
+        if (s2 > 0) fetch(s2 - 1);
+        __threadfence_block();
+        row = s_row;
+      }
       for (int t0 = 0; t0 < nst; t0 += 4) {
         const int t = t0 + lg;
         const int q1 = s2 + 1 + SLK_BW * t;
         const bool act = t < nst && q1 < m;
         const int r = q1 + li;
-        const bool in = act && li < SLK_BW && r < m && r < nn;
+        const bool in = act && r < m && r < nn;
         const float2 tf = act ? row[q1] : make_float2(0.0f, 0.0f);
         float2 vf = make_float2(0.0f, 0.0f);
         if (in) vf = li == 0 ? make_float2(1.0f, 0.0f) : row[r];
         const double2 xv = in ? x[r] : make_double2(0.0, 0.0);
-        double px = (double)vf.x * xv.x + (double)vf.y * xv.y;
-        double py = (double)vf.x * xv.y - (double)vf.y * xv.x;
-#pragma unroll
-        for (int o = 1; o < 16; o <<= 1) {
-          px += __shfl_xor(px, o, 64);
-          py += __shfl_xor(py, o, 64);
-        }
+        const double px = sum16((double)vf.x * xv.x + (double)vf.y * xv.y);
+        const double py = sum16((double)vf.x * xv.y - (double)vf.y * xv.x);
         const double sx = (double)tf.x * px - (double)tf.y * py, sy = (double)tf.x * py + (double)tf.y * px;
         if (in) {
           double2 o = xv;
@@ -998,7 +1016,8 @@ int slk_refine_edges_mixed(const void* H, const void* S, int n, int ld, int batc
   c.meta = reinterpret_cast<int*>(c.part + (size_t)n_spec * 2 * c.nblk * 2);
   // x in LDS when every structure fits (m <= ld <= RF_LDS_X), else in the scratch slot
   c.lds_x = ld <= RF_LDS_X ? 1 : 0;
-  const size_t lds = c.lds_x ? ld * sizeof(double2) : 0;
+  // x, and after the two-stage reduction one staged row of chase reflectors
+  const size_t lds = c.lds_x ? ld * (sizeof(double2) + (band ? sizeof(float2) : 0)) : 0;
   const dim3 g(n_spec, 2);
   if (ld <= 64 * 4)
     hipLaunchKernelGGL(refine_tri_kernel<4>, g, dim3(64), lds, stream, c);
