@@ -39,7 +39,8 @@ struct BandCtx {
   C2 *Vc, *Wc;   // [batch][2 BW][ld]: [V | W] and [W | V] (column k at Vc[k*ld + r]); X goes to Wc first
   C2* Yp;        // [batch][nch][ld][BW]: split-K partials of A22 V
   C2* Zp;        // [batch][nrb][BW][BW]: per row block partials of V^H X
-  C2* TT;        // [batch][BW][BW]: block reflector factor, row-major
+  C2* TT;        // [batch][npan][BW][BW]: block reflector factor of each panel, row-major
+  int npan;      // panels per matrix (stride of TT)
   C2* TAU;       // [batch][ld]: stage-1 tau of column j
   C2* VT;        // nullable [batch][ld][ld]: stage-1 reflector of column j as row j
   C2* BND;       // [batch][ld][BND_LD]: compact band for the chase
@@ -305,7 +306,7 @@ __global__ __launch_bounds__(1024) void panel_qr_kernel(BandCtx<T> c, int j0) {
       if (c.VT != nullptr && w < kb && r >= w) c.VT[(size_t)b * ld * ld + (size_t)(j0 + w) * ld + r0 + r] = v;
     });
   if (tid < BW) c.TAU[(size_t)b * ld + j0 + tid] = tid < kb ? s_tau[tid] : zero;
-  if (tid < BW * BW) c.TT[(size_t)b * BW * BW + tid] = s_T[tid / BW][tid % BW];
+  if (tid < BW * BW) c.TT[((size_t)b * c.npan + j0 / BW) * BW * BW + tid] = s_T[tid / BW][tid % BW];
 }
 
 // ---------------------------------------------------------------------------
@@ -390,7 +391,7 @@ __global__ __launch_bounds__(256) void band_x_kernel(BandCtx<T> c, int r0, int n
   __shared__ C2 s_T[BW][BW];
   __shared__ C2 s_y[64][BW + 1], s_v[64][BW + 1], s_x[64][BW + 1];
   const int tid = threadIdx.x;
-  s_T[tid / BW][tid % BW] = c.TT[(size_t)b * BW * BW + tid];
+  s_T[tid / BW][tid % BW] = c.TT[((size_t)b * c.npan + (r0 - BW) / BW) * BW * BW + tid];
   const int rl = tid >> 2, c0 = (tid & 3) * 4;
   const int gr = row0 + rl;
   const C2* vc = c.Vc + (size_t)b * 2 * BW * ld;
@@ -445,7 +446,7 @@ __global__ __launch_bounds__(256) void band_w_kernel(BandCtx<T> c, int r0, int n
   C2 z = mk<T>(0, 0);
   for (int q = 0; q < nrbk; ++q) z = c_add(z, c.Zp[((size_t)b * c.nrb + q) * BW * BW + tid]);
   s_Z[i][kk] = z;
-  s_T[i][kk] = c.TT[(size_t)b * BW * BW + tid];
+  s_T[i][kk] = c.TT[((size_t)b * c.npan + (r0 - BW) / BW) * BW * BW + tid];
   const int rl = tid >> 2, c0 = (tid & 3) * 4;
   const int gr = row0 + rl;
   const C2* vc = c.Vc + (size_t)b * 2 * BW * ld;
@@ -665,8 +666,13 @@ __global__ __launch_bounds__(CH_THREADS) void band_chase_kernel(BandCtx<T> c) {
 
 size_t band_partials_elems(int ld) { return (size_t)((ld + AV_KCH - 1) / AV_KCH) * ld * BW; }
 // compact band, Zp, T
+int band_panels(int ld) { return (ld + BW - 1) / BW; }
+// compact band, Zp, the panels' T factors
 size_t band_scratch_elems(int ld) {
-  return (size_t)ld * BND_LD + (size_t)((ld + 63) / 64) * BW * BW + (size_t)BW * BW;
+  return (size_t)ld * BND_LD + (size_t)((ld + 63) / 64) * BW * BW + (size_t)band_panels(ld) * BW * BW;
+}
+size_t band_tt_offset(int ld, int batch) {
+  return (size_t)batch * ld * BND_LD + (size_t)batch * ((ld + 63) / 64) * BW * BW;
 }
 
 // A22 -= [V | W] [W | V]^H = V W^H + W V^H, one GEMM with K = 2 BW (full tiles)
@@ -725,9 +731,8 @@ static void band_impl(hipStream_t st, typename C2T<T>::t* A, int n, int ld, int 
   c.E = E;
   c.nch = (ld + AV_KCH - 1) / AV_KCH;
   c.nrb = (ld + 63) / 64;
-  // scratch per matrix: band, then Zp, then T (strided by band_scratch_elems)
-  const size_t per = band_scratch_elems(ld);
-  (void)per;
+  c.npan = band_panels(ld);
+  // scratch: band, then Zp, then the panels' T (band_scratch_elems per matrix)
   c.BND = scratch;
   c.Zp = scratch + (size_t)batch * ld * BND_LD;
   c.TT = c.Zp + (size_t)batch * c.nrb * BW * BW;

@@ -64,3 +64,8 @@ void launch_band_tridiag_f32(hipStream_t st, float2* C, int n, int ld, int batch
                              float2* Vc, float2* Wc, float2* Yp, float2* scratch, float2* VT);
 size_t band_partials_elems(int ld);
 size_t band_scratch_elems(int ld);
+// Panels of the band reduction per matrix, and the element offset inside the
+// band scratch (of `batch` matrices) of the panels' block-reflector factors
+// T ([batch][band_panels(ld)][SLK_BW][SLK_BW], row-major, Q_p = I - V_p T_p V_p^H).
+int band_panels(int ld);
+size_t band_tt_offset(int ld, int batch);

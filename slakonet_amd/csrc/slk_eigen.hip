@@ -530,64 +530,14 @@ __global__ __launch_bounds__(64) void refine_tri_kernel(RefineCtx c) {
     for (int i = lane; i < m; i += 64) y[i] *= nr;
     __threadfence_block();
   }
+  if (c.V2 != nullptr) {
+    // two-stage reduction: refine_band_kernel carries y back (x slot of scratch)
+    double2* xg = reinterpret_cast<double2*>(base + 6 * (size_t)ld);
+    for (int r = lane; r < m; r += 64) xg[r] = make_double2(y[r], 0.0);
+    return;
+  }
   for (int r = lane; r < m; r += 64) x[r] = make_double2(y[r], 0.0);
   __threadfence_block();
-  if (c.V2 != nullptr) {
-    // band chase: x = Q2 x, sweeps last to first; the blocks of one sweep
-    // (rows s+1+BW t .. s+BW(t+1)) are disjoint, one per 16-lane group.
-    // NQ > 0 (x in LDS): row s of the reflectors is staged in LDS, and row s-1
-    // is loaded into registers while sweep s is applied.
-    const float2* v2 = c.V2 + mat * ld * ld;
-    const int li = lane & 15, lg = lane >> 4;
-    const int nn = c.n;
-    float2* s_row = reinterpret_cast<float2*>(s_x + ld);
-    constexpr int NP = NQ > 0 ? NQ : 1;
-    float2 pre[NP];
-    auto fetch = [&](int s2) {
-#pragma unroll
-      for (int q = 0; q < NP; ++q) {
-        const int r = s2 + 1 + lane + 64 * q;
-        pre[q] = r < m ? v2[(size_t)s2 * ld + r] : make_float2(0.0f, 0.0f);
-      }
-    };
-    if (NQ > 0 && m >= 2) fetch(m - 2);
-    for (int s2 = m - 2; s2 >= 0; --s2) {
-      const int rr = nn - 3 - s2;
-      const int nst = 1 + (rr > 0 ? rr / SLK_BW : 0);
-      const float2* row = v2 + (size_t)s2 * ld;
-      if (NQ > 0) {
-#pragma unroll
-        for (int q = 0; q < NP; ++q) {
-          const int r = s2 + 1 + lane + 64 * q;
-          if (r < m) s_row[r] = pre[q];
-        }
-        if (s2 > 0) fetch(s2 - 1);
-        __threadfence_block();
-        row = s_row;
-      }
-      for (int t0 = 0; t0 < nst; t0 += 4) {
-        const int t = t0 + lg;
-        const int q1 = s2 + 1 + SLK_BW * t;
-        const bool act = t < nst && q1 < m;
-        const int r = q1 + li;
-        const bool in = act && r < m && r < nn;
-        const float2 tf = act ? row[q1] : make_float2(0.0f, 0.0f);
-        float2 vf = make_float2(0.0f, 0.0f);
-        if (in) vf = li == 0 ? make_float2(1.0f, 0.0f) : row[r];
-        const double2 xv = in ? x[r] : make_double2(0.0, 0.0);
-        const double px = sum16((double)vf.x * xv.x + (double)vf.y * xv.y);
-        const double py = sum16((double)vf.x * xv.y - (double)vf.y * xv.x);
-        const double sx = (double)tf.x * px - (double)tf.y * py, sy = (double)tf.x * py + (double)tf.y * px;
-        if (in) {
-          double2 o = xv;
-          o.x -= (double)vf.x * sx - (double)vf.y * sy;
-          o.y -= (double)vf.x * sy + (double)vf.y * sx;
-          x[r] = o;
-        }
-      }
-      __threadfence_block();
-    }
-  }
   // x = Q1 x, reflectors last to first: x -= v tau (v^H x), v = row j of VT
   const float2* vt = c.R + mat * ld * ld;
   const float2* tau = c.TAU + mat * ld;
@@ -666,6 +616,131 @@ __global__ __launch_bounds__(64) void refine_tri_kernel(RefineCtx c) {
     double2* xg = reinterpret_cast<double2*>(base + 6 * (size_t)ld);
     for (int r = lane; r < m; r += 64) xg[r] = x[r];
   }
+}
+
+// Back-transform after the two-stage reduction, one 256-thread workgroup per
+// (structure, edge), x in LDS:
+//   x = Q2 x: chase reflectors, sweeps last to first; the blocks of one sweep
+//     are disjoint, one per 16-lane group; row s of the reflectors is staged
+//     in LDS while row s-1 is loaded into registers;
+//   x = Q1 x: stage-1 reflectors one panel at a time in compact-WY form,
+//     x -= V_p (T_p (V_p^H x)), panels last to first (T_p from the reduction).
+#define RB_THREADS 256
+#define RB_LDS_MAX 2688     // x (FP64) and one reflector row (FP32) in 64 KB of LDS
+template <int NP>
+__global__ __launch_bounds__(RB_THREADS) void refine_band_kernel(RefineCtx c, const float2* TT, int npan) {
+  const int s = blockIdx.x, edge = blockIdx.y, v = s * 2 + edge;
+  if (c.meta[2 * v] < 0) return;
+  const size_t mat = (size_t)c.meta[2 * v + 1];
+  const int m = c.spec[s].nb, ld = c.ld, nn = c.n;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  extern __shared__ double2 rb_smem[];
+  double2* x = rb_smem;
+  float2* s_row = reinterpret_cast<float2*>(rb_smem + ld);
+  __shared__ double2 s_red[RB_THREADS / 64][SLK_BW];
+  __shared__ double2 s_t[SLK_BW];
+  double2* xg = reinterpret_cast<double2*>(rf_base(c, v) + 6 * (size_t)ld);
+  for (int r = tid; r < m; r += RB_THREADS) x[r] = xg[r];
+  const float2* v2 = c.V2 + mat * ld * ld;
+  float2 pre[NP];
+  auto fetch = [&](int s2) {
+#pragma unroll
+    for (int q = 0; q < NP; ++q) {
+      const int r = s2 + 1 + tid + RB_THREADS * q;
+      pre[q] = r < m ? v2[(size_t)s2 * ld + r] : make_float2(0.0f, 0.0f);
+    }
+  };
+  if (m >= 2) fetch(m - 2);
+  const int li = tid & 15, g = tid >> 4;
+  for (int s2 = m - 2; s2 >= 0; --s2) {
+#pragma unroll
+    for (int q = 0; q < NP; ++q) {
+      const int r = s2 + 1 + tid + RB_THREADS * q;
+      if (r < m) s_row[r] = pre[q];
+    }
+    if (s2 > 0) fetch(s2 - 1);
+    __syncthreads();
+    const int rr = nn - 3 - s2;
+    const int nst = 1 + (rr > 0 ? rr / SLK_BW : 0);
+    for (int t0 = 0; t0 < nst; t0 += RB_THREADS / 16) {
+      const int t = t0 + g;
+      const int q1 = s2 + 1 + SLK_BW * t;
+      const bool act = t < nst && q1 < m;
+      const int r = q1 + li;
+      const bool in = act && r < m && r < nn;
+      const float2 tf = act ? s_row[q1] : make_float2(0.0f, 0.0f);
+      float2 vf = make_float2(0.0f, 0.0f);
+      if (in) vf = li == 0 ? make_float2(1.0f, 0.0f) : s_row[r];
+      const double2 xv = in ? x[r] : make_double2(0.0, 0.0);
+      const double px = sum16((double)vf.x * xv.x + (double)vf.y * xv.y);
+      const double py = sum16((double)vf.x * xv.y - (double)vf.y * xv.x);
+      const double sx = (double)tf.x * px - (double)tf.y * py, sy = (double)tf.x * py + (double)tf.y * px;
+      if (in) {
+        double2 o = xv;
+        o.x -= (double)vf.x * sx - (double)vf.y * sy;
+        o.y -= (double)vf.x * sy + (double)vf.y * sx;
+        x[r] = o;
+      }
+    }
+    __syncthreads();
+  }
+  // x = Q1 x, panel by panel (reflector j0+k: row j0+k of VT, entries r >= j0+k+BW)
+  const float2* vt = c.R + mat * ld * ld;
+  const float2* tt = TT + mat * (size_t)npan * SLK_BW * SLK_BW;
+  for (int p = (m - 1 - SLK_BW) / SLK_BW; p >= 0 && m - 1 >= SLK_BW; --p) {
+    const int j0 = p * SLK_BW, r0 = j0 + SLK_BW;
+    double2 acc[SLK_BW];
+#pragma unroll
+    for (int k = 0; k < SLK_BW; ++k) acc[k] = make_double2(0.0, 0.0);
+    for (int r = r0 + tid; r < m; r += RB_THREADS) {
+      const double2 xv = x[r];
+#pragma unroll
+      for (int k = 0; k < SLK_BW; ++k)
+        if (r >= r0 + k) {
+          const float2 vf = vt[(size_t)(j0 + k) * ld + r];
+          acc[k].x += (double)vf.x * xv.x + (double)vf.y * xv.y;
+          acc[k].y += (double)vf.x * xv.y - (double)vf.y * xv.x;
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < SLK_BW; ++k) {
+      double ax = sum16(acc[k].x), ay = sum16(acc[k].y);
+      ax += __shfl_xor(ax, 16, 64);
+      ay += __shfl_xor(ay, 16, 64);
+      ax += __shfl_xor(ax, 32, 64);
+      ay += __shfl_xor(ay, 32, 64);
+      if (lane == 0) s_red[wv][k] = make_double2(ax, ay);
+    }
+    __syncthreads();
+    if (tid < SLK_BW) {
+      double2 t = make_double2(0.0, 0.0);
+#pragma unroll
+      for (int l = 0; l < SLK_BW; ++l) {
+        double2 yl = make_double2(0.0, 0.0);
+#pragma unroll
+        for (int w = 0; w < RB_THREADS / 64; ++w) yl = c_add(yl, s_red[w][l]);
+        const float2 tf = tt[(size_t)p * SLK_BW * SLK_BW + tid * SLK_BW + l];
+        t.x += (double)tf.x * yl.x - (double)tf.y * yl.y;
+        t.y += (double)tf.x * yl.y + (double)tf.y * yl.x;
+      }
+      s_t[tid] = t;
+    }
+    __syncthreads();
+    for (int r = r0 + tid; r < m; r += RB_THREADS) {
+      double2 xv = x[r];
+#pragma unroll
+      for (int k = 0; k < SLK_BW; ++k)
+        if (r >= r0 + k) {
+          const float2 vf = vt[(size_t)(j0 + k) * ld + r];
+          const double2 tk = s_t[k];
+          xv.x -= (double)vf.x * tk.x - (double)vf.y * tk.y;
+          xv.y -= (double)vf.x * tk.y + (double)vf.y * tk.x;
+        }
+      x[r] = xv;
+    }
+    __syncthreads();
+  }
+  for (int r = tid; r < m; r += RB_THREADS) xg[r] = x[r];
 }
 
 // z[c] = sum_{r >= c} conj(W[r][c]) x[r] for the RF_BLK columns of this block
@@ -1015,11 +1090,25 @@ int slk_refine_edges_mixed(const void* H, const void* S, int n, int ld, int batc
   c.part = scratch + (size_t)n_spec * 2 * RF_SCR * ld;
   c.meta = reinterpret_cast<int*>(c.part + (size_t)n_spec * 2 * c.nblk * 2);
   // x in LDS when every structure fits (m <= ld <= RF_LDS_X), else in the scratch slot
-  c.lds_x = ld <= RF_LDS_X ? 1 : 0;
-  // x, and after the two-stage reduction one staged row of chase reflectors
-  const size_t lds = c.lds_x ? ld * (sizeof(double2) + (band ? sizeof(float2) : 0)) : 0;
+  c.lds_x = (ld <= RF_LDS_X && !band) ? 1 : 0;
+  const size_t lds = c.lds_x ? ld * sizeof(double2) : 0;
   const dim3 g(n_spec, 2);
-  if (ld <= 64 * 4)
+  if (band) {
+    // tridiagonal solve only, then the two-stage back-transform
+    if (ld > RB_LDS_MAX) return 3;
+    const size_t lds2 = ld * (sizeof(double2) + sizeof(float2));
+    const float2* TT = s.fBand + band_tt_offset(ld, batch);
+    const int npan = band_panels(ld);
+    hipLaunchKernelGGL(refine_tri_kernel<0>, g, dim3(64), 0, stream, c);
+    if (ld <= RB_THREADS * 2)
+      hipLaunchKernelGGL(refine_band_kernel<2>, g, dim3(RB_THREADS), lds2, stream, c, TT, npan);
+    else if (ld <= RB_THREADS * 4)
+      hipLaunchKernelGGL(refine_band_kernel<4>, g, dim3(RB_THREADS), lds2, stream, c, TT, npan);
+    else if (ld <= RB_THREADS * 10)
+      hipLaunchKernelGGL(refine_band_kernel<10>, g, dim3(RB_THREADS), lds2, stream, c, TT, npan);
+    else
+      hipLaunchKernelGGL(refine_band_kernel<16>, g, dim3(RB_THREADS), lds2, stream, c, TT, npan);
+  } else if (ld <= 64 * 4)
     hipLaunchKernelGGL(refine_tri_kernel<4>, g, dim3(64), lds, stream, c);
   else if (ld <= 64 * 8)
     hipLaunchKernelGGL(refine_tri_kernel<8>, g, dim3(64), lds, stream, c);
