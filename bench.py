@@ -117,6 +117,11 @@ def run_serve(tables, inputs, arrivals, grid):
 
 HETRD_SAMPLE_EVERY = 8
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hetrd_step_traffic.json")
+GEMM_TRAFFIC_FILE = os.path.join(ROOT, "profiles", "gemm_f64_traffic.json")
+FP64_MFMA_PEAK_TFS = 78.6
+# one compute unit's FP32 VALU rate: 4 SIMDs x 16 lanes x 2 flops (FMA) x 2.4 GHz
+CU_FP32_VALU_PEAK_TFS = 4 * 16 * 2 * 2.4e9 / 1e12
+SAMPLE_KINDS = 4   # 0 one-stage mat-vec, 1 build, 2 FP64 GEMM, 3 band chase
 
 
 def start_sampling(every, capacity):
@@ -138,7 +143,7 @@ def stop_sampling():
     from slakonet_amd import _lib, engine
     lib = _lib.lib()
     res = []
-    for kind in (0, 1):
+    for kind in range(SAMPLE_KINDS):
         buf = (C.c_double * 7)()
         _lib.check(lib.slk_sampler_read(kind, buf), "slk_sampler_read")
         res.append(tuple(buf))
@@ -147,38 +152,78 @@ def stop_sampling():
     return res
 
 
-def roofline_fields(hetrd, build):
-    """roofline (dominant kernel: the tridiagonalisation mat-vec) and
-    build_roofline from the live samples; traffic = measured HBM bytes per
-    algorithmic byte (counter runs, profiles/hetrd_step_traffic.json) times
-    the algorithmic bytes per launch."""
+def _ratio(path, key):
+    if os.path.exists(path):
+        with open(path) as f:
+            return json.load(f).get(key)
+    return None
+
+
+def _window(sample):
+    """(sampled launches, avg window ms, avg event ms, amount per sampled
+    launch, launches counted, amount of all counted launches)."""
+    n, ms, amount, n_all, amount_all, ev_ms, _ = sample
+    if n <= 0 or ms <= 0:
+        return None
+    return n, ms / n, ev_ms / n, amount / n, n_all, amount_all
+
+
+def roofline_fields(hetrd, build, gemm, chase):
+    """Rooflines from the live samples.
+
+    roofline: the FP64 MFMA GEMM (Cholesky trailing updates and L^-1), the
+    largest device-wide MFMA consumer, against FP64 dense MFMA peak; traffic
+    = measured HBM bytes per algorithmic flop (counter runs,
+    profiles/gemm_f64_traffic.json) times the flops per launch.
+    roofline_chase: the bulge chase (the longest-running kernel), one
+    workgroup per matrix, issue-bound on its compute unit: the library counts
+    flops per workgroup, so flops / window is one workgroup's rate, against
+    one CU's FP32 VALU rate.
+    roofline_hetrd_step: the one-stage mat-vec (eigenvector solves and
+    n > 2304), when it ran. build_roofline: the H/S build (HBM)."""
     out = {}
-    n, ms, nbytes, n_all, bytes_all, ev_ms, _ = hetrd
-    if n_all > 0:
-        ach = nbytes / (ms * 1e-3) / 1e9 if n > 0 and ms > 0 else None
-        per_launch = nbytes / n if n > 0 else bytes_all / n_all
-        traffic = None
-        if os.path.exists(TRAFFIC_FILE):
-            with open(TRAFFIC_FILE) as f:
-                ratio = json.load(f).get("hbm_bytes_per_algorithmic_byte")
-            if ratio:
-                traffic = ratio * per_launch
-        out["roofline"] = {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                           "frac": None if ach is None else ach / HBM_PEAK_GBS, "traffic": traffic,
-                           "kernel": "hetrd_step_kernel<float> (tridiagonalisation trailing mat-vec)",
-                           "avg_launch_us": 1000.0 * ms / n if n > 0 else None,
-                           "avg_launch_us_events": 1000.0 * ev_ms / n if n > 0 else None,
-                           "algorithmic_bytes_per_launch": per_launch, "launches_sampled": int(n),
+    g = _window(gemm)
+    if g is not None:
+        n, ms, ev, fl, n_all, fl_all = g
+        ach = fl / (ms * 1e-3) / 1e12
+        ratio = _ratio(GEMM_TRAFFIC_FILE, "hbm_bytes_per_flop")
+        out["roofline"] = {"bound": "mfma", "achieved": ach, "peak": FP64_MFMA_PEAK_TFS, "unit": "TFLOP/s",
+                           "frac": ach / FP64_MFMA_PEAK_TFS, "traffic": ratio * fl if ratio else None,
+                           "kernel": "gemm_mfma_kernel<double> (Cholesky trailing updates, L^-1)",
+                           "avg_launch_us": 1000.0 * ms, "avg_launch_us_events": 1000.0 * ev,
+                           "algorithmic_flops_per_launch": fl, "launches_sampled": int(n),
                            "sample_every": HETRD_SAMPLE_EVERY, "launches_total": int(n_all),
-                           "algorithmic_bytes_total": bytes_all}
-    bn, bms, bbytes, bn_all, bbytes_all, bev_ms, _ = build
-    if bn > 0 and bms > 0:
-        ach = bbytes / (bms * 1e-3) / 1e9
+                           "algorithmic_flops_total": fl_all,
+                           "est_kernel_ms_total": ms * n_all}
+    ch = _window(chase)
+    if ch is not None:
+        n, ms, ev, fl, n_all, fl_all = ch
+        ach = fl / (ms * 1e-3) / 1e12
+        out["roofline_chase"] = {"bound": "valu (one CU)", "achieved": ach, "peak": CU_FP32_VALU_PEAK_TFS,
+                                 "unit": "TFLOP/s", "frac": ach / CU_FP32_VALU_PEAK_TFS,
+                                 "kernel": "band_chase_kernel<float> (band -> tridiagonal, one workgroup per matrix)",
+                                 "avg_launch_us": 1000.0 * ms, "avg_launch_us_events": 1000.0 * ev,
+                                 "algorithmic_flops_per_workgroup": fl, "launches_sampled": int(n),
+                                 "launches_total": int(n_all), "est_kernel_ms_total": ms * n_all}
+    h = _window(hetrd)
+    if h is not None:
+        n, ms, ev, nbytes, n_all, bytes_all = h
+        ach = nbytes / (ms * 1e-3) / 1e9
+        ratio = _ratio(TRAFFIC_FILE, "hbm_bytes_per_algorithmic_byte")
+        out["roofline_hetrd_step"] = {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                      "frac": ach / HBM_PEAK_GBS, "traffic": ratio * nbytes if ratio else None,
+                                      "kernel": "hetrd_step_kernel<float> (one-stage trailing mat-vec)",
+                                      "avg_launch_us": 1000.0 * ms, "launches_sampled": int(n),
+                                      "launches_total": int(n_all)}
+    b = _window(build)
+    if b is not None:
+        n, ms, ev, nbytes, n_all, _ = b
+        ach = nbytes / (ms * 1e-3) / 1e9
         out["build_roofline"] = {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                                  "frac": ach / HBM_PEAK_GBS, "kernel": "slk_build_kernel<complex128>",
-                                 "avg_launch_us": 1000.0 * bms / bn, "avg_launch_us_events": 1000.0 * bev_ms / bn,
-                                 "algorithmic_bytes_per_launch": bbytes / bn,
-                                 "launches_sampled": int(bn), "launches_total": int(bn_all)}
+                                 "avg_launch_us": 1000.0 * ms, "avg_launch_us_events": 1000.0 * ev,
+                                 "algorithmic_bytes_per_launch": nbytes,
+                                 "launches_sampled": int(n), "launches_total": int(n_all)}
     return out
 
 
@@ -385,6 +430,9 @@ def main():
     }
     if samples is not None:
         out.update(roofline_fields(*samples))
+        # launches and algorithmic amount counted per kind (also with no timed samples)
+        out["sampler_counts"] = {name: [int(smp[3]), smp[4]] for name, smp in
+                                 zip(("hetrd_step", "build", "gemm_f64", "chase"), samples)}
     if args.cpu_seconds > 0:
         out["cpu_baseline"] = cpu_baseline(structs, skfs, args.cpu_seconds)
     print(json.dumps(out))

@@ -224,15 +224,22 @@ int slk_refine_edges_mixed(const void* H, const void* S, int n, int ld, int batc
  * sampled kernel kind is counted with its algorithmic bytes, and every
  * `every`-th one (at most `capacity`) is started with a HIP start/stop event
  * pair that the runtime records at the kernel's own begin and end on its
- * launch stream. Kinds: 0 = tridiagonalisation mat-vec (bytes = one triangle
- * of the trailing Hermitian block per matrix), 1 = H/S build (bytes from
- * slk_sampler_hint, per calling thread: the H and S elements written).
+ * launch stream, and records the launch's execution window (earliest
+ * workgroup start, latest workgroup end on the GPU's constant-rate wall
+ * clock). Kinds and their algorithmic amount: 0 = one-stage
+ * tridiagonalisation mat-vec (bytes: one triangle of the trailing Hermitian
+ * block per matrix), 1 = H/S build (bytes from slk_sampler_hint, per calling
+ * thread: the H and S elements written), 2 = FP64 MFMA GEMM (flops of the
+ * executed 64x64 output tiles), 3 = band chase (flops of the steps of one
+ * matrix, i.e. per workgroup).
  * enable = 0 disables and frees. Thread-safe. */
 int slk_sampler_reset(int enable, int every, int capacity);
 int slk_sampler_hint(int kind, double bytes);
-/* out[0] = event samples, out[1] = their summed milliseconds, out[2] = their
- * summed bytes, out[3] = launches counted, out[4] = bytes of all counted
- * launches; waits for the sampled launches to complete. */
+/* out[0] = sampled launches, out[1] = their summed execution-window
+ * milliseconds, out[2] = their summed algorithmic amount, out[3] = launches
+ * counted, out[4] = amount of all counted launches, out[5] = summed event-pair
+ * milliseconds of the sampled launches, out[6] = wall clock rate (kHz);
+ * waits for the sampled launches to complete. */
 int slk_sampler_read(int kind, double* out);
 
 /* Atom-type projected DOS from the eigenvectors of one structure (V: the

@@ -23,6 +23,12 @@
 // are disjoint), so 16 sweeps run at once, in lock-step rounds of one step.
 #include "slk_common.h"
 #include "slk_dense.h"
+#include <hip/hip_ext.h>
+
+#include <mutex>
+#include <unordered_map>
+
+#include "slk_sampler.h"
 
 #define BW SLK_BW
 #define AV_KT 64                      // k per LDS tile of band_av
@@ -599,8 +605,9 @@ __device__ __forceinline__ int chase_nsteps(int n, int s) {
 // global memory.
 #define CH_SPIN_MAX (1 << 22)
 template <typename T, bool LDSB>
-__global__ __launch_bounds__(CH_THREADS) void band_chase_kernel(BandCtx<T> c) {
+__global__ __launch_bounds__(CH_THREADS) void band_chase_kernel(BandCtx<T> c, unsigned long long* clk) {
   using C2 = typename C2T<T>::t;
+  sample_start(clk);
   extern __shared__ double ch_smem[];
   __shared__ int s_prog[CH_WAVES];
   __shared__ int s_fail;
@@ -662,6 +669,36 @@ __global__ __launch_bounds__(CH_THREADS) void band_chase_kernel(BandCtx<T> c) {
     E[n - 1] = 0.0;
     if (s_fail) D[0] = __builtin_nan("");
   }
+  sample_end(clk);
+}
+
+// Algorithmic flops of the chase of one n x n band (cached per n): per step
+// with L reflector rows, nbc bulge columns and Le rows below, 2 complex
+// multiply-adds per bulge element (inner product, update), L^2 for D v and
+// L(L+1) for the rank-2 update of D's lower triangle, 2 per element of the
+// block below, and ~10 L for the reflector; 8 real flops per multiply-add.
+static double chase_flops(int n) {
+  static std::mutex mu;
+  static std::unordered_map<int, double> cache;
+  std::lock_guard<std::mutex> lock(mu);
+  const auto it = cache.find(n);
+  if (it != cache.end()) return it->second;
+  double f = 0.0;
+  for (int s = 0; s < n - 1; ++s) {
+    const int r = n - 3 - s, nst = 1 + (r > 0 ? r / BW : 0);
+    int c1 = s, q1 = s + 1, q2 = min(s + BW, n - 1);
+    for (int cs = 0; cs < nst; ++cs) {
+      if (cs > 0) {
+        c1 = q1;
+        q1 = q2 + 1;
+        q2 = min(q2 + BW, n - 1);
+      }
+      const double L = q2 - q1 + 1, nbc = q1 - c1 - 1, Le = min(BW, n - 1 - q2);
+      f += 16.0 * nbc * L + 8.0 * L * L + 8.0 * L * (L + 1) + 16.0 * Le * L + 10.0 * L;
+    }
+  }
+  cache[n] = f;
+  return f;
 }
 
 size_t band_partials_elems(int ld) { return (size_t)((ld + AV_KCH - 1) / AV_KCH) * ld * BW; }
@@ -751,10 +788,24 @@ static void band_impl(hipStream_t st, typename C2T<T>::t* A, int n, int ld, int 
   }
   // the band (and the schedule) in LDS when it fits, else in global memory
   const size_t lds = (size_t)n * BND_LD * sizeof(C2);
-  if (lds <= CH_LDS_MAX && chase_lds_ok<T>())
-    hipLaunchKernelGGL((band_chase_kernel<T, true>), dim3(batch), dim3(CH_THREADS), lds, st, c);
-  else
-    hipLaunchKernelGGL((band_chase_kernel<T, false>), dim3(batch), dim3(CH_THREADS), 0, st, c);
+  SampleSlot sl;
+  // algorithmic amount per workgroup (one matrix): each workgroup runs on its
+  // own compute unit, so flops / window is the per-CU rate
+  const bool sampled = sampler_enabled() && sampler_take(SLK_SAMPLE_CHASE, chase_flops(n), &sl);
+  unsigned long long* clk = sampled ? sl.clk : nullptr;
+  if (lds <= CH_LDS_MAX && chase_lds_ok<T>()) {
+    if (sampled)
+      hipExtLaunchKernelGGL((band_chase_kernel<T, true>), dim3(batch), dim3(CH_THREADS), lds, st, sl.e0, sl.e1, 0, c,
+                            clk);
+    else
+      hipLaunchKernelGGL((band_chase_kernel<T, true>), dim3(batch), dim3(CH_THREADS), lds, st, c, clk);
+  } else {
+    if (sampled)
+      hipExtLaunchKernelGGL((band_chase_kernel<T, false>), dim3(batch), dim3(CH_THREADS), 0, st, sl.e0, sl.e1, 0, c,
+                            clk);
+    else
+      hipLaunchKernelGGL((band_chase_kernel<T, false>), dim3(batch), dim3(CH_THREADS), 0, st, c, clk);
+  }
   (void)sizeof(C2);
 }
 

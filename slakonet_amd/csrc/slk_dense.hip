@@ -5,6 +5,9 @@
 // matrix-vector product; per 32-column panel one rank-2k update on the GEMM).
 #include "slk_common.h"
 #include "slk_dense.h"
+#include <hip/hip_ext.h>
+
+#include "slk_sampler.h"
 
 #define GT 64
 #define GK 16
@@ -54,11 +57,16 @@ __global__ __launch_bounds__(256) void gemm_mfma_kernel(int M, int N, int K, T a
                                                         long long sA, int opA,
                                                         const typename CT<T>::c2* __restrict__ B, int ldb,
                                                         long long sB, int opB, T beta, typename CT<T>::c2* C,
-                                                        int ldc, long long sC, int klim, int tri) {
+                                                        int ldc, long long sC, int klim, int tri,
+                                                        unsigned long long* clk) {
   using C2 = typename CT<T>::c2;
   using V4 = typename CT<T>::v4;
   const int m0 = blockIdx.y * GT, n0 = blockIdx.x * GT;
-  if (tri && n0 > m0 + GT - 1) return;
+  sample_start(clk);
+  if (tri && n0 > m0 + GT - 1) {
+    sample_end(clk);
+    return;
+  }
   __shared__ T Ar[GK][GT + 4], Ai[GK][GT + 4], Br[GK][GT + 4], Bi[GK][GT + 4];
   const int b = blockIdx.z;
   const C2* a = A + (size_t)b * sA;
@@ -146,6 +154,28 @@ __global__ __launch_bounds__(256) void gemm_mfma_kernel(int M, int N, int K, T a
           c[(size_t)gm * ldc + gn] = r;
         }
       }
+  if (clk != nullptr) {
+    __syncthreads();
+    sample_end(clk);
+  }
+}
+
+// Flops of the executed 64x64 output tiles (edge tiles clipped to M x N,
+// K ranges as limited by klim): 8 real flops per complex multiply-add.
+static double gemm_tile_flops(int batch, int M, int N, int K, int klim, int tri) {
+  if (!klim && !tri) return 8.0 * batch * (double)M * N * K;
+  double f = 0.0;
+  for (int m0 = 0; m0 < M; m0 += GT)
+    for (int n0 = 0; n0 < N; n0 += GT) {
+      if (tri && n0 > m0 + GT - 1) continue;
+      int kbeg = 0, kend = K;
+      if (klim & KLIM_A_LOWER) kend = min(kend, m0 + GT);
+      if (klim & KLIM_A_UPPER) kbeg = max(kbeg, m0);
+      if (klim & KLIM_B_UPPER) kend = min(kend, n0 + GT);
+      if (klim & KLIM_B_LOWER) kbeg = max(kbeg, n0);
+      if (kend > kbeg) f += 8.0 * min(GT, M - m0) * min(GT, N - n0) * (kend - kbeg);
+    }
+  return f * batch;
 }
 
 void launch_zgemm(hipStream_t st, int batch, int M, int N, int K, double alpha, const double2* A, int lda,
@@ -153,8 +183,13 @@ void launch_zgemm(hipStream_t st, int batch, int M, int N, int K, double alpha, 
                   int ldc, long long sC, int klim, int tri) {
   if (M <= 0 || N <= 0 || batch <= 0) return;
   dim3 g((N + GT - 1) / GT, (M + GT - 1) / GT, batch);
-  hipLaunchKernelGGL(gemm_mfma_kernel<double>, g, dim3(256), 0, st, M, N, K, alpha, A, lda, sA, opA, B, ldb, sB,
-                     opB, beta, C, ldc, sC, klim, tri);
+  SampleSlot sl;
+  if (sampler_enabled() && sampler_take(SLK_SAMPLE_GEMM_F64, gemm_tile_flops(batch, M, N, K, klim, tri), &sl))
+    hipExtLaunchKernelGGL(gemm_mfma_kernel<double>, g, dim3(256), 0, st, sl.e0, sl.e1, 0, M, N, K, alpha, A, lda, sA,
+                          opA, B, ldb, sB, opB, beta, C, ldc, sC, klim, tri, sl.clk);
+  else
+    hipLaunchKernelGGL(gemm_mfma_kernel<double>, g, dim3(256), 0, st, M, N, K, alpha, A, lda, sA, opA, B, ldb, sB,
+                       opB, beta, C, ldc, sC, klim, tri, (unsigned long long*)nullptr);
 }
 
 void launch_cgemm(hipStream_t st, int batch, int M, int N, int K, float alpha, const float2* A, int lda,
@@ -163,7 +198,7 @@ void launch_cgemm(hipStream_t st, int batch, int M, int N, int K, float alpha, c
   if (M <= 0 || N <= 0 || batch <= 0) return;
   dim3 g((N + GT - 1) / GT, (M + GT - 1) / GT, batch);
   hipLaunchKernelGGL(gemm_mfma_kernel<float>, g, dim3(256), 0, st, M, N, K, alpha, A, lda, sA, opA, B, ldb, sB,
-                     opB, beta, C, ldc, sC, klim, tri);
+                     opB, beta, C, ldc, sC, klim, tri, (unsigned long long*)nullptr);
 }
 
 __global__ void zero_kernel(uint4* p, size_t n16) {

@@ -10,7 +10,15 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
-enum { SLK_SAMPLE_HETRD_STEP = 0, SLK_SAMPLE_BUILD = 1, SLK_SAMPLE_KINDS = 2 };
+// Kinds and their algorithmic amount: bytes for the memory-bound kinds (0, 1),
+// flops for the FP64 MFMA GEMM (2) and the bulge chase (3).
+enum {
+  SLK_SAMPLE_HETRD_STEP = 0,
+  SLK_SAMPLE_BUILD = 1,
+  SLK_SAMPLE_GEMM_F64 = 2,
+  SLK_SAMPLE_CHASE = 3,
+  SLK_SAMPLE_KINDS = 4
+};
 
 // Workgroups of a sampled launch spread their clock atomics over
 // SLK_CLK_SUB sub-slots (one 128-byte line each) to avoid contention on one
@@ -27,6 +35,8 @@ struct SampleSlot {
 // the pending hint of that kind). Returns true and a slot when this launch
 // is sampled.
 bool sampler_take(int kind, double bytes, SampleSlot* slot);
+// Cheap check before computing an expensive algorithmic amount.
+bool sampler_enabled();
 
 // Device side: call at the start and at the end of every workgroup of a
 // sampled launch (clk != nullptr); the end call must follow a barrier.

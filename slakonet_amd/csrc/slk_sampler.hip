@@ -44,6 +44,8 @@ void free_pool() {
 }
 }  // namespace
 
+bool sampler_enabled() { return g.enabled; }
+
 bool sampler_take(int kind, double bytes, SampleSlot* slot) {
   if (!g.enabled || kind < 0 || kind >= SLK_SAMPLE_KINDS) return false;
   if (bytes < 0) bytes = t_hint[kind];
