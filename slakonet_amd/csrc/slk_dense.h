@@ -27,6 +27,7 @@ void launch_potrf_blocked(hipStream_t st, const double2* S, double2* L, double2*
                           int* info);
 
 // W = L^-1 (lower) from L and the diagonal-block inverses already in W.
+// T: temporary of ld * ld / 2 complex128 per matrix (stride ld * ld / 2).
 void launch_trinv_blocked(hipStream_t st, const double2* L, double2* W, double2* T, int n, int ld, int batch);
 
 // Blocked Householder tridiagonalisation of the Hermitian C (in place): d, e

@@ -326,17 +326,28 @@ void launch_potrf_blocked(hipStream_t st, const double2* S, double2* L, double2*
   }
 }
 
+// W[o:o+m, o:o+m] = L[o:o+m, o:o+m]^-1 for the lower triangular L whose
+// NB x NB diagonal-block inverses are already in W. Recursive halving:
+// W21 = -W22 (L21 W11), two GEMMs per split with triangular K-limits, so the
+// flops sit in a few large products instead of one thin block row per NB
+// rows. T holds L21 W11 (row stride ld, at most ld/2 rows).
+static void trinv_rec(hipStream_t st, const double2* L, double2* W, double2* T, int o, int m, int ld, int batch,
+                      long long sM, long long sT) {
+  if (m <= NB) return;
+  const int n1 = ((m + 1) / 2 + NB - 1) / NB * NB, n2 = m - n1;
+  trinv_rec(st, L, W, T, o, n1, ld, batch, sM, sT);
+  trinv_rec(st, L, W, T, o + n1, n2, ld, batch, sM, sT);
+  // T = L21 W11 (W11 lower: K starts at the column block)
+  launch_zgemm(st, batch, n2, n1, n1, 1.0, L + (size_t)(o + n1) * ld + o, ld, sM, OP_N, W + (size_t)o * ld + o, ld,
+               sM, OP_N, 0.0, T, ld, sT, KLIM_B_LOWER, 0);
+  // W21 = -W22 T (W22 lower: K ends at the row block)
+  launch_zgemm(st, batch, n2, n1, n2, -1.0, W + (size_t)(o + n1) * ld + o + n1, ld, sM, OP_N, T, ld, sT, OP_N, 0.0,
+               W + (size_t)(o + n1) * ld + o, ld, sM, KLIM_A_LOWER, 0);
+}
+
 void launch_trinv_blocked(hipStream_t st, const double2* L, double2* W, double2* T, int n, int ld, int batch) {
-  const long long sM = (long long)ld * ld, sT = (long long)NB * ld;
-  for (int i0 = NB; i0 < n; i0 += NB) {
-    const int mi = min(NB, n - i0);
-    // T = L[i0:i0+mi, 0:i0] * W[0:i0, 0:i0]  (W lower)
-    launch_zgemm(st, batch, mi, i0, i0, 1.0, L + (size_t)i0 * ld, ld, sM, OP_N, W, ld, sM, OP_N, 0.0, T, ld, sT,
-                 KLIM_B_LOWER, 0);
-    // W[i0:i0+mi, 0:i0] = -Linv_ii * T
-    launch_zgemm(st, batch, mi, i0, mi, -1.0, W + (size_t)i0 * ld + i0, ld, sM, OP_N, T, ld, sT, OP_N, 0.0,
-                 W + (size_t)i0 * ld, ld, sM, KLIM_A_LOWER, 0);
-  }
+  const long long sM = (long long)ld * ld, sT = (long long)ld * ld / 2;
+  trinv_rec(st, L, W, T, 0, n, ld, batch, sM, sT);
 }
 
 

@@ -959,7 +959,9 @@ static int eigh_impl(const void* H, const void* S, int n, int ld, int batch, dou
   if (ev) (void)hipEventRecord(ev[0], stream);
   launch_potrf_blocked(stream, sp, s.B0, s.B1, n, ld, batch, info);
   if (ev) (void)hipEventRecord(ev[1], stream);
-  launch_trinv_blocked(stream, s.B0, s.B1, s.T, n, ld, batch);
+  // the trinv temporary (ld * ld / 2 complex128 per matrix) borrows the
+  // product buffer of the reduction, which is free until then
+  launch_trinv_blocked(stream, s.B0, s.B1, mixed ? reinterpret_cast<double2*>(s.F2) : s.B2, n, ld, batch);
   if (ev) (void)hipEventRecord(ev[2], stream);
   if (!mixed) {
     // B2 = W H (W lower), B0 = B2 W^H (W^H upper)
