@@ -26,6 +26,11 @@ void launch_zero(hipStream_t st, void* p, size_t bytes);
 void launch_potrf_blocked(hipStream_t st, const double2* S, double2* L, double2* W, int n, int ld, int batch,
                           int* info);
 
+// Cholesky and inverse together (recursive halving, large GEMMs): L, W = L^-1,
+// info as launch_potrf_blocked. T: ld * ld / 2 complex128 per matrix.
+void launch_chol_inv(hipStream_t st, const double2* S, double2* L, double2* W, double2* T, int n, int ld, int batch,
+                     int* info);
+
 // W = L^-1 (lower) from L and the diagonal-block inverses already in W.
 // T: temporary of ld * ld / 2 complex128 per matrix (stride ld * ld / 2).
 void launch_trinv_blocked(hipStream_t st, const double2* L, double2* W, double2* T, int n, int ld, int batch);

@@ -957,11 +957,11 @@ static int eigh_impl(const void* H, const void* S, int n, int ld, int batch, dou
   const double2* h = reinterpret_cast<const double2*>(H);
   const double2* sp = reinterpret_cast<const double2*>(S);
   if (ev) (void)hipEventRecord(ev[0], stream);
-  launch_potrf_blocked(stream, sp, s.B0, s.B1, n, ld, batch, info);
+  // Cholesky and L^-1 in one recursion (the "potrf" stage; "trinv" is empty).
+  // The temporary (ld * ld / 2 complex128 per matrix) borrows the product
+  // buffer of the reduction, which is free until then.
+  launch_chol_inv(stream, sp, s.B0, s.B1, mixed ? reinterpret_cast<double2*>(s.F2) : s.B2, n, ld, batch, info);
   if (ev) (void)hipEventRecord(ev[1], stream);
-  // the trinv temporary (ld * ld / 2 complex128 per matrix) borrows the
-  // product buffer of the reduction, which is free until then
-  launch_trinv_blocked(stream, s.B0, s.B1, mixed ? reinterpret_cast<double2*>(s.F2) : s.B2, n, ld, batch);
   if (ev) (void)hipEventRecord(ev[2], stream);
   if (!mixed) {
     // B2 = W H (W lower), B0 = B2 W^H (W^H upper)
