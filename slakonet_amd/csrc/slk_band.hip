@@ -588,6 +588,124 @@ __device__ __forceinline__ T chase_step(typename C2T<T>::t* bnd, typename C2T<T>
   return h.beta;
 }
 
+// Per-lane constant parts of the element offsets of a full step: every offset
+// is a per-step uniform base plus one of these (the band is column-major with
+// BND_LD entries per column, so element (r, col) sits at 31 col + r + col).
+struct ChaseLane {
+  int ob[4];     // bulge block, column c1+1+li, row lg+4t: base 31 (c1+1) + q1
+  int od[4];     // diagonal block (lower element of (li, lg+4t)): base 32 q1
+  int oe[4];     // block below, row q2+1+li, column q1+lg+4t: base 32 q1 + BW
+  bool up[4];    // (li, lg+4t) lies above the diagonal: stored conjugated
+};
+__device__ __forceinline__ ChaseLane chase_lane(int lane) {
+  ChaseLane o;
+  const int li = lane & 15, lg = lane >> 4;
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {
+    const int r = lg + 4 * t;
+    o.ob[t] = 31 * li + r;
+    o.up[t] = r > li;
+    o.od[t] = o.up[t] ? 31 * li + r : 31 * r + li;
+    o.oe[t] = 31 * r + li;
+  }
+  return o;
+}
+
+// Reciprocal for the FP32 reflector (one-ulp hardware reciprocal; the FP64
+// path keeps the IEEE division).
+__device__ __forceinline__ float rcp_t(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ double rcp_t(double x) { return 1.0 / x; }
+
+// A full step (L = BW reflector rows, BW-1 bulge columns, BW rows below),
+// with the addressing reduced to uniform bases plus ChaseLane offsets.
+template <typename T>
+__device__ __forceinline__ T chase_step_full(typename C2T<T>::t* bnd, typename C2T<T>::t* v2row,
+                                             typename C2T<T>::t* sv, int c1, int q1, int lane, const ChaseLane& o) {
+  using C2 = typename C2T<T>::t;
+  const int li = lane & 15, lg = lane >> 4;
+  const C2 zero = mk<T>(0, 0);
+  C2* bx = bnd + 31 * c1 + q1;            // column c1, rows q1 + li
+  C2* bb = bnd + 31 * (c1 + 1) + q1;      // bulge block
+  C2* bd = bnd + 32 * q1;                 // diagonal block
+  C2* be = bnd + 32 * q1 + BW;            // block below
+  const C2 x = bx[li];
+  C2 eb[4], ed[4], ee[4];
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {
+    eb[t] = li < BW - 1 ? bb[o.ob[t]] : zero;
+    C2 d = bd[o.od[t]];
+    if (o.up[t]) d.y = -d.y;
+    ed[t] = d;
+    ee[t] = be[o.oe[t]];
+  }
+  // (a) reflector
+  const T xn2 = sum16(li >= 1 ? x.x * x.x + x.y * x.y : T(0));
+  const C2 alpha = mk<T>(__shfl(x.x, 0, 64), __shfl(x.y, 0, 64));
+  C2 tau, scale;
+  T beta;
+  bool skip;
+  if (xn2 == T(0) && alpha.y == T(0)) {
+    skip = true;
+    beta = alpha.x;
+    tau = zero;
+    scale = zero;
+  } else {
+    skip = false;
+    const T nrm = sqrt(alpha.x * alpha.x + alpha.y * alpha.y + xn2);
+    beta = alpha.x >= T(0) ? -nrm : nrm;
+    const T ib = rcp_t(beta);
+    tau = mk<T>((beta - alpha.x) * ib, -alpha.y * ib);
+    const T dx = alpha.x - beta, dy = alpha.y, idd = rcp_t(dx * dx + dy * dy);
+    scale = mk<T>(dx * idd, -dy * idd);
+  }
+  const C2 v = li == 0 ? mk<T>(1, 0) : (skip ? zero : c_mul(x, scale));
+  if (lg == 0) {
+    bx[li] = li == 0 ? mk<T>(beta, 0) : zero;
+    if (v2row != nullptr) v2row[q1 + li] = li == 0 ? tau : v;
+    sv[li] = v;
+  }
+  __builtin_amdgcn_wave_barrier();
+  C2 vr[4];
+#pragma unroll
+  for (int t = 0; t < 4; ++t) vr[t] = sv[lg + 4 * t];
+  C2 db = zero, y = zero, z = zero;
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {
+    db = c_add(db, c_cmul(vr[t], eb[t]));
+    y = c_add(y, c_mul(ed[t], vr[t]));
+    z = c_add(z, c_mul(ee[t], vr[t]));
+  }
+  db = sum4g(db);
+  y = sum4g(y);
+  z = sum4g(z);
+  const C2 fb = c_mul(c_conj(tau), db);
+#pragma unroll
+  for (int t = 0; t < 4; ++t)
+    if (li < BW - 1) bb[o.ob[t]] = c_sub(eb[t], c_mul(vr[t], fb));
+  const T vhy = sum16(v.x * y.x + v.y * y.y);
+  const T t2 = T(0.5) * (tau.x * tau.x + tau.y * tau.y) * vhy;
+  const C2 ty = c_mul(tau, y);
+  const C2 w = mk<T>(ty.x - t2 * v.x, ty.y - t2 * v.y);
+  if (lg == 0) sv[BW + li] = w;
+  __builtin_amdgcn_wave_barrier();
+  const C2 tz = c_mul(tau, z);
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {
+    const int r = lg + 4 * t;
+    const C2 wr = sv[BW + r];
+    if (!o.up[t]) {
+      const C2 t1 = c_mulc(v, wr), t3 = c_mulc(w, vr[t]);
+      C2 e = ed[t];
+      e.x -= t1.x + t3.x;
+      e.y -= t1.y + t3.y;
+      if (r == li) e.y = T(0);
+      bd[o.od[t]] = e;
+    }
+    be[o.oe[t]] = c_sub(ee[t], c_mulc(tz, vr[t]));
+  }
+  return beta;
+}
+
 // Steps of sweep s: step 0, then one per BW rows while at least two rows remain.
 __device__ __forceinline__ int chase_nsteps(int n, int s) {
   const int r = n - 3 - s;
@@ -628,6 +746,7 @@ __global__ __launch_bounds__(CH_THREADS) void band_chase_kernel(BandCtx<T> c, un
   __syncthreads();
   const int ns = n - 1;   // sweeps 0 .. n-2
   const int pw = (w + CH_WAVES - 1) % CH_WAVES;   // wave of the predecessor sweep
+  const ChaseLane lo = LDSB ? chase_lane(lane) : ChaseLane{};
   bool ok = true;
   for (int s = w; s < ns; s += CH_WAVES) {
     const int nst = chase_nsteps(n, s);
@@ -655,8 +774,15 @@ __global__ __launch_bounds__(CH_THREADS) void band_chase_kernel(BandCtx<T> c, un
       }
       if (cs == 0 && lane == 0) D[s] = (double)bnd[(size_t)s * BND_LD].x;
       const bool full = cs > 0 && q2 - q1 + 1 == BW && q1 - c1 == BW && q2 + BW <= n - 1;
-      const T beta = full ? chase_step<T, true>(bnd, a + (size_t)s * ld, s_vw[w], n, c1, q1, q2, lane)
-                          : chase_step<T, false>(bnd, a + (size_t)s * ld, s_vw[w], n, c1, q1, q2, lane);
+      // LDS band: offsets precomputed per lane; global band: the register
+      // budget of the precomputed offsets costs more than it saves
+      T beta;
+      if (LDSB && full)
+        beta = chase_step_full<T>(bnd, a + (size_t)s * ld, s_vw[w], c1, q1, lane, lo);
+      else if (full)
+        beta = chase_step<T, true>(bnd, a + (size_t)s * ld, s_vw[w], n, c1, q1, q2, lane);
+      else
+        beta = chase_step<T, false>(bnd, a + (size_t)s * ld, s_vw[w], n, c1, q1, q2, lane);
       if (cs == 0 && lane == 0) E[s] = (double)beta;
       __hip_atomic_store(&s_prog[w], (s << 12) + (cs + 1 == nst ? 4095 : cs + 1), __ATOMIC_RELEASE,
                          __HIP_MEMORY_SCOPE_WORKGROUP);
