@@ -20,20 +20,11 @@ void launch_cgemm(hipStream_t st, int batch, int M, int N, int K, float alpha, c
 // Zero-fill (no memset nodes so that the solve can be graph-captured).
 void launch_zero(hipStream_t st, void* p, size_t bytes);
 
-// Blocked Cholesky S = L L^H into L (copy of S), writing the inverses of the
-// 32x32 diagonal blocks into the diagonal blocks of W; info[b] = first failing
-// pivot (1-based) or 0.
-void launch_potrf_blocked(hipStream_t st, const double2* S, double2* L, double2* W, int n, int ld, int batch,
-                          int* info);
-
-// Cholesky and inverse together (recursive halving, large GEMMs): L, W = L^-1,
-// info as launch_potrf_blocked. T: ld * ld / 2 complex128 per matrix.
+// Cholesky S = L L^H (into L, a copy of S; lower part) and W = L^-1 together
+// (recursive halving, large GEMMs); info[b] = first failing pivot (1-based)
+// or 0. T: temporary of ld * ld / 2 complex128 per matrix.
 void launch_chol_inv(hipStream_t st, const double2* S, double2* L, double2* W, double2* T, int n, int ld, int batch,
                      int* info);
-
-// W = L^-1 (lower) from L and the diagonal-block inverses already in W.
-// T: temporary of ld * ld / 2 complex128 per matrix (stride ld * ld / 2).
-void launch_trinv_blocked(hipStream_t st, const double2* L, double2* W, double2* T, int n, int ld, int batch);
 
 // Blocked Householder tridiagonalisation of the Hermitian C (in place): d, e
 // (real tridiagonal), tau, reflectors in the strict lower part of C.

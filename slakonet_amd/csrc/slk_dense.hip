@@ -1,8 +1,7 @@
-// Blocked dense kernels of the batched eigensolver (gfx950): complex GEMM,
-// blocked Cholesky with diagonal-block inverses, blocked triangular inverse,
-// and the blocked Householder tridiagonalisation (LAPACK zhetrd/zlatrd
-// structure: per column one reflector step + one multi-workgroup trailing
-// matrix-vector product; per 32-column panel one rank-2k update on the GEMM).
+// Dense kernels of the batched eigensolver (gfx950): the batched complex MFMA
+// GEMM, and the Cholesky factorisation with its inverse (recursive halving
+// over 32x32 leaves factored and inverted in LDS). The tridiagonalisations
+// are in slk_hetrd.hip (one-stage) and slk_band.hip (two-stage).
 #include "slk_common.h"
 #include "slk_dense.h"
 #include <hip/hip_ext.h>
@@ -281,50 +280,7 @@ __global__ __launch_bounds__(256) void potrf_diag_kernel(double2* L, double2* W,
 }
 
 // L[r, j0:j0+NB] = A[r, j0:j0+NB] * Linv^H for the rows below the block.
-__global__ __launch_bounds__(256) void trsm_panel_kernel(double2* L, const double2* W, int n, int ld, int j0) {
-  const int b = blockIdx.y;
-  double2* a = L + (size_t)b * ld * ld;
-  const double2* w = W + (size_t)b * ld * ld;
-  const int r0 = j0 + NB + blockIdx.x * 32;
-  const int nr = min(32, n - r0);
-  __shared__ double2 Li[NB][NB + 1];
-  __shared__ double2 Ar[32][NB + 1];
-  const int tid = threadIdx.x;
-  for (int idx = tid; idx < NB * NB; idx += 256) {
-    const int r = idx / NB, c = idx % NB;
-    Li[r][c] = w[(size_t)(j0 + r) * ld + j0 + c];
-    Ar[r][c] = r < nr ? a[(size_t)(r0 + r) * ld + j0 + c] : make_double2(0.0, 0.0);
-  }
-  __syncthreads();
-  for (int q = 0; q < 4; ++q) {
-    const int idx = tid + 256 * q;
-    const int r = idx / NB, c = idx % NB;
-    double2 acc = make_double2(0.0, 0.0);
-    for (int k = 0; k <= c; ++k) {
-      const double2 x = Ar[r][k], y = Li[c][k];
-      acc.x += x.x * y.x + x.y * y.y;
-      acc.y += x.y * y.x - x.x * y.y;
-    }
-    if (r < nr) a[(size_t)(r0 + r) * ld + j0 + c] = acc;
-  }
-}
 
-void launch_potrf_blocked(hipStream_t st, const double2* S, double2* L, double2* W, int n, int ld, int batch,
-                          int* info) {
-  // L = S (lower part used), W = 0 (diagonal blocks filled below)
-  (void)hipMemcpyAsync(L, S, (size_t)batch * ld * ld * sizeof(double2), hipMemcpyDeviceToDevice, st);
-  launch_zero(st, W, (size_t)batch * ld * ld * sizeof(double2));
-  for (int j0 = 0; j0 < n; j0 += NB) {
-    hipLaunchKernelGGL(potrf_diag_kernel, dim3(batch), dim3(256), 0, st, L, W, n, ld, j0, info);
-    const int j1 = j0 + NB;
-    if (j1 >= n) break;
-    hipLaunchKernelGGL(trsm_panel_kernel, dim3((n - j1 + 31) / 32, batch), dim3(256), 0, st, L, W, n, ld, j0);
-    // trailing lower update A22 -= L21 L21^H
-    launch_zgemm(st, batch, n - j1, n - j1, NB, -1.0, L + (size_t)j1 * ld + j0, ld, (long long)ld * ld, OP_N,
-                 L + (size_t)j1 * ld + j0, ld, (long long)ld * ld, OP_C, 1.0, L + (size_t)j1 * ld + j1, ld,
-                 (long long)ld * ld, 0, 1);
-  }
-}
 
 // dst = src for a rows x cols block (row strides lds / ldd, matrix strides sS / sD).
 __global__ void copy_block_kernel(const double2* __restrict__ src, int lds, long long sS, double2* __restrict__ dst,
@@ -378,29 +334,6 @@ void launch_chol_inv(hipStream_t st, const double2* S, double2* L, double2* W, d
   chol_inv_rec(st, L, W, T, 0, n, n, ld, batch, (long long)ld * ld, (long long)ld * ld / 2, info);
 }
 
-// W[o:o+m, o:o+m] = L[o:o+m, o:o+m]^-1 for the lower triangular L whose
-// NB x NB diagonal-block inverses are already in W. Recursive halving:
-// W21 = -W22 (L21 W11), two GEMMs per split with triangular K-limits, so the
-// flops sit in a few large products instead of one thin block row per NB
-// rows. T holds L21 W11 (row stride ld, at most ld/2 rows).
-static void trinv_rec(hipStream_t st, const double2* L, double2* W, double2* T, int o, int m, int ld, int batch,
-                      long long sM, long long sT) {
-  if (m <= NB) return;
-  const int n1 = ((m + 1) / 2 + NB - 1) / NB * NB, n2 = m - n1;
-  trinv_rec(st, L, W, T, o, n1, ld, batch, sM, sT);
-  trinv_rec(st, L, W, T, o + n1, n2, ld, batch, sM, sT);
-  // T = L21 W11 (W11 lower: K starts at the column block)
-  launch_zgemm(st, batch, n2, n1, n1, 1.0, L + (size_t)(o + n1) * ld + o, ld, sM, OP_N, W + (size_t)o * ld + o, ld,
-               sM, OP_N, 0.0, T, ld, sT, KLIM_B_LOWER, 0);
-  // W21 = -W22 T (W22 lower: K ends at the row block)
-  launch_zgemm(st, batch, n2, n1, n2, -1.0, W + (size_t)(o + n1) * ld + o + n1, ld, sM, OP_N, T, ld, sT, OP_N, 0.0,
-               W + (size_t)(o + n1) * ld + o, ld, sM, KLIM_A_LOWER, 0);
-}
-
-void launch_trinv_blocked(hipStream_t st, const double2* L, double2* W, double2* T, int n, int ld, int batch) {
-  const long long sM = (long long)ld * ld, sT = (long long)ld * ld / 2;
-  trinv_rec(st, L, W, T, 0, n, ld, batch, sM, sT);
-}
 
 
 extern "C" {
