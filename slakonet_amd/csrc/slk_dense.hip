@@ -214,68 +214,65 @@ void launch_zero(hipStream_t st, void* p, size_t bytes) {
 }
 
 // ---------------------------------------------------------------------------
-// Cholesky of the NB x NB diagonal block at j0 in LDS, plus its inverse.
+// Leaf of the Cholesky recursion: the NB x NB diagonal block at j0 (its lower
+// part current) is factored, L_jj, and inverted, W_jj = L_jj^-1, by one
+// workgroup of NB*NB threads per matrix, thread (r, c) holding A[r][c] and
+// W[r][c] in registers. Step k reads column k of A and row k of W, published
+// in LDS at the end of step k-1, and each thread applies its part of
+//   L[r][k] = A[r][k] / l_kk                       (r > k)
+//   A[r][c] -= L[r][k] conj(L[c][k])               (k < c <= r)
+//   W[k][:] /= l_kk,  W[r][:] -= L[r][k] W[k][:]   (r > k)
+// the second line the Cholesky trailing update, the last the same row
+// operations applied to the identity (forward substitution). The two LDS
+// vectors alternate by step parity, so one barrier per step suffices.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void potrf_diag_kernel(double2* L, double2* W, int n, int ld, int j0, int* info) {
+__global__ __launch_bounds__(NB * NB) void potrf_diag_kernel(double2* L, double2* W, int n, int ld, int j0, int* info) {
   const int b = blockIdx.x;
   double2* a = L + (size_t)b * ld * ld;
   double2* w = W + (size_t)b * ld * ld;
   const int nbk = min(NB, n - j0);
-  __shared__ double2 As[NB][NB + 1];
-  __shared__ double2 Li[NB][NB + 1];
-  __shared__ double s_inv;
-  const int tid = threadIdx.x;
-  for (int idx = tid; idx < NB * NB; idx += blockDim.x) {
-    const int r = idx / NB, c = idx % NB;
-    As[r][c] = (r < nbk && c <= r) ? a[(size_t)(j0 + r) * ld + j0 + c] : make_double2(0.0, 0.0);
-  }
+  __shared__ double2 s_col[2][NB];
+  __shared__ double2 s_row[2][NB];
+  const int r = threadIdx.x / NB, c = threadIdx.x % NB;
+  const bool in = r < nbk && c < nbk;
+  const size_t off = (size_t)(j0 + r) * ld + j0 + c;
+  double2 x = (in && c <= r) ? a[off] : make_double2(0.0, 0.0);
+  double2 y = make_double2(in && r == c ? 1.0 : 0.0, 0.0);
+  if (c == 0) s_col[0][r] = x;
+  if (r == 0) s_row[0][c] = y;
   __syncthreads();
   for (int k = 0; k < nbk; ++k) {
-    if (tid == 0) {
-      double d = As[k][k].x;
-      if (!(d > 0.0)) {
-        if (info[b] == 0) info[b] = j0 + k + 1;
-        d = 1.0;
-      }
-      const double l = sqrt(d);
-      As[k][k] = make_double2(l, 0.0);
-      s_inv = 1.0 / l;
+    const int p = k & 1;
+    double d = s_col[p][k].x;
+    if (!(d > 0.0)) {
+      if (threadIdx.x == 0 && info[b] == 0) info[b] = j0 + k + 1;
+      d = 1.0;
     }
-    __syncthreads();
-    if (tid > k && tid < nbk) {
-      const double s = s_inv;
-      As[tid][k] = make_double2(As[tid][k].x * s, As[tid][k].y * s);
+    const double l = sqrt(d), il = 1.0 / l;
+    const double2 cr = s_col[p][r], cc = s_col[p][c], wk = s_row[p][c];
+    const double2 lr = make_double2(cr.x * il, cr.y * il);
+    const double2 lc = make_double2(cc.x * il, cc.y * il);
+    const double2 wkn = make_double2(wk.x * il, wk.y * il);
+    if (c == k) {
+      if (r == k) x = make_double2(l, 0.0);
+      else if (r > k) x = lr;
+    } else if (c > k && r >= c) {
+      x.x -= lr.x * lc.x + lr.y * lc.y;
+      x.y -= lr.y * lc.x - lr.x * lc.y;
     }
-    __syncthreads();
-    for (int idx = tid; idx < nbk * nbk; idx += blockDim.x) {
-      const int r = idx / nbk, c = idx % nbk;
-      if (c > k && r >= c) {
-        const double2 x = As[r][k], y = As[c][k];
-        As[r][c].x -= x.x * y.x + x.y * y.y;
-        As[r][c].y -= x.y * y.x - x.x * y.y;
-      }
+    if (r == k) {
+      y = wkn;
+    } else if (r > k) {
+      y.x -= lr.x * wkn.x - lr.y * wkn.y;
+      y.y -= lr.x * wkn.y + lr.y * wkn.x;
     }
+    if (c == k + 1) s_col[p ^ 1][r] = x;
+    if (r == k + 1) s_row[p ^ 1][c] = y;
     __syncthreads();
   }
-  if (tid < nbk) {
-    const int c = tid;
-    for (int r = 0; r < c; ++r) Li[r][c] = make_double2(0.0, 0.0);
-    for (int r = c; r < nbk; ++r) {
-      double2 acc = make_double2(r == c ? 1.0 : 0.0, 0.0);
-      for (int k = c; k < r; ++k) {
-        const double2 lv = As[r][k], wv = Li[k][c];
-        acc.x -= lv.x * wv.x - lv.y * wv.y;
-        acc.y -= lv.x * wv.y + lv.y * wv.x;
-      }
-      const double dinv = 1.0 / As[r][r].x;
-      Li[r][c] = make_double2(acc.x * dinv, acc.y * dinv);
-    }
-  }
-  __syncthreads();
-  for (int idx = tid; idx < nbk * nbk; idx += blockDim.x) {
-    const int r = idx / nbk, c = idx % nbk;
-    if (c <= r) a[(size_t)(j0 + r) * ld + j0 + c] = As[r][c];
-    w[(size_t)(j0 + r) * ld + j0 + c] = Li[r][c];
+  if (in) {
+    if (c <= r) a[off] = x;
+    w[off] = y;
   }
 }
 
@@ -302,7 +299,7 @@ __global__ void copy_block_kernel(const double2* __restrict__ src, int lds, long
 static void chol_inv_rec(hipStream_t st, double2* L, double2* W, double2* T, int o, int m, int n, int ld, int batch,
                          long long sM, long long sT, int* info) {
   if (m <= NB) {
-    hipLaunchKernelGGL(potrf_diag_kernel, dim3(batch), dim3(256), 0, st, L, W, n, ld, o, info);
+    hipLaunchKernelGGL(potrf_diag_kernel, dim3(batch), dim3(NB * NB), 0, st, L, W, n, ld, o, info);
     return;
   }
   const int n1 = ((m + 1) / 2 + NB - 1) / NB * NB, n2 = m - n1;

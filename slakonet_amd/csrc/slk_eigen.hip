@@ -47,22 +47,16 @@ __device__ __forceinline__ int sturm_count(const double* d, const double* e2, in
   return cnt;
 }
 
-// Groups of G lanes (G a power of two <= 64) share one eigenvalue; each step
-// evaluates G Sturm counts at equidistant interior points and keeps the
-// sub-interval where the count crosses the eigenvalue index ((G+1)x shrink per
-// step). G = 1 is plain bisection with 64 eigenvalues per wave; larger G
-// trades total work for parallelism when few eigenvalues are in flight.
-// LDS = true: d and e^2 staged in LDS (2n doubles, n <= 4096); false: read
-// from global memory (d in place, e^2 from E2g written by square_e_kernel).
+// Gershgorin interval [lo, hi] of the tridiagonal (widened by a relative
+// margin) and the pivot floor, reduced over the workgroup; with LDS = true d
+// and e^2 are staged in sd / se on the way. Every kernel that calls it with
+// the same blockDim gets the same bits, so grid points derived from [lo, hi]
+// agree between kernels.
+struct TriInterval {
+  double lo, hi, piv;
+};
 template <bool LDS>
-__global__ __launch_bounds__(EIG_THREADS) void bisect_kernel(const double* D, const double* Ed, int n, int ld,
-                                                             double* Wout, int wld, int G, const double* E2g) {
-  const int b = blockIdx.y;
-  const double* d = D + (size_t)b * ld;
-  const double* e = Ed + (size_t)b * ld;
-  extern __shared__ double sm[];
-  double* sd = LDS ? sm : const_cast<double*>(d);
-  double* se = LDS ? sm + n : const_cast<double*>(E2g + (size_t)b * ld);
+__device__ TriInterval tri_interval(const double* d, const double* e, int n, double* sd, double* se) {
   __shared__ double s_red[3][EIG_THREADS / 64];
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   double lo = 1e300, hi = -1e300, emax = 0.0;
@@ -91,19 +85,86 @@ __global__ __launch_bounds__(EIG_THREADS) void bisect_kernel(const double* D, co
     emax = fmax(emax, s_red[2][q]);
   }
   const double span = fmax(fabs(lo), fabs(hi));
-  lo = lo - 2.0 * 2.2e-16 * span * n - 1e-300;
-  hi = hi + 2.0 * 2.2e-16 * span * n + 1e-300;
-  const double piv = fmax(2.2e-308, 2.2e-308 * emax) * 1e2;
+  TriInterval t;
+  t.lo = lo - 2.0 * 2.2e-16 * span * n - 1e-300;
+  t.hi = hi + 2.0 * 2.2e-16 * span * n + 1e-300;
+  t.piv = fmax(2.2e-308, 2.2e-308 * emax) * 1e2;
+  return t;
+}
+
+// Point j of the equidistant grid lo = x_0 < ... < x_M = hi (explicit fma so
+// that every kernel rounds it the same way).
+__device__ __forceinline__ double grid_point(const TriInterval& t, int j, int M) {
+  return fma(t.hi - t.lo, (double)j / (double)M, t.lo);
+}
+
+// Sturm counts at the M + 1 grid points: cnt[b][j] = #eigenvalues below x_j.
+// One thread per point; they give every eigenvalue a starting cell of width
+// (hi - lo) / M for the bisection, in place of log2(M) halvings per eigenvalue.
+template <bool LDS>
+__global__ __launch_bounds__(EIG_THREADS) void sturm_grid_kernel(const double* D, const double* Ed, int n, int ld,
+                                                                 int M, int* cnt, const double* E2g) {
+  const int b = blockIdx.y;
+  extern __shared__ double sm[];
+  const double* d = D + (size_t)b * ld;
+  double* sd = LDS ? sm : const_cast<double*>(d);
+  double* se = LDS ? sm + n : const_cast<double*>(E2g + (size_t)b * ld);
+  const TriInterval t = tri_interval<LDS>(d, Ed + (size_t)b * ld, n, sd, se);
+  __syncthreads();
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j > M) return;
+  int c;
+  if (j == 0) c = 0;
+  else if (j == M) c = n;
+  else c = sturm_count(sd, se, n, grid_point(t, j, M), t.piv);
+  cnt[(size_t)b * (M + 1) + j] = c;
+}
+
+// Groups of G lanes (G a power of two <= 64) share one eigenvalue; each step
+// evaluates G Sturm counts at equidistant interior points and keeps the
+// sub-interval where the count crosses the eigenvalue index ((G+1)x shrink per
+// step). G = 1 is plain bisection with 64 eigenvalues per wave; larger G
+// trades total work for parallelism when few eigenvalues are in flight.
+// Eigenvalue idx starts from the grid cell [x_j, x_j+1] whose counts bracket
+// it, cnt[j] <= idx < cnt[j+1] (binary search over the grid counts), and
+// stops once the interval is below rtol relative (plus the pivot floor).
+// LDS = true: d and e^2 staged in LDS (2n doubles, n <= 4096); false: read
+// from global memory (d in place, e^2 from E2g written by square_e_kernel).
+template <bool LDS>
+__global__ __launch_bounds__(EIG_THREADS) void bisect_kernel(const double* D, const double* Ed, int n, int ld,
+                                                             double* Wout, int wld, int G, const double* E2g,
+                                                             const int* cnt, int M, double rtol) {
+  const int b = blockIdx.y;
+  const double* d = D + (size_t)b * ld;
+  extern __shared__ double sm[];
+  double* sd = LDS ? sm : const_cast<double*>(d);
+  double* se = LDS ? sm + n : const_cast<double*>(E2g + (size_t)b * ld);
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const TriInterval t = tri_interval<LDS>(d, Ed + (size_t)b * ld, n, sd, se);
   const int per_wave = 64 / G;
   const int grp = lane / G, sub = lane % G;
   const int idx = (blockIdx.x * (EIG_THREADS / 64) + wv) * per_wave + grp;
   if ((blockIdx.x * (EIG_THREADS / 64) + wv) * per_wave >= n) return;   // whole wave idle
+  double lo = t.lo, hi = t.hi;
+  const double piv = t.piv;
+  if (idx < n) {
+    const int* cb = cnt + (size_t)b * (M + 1);
+    if (cb[0] <= idx && idx < cb[M]) {
+      int jl = 0, jh = M;
+      while (jh - jl > 1) {
+        const int jm = (jl + jh) >> 1;
+        if (cb[jm] <= idx) jl = jm; else jh = jm;
+      }
+      lo = grid_point(t, jl, M);
+      hi = grid_point(t, jh, M);
+    }
+  }
   const unsigned long long gmask = G == 64 ? ~0ULL : ((1ULL << G) - 1ULL);
   const double inv = 1.0 / (double)(G + 1);
   bool done = idx >= n;
   for (int it = 0; it < 256; ++it) {
     const double width = hi - lo;
-    if (!done && width <= 4.0 * 2.2e-16 * fmax(fabs(lo), fabs(hi)) + piv) done = true;
+    if (!done && width <= rtol * fmax(fabs(lo), fabs(hi)) + piv) done = true;
     if (__ballot(!done) == 0ULL) break;
     const double x = lo + width * (double)(sub + 1) * inv;
     const bool above = !done && sturm_count(sd, se, n, x, piv) > idx;
@@ -855,13 +916,15 @@ __global__ void refine_commit_kernel(RefineCtx c, double* w, double* edges, int 
 extern "C" {
 
 // Workspace (per matrix). FP64 path: L/C, W, W*H (and Y for vectors), the
-// trinv temporary T, panel V and W, Y, the mat-vec partial sums, C12, TAU, D,
-// E, and the real Z for the vectors path. Mixed path: L and W (FP64), T, then
-// FP32 copies W32, H32 (reused for C), W32*H32 and FP32 panels.
+// bisection grid counts, panel V and W, Y, the mat-vec partial sums, C12, TAU,
+// D, E, and the real Z for the vectors path. Mixed path: L and W (FP64), the
+// grid counts, then FP32 copies W32, H32 (reused for C), W32*H32 and FP32
+// panels.
 struct EighWs {
-  double2 *B0, *B1, *B2, *T, *Vc, *Wc, *Y, *Yp, *C12, *TAU, *Band;
+  double2 *B0, *B1, *B2, *Vc, *Wc, *Y, *Yp, *C12, *TAU, *Band;
   float2 *F0, *F1, *F2, *fVc, *fWc, *fY, *fYp, *fC12, *fTAU, *fBand;
   double *D, *E, *Z;
+  int* SG;   // Sturm counts on the bisection start grid, 2 ld + 1 per matrix
 };
 
 // Carves the workspace (base may be null to only size it); returns bytes.
@@ -876,7 +939,7 @@ static size_t carve(char* base, int ld, int batch, int vectors, int mixed, EighW
   *w = EighWs{};
   w->B0 = reinterpret_cast<double2*>(take(matel * batch * sizeof(double2)));
   w->B1 = reinterpret_cast<double2*>(take(matel * batch * sizeof(double2)));
-  w->T = reinterpret_cast<double2*>(take((size_t)SLK_NB * ld * batch * sizeof(double2)));
+  w->SG = reinterpret_cast<int*>(take((size_t)(2 * ld + 1) * batch * sizeof(int)));
   w->D = reinterpret_cast<double*>(take((size_t)ld * batch * sizeof(double)));
   w->E = reinterpret_cast<double*>(take((size_t)ld * batch * sizeof(double)));
   if (!mixed) {
@@ -996,15 +1059,26 @@ static int eigh_impl(const void* H, const void* S, int n, int ld, int batch, dou
     const int G = bisect_group(n, batch);
     const int eig_per_block = (EIG_THREADS / 64) * (64 / G);
     const dim3 grid((n + eig_per_block - 1) / eig_per_block, batch);
+    const int M = 2 * n;
+    const dim3 ggrid((M + 1 + EIG_THREADS - 1) / EIG_THREADS, batch);
+    // the mixed path's tridiagonal carries FP32 rounding (~1e-7 of the
+    // spectral radius): its eigenvalues are resolved to 2^-36 relative
+    // instead of the FP64 path's 4 ulp
+    const double rtol = mixed ? 1.4551915228366852e-11 : 4.0 * 2.2e-16;
     if (n <= BISECT_LDS_MAX) {
-      hipLaunchKernelGGL(bisect_kernel<true>, grid, dim3(EIG_THREADS), 2 * n * sizeof(double), stream, s.D, s.E, n,
-                         ld, w, wld, G, (const double*)nullptr);
+      const size_t lds = 2 * n * sizeof(double);
+      hipLaunchKernelGGL(sturm_grid_kernel<true>, ggrid, dim3(EIG_THREADS), lds, stream, s.D, s.E, n, ld, M, s.SG,
+                         (const double*)nullptr);
+      hipLaunchKernelGGL(bisect_kernel<true>, grid, dim3(EIG_THREADS), lds, stream, s.D, s.E, n, ld, w, wld, G,
+                         (const double*)nullptr, (const int*)s.SG, M, rtol);
     } else {
       // e^2 into the (now free) mat-vec vector buffer, ld doubles per matrix
       double* e2 = mixed ? reinterpret_cast<double*>(s.fY) : reinterpret_cast<double*>(s.Y);
       hipLaunchKernelGGL(square_e_kernel, dim3((n + 255) / 256, batch), dim3(256), 0, stream, s.E, n, ld, e2);
-      hipLaunchKernelGGL(bisect_kernel<false>, grid, dim3(EIG_THREADS), 0, stream, s.D, s.E, n, ld, w, wld, G,
+      hipLaunchKernelGGL(sturm_grid_kernel<false>, ggrid, dim3(EIG_THREADS), 0, stream, s.D, s.E, n, ld, M, s.SG,
                          (const double*)e2);
+      hipLaunchKernelGGL(bisect_kernel<false>, grid, dim3(EIG_THREADS), 0, stream, s.D, s.E, n, ld, w, wld, G,
+                         (const double*)e2, (const int*)s.SG, M, rtol);
     }
     if (ev) (void)hipEventRecord(ev[5], stream);
   } else {

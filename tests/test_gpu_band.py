@@ -78,7 +78,7 @@ def test_two_stage_mixed_vs_lapack(n):
 def test_two_stage_padded_batch(precision):
     """Identity-padded matrices of different sizes in one batch: the structure
     block's eigenvalues match LAPACK on the unpadded block, the padding gives
-    exactly the pad value."""
+    the pad value to the bisection tolerance."""
     print(f"\ncase: two-stage padded batch {precision}", flush=True)
     ld, pad = 160, 4.0
     sizes = [160, 97, 130, 40]
@@ -97,4 +97,7 @@ def test_two_stage_padded_batch(precision):
         err = float(np.abs(w[b, :n] - refs[b]).max())
         assert err < tol, (b, n, err)
         if n < ld:
-            assert float(np.abs(w[b, n:] - pad).max()) < 1e-12
+            # the pad eigenvalue is exact in the tridiagonal; bisection resolves
+            # it to 4 ulp (FP64) or 2^-36 relative (mixed)
+            pad_tol = 1e-12 if precision == "fp64" else 2.0 ** -36 * pad * 2
+            assert float(np.abs(w[b, n:] - pad).max()) < pad_tol
