@@ -185,17 +185,16 @@ __global__ __launch_bounds__(1024) void panel_qr_kernel(BandCtx<T> c, int j0) {
   __shared__ C2 s_tau[BW];
   __shared__ C2 s_G[BW][BW];   // s_G[k][l] = v_l^H v_k, l < k
   __shared__ C2 s_T[BW][BW];
-  const C2 zero = mk<T>(0, 0);
   const int nq = RPT > 0 ? RPT : (m + 63) / 64;
   C2* col = a + (size_t)(r0 + lane) * ld + j0 + w;   // this lane's rows of column w
   C2 p[NR];
   if constexpr (RPT > 0) {
 #pragma unroll
-    for (int q = 0; q < RPT; ++q) p[q] = lane + 64 * q < m ? col[(size_t)64 * q * ld] : zero;
+    for (int q = 0; q < RPT; ++q) p[q] = load_or_zero(col + (size_t)64 * q * ld, lane + 64 * q < m);
   }
   auto get = [&](int q) -> C2 {
     if constexpr (RPT > 0) return p[q];
-    else return lane + 64 * q < m ? col[(size_t)64 * q * ld] : mk<T>(0, 0);
+    else return load_or_zero(col + (size_t)64 * q * ld, lane + 64 * q < m);
   };
   auto put = [&](int q, C2 v) {
     if constexpr (RPT > 0) p[q] = v;
@@ -203,14 +202,14 @@ __global__ __launch_bounds__(1024) void panel_qr_kernel(BandCtx<T> c, int j0) {
   };
   // T (Q = I - V T V^H): T[k][k] = tau_k, T[i][k] = -tau_k sum_{l=i}^{k-1} T[i][l] g_l^(k);
   // lane i of wave 0 owns row i
-  if (tid < BW * BW) s_T[tid / BW][tid % BW] = zero;
+  if (tid < BW * BW) s_T[tid / BW][tid % BW] = mk<T>(0, 0);
   auto t_column = [&](int k) {
     if (k < 0 || lane > k) return;
     if (lane == k) {
       s_T[k][k] = s_tau[k];
       return;
     }
-    C2 acc = zero;
+    C2 acc = mk<T>(0, 0);
 #pragma unroll
     for (int l = 0; l < BW; ++l)
       if (l >= lane && l < k) acc = c_add(acc, c_mul(s_T[lane][l], s_G[k][l]));
@@ -236,12 +235,12 @@ __global__ __launch_bounds__(1024) void panel_qr_kernel(BandCtx<T> c, int j0) {
           const int r = lane + 64 * q;
           if (r >= m) return;
           C2 e = get(q);
-          C2 vv = zero;
+          C2 vv = mk<T>(0, 0);
           if (r == k) {
             vv = mk<T>(1, 0);
             put(q, mk<T>(h.beta, 0));
           } else if (r > k) {
-            vv = h.skip ? zero : c_mul(e, h.scale);
+            vv = h.skip ? mk<T>(0, 0) : c_mul(e, h.scale);
             put(q, vv);
           }
           if constexpr (RPT > 0) s_v[r] = vv;
@@ -252,7 +251,7 @@ __global__ __launch_bounds__(1024) void panel_qr_kernel(BandCtx<T> c, int j0) {
     __syncthreads();
     if (w > k) {
       const C2 tau = s_tau[k];
-      C2 u = zero;
+      C2 u = mk<T>(0, 0);
       for_rows<RPT>(nq, [&](int q) {
           const int r = lane + 64 * q;
           if (r >= k && r < m) {
@@ -279,7 +278,7 @@ __global__ __launch_bounds__(1024) void panel_qr_kernel(BandCtx<T> c, int j0) {
         });
     } else if (w < k) {
       // g_w = v_w^H v_k over rows >= k (column w holds v_w below row w)
-      C2 g = zero;
+      C2 g = mk<T>(0, 0);
       for_rows<RPT>(nq, [&](int q) {
           const int r = lane + 64 * q;
           if (r >= k && r < m) {
@@ -305,13 +304,13 @@ __global__ __launch_bounds__(1024) void panel_qr_kernel(BandCtx<T> c, int j0) {
       if (r >= m) return;
       const C2 e = get(q);
       if constexpr (RPT > 0) col[(size_t)64 * q * ld] = e;
-      C2 v = zero;
-      if (w < kb) v = r > w ? e : (r == w ? mk<T>(1, 0) : zero);
+      C2 v = mk<T>(0, 0);
+      if (w < kb) v = r > w ? e : (r == w ? mk<T>(1, 0) : mk<T>(0, 0));
       vcb[(size_t)w * ld + r0 + r] = v;
       wcv[(size_t)w * ld + r0 + r] = v;
       if (c.VT != nullptr && w < kb && r >= w) c.VT[(size_t)b * ld * ld + (size_t)(j0 + w) * ld + r0 + r] = v;
     });
-  if (tid < BW) c.TAU[(size_t)b * ld + j0 + tid] = tid < kb ? s_tau[tid] : zero;
+  if (tid < BW) c.TAU[(size_t)b * ld + j0 + tid] = load_or_zero(s_tau + tid, tid < kb);
   if (tid < BW * BW) c.TT[((size_t)b * c.npan + j0 / BW) * BW * BW + tid] = s_T[tid / BW][tid % BW];
 }
 
@@ -321,7 +320,7 @@ __global__ __launch_bounds__(1024) void panel_qr_kernel(BandCtx<T> c, int j0) {
 // The next 64-column tile is fetched into registers while the current one is
 // multiplied from LDS.
 template <typename T>
-__global__ __launch_bounds__(256) void band_av_kernel(BandCtx<T> c, int r0) {
+__global__ __launch_bounds__(256, 3) void band_av_kernel(BandCtx<T> c, int r0) {
   using C2 = typename C2T<T>::t;
   using V4 = typename MF<T>::v4;
   const int n = c.n, ld = c.ld;
@@ -333,38 +332,83 @@ __global__ __launch_bounds__(256) void band_av_kernel(BandCtx<T> c, int r0) {
   const C2* vc = c.Vc + (size_t)b * 2 * BW * ld;
   __shared__ T Ar[64][AV_KT + 1], Ai[64][AV_KT + 1];
   __shared__ T Br[AV_KT][BW + 1], Bi[AV_KT][BW + 1];
+  __shared__ T s_dump[2][64];
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const int li = lane & 15, lk = lane >> 4;
   constexpr int NA = 64 * AV_KT / 256, NBT = AV_KT * BW / 256;
   C2 ra[NA], rb[NBT];
+  // Only the lower triangle of A22 is current (the rank-2k updates skip the
+  // tiles above the diagonal); element (r, k) with k > r is conj(A[k][r]).
+  // Row blocks and k tiles share the origin r0, so a 64 x 64 tile is wholly
+  // below the diagonal (read as stored), wholly above (fetched transposed,
+  // coalesced along r, stored transposed into LDS: mode 1) or a diagonal
+  // tile (mode 2: its lower half is fetched and also stored mirrored). The
+  // conjugation happens at the LDS store: no arithmetic on a fetched value
+  // before the multiply of the current tile, so the fetch stays in flight.
+  static_assert(AV_KT == 64, "the tile modes assume square 64 x 64 tiles");
+  int mode = 0;
   auto fetch = [&](int k0) {
+    mode = k0 >= row0 + 64 ? 1 : (k0 == row0 ? 2 : 0);
+    // memory element (R + hi, Cc + lo), hi = e / 64, lo = e % 64: one
+    // address stream for all modes (R, Cc swapped for a transposed tile),
+    // clamped into the matrix so that every load is unconditional; elements
+    // outside the tile's valid part are zeroed at the LDS store
+    const int R = mode == 1 ? k0 : row0, Cc = mode == 1 ? row0 : k0;
+    const int limR = mode == 1 ? kend : n, limC = mode == 1 ? n : kend;
+    const int h0 = tid / 64, lo = tid % 64;
+    const C2* p = a + min(Cc + lo, limC - 1);
 #pragma unroll
-    for (int i = 0; i < NA; ++i) {
-      const int e = tid + 256 * i;
-      const int gr = row0 + e / AV_KT, gk = k0 + e % AV_KT;
-      ra[i] = (gr < n && gk < kend) ? a[(size_t)gr * ld + gk] : mk<T>(0, 0);
-    }
+    for (int i = 0; i < NA; ++i) ra[i] = p[(size_t)min(R + h0 + 4 * i, limR - 1) * ld];
 #pragma unroll
     for (int i = 0; i < NBT; ++i) {
       const int e = tid + 256 * i;
-      const int gk = k0 + e % AV_KT;
-      rb[i] = gk < kend ? vc[(size_t)(e / AV_KT) * ld + gk] : mk<T>(0, 0);
+      rb[i] = vc[(size_t)(e / AV_KT) * ld + min(k0 + e % AV_KT, kend - 1)];
     }
   };
   V4 accR = V4{0, 0, 0, 0}, accI = V4{0, 0, 0, 0};
   fetch(kbeg);
   for (int k0 = kbeg; k0 < kend; k0 += AV_KT) {
+    {
+      // branch-free stores (hi is uniform over a wave, lo is the lane):
+      // element (hi, lo) goes to (hi, lo), or (lo, hi) conjugated for a
+      // transposed tile; the upper half of a diagonal tile is not stored
+      // (its place takes the mirror of the lower half) and goes to a
+      // per-lane dump slot instead
+      const bool tr = mode == 1;
+      const int R = tr ? k0 : row0, Cc = tr ? row0 : k0;
+      const int limR = tr ? kend : n, limC = tr ? n : kend;
 #pragma unroll
-    for (int i = 0; i < NA; ++i) {
-      const int e = tid + 256 * i;
-      Ar[e / AV_KT][e % AV_KT] = ra[i].x;
-      Ai[e / AV_KT][e % AV_KT] = ra[i].y;
+      for (int i = 0; i < NA; ++i) {
+        const int hi = wv + 4 * i, lo = lane;
+        const bool ok = R + hi < limR && Cc + lo < limC;
+        const T x = ok ? ra[i].x : T(0), y = ok ? ra[i].y : T(0);
+        const bool keep = mode != 2 || lo <= hi;
+        const int o = tr ? lo * (AV_KT + 1) + hi : hi * (AV_KT + 1) + lo;
+        T* pr = keep ? &Ar[0][0] + o : &s_dump[0][lane];
+        T* pi = keep ? &Ai[0][0] + o : &s_dump[1][lane];
+        *pr = x;
+        *pi = tr ? -y : y;
+      }
+      if (mode == 2) {
+#pragma unroll
+        for (int i = 0; i < NA; ++i) {
+          const int hi = wv + 4 * i, lo = lane;
+          const bool ok = R + hi < limR && Cc + lo < limC;
+          const bool mir = lo < hi;
+          const int o = lo * (AV_KT + 1) + hi;
+          T* pr = mir ? &Ar[0][0] + o : &s_dump[0][lane];
+          T* pi = mir ? &Ai[0][0] + o : &s_dump[1][lane];
+          *pr = ok ? ra[i].x : T(0);
+          *pi = ok ? -ra[i].y : T(0);
+        }
+      }
     }
 #pragma unroll
     for (int i = 0; i < NBT; ++i) {
       const int e = tid + 256 * i;
-      Br[e % AV_KT][e / AV_KT] = rb[i].x;
-      Bi[e % AV_KT][e / AV_KT] = rb[i].y;
+      const bool ok = k0 + e % AV_KT < kend;
+      Br[e % AV_KT][e / AV_KT] = ok ? rb[i].x : T(0);
+      Bi[e % AV_KT][e / AV_KT] = ok ? rb[i].y : T(0);
     }
     __syncthreads();
     if (k0 + AV_KT < kend) fetch(k0 + AV_KT);
@@ -512,19 +556,19 @@ __device__ __forceinline__ T chase_step(typename C2T<T>::t* bnd, typename C2T<T>
   const int nbc = FULL ? BW - 1 : q1 - c1 - 1;
   const C2 zero = mk<T>(0, 0);
   // every load of the step first
-  const C2 x = (FULL || li < L) ? bnd[c1 * BND_LD + q1 - c1 + li] : zero;
+  const C2 x = load_or_zero(bnd + c1 * BND_LD + q1 - c1 + li, FULL || li < L);
   C2 eb[4], ed[4], ee[4];
   const int cb = c1 + 1 + li;   // this lane's bulge column
 #pragma unroll
   for (int t = 0; t < 4; ++t) {
     const int r = lg + 4 * t;   // row (b) / column (c), (d) index
-    eb[t] = ((FULL || r < L) && li < nbc) ? bnd[cb * BND_LD + q1 + r - cb] : zero;
+    eb[t] = load_or_zero(bnd + cb * BND_LD + q1 + r - cb, (FULL || r < L) && li < nbc);
     const bool lo = r <= li;
     const int od = lo ? (q1 + r) * BND_LD + li - r : (q1 + li) * BND_LD + r - li;
-    C2 d = (FULL || (li < L && r < L)) ? bnd[od] : zero;
+    C2 d = load_or_zero(bnd + od, FULL || (li < L && r < L));
     if (!lo) d.y = -d.y;
     ed[t] = d;
-    ee[t] = ((FULL || r < L) && (FULL || li < Le)) ? bnd[(q1 + r) * BND_LD + q2 + 1 + li - q1 - r] : zero;
+    ee[t] = load_or_zero(bnd + (q1 + r) * BND_LD + q2 + 1 + li - q1 - r, (FULL || r < L) && (FULL || li < Le));
   }
   // (a)
   const T xn2 = sum16((li >= 1 && (FULL || li < L)) ? x.x * x.x + x.y * x.y : T(0));
@@ -632,7 +676,7 @@ __device__ __forceinline__ T chase_step_full(typename C2T<T>::t* bnd, typename C
   C2 eb[4], ed[4], ee[4];
 #pragma unroll
   for (int t = 0; t < 4; ++t) {
-    eb[t] = li < BW - 1 ? bb[o.ob[t]] : zero;
+    eb[t] = load_or_zero(bb + o.ob[t], li < BW - 1);
     C2 d = bd[o.od[t]];
     if (o.up[t]) d.y = -d.y;
     ed[t] = d;
@@ -838,19 +882,92 @@ size_t band_tt_offset(int ld, int batch) {
   return (size_t)batch * ld * BND_LD + (size_t)batch * ((ld + 63) / 64) * BW * BW;
 }
 
-// A22 -= [V | W] [W | V]^H = V W^H + W V^H, one GEMM with K = 2 BW (full tiles)
+// ---------------------------------------------------------------------------
+// Stage 1 trailing update A22 -= [V | W] [W | V]^H = V W^H + W V^H (K = 2 BW),
+// lower triangle only. Each wave owns a 32 x 32 block of A22: the block is
+// loaded straight into the MFMA accumulators, the operands straight into
+// registers in MFMA operand layout (lane l: A[l%16][l/16], B[l/16][l%16]),
+// with no LDS and no barrier, so one memory round trip covers the block.
+// Blocks wholly above the diagonal are skipped: every later reader of A22
+// (the next A22 V product, the panel QR, the band copy of the chase) reads
+// elements on or below the diagonal only.
+// ---------------------------------------------------------------------------
 template <typename T>
-static void rank2k_full(hipStream_t st, int batch, int m, typename C2T<T>::t* Vc, typename C2T<T>::t* Wc,
-                        typename C2T<T>::t* C, int ld, long long sP, long long sM);
-template <>
-void rank2k_full<double>(hipStream_t st, int batch, int m, double2* Vc, double2* Wc, double2* C, int ld,
-                         long long sP, long long sM) {
-  launch_zgemm(st, batch, m, m, 2 * BW, -1.0, Vc, ld, sP, OP_T, Wc, ld, sP, OP_CONJ, 1.0, C, ld, sM, 0, 0);
+__global__ __launch_bounds__(256) void rank2k_lower_kernel(int m, const typename C2T<T>::t* __restrict__ Vc,
+                                                           const typename C2T<T>::t* __restrict__ Wc,
+                                                           typename C2T<T>::t* __restrict__ C, int ld, long long sP,
+                                                           long long sM) {
+  using C2 = typename C2T<T>::t;
+  using V4 = typename MF<T>::v4;
+  constexpr int KK = 2 * BW / 4;                 // k-steps of 4
+  constexpr int KC = sizeof(T) == 4 ? KK : KK / 2;   // k-steps per operand batch
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int m0 = blockIdx.y * 64 + (wv >> 1) * 32, n0 = blockIdx.x * 64 + (wv & 1) * 32;
+  if (m0 >= m || n0 > m0 + 31) return;
+  const int b = blockIdx.z;
+  const C2* vc = Vc + (size_t)b * sP;
+  const C2* wc = Wc + (size_t)b * sP;
+  C2* c = C + (size_t)b * sM;
+  const int li = lane & 15, lk = lane >> 4;
+  const C2 zero = mk<T>(0, 0);
+  V4 accR[2][2], accI[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int r = m0 + 16 * i + MF<T>::row(lk, v), col = n0 + 16 * j + li;
+        const C2 x = load_or_zero(c + (size_t)r * ld + col, r < m && col < m);
+        accR[i][j][v] = x.x;
+        accI[i][j][v] = x.y;
+      }
+#pragma unroll
+  for (int kc = 0; kc < KK; kc += KC) {
+    C2 av[2][KC], bw[2][KC];
+#pragma unroll
+    for (int q = 0; q < KC; ++q) {
+      const size_t krow = (size_t)(4 * (kc + q) + lk) * ld;
+#pragma unroll
+      for (int i = 0; i < 2; ++i) {
+        const int r = m0 + 16 * i + li;
+        av[i][q] = load_or_zero(vc + krow + r, r < m);
+      }
+#pragma unroll
+      for (int j = 0; j < 2; ++j) {
+        const int col = n0 + 16 * j + li;
+        bw[j][q] = load_or_zero(wc + krow + col, col < m);
+      }
+    }
+    // C -= a conj(w): Re -= a.x w.x + a.y w.y, Im -= a.y w.x - a.x w.y
+#pragma unroll
+    for (int q = 0; q < KC; ++q)
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+          accR[i][j] = MF<T>::mfma(-av[i][q].x, bw[j][q].x, accR[i][j]);
+          accR[i][j] = MF<T>::mfma(-av[i][q].y, bw[j][q].y, accR[i][j]);
+          accI[i][j] = MF<T>::mfma(-av[i][q].y, bw[j][q].x, accI[i][j]);
+          accI[i][j] = MF<T>::mfma(av[i][q].x, bw[j][q].y, accI[i][j]);
+        }
+  }
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int r = m0 + 16 * i + MF<T>::row(lk, v), col = n0 + 16 * j + li;
+        if (r < m && col < m) c[(size_t)r * ld + col] = mk<T>(accR[i][j][v], accI[i][j][v]);
+      }
 }
-template <>
-void rank2k_full<float>(hipStream_t st, int batch, int m, float2* Vc, float2* Wc, float2* C, int ld, long long sP,
-                        long long sM) {
-  launch_cgemm(st, batch, m, m, 2 * BW, -1.0f, Vc, ld, sP, OP_T, Wc, ld, sP, OP_CONJ, 1.0f, C, ld, sM, 0, 0);
+
+template <typename T>
+static void rank2k_lower(hipStream_t st, int batch, int m, typename C2T<T>::t* Vc, typename C2T<T>::t* Wc,
+                         typename C2T<T>::t* C, int ld, long long sP, long long sM) {
+  const int t = (m + 63) / 64;
+  hipLaunchKernelGGL(rank2k_lower_kernel<T>, dim3(t, t, batch), dim3(256), 0, st, m, Vc, Wc, C, ld, sP, sM);
 }
 
 #define CH_LDS_MAX (150 * 1024)
@@ -910,7 +1027,7 @@ static void band_impl(hipStream_t st, typename C2T<T>::t* A, int n, int ld, int 
     hipLaunchKernelGGL(band_av_kernel<T>, dim3(nrbk, nchk, batch), dim3(256), 0, st, c, r0);
     hipLaunchKernelGGL(band_x_kernel<T>, dim3(nrbk, batch), dim3(256), 0, st, c, r0, nchk);
     hipLaunchKernelGGL(band_w_kernel<T>, dim3(nrbk, batch), dim3(256), 0, st, c, r0, nrbk);
-    rank2k_full<T>(st, batch, m, Vc + r0, Wc + r0, A + (size_t)r0 * ld + r0, ld, sP, sM);
+    rank2k_lower<T>(st, batch, m, Vc + r0, Wc + r0, A + (size_t)r0 * ld + r0, ld, sP, sM);
   }
   // the band (and the schedule) in LDS when it fits, else in global memory
   const size_t lds = (size_t)n * BND_LD * sizeof(C2);

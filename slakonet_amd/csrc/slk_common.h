@@ -18,6 +18,19 @@ __device__ __forceinline__ typename C2T<T>::t mk(T x, T y) {
   return r;
 }
 
+// *p when ok, else 0. Written as a branch: a conditional expression between
+// a load and a named zero value can be compiled into a load through a select
+// of two addresses (the zero spilled to private memory), i.e. flat loads that
+// serialise on their waits.
+template <typename C2>
+__device__ __forceinline__ C2 load_or_zero(const C2* p, bool ok) {
+  C2 v;
+  v.x = 0;
+  v.y = 0;
+  if (ok) v = *p;
+  return v;
+}
+
 __device__ __forceinline__ double2 c_mul(double2 a, double2 b) {
   return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }

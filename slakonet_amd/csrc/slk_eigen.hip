@@ -1026,14 +1026,20 @@ static int eigh_impl(const void* H, const void* S, int n, int ld, int batch, dou
   launch_chol_inv(stream, sp, s.B0, s.B1, mixed ? reinterpret_cast<double2*>(s.F2) : s.B2, n, ld, batch, info);
   if (ev) (void)hipEventRecord(ev[1], stream);
   if (ev) (void)hipEventRecord(ev[2], stream);
+  // The band reduction reads only the lower triangle of C = W H W^H. Its
+  // lower tiles need W H only on and below the diagonal tiles (C_ij, j <= i,
+  // sums (W H)_ik conj(W_jk) over k <= j), so both products skip the tiles
+  // above the diagonal (tri = 1) on that path.
+  const bool band = use_band(n, V != nullptr);
+  const int tri = band ? 1 : 0;
   if (!mixed) {
     // B2 = W H (W lower), B0 = B2 W^H (W^H upper)
     launch_zgemm(stream, batch, n, n, n, 1.0, s.B1, ld, sM, OP_N, h, ld, sM, OP_N, 0.0, s.B2, ld, sM, KLIM_A_LOWER,
-                 0);
+                 tri);
     launch_zgemm(stream, batch, n, n, n, 1.0, s.B2, ld, sM, OP_N, s.B1, ld, sM, OP_C, 0.0, s.B0, ld, sM,
-                 KLIM_B_UPPER, 0);
+                 KLIM_B_UPPER, tri);
     if (ev) (void)hipEventRecord(ev[3], stream);
-    if (use_band(n, V != nullptr))
+    if (band)
       // B2 (W H) is free after the reduction: it holds the split-K partials
       launch_band_tridiag(stream, s.B0, n, ld, batch, s.D, s.E, s.TAU, s.Vc, s.Wc, s.B2, s.Band, nullptr);
     else
@@ -1042,13 +1048,13 @@ static int eigh_impl(const void* H, const void* S, int n, int ld, int batch, dou
     launch_to_c64(stream, s.B1, s.F0, (size_t)batch * ld * ld);
     launch_to_c64(stream, h, s.F1, (size_t)batch * ld * ld);
     launch_cgemm(stream, batch, n, n, n, 1.0f, s.F0, ld, sM, OP_N, s.F1, ld, sM, OP_N, 0.0f, s.F2, ld, sM,
-                 KLIM_A_LOWER, 0);
+                 KLIM_A_LOWER, tri);
     launch_cgemm(stream, batch, n, n, n, 1.0f, s.F2, ld, sM, OP_N, s.F0, ld, sM, OP_C, 0.0f, s.F1, ld, sM,
-                 KLIM_B_UPPER, 0);
+                 KLIM_B_UPPER, tri);
     if (ev) (void)hipEventRecord(ev[3], stream);
     // F0 (W in FP32) is free after the reduction: it receives the reflector
     // rows; F2 (W32 H32) holds the split-K partials of the band reduction
-    if (use_band(n, 0))
+    if (band)
       launch_band_tridiag_f32(stream, s.F1, n, ld, batch, s.D, s.E, s.fTAU, s.fVc, s.fWc, s.F2, s.fBand, s.F0);
     else
       launch_hetrd_blocked_f32(stream, s.F1, n, ld, batch, s.D, s.E, s.fTAU, s.fVc, s.fWc, s.fY, s.fYp, s.fC12,
