@@ -24,23 +24,27 @@
 // Eigenvalues of the real symmetric tridiagonal (d, e) by bisection /
 // multisection on Sturm counts, ascending order.
 // ---------------------------------------------------------------------------
-// 1/q from the hardware approximation refined by two Newton steps (full
-// double precision for the pivots; |q| >= pivmin keeps it finite).
+// 1/q from the hardware approximation (relative error 2^-24.4, measured on
+// gfx950) refined by NEWTON Newton steps: 2 give full double precision
+// (2^-53), 1 gives 2^-48.7, a relative perturbation of e^2 of that size in
+// the Sturm recurrence (|q| >= pivmin keeps it finite).
+template <int NEWTON>
 __device__ __forceinline__ double rcp_refined(double q) {
   double r = __builtin_amdgcn_rcp(q);
-  r = fma(fma(-q, r, 1.0), r, r);
-  r = fma(fma(-q, r, 1.0), r, r);
+#pragma unroll
+  for (int i = 0; i < NEWTON; ++i) r = fma(fma(-q, r, 1.0), r, r);
   return r;
 }
 
 // Number of eigenvalues below x; e2[i] = e[i]^2.
+template <int NEWTON>
 __device__ __forceinline__ int sturm_count(const double* d, const double* e2, int n, double x, double pivmin) {
   int cnt = 0;
   double q = d[0] - x;
   if (fabs(q) < pivmin) q = -pivmin;
   if (q < 0.0) ++cnt;
   for (int i = 1; i < n; ++i) {
-    q = (d[i] - x) - e2[i - 1] * rcp_refined(q);
+    q = (d[i] - x) - e2[i - 1] * rcp_refined<NEWTON>(q);
     if (fabs(q) < pivmin) q = -pivmin;
     if (q < 0.0) ++cnt;
   }
@@ -101,7 +105,7 @@ __device__ __forceinline__ double grid_point(const TriInterval& t, int j, int M)
 // Sturm counts at the M + 1 grid points: cnt[b][j] = #eigenvalues below x_j.
 // One thread per point; they give every eigenvalue a starting cell of width
 // (hi - lo) / M for the bisection, in place of log2(M) halvings per eigenvalue.
-template <bool LDS>
+template <bool LDS, int NEWTON>
 __global__ __launch_bounds__(EIG_THREADS) void sturm_grid_kernel(const double* D, const double* Ed, int n, int ld,
                                                                  int M, int* cnt, const double* E2g) {
   const int b = blockIdx.y;
@@ -116,7 +120,7 @@ __global__ __launch_bounds__(EIG_THREADS) void sturm_grid_kernel(const double* D
   int c;
   if (j == 0) c = 0;
   else if (j == M) c = n;
-  else c = sturm_count(sd, se, n, grid_point(t, j, M), t.piv);
+  else c = sturm_count<NEWTON>(sd, se, n, grid_point(t, j, M), t.piv);
   cnt[(size_t)b * (M + 1) + j] = c;
 }
 
@@ -130,7 +134,7 @@ __global__ __launch_bounds__(EIG_THREADS) void sturm_grid_kernel(const double* D
 // stops once the interval is below rtol relative (plus the pivot floor).
 // LDS = true: d and e^2 staged in LDS (2n doubles, n <= 4096); false: read
 // from global memory (d in place, e^2 from E2g written by square_e_kernel).
-template <bool LDS>
+template <bool LDS, int NEWTON>
 __global__ __launch_bounds__(EIG_THREADS) void bisect_kernel(const double* D, const double* Ed, int n, int ld,
                                                              double* Wout, int wld, int G, const double* E2g,
                                                              const int* cnt, int M, double rtol) {
@@ -167,7 +171,7 @@ __global__ __launch_bounds__(EIG_THREADS) void bisect_kernel(const double* D, co
     if (!done && width <= rtol * fmax(fabs(lo), fabs(hi)) + piv) done = true;
     if (__ballot(!done) == 0ULL) break;
     const double x = lo + width * (double)(sub + 1) * inv;
-    const bool above = !done && sturm_count(sd, se, n, x, piv) > idx;
+    const bool above = !done && sturm_count<NEWTON>(sd, se, n, x, piv) > idx;
     const unsigned long long mask = (__ballot(above) >> (grp * G)) & gmask;
     if (!done) {
       const int first = mask ? (__ffsll((long long)mask) - 1) : G;
@@ -1071,20 +1075,36 @@ static int eigh_impl(const void* H, const void* S, int n, int ld, int batch, dou
     // spectral radius): its eigenvalues are resolved to 2^-36 relative
     // instead of the FP64 path's 4 ulp
     const double rtol = mixed ? 1.4551915228366852e-11 : 4.0 * 2.2e-16;
+    // one Newton step on the pivot reciprocals for the mixed path (its
+    // tolerance is far above the 2^-48.7 they leave), two for FP64
     if (n <= BISECT_LDS_MAX) {
       const size_t lds = 2 * n * sizeof(double);
-      hipLaunchKernelGGL(sturm_grid_kernel<true>, ggrid, dim3(EIG_THREADS), lds, stream, s.D, s.E, n, ld, M, s.SG,
-                         (const double*)nullptr);
-      hipLaunchKernelGGL(bisect_kernel<true>, grid, dim3(EIG_THREADS), lds, stream, s.D, s.E, n, ld, w, wld, G,
-                         (const double*)nullptr, (const int*)s.SG, M, rtol);
+      if (mixed) {
+        hipLaunchKernelGGL((sturm_grid_kernel<true, 1>), ggrid, dim3(EIG_THREADS), lds, stream, s.D, s.E, n, ld, M,
+                           s.SG, (const double*)nullptr);
+        hipLaunchKernelGGL((bisect_kernel<true, 1>), grid, dim3(EIG_THREADS), lds, stream, s.D, s.E, n, ld, w, wld, G,
+                           (const double*)nullptr, (const int*)s.SG, M, rtol);
+      } else {
+        hipLaunchKernelGGL((sturm_grid_kernel<true, 2>), ggrid, dim3(EIG_THREADS), lds, stream, s.D, s.E, n, ld, M,
+                           s.SG, (const double*)nullptr);
+        hipLaunchKernelGGL((bisect_kernel<true, 2>), grid, dim3(EIG_THREADS), lds, stream, s.D, s.E, n, ld, w, wld, G,
+                           (const double*)nullptr, (const int*)s.SG, M, rtol);
+      }
     } else {
       // e^2 into the (now free) mat-vec vector buffer, ld doubles per matrix
       double* e2 = mixed ? reinterpret_cast<double*>(s.fY) : reinterpret_cast<double*>(s.Y);
       hipLaunchKernelGGL(square_e_kernel, dim3((n + 255) / 256, batch), dim3(256), 0, stream, s.E, n, ld, e2);
-      hipLaunchKernelGGL(sturm_grid_kernel<false>, ggrid, dim3(EIG_THREADS), 0, stream, s.D, s.E, n, ld, M, s.SG,
-                         (const double*)e2);
-      hipLaunchKernelGGL(bisect_kernel<false>, grid, dim3(EIG_THREADS), 0, stream, s.D, s.E, n, ld, w, wld, G,
-                         (const double*)e2, (const int*)s.SG, M, rtol);
+      if (mixed) {
+        hipLaunchKernelGGL((sturm_grid_kernel<false, 1>), ggrid, dim3(EIG_THREADS), 0, stream, s.D, s.E, n, ld, M,
+                           s.SG, (const double*)e2);
+        hipLaunchKernelGGL((bisect_kernel<false, 1>), grid, dim3(EIG_THREADS), 0, stream, s.D, s.E, n, ld, w, wld, G,
+                           (const double*)e2, (const int*)s.SG, M, rtol);
+      } else {
+        hipLaunchKernelGGL((sturm_grid_kernel<false, 2>), ggrid, dim3(EIG_THREADS), 0, stream, s.D, s.E, n, ld, M,
+                           s.SG, (const double*)e2);
+        hipLaunchKernelGGL((bisect_kernel<false, 2>), grid, dim3(EIG_THREADS), 0, stream, s.D, s.E, n, ld, w, wld, G,
+                           (const double*)e2, (const int*)s.SG, M, rtol);
+      }
     }
     if (ev) (void)hipEventRecord(ev[5], stream);
   } else {
