@@ -271,6 +271,8 @@ __global__ __launch_bounds__(BUILD_THREADS) void slk_build_kernel(SlkBuildArgs A
   __shared__ int col_atom[MAXCOL];
   __shared__ int col_slot[MAXCOL];
   __shared__ int col_orb[MAXCOL];
+  __shared__ int pi_mask[MAXPI];         // slots present on the column atom
+  __shared__ int s_rng[MAXCOL][2];       // per column atom: its pair-images in the batch
 
   const SlkBuildWork wk = A.work[blockIdx.x];
   const SlkStructDesc sd = A.structs[wk.s];
@@ -280,6 +282,9 @@ __global__ __launch_bounds__(BUILD_THREADS) void slk_build_kernel(SlkBuildArgs A
   const int norb_i = A.tab.spec_norb[si];
   const double rix = A.atom_pos[3 * ai], riy = A.atom_pos[3 * ai + 1], riz = A.atom_pos[3 * ai + 2];
   const int nja = wk.j1 - wk.j0;
+  // slots (s, p, d components) present on atom i
+  int mask_i = 0;
+  for (int o = 0; o < norb_i; ++o) mask_i |= 1 << A.tab.spec_slot[si * 9 + o];
 
   // column map: tile column -> (atom, canonical slot, orbital index)
   if (tid == 0) {
@@ -365,6 +370,9 @@ __global__ __launch_bounds__(BUILD_THREADS) void slk_build_kernel(SlkBuildArgs A
           pi_j[p] = wk.j0 + jr;
           pi_t[p] = t;
           pi_u[p][0] = vx / r; pi_u[p][1] = vy / r; pi_u[p][2] = vz / r;
+          int mj = 0;
+          for (int o = 0; o < A.tab.spec_norb[sj]; ++o) mj |= 1 << A.tab.spec_slot[sj * 9 + o];
+          pi_mask[p] = mj;
         }
         int base = -1;
         double w[8];
@@ -374,6 +382,18 @@ __global__ __launch_bounds__(BUILD_THREADS) void slk_build_kernel(SlkBuildArgs A
         }
         pi_base[p][dir] = base;
         for (int m = 0; m < 8; ++m) pi_w[p][dir][m] = (base >= 0) ? w[m] : 0.0;
+      }
+      // pair-images are in candidate order, i.e. sorted by column atom: the
+      // range of each column atom within the batch
+      if (tid < nja) {
+        int lo = np, hi = np;
+        for (int p = 0; p < np; ++p)
+          if (s_list[b0 + p] >> 20 == tid) {
+            if (lo == np) lo = p;
+            hi = p + 1;
+          }
+        s_rng[tid][0] = lo;
+        s_rng[tid][1] = lo == np ? np : hi;
       }
       __syncthreads();
       // (b) integrals
@@ -408,6 +428,8 @@ __global__ __launch_bounds__(BUILD_THREADS) void slk_build_kernel(SlkBuildArgs A
       // (d) effective SK blocks (canonical slots) per pair-image
       for (int task = tid; task < np * 81; task += BUILD_THREADS) {
         const int p = task / 81, ab = task % 81, a = ab / 9, b = ab % 9;
+        // slot pairs absent from the (i, j) species pair are never read
+        if (!((mask_i >> a) & (pi_mask[p] >> b) & 1)) continue;
         const int la = c_slot_l[a], lb = c_slot_l[b];
         const int j = pi_j[p];
         const bool direct = (la < lb) || (la == lb && wk.i <= j);
@@ -435,8 +457,8 @@ __global__ __launch_bounds__(BUILD_THREADS) void slk_build_kernel(SlkBuildArgs A
           const int ml = e / wk.ncol, c = e % wk.ncol;
           const int a = A.tab.spec_slot[si * 9 + ml];
           const int j = col_atom[c], b = col_slot[c];
-          for (int p = 0; p < np; ++p) {
-            if (pi_j[p] != j) continue;
+          const int p1 = s_rng[j - wk.j0][1];
+          for (int p = s_rng[j - wk.j0][0]; p < p1; ++p) {
             const double eh = E[p][0][a * 9 + b], es = E[p][1][a * 9 + b];
 #pragma unroll
             for (int kk = 0; kk < KC; ++kk) {
@@ -599,6 +621,9 @@ __global__ __launch_bounds__(BUILD_THREADS) void slk_adjoint_kernel(SlkBuildArgs
   const int oi = A.atom_orb[ai];
   const double rix = A.atom_pos[3 * ai], riy = A.atom_pos[3 * ai + 1], riz = A.atom_pos[3 * ai + 2];
   const int nja = wk.j1 - wk.j0;
+  // slots (s, p, d components) present on atom i
+  int mask_i = 0;
+  for (int o = 0; o < norb_i; ++o) mask_i |= 1 << A.tab.spec_slot[si * 9 + o];
   const size_t mat_el = (size_t)sd.ld * sd.ld;
   if (tid == 0) {
     int c = 0;
