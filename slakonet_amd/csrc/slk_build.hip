@@ -258,7 +258,6 @@ template <typename OUT>
 __global__ __launch_bounds__(BUILD_THREADS) void slk_build_kernel(SlkBuildArgs A, unsigned long long* clk) {
   sample_start(clk);
   __shared__ int s_list[SEG];            // compacted candidates (packed jrel<<20 | t)
-  __shared__ int s_cnt[BUILD_THREADS];
   __shared__ int s_total;
   __shared__ int pi_j[MAXPI];
   __shared__ int pi_t[MAXPI];
@@ -272,6 +271,8 @@ __global__ __launch_bounds__(BUILD_THREADS) void slk_build_kernel(SlkBuildArgs A
   __shared__ int col_slot[MAXCOL];
   __shared__ int col_orb[MAXCOL];
   __shared__ int pi_mask[MAXPI];         // slots present on the column atom
+  __shared__ int s_jspec[MAXCOL], s_jnorb[MAXCOL], s_jorb[MAXCOL];
+  __shared__ int s_wsum[BUILD_THREADS / 64];
   __shared__ int s_rng[MAXCOL][2];       // per column atom: its pair-images in the batch
 
   const SlkBuildWork wk = A.work[blockIdx.x];
@@ -286,18 +287,29 @@ __global__ __launch_bounds__(BUILD_THREADS) void slk_build_kernel(SlkBuildArgs A
   int mask_i = 0;
   for (int o = 0; o < norb_i; ++o) mask_i |= 1 << A.tab.spec_slot[si * 9 + o];
 
-  // column map: tile column -> (atom, canonical slot, orbital index)
-  if (tid == 0) {
-    int c = 0;
-    for (int j = wk.j0; j < wk.j1; ++j) {
-      const int aj = sd.atom0 + j;
-      const int sj = A.atom_spec[aj];
-      const int nb = A.tab.spec_norb[sj];
-      for (int o = 0; o < nb && c < MAXCOL; ++o, ++c) {
-        col_atom[c] = j;
-        col_slot[c] = A.tab.spec_slot[sj * 9 + o];
-        col_orb[c] = A.atom_orb[aj] + o;
-      }
+  // column map: tile column -> (atom, canonical slot, orbital index). The
+  // column atoms' species and offsets are loaded in parallel, then each
+  // column finds its atom by a scan of the orbital counts (columns are the
+  // consecutive orbitals of atoms j0 .. j1-1, cut at MAXCOL).
+  if (tid < nja) {
+    const int aj = sd.atom0 + wk.j0 + tid;
+    const int sj = A.atom_spec[aj];
+    s_jspec[tid] = sj;
+    s_jnorb[tid] = A.tab.spec_norb[sj];
+    s_jorb[tid] = A.atom_orb[aj];
+  }
+  __syncthreads();
+  if (tid < MAXCOL) {
+    int c0 = 0, t = 0;
+    for (; t < nja; ++t) {
+      if (tid < c0 + s_jnorb[t]) break;
+      c0 += s_jnorb[t];
+    }
+    if (t < nja) {
+      const int o = tid - c0;
+      col_atom[tid] = wk.j0 + t;
+      col_slot[tid] = A.tab.spec_slot[s_jspec[t] * 9 + o];
+      col_orb[tid] = s_jorb[t] + o;
     }
   }
 
@@ -327,15 +339,24 @@ __global__ __launch_bounds__(BUILD_THREADS) void slk_build_kernel(SlkBuildArgs A
       const double r = sqrt(vx * vx + vy * vy + vz * vz);
       if (r < A.cutoff && r > 0.0) ++mine;
     }
-    s_cnt[tid] = mine;
-    __syncthreads();
-    if (tid == 0) {
-      int acc = 0;
-      for (int q = 0; q < BUILD_THREADS; ++q) { int v = s_cnt[q]; s_cnt[q] = acc; acc += v; }
-      s_total = acc;
+    // exclusive prefix over the workgroup: wave scan, then wave offsets
+    const int lane = tid & 63, wv = tid >> 6;
+    int inc = mine;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      const int v = __shfl_up(inc, d, 64);
+      if (lane >= d) inc += v;
     }
+    if (lane == 63) s_wsum[wv] = inc;
     __syncthreads();
-    int pos = s_cnt[tid];
+    int woff = 0, total_all = 0;
+#pragma unroll
+    for (int q = 0; q < BUILD_THREADS / 64; ++q) {
+      if (q < wv) woff += s_wsum[q];
+      total_all += s_wsum[q];
+    }
+    if (tid == 0) s_total = total_all;
+    int pos = woff + inc - mine;
     for (int c = beg; c < end; ++c) {
       const long long g = seg0 + c;
       const int jr = (int)(g / sd.nt), t = (int)(g % sd.nt);
