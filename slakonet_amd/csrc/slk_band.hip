@@ -2,25 +2,29 @@
 // form for the eigenvalue path (FP64 or FP32 arithmetic; d, e always FP64).
 //
 // Stage 1, dense -> band of width BW = 16, per 16-column panel j0
-// (rows r0 = j0 + BW .. n-1, m = n - r0; LAPACK zhetrd_he2hb structure):
-//   panel_qr   one workgroup per matrix: Householder QR of the m x BW panel
-//              held in registers (thread = (row group, column)), one block
-//              reduction for the column norm and one for the 15 inner products
-//              per column; the block reflector factor T (Q = I - V T V^H) is
-//              built alongside. R goes into the band, the reflectors below it.
-//   band_av    Y = A22 V on MFMA, 64 rows x 256-column chunks (split K).
+// (rows r0 = j0 + BW .. n-1, m = n - r0; LAPACK zhetrd_he2hb structure),
+// on the lower triangle of A only:
+//   panel_qr   one workgroup per matrix: Householder QR of the m x BW panel,
+//              wave w owning column w, lanes owning rows (in registers up to
+//              a size tier); one barrier per column, the reflector published
+//              in LDS double-buffered by column parity; the block reflector
+//              factor T (Q = I - V T V^H) built by wave 0 one column behind.
+//              R goes into the band, the reflectors below it.
+//   band_av    Y = A22 V on MFMA, 64 rows x 128-column chunks (split K),
+//              A22 read as Hermitian from its lower triangle.
 //   band_x     X = Y T, per 64-row block partial Z = V^H X.
 //   band_w     W = X - V (T^H Z) / 2.
-//   rank-2k    A22 -= V W^H + W V^H on the MFMA GEMM (full tiles, so A22
-//              stays fully stored for the next panel's product).
+//   rank-2k    A22 -= V W^H + W V^H on the 32 x 32 blocks touching the lower
+//              triangle (rank2k_lower_kernel, registers only).
 // Stage 2, band -> tridiagonal: band_chase, one workgroup (16 waves) per
-// matrix on a compact copy of the band (column j holds A[j..j+2BW-1][j]).
-// Sweep s annihilates column s below the subdiagonal with one reflector and
-// chases the bulge down in steps of BW rows; each step is one wave: reflector
-// from the bulge column, left application to the bulge block, two-sided
-// application to the diagonal block, right application to the block below.
-// Sweep s+1 may run step c together with sweep s's step c+3 (their regions
-// are disjoint), so 16 sweeps run at once, in lock-step rounds of one step.
+// matrix on a compact copy of the band (column j holds A[j..j+2BW-1][j]),
+// in LDS when it fits. Sweep s annihilates column s below the subdiagonal
+// with one reflector and chases the bulge down in steps of BW rows; each step
+// is one wave: reflector from the bulge column, left application to the bulge
+// block, two-sided application to the diagonal block, right application to
+// the block below. Sweep s may run step c once sweep s-1 has finished step
+// c+2 (their regions are then disjoint); waves publish their progress in LDS
+// and wait on their predecessor's counter, with a bounded wait.
 #include "slk_common.h"
 #include "slk_dense.h"
 #include <hip/hip_ext.h>
