@@ -10,12 +10,15 @@
 //              in LDS double-buffered by column parity; the block reflector
 //              factor T (Q = I - V T V^H) built by wave 0 one column behind.
 //              R goes into the band, the reflectors below it.
-//   band_av    Y = A22 V on MFMA, 64 rows x 128-column chunks (split K),
-//              A22 read as Hermitian from its lower triangle.
+//   band_av    (first panel) Y = A22 V on MFMA, 64 rows x 128-column chunks
+//              (split K), A22 read as Hermitian from its lower triangle.
 //   band_x     X = Y T, per 64-row block partial Z = V^H X.
 //   band_w     W = X - V (T^H Z) / 2.
-//   rank-2k    A22 -= V W^H + W V^H on the 32 x 32 blocks touching the lower
-//              triangle (rank2k_lower_kernel, registers only).
+//   update     A22 -= V W^H + W V^H with look-ahead: the next panel's 16
+//              columns first (strip_update), then the next panel's QR, then
+//              the rest of A22 in one pass that also forms the next panel's
+//              partials of A22 V (fused_update_av), on the 32 x 32 blocks
+//              touching the lower triangle.
 // Stage 2, band -> tridiagonal: band_chase, one workgroup (16 waves) per
 // matrix on a compact copy of the band (column j holds A[j..j+2BW-1][j]),
 // in LDS when it fits. Sweep s annihilates column s below the subdiagonal
@@ -450,16 +453,32 @@ __global__ __launch_bounds__(256) void band_x_kernel(BandCtx<T> c, int r0, int n
   const int gr = row0 + rl;
   const C2* vc = c.Vc + (size_t)b * 2 * BW * ld;
   C2* wc = c.Wc + (size_t)b * 2 * BW * ld;
+  // the 4 columns of this thread are adjacent: one 4-element load per chunk,
+  // 8 chunks in flight
+  {
+    C2 y[4], v[4];
 #pragma unroll
-  for (int q = 0; q < 4; ++q) {
-    const int cc = c0 + q;
-    C2 y = mk<T>(0, 0), v = mk<T>(0, 0);
-    if (gr < n) {
-      for (int ch = 0; ch < nchk; ++ch) y = c_add(y, c.Yp[(((size_t)b * c.nch + ch) * ld + gr) * BW + cc]);
-      v = vc[(size_t)cc * ld + gr];
+    for (int q = 0; q < 4; ++q) {
+      y[q] = mk<T>(0, 0);
+      v[q] = mk<T>(0, 0);
     }
-    s_y[rl][cc] = y;
-    s_v[rl][cc] = v;
+    if (gr < n) {
+      const C2* yp = c.Yp + ((size_t)b * c.nch * ld + gr) * BW + c0;
+      const size_t cstride = (size_t)ld * BW;
+#pragma unroll 8
+      for (int ch = 0; ch < nchk; ++ch) {
+        const C2* p = yp + ch * cstride;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) y[q] = c_add(y[q], p[q]);
+      }
+#pragma unroll
+      for (int q = 0; q < 4; ++q) v[q] = vc[(size_t)(c0 + q) * ld + gr];
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      s_y[rl][c0 + q] = y[q];
+      s_v[rl][c0 + q] = v[q];
+    }
   }
   __syncthreads();
 #pragma unroll
@@ -875,77 +894,153 @@ static double chase_flops(int n) {
   return f;
 }
 
-size_t band_partials_elems(int ld) { return (size_t)((ld + AV_KCH - 1) / AV_KCH) * ld * BW; }
+// partials of A22 V: one slot of ld x BW per 64-row chunk of K
+size_t band_partials_elems(int ld) { return (size_t)((ld + 63) / 64) * ld * BW; }
 // compact band, Zp, T
 int band_panels(int ld) { return (ld + BW - 1) / BW; }
-// compact band, Zp, the panels' T factors
+// compact band, Zp, the panels' T factors, the second [V|W], [W|V] pair
 size_t band_scratch_elems(int ld) {
-  return (size_t)ld * BND_LD + (size_t)((ld + 63) / 64) * BW * BW + (size_t)band_panels(ld) * BW * BW;
+  return (size_t)ld * BND_LD + (size_t)((ld + 63) / 64) * BW * BW + (size_t)band_panels(ld) * BW * BW +
+         (size_t)2 * (2 * BW) * ld;
 }
 size_t band_tt_offset(int ld, int batch) {
   return (size_t)batch * ld * BND_LD + (size_t)batch * ((ld + 63) / 64) * BW * BW;
 }
 
 // ---------------------------------------------------------------------------
-// Stage 1 trailing update A22 -= [V | W] [W | V]^H = V W^H + W V^H (K = 2 BW),
-// lower triangle only. Each wave owns a 32 x 32 block of A22: the block is
-// loaded straight into the MFMA accumulators, the operands straight into
-// registers in MFMA operand layout (lane l: A[l%16][l/16], B[l/16][l%16]),
-// with no LDS and no barrier, so one memory round trip covers the block.
-// Blocks wholly above the diagonal are skipped: every later reader of A22
-// (the next A22 V product, the panel QR, the band copy of the chase) reads
-// elements on or below the diagonal only.
+// Look-ahead form of stage 1. Panel p's update of A22(p) is split: the next
+// panel's 16 columns first (strip_update_kernel), so that its QR can run, then
+// the rest, A22(p+1), in one pass that also forms the next panel's product
+// Y = A22(p+1) V_{p+1} (fused_update_av_kernel). A22(p+1) is read and written
+// once per panel instead of read twice and written once.
 // ---------------------------------------------------------------------------
+
+// C[0:m, 0:16] -= [V | W] [W | V]^H restricted to the first 16 columns of
+// A22(p) (C at its origin, operands at its first row). One 16 x 16 block per
+// wave, 64 rows per workgroup.
 template <typename T>
-__global__ __launch_bounds__(256) void rank2k_lower_kernel(int m, const typename C2T<T>::t* __restrict__ Vc,
+__global__ __launch_bounds__(256) void strip_update_kernel(int m, const typename C2T<T>::t* __restrict__ Vc,
                                                            const typename C2T<T>::t* __restrict__ Wc,
                                                            typename C2T<T>::t* __restrict__ C, int ld, long long sP,
                                                            long long sM) {
   using C2 = typename C2T<T>::t;
   using V4 = typename MF<T>::v4;
-  constexpr int KK = 2 * BW / 4;                 // k-steps of 4
-  constexpr int KC = sizeof(T) == 4 ? KK : KK / 2;   // k-steps per operand batch
+  constexpr int KK = 2 * BW / 4;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  const int m0 = blockIdx.y * 64 + (wv >> 1) * 32, n0 = blockIdx.x * 64 + (wv & 1) * 32;
-  if (m0 >= m || n0 > m0 + 31) return;
+  const int m0 = blockIdx.x * 64 + 16 * wv;
+  if (m0 >= m) return;
   const int b = blockIdx.z;
   const C2* vc = Vc + (size_t)b * sP;
   const C2* wc = Wc + (size_t)b * sP;
   C2* c = C + (size_t)b * sM;
   const int li = lane & 15, lk = lane >> 4;
-  const C2 zero = mk<T>(0, 0);
-  V4 accR[2][2], accI[2][2];
+  const int ncol = m < BW ? m : BW;
+  V4 accR, accI;
 #pragma unroll
-  for (int i = 0; i < 2; ++i)
+  for (int v = 0; v < 4; ++v) {
+    const int r = m0 + MF<T>::row(lk, v);
+    const C2 x = load_or_zero(c + (size_t)r * ld + li, r < m && li < ncol);
+    accR[v] = x.x;
+    accI[v] = x.y;
+  }
+  C2 av[KK], bw[KK];
 #pragma unroll
-    for (int j = 0; j < 2; ++j)
+  for (int q = 0; q < KK; ++q) {
+    const size_t krow = (size_t)(4 * q + lk) * ld;
+    av[q] = load_or_zero(vc + krow + m0 + li, m0 + li < m);
+    bw[q] = load_or_zero(wc + krow + li, li < ncol);
+  }
 #pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        const int r = m0 + 16 * i + MF<T>::row(lk, v), col = n0 + 16 * j + li;
-        const C2 x = load_or_zero(c + (size_t)r * ld + col, r < m && col < m);
-        accR[i][j][v] = x.x;
-        accI[i][j][v] = x.y;
-      }
+  for (int q = 0; q < KK; ++q) {
+    accR = MF<T>::mfma(-av[q].x, bw[q].x, accR);
+    accR = MF<T>::mfma(-av[q].y, bw[q].y, accR);
+    accI = MF<T>::mfma(-av[q].y, bw[q].x, accI);
+    accI = MF<T>::mfma(av[q].x, bw[q].y, accI);
+  }
 #pragma unroll
-  for (int kc = 0; kc < KK; kc += KC) {
-    C2 av[2][KC], bw[2][KC];
+  for (int v = 0; v < 4; ++v) {
+    const int r = m0 + MF<T>::row(lk, v);
+    if (r < m && li < ncol) c[(size_t)r * ld + li] = mk<T>(accR[v], accI[v]);
+  }
+}
+
+// One 64 x 64 tile (I, J), J <= I, of A22(p+1) (origin r0n, order mn):
+//   1. A -= [V | W]_p [W | V]_p^H on its 32 x 32 blocks touching the lower
+//      triangle, each wave loading its block straight into the MFMA
+//      accumulators and the operands straight into registers in MFMA operand
+//      layout; the result is stored back and kept in LDS;
+//   2. a diagonal tile gets its upper half from the conjugate of the lower
+//      one (elements above the diagonal are not kept current in A);
+//   3. the partials of Y = A22(p+1) V_{p+1}: rows of block I from A_IJ V_J
+//      into chunk J, and (I != J) rows of block J from A_IJ^H V_I into chunk
+//      I. Every (row block, chunk) slot is written by exactly one tile.
+// Vp, Wp: panel p's [V | W], [W | V] at row r0n; Vn: V_{p+1} at row r0n.
+template <typename T>
+__global__ __launch_bounds__(256) void fused_update_av_kernel(const typename C2T<T>::t* __restrict__ Vp,
+                                                              const typename C2T<T>::t* __restrict__ Wp,
+                                                              const typename C2T<T>::t* __restrict__ Vn,
+                                                              typename C2T<T>::t* __restrict__ A,
+                                                              typename C2T<T>::t* __restrict__ Yp, int n, int ld,
+                                                              int r0n, int nch, long long sP, long long sM) {
+  using C2 = typename C2T<T>::t;
+  using V4 = typename MF<T>::v4;
+  constexpr int KK = 2 * BW / 4;
+  const int I = blockIdx.y, J = blockIdx.x;
+  if (J > I) return;
+  const int mn = n - r0n;
+  const int b = blockIdx.z;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int li = lane & 15, lk = lane >> 4;
+  const C2* vp = Vp + (size_t)b * sP;
+  const C2* wp = Wp + (size_t)b * sP;
+  const C2* vn = Vn + (size_t)b * sP;
+  C2* a = A + (size_t)b * sM + (size_t)r0n * ld + r0n;
+  __shared__ T Cr[64][65], Ci[64][65];
+  __shared__ T VJr[64][BW + 1], VJi[64][BW + 1], VIr[64][BW + 1], VIi[64][BW + 1];
+  // V_{p+1} rows of blocks J and I (zero beyond the matrix)
+  for (int e = tid; e < 64 * BW; e += 256) {
+    const int col = e / 64, k = e % 64;
+    const int rj = 64 * J + k, ri = 64 * I + k;
+    const C2 xj = load_or_zero(vn + (size_t)col * ld + rj, rj < mn);
+    const C2 xi = load_or_zero(vn + (size_t)col * ld + ri, ri < mn);
+    VJr[k][col] = xj.x;
+    VJi[k][col] = xj.y;
+    VIr[k][col] = xi.x;
+    VIi[k][col] = xi.y;
+  }
+  // 1. rank-2k on this wave's 32 x 32 block
+  const int wr = (wv >> 1) * 32, wc = (wv & 1) * 32;
+  const int m0 = 64 * I + wr, n0 = 64 * J + wc;
+  if (n0 <= m0 + 31) {
+    C2 av[2][KK], bw[2][KK];
 #pragma unroll
-    for (int q = 0; q < KC; ++q) {
-      const size_t krow = (size_t)(4 * (kc + q) + lk) * ld;
+    for (int q = 0; q < KK; ++q) {
+      const size_t krow = (size_t)(4 * q + lk) * ld;
 #pragma unroll
       for (int i = 0; i < 2; ++i) {
         const int r = m0 + 16 * i + li;
-        av[i][q] = load_or_zero(vc + krow + r, r < m);
+        av[i][q] = load_or_zero(vp + krow + r, r < mn);
       }
 #pragma unroll
       for (int j = 0; j < 2; ++j) {
         const int col = n0 + 16 * j + li;
-        bw[j][q] = load_or_zero(wc + krow + col, col < m);
+        bw[j][q] = load_or_zero(wp + krow + col, col < mn);
       }
     }
-    // C -= a conj(w): Re -= a.x w.x + a.y w.y, Im -= a.y w.x - a.x w.y
+    V4 accR[2][2], accI[2][2];
 #pragma unroll
-    for (int q = 0; q < KC; ++q)
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int j = 0; j < 2; ++j)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int r = m0 + 16 * i + MF<T>::row(lk, v), col = n0 + 16 * j + li;
+          const C2 x = load_or_zero(a + (size_t)r * ld + col, r < mn && col < mn);
+          accR[i][j][v] = x.x;
+          accI[i][j][v] = x.y;
+        }
+#pragma unroll
+    for (int q = 0; q < KK; ++q)
 #pragma unroll
       for (int i = 0; i < 2; ++i)
 #pragma unroll
@@ -955,23 +1050,70 @@ __global__ __launch_bounds__(256) void rank2k_lower_kernel(int m, const typename
           accI[i][j] = MF<T>::mfma(-av[i][q].y, bw[j][q].x, accI[i][j]);
           accI[i][j] = MF<T>::mfma(av[i][q].x, bw[j][q].y, accI[i][j]);
         }
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int j = 0; j < 2; ++j)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int rl = wr + 16 * i + MF<T>::row(lk, v), cl = wc + 16 * j + li;
+          const int r = 64 * I + rl, col = 64 * J + cl;
+          if (r < mn && col < mn) a[(size_t)r * ld + col] = mk<T>(accR[i][j][v], accI[i][j][v]);
+          Cr[rl][cl] = accR[i][j][v];
+          Ci[rl][cl] = accI[i][j][v];
+        }
   }
-#pragma unroll
-  for (int i = 0; i < 2; ++i)
-#pragma unroll
-    for (int j = 0; j < 2; ++j)
-#pragma unroll
-      for (int v = 0; v < 4; ++v) {
-        const int r = m0 + 16 * i + MF<T>::row(lk, v), col = n0 + 16 * j + li;
-        if (r < m && col < m) c[(size_t)r * ld + col] = mk<T>(accR[i][j][v], accI[i][j][v]);
+  __syncthreads();
+  // 2. Hermitian completion of a diagonal tile
+  if (I == J) {
+    for (int e = tid; e < 64 * 64; e += 256) {
+      const int r = e / 64, col = e % 64;
+      if (col > r) {
+        Cr[r][col] = Cr[col][r];
+        Ci[r][col] = -Ci[col][r];
       }
-}
-
-template <typename T>
-static void rank2k_lower(hipStream_t st, int batch, int m, typename C2T<T>::t* Vc, typename C2T<T>::t* Wc,
-                         typename C2T<T>::t* C, int ld, long long sP, long long sM) {
-  const int t = (m + 63) / 64;
-  hipLaunchKernelGGL(rank2k_lower_kernel<T>, dim3(t, t, batch), dim3(256), 0, st, m, Vc, Wc, C, ld, sP, sM);
+    }
+    __syncthreads();
+  }
+  // 3. partials of Y: this wave's 16 rows of each product
+  const int rw = 16 * wv;
+  {
+    V4 yR = V4{0, 0, 0, 0}, yI = V4{0, 0, 0, 0};
+#pragma unroll
+    for (int k4 = 0; k4 < 64; k4 += 4) {
+      const T ar = Cr[rw + li][k4 + lk], ai = Ci[rw + li][k4 + lk];
+      const T br = VJr[k4 + lk][li], bi = VJi[k4 + lk][li];
+      yR = MF<T>::mfma(ar, br, yR);
+      yR = MF<T>::mfma(-ai, bi, yR);
+      yI = MF<T>::mfma(ar, bi, yI);
+      yI = MF<T>::mfma(ai, br, yI);
+    }
+    C2* yp = Yp + ((size_t)b * nch + J) * ld * BW;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int r = 64 * I + rw + MF<T>::row(lk, v);
+      if (r < mn) yp[(size_t)(r0n + r) * BW + li] = mk<T>(yR[v], yI[v]);
+    }
+  }
+  if (I != J) {
+    V4 yR = V4{0, 0, 0, 0}, yI = V4{0, 0, 0, 0};
+#pragma unroll
+    for (int k4 = 0; k4 < 64; k4 += 4) {
+      // (A^H)[i][k] = conj(A[k][i])
+      const T ar = Cr[k4 + lk][rw + li], ai = -Ci[k4 + lk][rw + li];
+      const T br = VIr[k4 + lk][li], bi = VIi[k4 + lk][li];
+      yR = MF<T>::mfma(ar, br, yR);
+      yR = MF<T>::mfma(-ai, bi, yR);
+      yI = MF<T>::mfma(ar, bi, yI);
+      yI = MF<T>::mfma(ai, br, yI);
+    }
+    C2* yp = Yp + ((size_t)b * nch + I) * ld * BW;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int r = 64 * J + rw + MF<T>::row(lk, v);
+      if (r < mn) yp[(size_t)(r0n + r) * BW + li] = mk<T>(yR[v], yI[v]);
+    }
+  }
 }
 
 #define CH_LDS_MAX (150 * 1024)
@@ -1013,25 +1155,46 @@ static void band_impl(hipStream_t st, typename C2T<T>::t* A, int n, int ld, int 
   c.VT = VT;
   c.D = D;
   c.E = E;
-  c.nch = (ld + AV_KCH - 1) / AV_KCH;
+  c.nch = (ld + 63) / 64;
   c.nrb = (ld + 63) / 64;
   c.npan = band_panels(ld);
-  // scratch: band, then Zp, then the panels' T (band_scratch_elems per matrix)
+  // scratch: band, then Zp, then the panels' T, then the second [V|W], [W|V]
+  // pair (band_scratch_elems per matrix)
   c.BND = scratch;
   c.Zp = scratch + (size_t)batch * ld * BND_LD;
   c.TT = c.Zp + (size_t)batch * c.nrb * BW * BW;
+  C2* Vc2 = c.TT + (size_t)batch * c.npan * BW * BW;
+  C2* Wc2 = Vc2 + (size_t)batch * 2 * BW * ld;
   const long long sP = (long long)2 * BW * ld, sM = (long long)ld * ld;
-  for (int j0 = 0;; j0 += BW) {
-    const int r0 = j0 + BW, m = n - r0;
-    // m = 1 gives a one-element (phase) reflector: it leaves the band as it is
-    // and defines tau for every column whose reflector starts inside the matrix
-    if (m < 1) break;
-    launch_panel_qr<T>(st, c, batch, j0, m);
+  // panel p works in the buffer pair of its parity: the next panel's QR
+  // writes its V while panel p's [V | W] is still needed
+  BandCtx<T> cx[2] = {c, c};
+  cx[1].Vc = Vc2;
+  cx[1].Wc = Wc2;
+  // m = 1 gives a one-element (phase) reflector: it leaves the band as it is
+  // and defines tau for every column whose reflector starts inside the matrix
+  if (n - BW >= 1) {
+    const int m = n - BW;
+    launch_panel_qr<T>(st, cx[0], batch, 0, m);
     const int nrbk = (m + 63) / 64, nchk = (m + AV_KCH - 1) / AV_KCH;
-    hipLaunchKernelGGL(band_av_kernel<T>, dim3(nrbk, nchk, batch), dim3(256), 0, st, c, r0);
-    hipLaunchKernelGGL(band_x_kernel<T>, dim3(nrbk, batch), dim3(256), 0, st, c, r0, nchk);
-    hipLaunchKernelGGL(band_w_kernel<T>, dim3(nrbk, batch), dim3(256), 0, st, c, r0, nrbk);
-    rank2k_lower<T>(st, batch, m, Vc + r0, Wc + r0, A + (size_t)r0 * ld + r0, ld, sP, sM);
+    hipLaunchKernelGGL(band_av_kernel<T>, dim3(nrbk, nchk, batch), dim3(256), 0, st, cx[0], BW);
+    hipLaunchKernelGGL(band_x_kernel<T>, dim3(nrbk, batch), dim3(256), 0, st, cx[0], BW, nchk);
+    hipLaunchKernelGGL(band_w_kernel<T>, dim3(nrbk, batch), dim3(256), 0, st, cx[0], BW, nrbk);
+    for (int p = 0;; ++p) {
+      const BandCtx<T>& cp = cx[p & 1];
+      const BandCtx<T>& cn = cx[(p + 1) & 1];
+      const int r0 = BW * (p + 1), mp = n - r0;       // A22(p)
+      hipLaunchKernelGGL(strip_update_kernel<T>, dim3((mp + 63) / 64, 1, batch), dim3(256), 0, st, mp, cp.Vc + r0,
+                         cp.Wc + r0, A + (size_t)r0 * ld + r0, ld, sP, sM);
+      const int r0n = r0 + BW, mn = n - r0n;          // A22(p + 1)
+      if (mn < 1) break;
+      launch_panel_qr<T>(st, cn, batch, r0, mn);
+      const int t = (mn + 63) / 64;
+      hipLaunchKernelGGL(fused_update_av_kernel<T>, dim3(t, t, batch), dim3(256), 0, st, cp.Vc + r0n, cp.Wc + r0n,
+                         cn.Vc + r0n, A, c.Yp, n, ld, r0n, c.nch, sP, sM);
+      hipLaunchKernelGGL(band_x_kernel<T>, dim3(t, batch), dim3(256), 0, st, cn, r0n, t);
+      hipLaunchKernelGGL(band_w_kernel<T>, dim3(t, batch), dim3(256), 0, st, cn, r0n, t);
+    }
   }
   // the band (and the schedule) in LDS when it fits, else in global memory
   const size_t lds = (size_t)n * BND_LD * sizeof(C2);

@@ -53,7 +53,7 @@ constexpr int SLK_BW = 16;   // band width of the two-stage reduction
 // reflector of column j as row j of VT (entries r >= j + SLK_BW, explicit 1
 // at j + SLK_BW; VT nullable). Chase reflectors: row s of C's strict upper
 // triangle holds sweep s's reflectors, tau in place of each leading 1.
-// Workspace: Vc, Wc [SLK_BW][ld] and Yp [band_partials_elems(ld)] per
+// Workspace: Vc, Wc [2 SLK_BW][ld] and Yp [band_partials_elems(ld)] per
 // matrix, band scratch [band_scratch_elems(ld)] per matrix.
 void launch_band_tridiag(hipStream_t st, double2* C, int n, int ld, int batch, double* D, double* E, double2* TAU,
                          double2* Vc, double2* Wc, double2* Yp, double2* scratch, double2* VT);
