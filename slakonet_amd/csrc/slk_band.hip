@@ -658,23 +658,34 @@ __device__ __forceinline__ T chase_step(typename C2T<T>::t* bnd, typename C2T<T>
 // Per-lane constant parts of the element offsets of a full step: every offset
 // is a per-step uniform base plus one of these (the band is column-major with
 // BND_LD entries per column, so element (r, col) sits at 31 col + r + col).
+// Per-lane parts of the element offsets of a full step. Every offset is a
+// per-step uniform base plus a lane part; the band is column-major with BND_LD
+// entries per column, so element (r, col) sits at 31 col + r + col. For row
+// r = lg + 4t of the lane's column li, the bulge and below-block offsets are
+// ob0 + 4t and oe0 + 124t (constant steps, folded into the instructions'
+// immediate offsets); the diagonal block reads the lower element, which is
+// one of the two. Offsets are applied in bytes as unsigned 32-bit values, so
+// that a global band is addressed as a uniform base plus one 32-bit VGPR.
 struct ChaseLane {
-  int ob[4];     // bulge block, column c1+1+li, row lg+4t: base 31 (c1+1) + q1
-  int od[4];     // diagonal block (lower element of (li, lg+4t)): base 32 q1
-  int oe[4];     // block below, row q2+1+li, column q1+lg+4t: base 32 q1 + BW
-  bool up[4];    // (li, lg+4t) lies above the diagonal: stored conjugated
+  unsigned ob0;   // bulge block, column c1+1+li, row lg: base 31 (c1+1) + q1
+  unsigned oe0;   // block below, row q2+1+li, column q1+lg: base 32 q1 + BW
+  int li, lg;
 };
+__device__ __forceinline__ bool cl_up(const ChaseLane& o, int t) { return o.lg + 4 * t > o.li; }
+__device__ __forceinline__ unsigned cl_ob(const ChaseLane& o, int t) { return o.ob0 + 4u * t; }
+__device__ __forceinline__ unsigned cl_oe(const ChaseLane& o, int t) { return o.oe0 + 124u * t; }
+// diagonal block (lower element of (li, lg+4t)): base 32 q1
+__device__ __forceinline__ unsigned cl_od(const ChaseLane& o, int t) { return cl_up(o, t) ? cl_ob(o, t) : cl_oe(o, t); }
+template <typename C2>
+__device__ __forceinline__ C2& at_off(C2* base, unsigned el) {
+  return *reinterpret_cast<C2*>(reinterpret_cast<char*>(base) + el * (unsigned)sizeof(C2));
+}
 __device__ __forceinline__ ChaseLane chase_lane(int lane) {
   ChaseLane o;
-  const int li = lane & 15, lg = lane >> 4;
-#pragma unroll
-  for (int t = 0; t < 4; ++t) {
-    const int r = lg + 4 * t;
-    o.ob[t] = 31 * li + r;
-    o.up[t] = r > li;
-    o.od[t] = o.up[t] ? 31 * li + r : 31 * r + li;
-    o.oe[t] = 31 * r + li;
-  }
+  o.li = lane & 15;
+  o.lg = lane >> 4;
+  o.ob0 = 31u * o.li + o.lg;
+  o.oe0 = 31u * o.lg + o.li;
   return o;
 }
 
@@ -699,11 +710,11 @@ __device__ __forceinline__ T chase_step_full(typename C2T<T>::t* bnd, typename C
   C2 eb[4], ed[4], ee[4];
 #pragma unroll
   for (int t = 0; t < 4; ++t) {
-    eb[t] = load_or_zero(bb + o.ob[t], li < BW - 1);
-    C2 d = bd[o.od[t]];
-    if (o.up[t]) d.y = -d.y;
+    eb[t] = load_or_zero(&at_off(bb, cl_ob(o, t)), li < BW - 1);
+    C2 d = at_off(bd, cl_od(o, t));
+    if (cl_up(o, t)) d.y = -d.y;
     ed[t] = d;
-    ee[t] = be[o.oe[t]];
+    ee[t] = at_off(be, cl_oe(o, t));
   }
   // (a) reflector
   const T xn2 = sum16(li >= 1 ? x.x * x.x + x.y * x.y : T(0));
@@ -748,7 +759,7 @@ __device__ __forceinline__ T chase_step_full(typename C2T<T>::t* bnd, typename C
   const C2 fb = c_mul(c_conj(tau), db);
 #pragma unroll
   for (int t = 0; t < 4; ++t)
-    if (li < BW - 1) bb[o.ob[t]] = c_sub(eb[t], c_mul(vr[t], fb));
+    if (li < BW - 1) at_off(bb, cl_ob(o, t)) = c_sub(eb[t], c_mul(vr[t], fb));
   const T vhy = sum16(v.x * y.x + v.y * y.y);
   const T t2 = T(0.5) * (tau.x * tau.x + tau.y * tau.y) * vhy;
   const C2 ty = c_mul(tau, y);
@@ -760,15 +771,15 @@ __device__ __forceinline__ T chase_step_full(typename C2T<T>::t* bnd, typename C
   for (int t = 0; t < 4; ++t) {
     const int r = lg + 4 * t;
     const C2 wr = sv[BW + r];
-    if (!o.up[t]) {
+    if (!cl_up(o, t)) {
       const C2 t1 = c_mulc(v, wr), t3 = c_mulc(w, vr[t]);
       C2 e = ed[t];
       e.x -= t1.x + t3.x;
       e.y -= t1.y + t3.y;
       if (r == li) e.y = T(0);
-      bd[o.od[t]] = e;
+      at_off(bd, cl_od(o, t)) = e;
     }
-    be[o.oe[t]] = c_sub(ee[t], c_mulc(tz, vr[t]));
+    at_off(be, cl_oe(o, t)) = c_sub(ee[t], c_mulc(tz, vr[t]));
   }
   return beta;
 }
@@ -813,7 +824,7 @@ __global__ __launch_bounds__(CH_THREADS) void band_chase_kernel(BandCtx<T> c, un
   __syncthreads();
   const int ns = n - 1;   // sweeps 0 .. n-2
   const int pw = (w + CH_WAVES - 1) % CH_WAVES;   // wave of the predecessor sweep
-  const ChaseLane lo = LDSB ? chase_lane(lane) : ChaseLane{};
+  const ChaseLane lo = chase_lane(lane);
   bool ok = true;
   for (int s = w; s < ns; s += CH_WAVES) {
     const int nst = chase_nsteps(n, s);
@@ -841,13 +852,10 @@ __global__ __launch_bounds__(CH_THREADS) void band_chase_kernel(BandCtx<T> c, un
       }
       if (cs == 0 && lane == 0) D[s] = (double)bnd[(size_t)s * BND_LD].x;
       const bool full = cs > 0 && q2 - q1 + 1 == BW && q1 - c1 == BW && q2 + BW <= n - 1;
-      // LDS band: offsets precomputed per lane; global band: the register
-      // budget of the precomputed offsets costs more than it saves
+      // full steps: offsets precomputed per lane (32-bit, in LDS or global)
       T beta;
-      if (LDSB && full)
+      if (full)
         beta = chase_step_full<T>(bnd, a + (size_t)s * ld, s_vw[w], c1, q1, lane, lo);
-      else if (full)
-        beta = chase_step<T, true>(bnd, a + (size_t)s * ld, s_vw[w], n, c1, q1, q2, lane);
       else
         beta = chase_step<T, false>(bnd, a + (size_t)s * ld, s_vw[w], n, c1, q1, q2, lane);
       if (cs == 0 && lane == 0) E[s] = (double)beta;

@@ -194,11 +194,15 @@ __global__ void square_e_kernel(const double* Ed, int n, int ld, double* E2) {
   }
 }
 
-// lanes per eigenvalue so that about 2^17 lanes are busy
+// Lanes per eigenvalue so that about 2^14 lanes are busy. G lanes shrink the
+// interval (G+1)x per step for G Sturm counts, against 2x for one: G = 32
+// spends ~6x the counts of plain bisection for ~4.6x fewer steps. Batches
+// share the device with other streams, so the work matters more than the
+// latency of this stage (< 1 ms of a large matrix's solve either way).
 static int bisect_group(int n, int batch) {
   const double total = (double)n * batch;
   int g = 1;
-  while (g < 64 && total * g * 2 <= 131072.0) g *= 2;
+  while (g < 64 && total * g * 2 <= 16384.0) g *= 2;
   return g;
 }
 
