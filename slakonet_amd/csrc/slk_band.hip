@@ -698,7 +698,8 @@ __device__ __forceinline__ double rcp_t(double x) { return 1.0 / x; }
 // with the addressing reduced to uniform bases plus ChaseLane offsets.
 template <typename T>
 __device__ __forceinline__ T chase_step_full(typename C2T<T>::t* bnd, typename C2T<T>::t* v2row,
-                                             typename C2T<T>::t* sv, int c1, int q1, int lane, const ChaseLane& o) {
+                                             typename C2T<T>::t* sv, int c1, int q1, int lane, const ChaseLane& o,
+                                             typename C2T<T>::t (*stg)[BW + 1], bool from_stage, bool to_stage) {
   using C2 = typename C2T<T>::t;
   const int li = lane & 15, lg = lane >> 4;
   const C2 zero = mk<T>(0, 0);
@@ -706,11 +707,16 @@ __device__ __forceinline__ T chase_step_full(typename C2T<T>::t* bnd, typename C
   C2* bb = bnd + 31 * (c1 + 1) + q1;      // bulge block
   C2* bd = bnd + 32 * q1;                 // diagonal block
   C2* be = bnd + 32 * q1 + BW;            // block below
-  const C2 x = bx[li];
+  // from_stage: the previous step of this sweep left its updated block below
+  // (this step's column c1 and bulge block) in the wave's stage, not in the band
+  const C2 x = from_stage ? stg[li][0] : bx[li];
   C2 eb[4], ed[4], ee[4];
 #pragma unroll
   for (int t = 0; t < 4; ++t) {
-    eb[t] = load_or_zero(&at_off(bb, cl_ob(o, t)), li < BW - 1);
+    if (from_stage)
+      eb[t] = li < BW - 1 ? stg[lg + 4 * t][1 + li] : zero;
+    else
+      eb[t] = load_or_zero(&at_off(bb, cl_ob(o, t)), li < BW - 1);
     C2 d = at_off(bd, cl_od(o, t));
     if (cl_up(o, t)) d.y = -d.y;
     ed[t] = d;
@@ -779,8 +785,15 @@ __device__ __forceinline__ T chase_step_full(typename C2T<T>::t* bnd, typename C
       if (r == li) e.y = T(0);
       at_off(bd, cl_od(o, t)) = e;
     }
-    at_off(be, cl_oe(o, t)) = c_sub(ee[t], c_mulc(tz, vr[t]));
+    // to_stage: the next step of this sweep (full, same wave) takes the
+    // block below from the stage and writes it back to the band itself
+    const C2 ev = c_sub(ee[t], c_mulc(tz, vr[t]));
+    if (to_stage)
+      stg[li][r] = ev;
+    else
+      at_off(be, cl_oe(o, t)) = ev;
   }
+  __builtin_amdgcn_wave_barrier();
   return beta;
 }
 
@@ -808,6 +821,9 @@ __global__ __launch_bounds__(CH_THREADS) void band_chase_kernel(BandCtx<T> c, un
   __shared__ int s_prog[CH_WAVES];
   __shared__ int s_fail;
   __shared__ C2 s_vw[CH_WAVES][2 * BW];   // per-wave v and w of the current step
+  // global band: per-wave hand-over of a full step's block below to the next
+  // full step of the same sweep (LDS band: not needed)
+  __shared__ C2 s_stage[LDSB ? 1 : CH_WAVES][BW][BW + 1];
   const int b = blockIdx.x;
   const int n = c.n, ld = c.ld;
   C2* a = c.A + (size_t)b * ld * ld;
@@ -829,6 +845,7 @@ __global__ __launch_bounds__(CH_THREADS) void band_chase_kernel(BandCtx<T> c, un
   for (int s = w; s < ns; s += CH_WAVES) {
     const int nst = chase_nsteps(n, s);
     int c1 = 0, q1 = 0, q2 = 0;
+    bool staged = false;   // the previous step left the block below in s_stage
     for (int cs = 0; cs < nst; ++cs) {
       if (s > 0 && ok) {
         const int need = ((s - 1) << 12) + min(cs + 3, 4095);
@@ -852,12 +869,19 @@ __global__ __launch_bounds__(CH_THREADS) void band_chase_kernel(BandCtx<T> c, un
       }
       if (cs == 0 && lane == 0) D[s] = (double)bnd[(size_t)s * BND_LD].x;
       const bool full = cs > 0 && q2 - q1 + 1 == BW && q1 - c1 == BW && q2 + BW <= n - 1;
-      // full steps: offsets precomputed per lane (32-bit, in LDS or global)
+      // full steps: offsets precomputed per lane (32-bit, in LDS or global).
+      // The next step is full too when q1 + 47 <= n - 1 (its q2 + BW): then a
+      // global-band step hands its block below over through the wave's stage.
       T beta;
-      if (full)
-        beta = chase_step_full<T>(bnd, a + (size_t)s * ld, s_vw[w], c1, q1, lane, lo);
-      else
+      if (full) {
+        const bool to_stage = !LDSB && cs + 1 < nst && q1 + 3 * BW - 1 <= n - 1;
+        beta = chase_step_full<T>(bnd, a + (size_t)s * ld, s_vw[w], c1, q1, lane, lo, s_stage[LDSB ? 0 : w],
+                                  staged, to_stage);
+        staged = to_stage;
+      } else {
         beta = chase_step<T, false>(bnd, a + (size_t)s * ld, s_vw[w], n, c1, q1, q2, lane);
+        staged = false;
+      }
       if (cs == 0 && lane == 0) E[s] = (double)beta;
       __hip_atomic_store(&s_prog[w], (s << 12) + (cs + 1 == nst ? 4095 : cs + 1), __ATOMIC_RELEASE,
                          __HIP_MEMORY_SCOPE_WORKGROUP);
