@@ -80,6 +80,7 @@ class ServingEngine:
         # one launch thread per stream keeps each stream's launch order
         self.launchers = [ThreadPoolExecutor(max_workers=1) for _ in range(n_streams)]
         self._pending = [0] * n_streams
+        self._pending_cost = [0.0] * n_streams
         self.max_inflight = max_inflight
         self.budget = hbm_budget_bytes
         self.max_batch = max_batch_matrices
@@ -101,13 +102,25 @@ class ServingEngine:
         # H, S + eigensolver workspace (3 matrices + panels) + eigenvalues
         return int(nmat * ld * ld * 16 * 5.3) + nmat * ld * 8
 
+    @staticmethod
+    def batch_cost(ld: int, nmat: int) -> float:
+        """Relative time a batch holds its stream: the per-matrix chains (chase,
+        panel QRs) grow as ld^2 whatever the batch size, the arithmetic as
+        nmat * ld^3."""
+        return float(ld) * ld + nmat * float(ld) ** 3 / 4096.0
+
     def _submit(self, reqs: List[Request], ld: int) -> Future:
-        """Queue a batch on the least-loaded stream's launch thread."""
-        i = min(range(len(self.streams)), key=lambda q: (self._pending[q], (q - self._rr) % len(self.streams)))
+        """Queue a batch on the stream with the least pending cost, so that a
+        small batch does not wait behind a large one on the same stream."""
+        nmat = sum(len(r.inp.kpoints) for r in reqs)
+        cost = self.batch_cost(ld, nmat)
+        i = min(range(len(self.streams)), key=lambda q: (self._pending_cost[q], (q - self._rr) % len(self.streams)))
         self._rr += 1
         self._pending[i] += 1
+        self._pending_cost[i] += cost
         fut = self.launchers[i].submit(self._launch, reqs, ld, self.streams[i])
         fut.stream_index = i
+        fut.cost = cost
         return fut
 
     def close(self):
@@ -196,6 +209,7 @@ class ServingEngine:
                 b = fut.result()          # re-raises a launch failure
                 if b.event.query():
                     self._pending[fut.stream_index] -= 1
+                    self._pending_cost[fut.stream_index] -= fut.cost
                     b.t_launch = t_sub
                     self._finish(b, results, done_t, time.perf_counter() - t0)
                     inflight.remove(item)
