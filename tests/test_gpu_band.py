@@ -101,3 +101,31 @@ def test_two_stage_padded_batch(precision):
             # it to 4 ulp (FP64) or 2^-36 relative (mixed)
             pad_tol = 1e-12 if precision == "fp64" else 2.0 ** -36 * pad * 2
             assert float(np.abs(w[b, n:] - pad).max()) < pad_tol
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_degenerate_spectrum(precision):
+    """Eigenvalues of multiplicity 1 to 4 (H = Q D Q^H with a random unitary
+    Q, S = I): the Sturm-count grid puts several eigenvalues in one cell and
+    the bisection must still resolve each index to its repeated value."""
+    print(f"\ncase: degenerate spectrum {precision}", flush=True)
+    rng = np.random.default_rng(11)
+    n = 200
+    vals = []
+    lev = -3.0
+    while len(vals) < n:
+        vals += [lev] * int(rng.integers(1, 5))
+        lev += float(rng.uniform(0.01, 0.2))
+    d = np.array(vals[:n])
+    z = rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n))
+    q, _ = np.linalg.qr(z)
+    h = (q * d) @ q.conj().T
+    h = 0.5 * (h + h.conj().T)
+    s = np.eye(n, dtype=complex)
+    w = _solve(h[None], s[None], precision, 2)[0]
+    ref = np.sort(d)
+    scale = float(np.abs(ref).max())
+    tol = FP64_ATOL if precision == "fp64" else MIXED_REL * scale
+    err = float(np.abs(w[:n] - ref).max())
+    print(f"|w - exact| {err:.2e} Ha (tol {tol:.1e})")
+    assert err < tol
