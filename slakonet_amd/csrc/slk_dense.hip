@@ -9,14 +9,17 @@
 #include "slk_sampler.h"
 
 #define GT 64
-#define GK 16
 #define NB SLK_NB
 
 // ---------------------------------------------------------------------------
 // Complex GEMM on MFMA: C = alpha opA(A) opB(B) + beta C.
-// 64x64 output tile per 256-thread workgroup (4 waves, 32x32 each as 2x2
-// 16x16 MFMA blocks), K staged through LDS 16 at a time as separate real and
-// imaginary planes. Complex product = 4 real MFMAs per block:
+// TM x TM output tile per 256-thread workgroup, 4 waves each owning a
+// (TM/2)^2 quadrant as 16x16 MFMA blocks: TM = 64 (2x2 blocks per wave, K
+// staged through LDS 16 at a time) for grids that fill the device, TM = 32
+// (one block per wave, K 32 at a time) for launches with fewer than 256
+// 64x64 tiles, where four times the workgroups shorten the launch. Real and
+// imaginary parts sit in separate LDS planes. Complex product = 4 real MFMAs
+// per block:
 // Re += Ar Br - Ai Bi, Im += Ar Bi + Ai Br.
 // Operand layout of the 16x16x4 MFMAs (f64 and f32): lane l supplies
 // A[l%16][l/16] and B[l/16][l%16]; it receives D[row(l/16, v)][l%16], v < 4.
@@ -45,7 +48,7 @@ template <> struct CT<float> {
 // The next K tile is fetched into registers while the current one is
 // multiplied from LDS (measured faster than fetch-stage-multiply in turn at
 // every size from 64^3 x 8 to 2048^3, FP64 and FP32).
-template <typename T>
+template <typename T, int TM>
 __global__ __launch_bounds__(256) void gemm_mfma_kernel(int M, int N, int K, T alpha,
                                                         const typename CT<T>::c2* __restrict__ A, int lda,
                                                         long long sA, int opA,
@@ -55,46 +58,48 @@ __global__ __launch_bounds__(256) void gemm_mfma_kernel(int M, int N, int K, T a
                                                         unsigned long long* clk) {
   using C2 = typename CT<T>::c2;
   using V4 = typename CT<T>::v4;
-  const int m0 = blockIdx.y * GT, n0 = blockIdx.x * GT;
+  constexpr int KD = 1024 / TM;   // K depth of one LDS stage
+  constexpr int BQ = TM / 32;     // 16x16 blocks per wave, per dimension
+  const int m0 = blockIdx.y * TM, n0 = blockIdx.x * TM;
   sample_start(clk);
-  if (tri && n0 > m0 + GT - 1) {
+  if (tri && n0 > m0 + TM - 1) {
     sample_end(clk);
     return;
   }
-  __shared__ T Ar[GK][GT + 4], Ai[GK][GT + 4], Br[GK][GT + 4], Bi[GK][GT + 4];
+  __shared__ T Ar[KD][TM + 4], Ai[KD][TM + 4], Br[KD][TM + 4], Bi[KD][TM + 4];
   const int b = blockIdx.z;
   const C2* a = A + (size_t)b * sA;
   const C2* bb = B + (size_t)b * sB;
   C2* c = C + (size_t)b * sC;
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-  const int wr = (wv >> 1) * 32, wc = (wv & 1) * 32;
+  const int wr = (wv >> 1) * (TM / 2), wc = (wv & 1) * (TM / 2);
   const int li = lane & 15, lk = lane >> 4;
-  V4 accR[2][2], accI[2][2];
+  V4 accR[BQ][BQ], accI[BQ][BQ];
 #pragma unroll
-  for (int i = 0; i < 2; ++i)
+  for (int i = 0; i < BQ; ++i)
 #pragma unroll
-    for (int j = 0; j < 2; ++j) {
+    for (int j = 0; j < BQ; ++j) {
       accR[i][j] = V4{0, 0, 0, 0};
       accI[i][j] = V4{0, 0, 0, 0};
     }
   int kbeg = 0, kend = K;
-  if (klim & KLIM_A_LOWER) kend = min(kend, m0 + GT);
+  if (klim & KLIM_A_LOWER) kend = min(kend, m0 + TM);
   if (klim & KLIM_A_UPPER) kbeg = max(kbeg, m0);
-  if (klim & KLIM_B_UPPER) kend = min(kend, n0 + GT);
+  if (klim & KLIM_B_UPPER) kend = min(kend, n0 + TM);
   if (klim & KLIM_B_LOWER) kbeg = max(kbeg, n0);
-  kbeg = (kbeg / GK) * GK;
+  kbeg = (kbeg / KD) * KD;
   // element e of a K tile: (mm, kk) of op(A), (kk, nn) of op(B), coalesced
   // along the stored rows; loads are unconditional at clamped indices (the
   // values outside op(A), op(B) are zeroed and conjugated at the LDS store,
   // so that no arithmetic waits on a fetch in flight)
-  constexpr int NL = GT * GK / 256;
+  constexpr int NL = TM * KD / 256;
   const bool ta = opA & OP_T, tb = opB & OP_T;
   C2 pa[NL], pb[NL];
   auto a_idx = [&](int e, int& mm, int& kk) {
-    if (ta) { kk = e / GT; mm = e % GT; } else { mm = e / GK; kk = e % GK; }
+    if (ta) { kk = e / TM; mm = e % TM; } else { mm = e / KD; kk = e % KD; }
   };
   auto b_idx = [&](int e, int& nn, int& kk) {
-    if (tb) { nn = e / GK; kk = e % GK; } else { kk = e / GT; nn = e % GT; }
+    if (tb) { nn = e / KD; kk = e % KD; } else { kk = e / TM; nn = e % TM; }
   };
   auto fetch = [&](int k0) {
 #pragma unroll
@@ -133,28 +138,28 @@ __global__ __launch_bounds__(256) void gemm_mfma_kernel(int M, int N, int K, T a
     }
   };
   if (kbeg < kend) fetch(kbeg);
-  for (int k0 = kbeg; k0 < kend; k0 += GK) {
+  for (int k0 = kbeg; k0 < kend; k0 += KD) {
     stage(k0);
     __syncthreads();
-    if (k0 + GK < kend) fetch(k0 + GK);
+    if (k0 + KD < kend) fetch(k0 + KD);
 #pragma unroll
-    for (int k4 = 0; k4 < GK; k4 += 4) {
-      T ar[2], ai[2], nai[2], br[2], bi[2];
+    for (int k4 = 0; k4 < KD; k4 += 4) {
+      T ar[BQ], ai[BQ], nai[BQ], br[BQ], bi[BQ];
 #pragma unroll
-      for (int i = 0; i < 2; ++i) {
+      for (int i = 0; i < BQ; ++i) {
         ar[i] = Ar[k4 + lk][wr + 16 * i + li];
         ai[i] = Ai[k4 + lk][wr + 16 * i + li];
         nai[i] = -ai[i];
       }
 #pragma unroll
-      for (int j = 0; j < 2; ++j) {
+      for (int j = 0; j < BQ; ++j) {
         br[j] = Br[k4 + lk][wc + 16 * j + li];
         bi[j] = Bi[k4 + lk][wc + 16 * j + li];
       }
 #pragma unroll
-      for (int i = 0; i < 2; ++i)
+      for (int i = 0; i < BQ; ++i)
 #pragma unroll
-        for (int j = 0; j < 2; ++j) {
+        for (int j = 0; j < BQ; ++j) {
           accR[i][j] = CT<T>::mfma(ar[i], br[j], accR[i][j]);
           accR[i][j] = CT<T>::mfma(nai[i], bi[j], accR[i][j]);
           accI[i][j] = CT<T>::mfma(ar[i], bi[j], accI[i][j]);
@@ -164,9 +169,9 @@ __global__ __launch_bounds__(256) void gemm_mfma_kernel(int M, int N, int K, T a
     __syncthreads();
   }
 #pragma unroll
-  for (int i = 0; i < 2; ++i)
+  for (int i = 0; i < BQ; ++i)
 #pragma unroll
-    for (int j = 0; j < 2; ++j)
+    for (int j = 0; j < BQ; ++j)
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
         const int gm = m0 + wr + 16 * i + CT<T>::row(lk, v), gn = n0 + wc + 16 * j + li;
@@ -188,33 +193,44 @@ __global__ __launch_bounds__(256) void gemm_mfma_kernel(int M, int N, int K, T a
   }
 }
 
-// Flops of the executed 64x64 output tiles (edge tiles clipped to M x N,
+// Flops of the executed TM x TM output tiles (edge tiles clipped to M x N,
 // K ranges as limited by klim): 8 real flops per complex multiply-add.
-static double gemm_tile_flops(int batch, int M, int N, int K, int klim, int tri) {
+static double gemm_tile_flops(int batch, int M, int N, int K, int klim, int tri, int tm) {
   if (!klim && !tri) return 8.0 * batch * (double)M * N * K;
   double f = 0.0;
-  for (int m0 = 0; m0 < M; m0 += GT)
-    for (int n0 = 0; n0 < N; n0 += GT) {
-      if (tri && n0 > m0 + GT - 1) continue;
+  for (int m0 = 0; m0 < M; m0 += tm)
+    for (int n0 = 0; n0 < N; n0 += tm) {
+      if (tri && n0 > m0 + tm - 1) continue;
       int kbeg = 0, kend = K;
-      if (klim & KLIM_A_LOWER) kend = min(kend, m0 + GT);
+      if (klim & KLIM_A_LOWER) kend = min(kend, m0 + tm);
       if (klim & KLIM_A_UPPER) kbeg = max(kbeg, m0);
-      if (klim & KLIM_B_UPPER) kend = min(kend, n0 + GT);
+      if (klim & KLIM_B_UPPER) kend = min(kend, n0 + tm);
       if (klim & KLIM_B_LOWER) kbeg = max(kbeg, n0);
-      if (kend > kbeg) f += 8.0 * min(GT, M - m0) * min(GT, N - n0) * (kend - kbeg);
+      if (kend > kbeg) f += 8.0 * min(tm, M - m0) * min(tm, N - n0) * (kend - kbeg);
     }
   return f * batch;
+}
+
+// 64x64 tiles once they fill the device's 256 CUs, 32x32 below.
+static int gemm_tile(int batch, int M, int N, int tri) {
+  const long long tm = (M + GT - 1) / GT, tn = (N + GT - 1) / GT;
+  long long tiles = tm * tn;
+  if (tri) tiles = 0;
+  if (tri)
+    for (long long i = 0; i < tm; ++i) tiles += min(tn, i + 1);
+  return tiles * batch >= 256 ? GT : 32;
 }
 
 void launch_zgemm(hipStream_t st, int batch, int M, int N, int K, double alpha, const double2* A, int lda,
                   long long sA, int opA, const double2* B, int ldb, long long sB, int opB, double beta, double2* C,
                   int ldc, long long sC, int klim, int tri) {
   if (M <= 0 || N <= 0 || batch <= 0) return;
-  dim3 g((N + GT - 1) / GT, (M + GT - 1) / GT, batch);
+  const int tm = gemm_tile(batch, M, N, tri);
+  dim3 g((N + tm - 1) / tm, (M + tm - 1) / tm, batch);
   SampleSlot sl;
   const bool sampled =
-      sampler_enabled() && sampler_take(SLK_SAMPLE_GEMM_F64, gemm_tile_flops(batch, M, N, K, klim, tri), &sl);
-  auto kern = gemm_mfma_kernel<double>;
+      sampler_enabled() && sampler_take(SLK_SAMPLE_GEMM_F64, gemm_tile_flops(batch, M, N, K, klim, tri, tm), &sl);
+  auto kern = tm == GT ? gemm_mfma_kernel<double, GT> : gemm_mfma_kernel<double, 32>;
   if (sampled)
     hipExtLaunchKernelGGL(kern, g, dim3(256), 0, st, sl.e0, sl.e1, 0, M, N, K, alpha, A, lda, sA, opA, B, ldb, sB,
                           opB, beta, C, ldc, sC, klim, tri, sl.clk);
@@ -227,8 +243,9 @@ void launch_cgemm(hipStream_t st, int batch, int M, int N, int K, float alpha, c
                   long long sA, int opA, const float2* B, int ldb, long long sB, int opB, float beta, float2* C,
                   int ldc, long long sC, int klim, int tri) {
   if (M <= 0 || N <= 0 || batch <= 0) return;
-  dim3 g((N + GT - 1) / GT, (M + GT - 1) / GT, batch);
-  auto kern = gemm_mfma_kernel<float>;
+  const int tm = gemm_tile(batch, M, N, tri);
+  dim3 g((N + tm - 1) / tm, (M + tm - 1) / tm, batch);
+  auto kern = tm == GT ? gemm_mfma_kernel<float, GT> : gemm_mfma_kernel<float, 32>;
   hipLaunchKernelGGL(kern, g, dim3(256), 0, st, M, N, K, alpha, A, lda, sA, opA, B, ldb, sB, opB, beta, C, ldc, sC,
                      klim, tri, (unsigned long long*)nullptr);
 }

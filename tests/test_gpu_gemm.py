@@ -1,4 +1,5 @@
-"""MFMA complex GEMM (C-ABI) against numpy: every operand form, ragged sizes."""
+"""MFMA complex GEMM (C-ABI) against numpy: every operand form, ragged sizes, both
+tile shapes (32x32 below 256 64x64 tiles, 64x64 from there: 1030 x 520 x 2)."""
 import ctypes as C
 
 import numpy as np
@@ -20,7 +21,7 @@ def _op(x, op):
 
 @pytest.mark.parametrize("dtype", ["c128", "c64"])
 @pytest.mark.parametrize("opa,opb", [(0, 0), (1, 0), (3, 2), (0, 3), (2, 1)])
-@pytest.mark.parametrize("mnk", [(1, 1, 1), (17, 70, 33), (64, 64, 16), (129, 65, 100)])
+@pytest.mark.parametrize("mnk", [(1, 1, 1), (17, 70, 33), (64, 64, 16), (129, 65, 100), (1030, 520, 40)])
 def test_gemm_matches_numpy(dtype, opa, opb, mnk):
     M, N, K = mnk
     rng = np.random.default_rng(M * 7 + N + K)
