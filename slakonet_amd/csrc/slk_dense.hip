@@ -306,7 +306,9 @@ __global__ __launch_bounds__(LEAF_THREADS) void potrf_diag_kernel(double2* L, do
       if (threadIdx.x == 0 && info[b] == 0) info[b] = j0 + k + 1;
       d = 1.0;
     }
-    const double l = sqrt(d), il = 1.0 / l;
+    // one reciprocal square root per step instead of a square root and an
+    // IEEE division on the step's serial path
+    const double il = rsqrt(d), l = d * il;
     const double2 cc = s_col[p][c], wk = s_row[p][c];
     const double2 lc = make_double2(cc.x * il, cc.y * il);
     const double2 wkn = make_double2(wk.x * il, wk.y * il);
