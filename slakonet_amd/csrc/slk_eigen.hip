@@ -1035,12 +1035,12 @@ static int eigh_impl(const void* H, const void* S, int n, int ld, int batch, dou
   launch_chol_inv(stream, sp, s.B0, s.B1, mixed ? reinterpret_cast<double2*>(s.F2) : s.B2, n, ld, batch, info);
   if (ev) (void)hipEventRecord(ev[1], stream);
   if (ev) (void)hipEventRecord(ev[2], stream);
-  // The band reduction reads only the lower triangle of C = W H W^H. Its
-  // lower tiles need W H only on and below the diagonal tiles (C_ij, j <= i,
-  // sums (W H)_ik conj(W_jk) over k <= j), so both products skip the tiles
-  // above the diagonal (tri = 1) on that path.
+  // Both tridiagonalisations (band and one-stage) read only the lower
+  // triangle of C = W H W^H. Its lower tiles need W H only on and below the
+  // diagonal tiles (C_ij, j <= i, sums (W H)_ik conj(W_jk) over k <= j), so
+  // both products skip the tiles above the diagonal (tri = 1).
   const bool band = use_band(n, V != nullptr);
-  const int tri = band ? 1 : 0;
+  const int tri = 1;
   if (!mixed) {
     // B2 = W H (W lower), B0 = B2 W^H (W^H upper)
     launch_zgemm(stream, batch, n, n, n, 1.0, s.B1, ld, sM, OP_N, h, ld, sM, OP_N, 0.0, s.B2, ld, sM, KLIM_A_LOWER,
