@@ -428,6 +428,7 @@ struct RefineCtx {
   double* part;             // [n_spec * 2][nblk][2]: partial num, den
   int nblk;
   int lds_x;                // 1: refine_tri was launched with ld complex doubles of LDS for x
+  int split;                // 1: refine_tri stops after the tridiagonal solve (refine_q1_kernel follows)
 };
 
 __device__ __forceinline__ double* rf_base(const RefineCtx& c, int v) {
@@ -599,8 +600,9 @@ __global__ __launch_bounds__(64) void refine_tri_kernel(RefineCtx c) {
     for (int i = lane; i < m; i += 64) y[i] *= nr;
     __threadfence_block();
   }
-  if (c.V2 != nullptr) {
-    // two-stage reduction: refine_band_kernel carries y back (x slot of scratch)
+  if (c.V2 != nullptr || c.split) {
+    // two-stage reduction, or one-stage with a large ld: refine_band_kernel or
+    // refine_q1_kernel carries y back (x slot of scratch)
     double2* xg = reinterpret_cast<double2*>(base + 6 * (size_t)ld);
     for (int r = lane; r < m; r += 64) xg[r] = make_double2(y[r], 0.0);
     return;
@@ -811,6 +813,95 @@ __global__ __launch_bounds__(RB_THREADS) void refine_band_kernel(RefineCtx c, co
     __syncthreads();
   }
   for (int r = tid; r < m; r += RB_THREADS) xg[r] = x[r];
+}
+
+// Back-transform after the one-stage reduction for large ld, one 1024-thread
+// workgroup per (structure, edge): x = Q y with the FP32 reflectors (rows of
+// VT, last to first), x -= v tau (v^H x). Thread t keeps rows t + 1024 q
+// (q < NQ) of x in registers; with PF it fetches its elements of the next
+// reflector (and the next tau) while the current one is applied. One barrier
+// per reflector: the wave partials of v^H x alternate between two LDS rows.
+#define RQ_THREADS 1024
+#define RQ_MIN_LD (64 * 40)   // below: refine_tri's one-wave paths
+template <int NQ, bool PF>
+__global__ __launch_bounds__(RQ_THREADS) void refine_q1_kernel(RefineCtx c) {
+  const int s = blockIdx.x, edge = blockIdx.y, v = s * 2 + edge;
+  if (c.meta[2 * v] < 0) return;
+  const size_t mat = (size_t)c.meta[2 * v + 1];
+  const int m = c.spec[s].nb, ld = c.ld, off = c.off;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  __shared__ double2 s_red[2][RQ_THREADS / 64];
+  double2* xg = reinterpret_cast<double2*>(rf_base(c, v) + 6 * (size_t)ld);
+  double2 x[NQ];
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) {
+    const int r = tid + RQ_THREADS * q;
+    x[q] = r < m ? xg[r] : make_double2(0.0, 0.0);
+  }
+  const float2* vt = c.R + mat * ld * ld;
+  const float2* tau = c.TAU + mat * ld;
+  // elements of reflector j outside rows [j + off, m) are zero
+  auto load = [&](int j, float2* dst) {
+    const float2* vr = vt + (size_t)j * ld;
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+      const int r = tid + RQ_THREADS * q;
+      dst[q] = (r >= j + off && r < m) ? vr[r] : make_float2(0.0f, 0.0f);
+    }
+  };
+  float2 vn[NQ];
+  float2 tn = make_float2(0.0f, 0.0f);
+  const int j1 = m - 1 - off;
+  if (PF && j1 >= 0) {
+    load(j1, vn);
+    tn = tau[j1];
+  }
+  int par = 0;
+  for (int j = j1; j >= 0; --j) {
+    float2 vc[NQ];
+    float2 tf;
+    if (PF) {
+#pragma unroll
+      for (int q = 0; q < NQ; ++q) vc[q] = vn[q];
+      tf = tn;
+      if (j > 0) {
+        load(j - 1, vn);
+        tn = tau[j - 1];
+      }
+    } else {
+      load(j, vc);
+      tf = tau[j];
+    }
+    if (tf.x == 0.0f && tf.y == 0.0f) continue;   // uniform: H_j = I
+    double px = 0.0, py = 0.0;
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+      px += (double)vc[q].x * x[q].x + (double)vc[q].y * x[q].y;
+      py += (double)vc[q].x * x[q].y - (double)vc[q].y * x[q].x;
+    }
+    px = wave_sum(px);
+    py = wave_sum(py);
+    if (lane == 0) s_red[par][wv] = make_double2(px, py);
+    __syncthreads();
+    double dx = 0.0, dy = 0.0;
+#pragma unroll
+    for (int w = 0; w < RQ_THREADS / 64; ++w) {
+      dx += s_red[par][w].x;
+      dy += s_red[par][w].y;
+    }
+    par ^= 1;
+    const double sx = (double)tf.x * dx - (double)tf.y * dy, sy = (double)tf.x * dy + (double)tf.y * dx;
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+      x[q].x -= (double)vc[q].x * sx - (double)vc[q].y * sy;
+      x[q].y -= (double)vc[q].x * sy + (double)vc[q].y * sx;
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) {
+    const int r = tid + RQ_THREADS * q;
+    if (r < m) xg[r] = x[q];
+  }
 }
 
 // z[c] = sum_{r >= c} conj(W[r][c]) x[r] for the RF_BLK columns of this block
@@ -1197,7 +1288,8 @@ int slk_refine_edges_mixed(const void* H, const void* S, int n, int ld, int batc
   c.part = scratch + (size_t)n_spec * 2 * RF_SCR * ld;
   c.meta = reinterpret_cast<int*>(c.part + (size_t)n_spec * 2 * c.nblk * 2);
   // x in LDS when every structure fits (m <= ld <= RF_LDS_X), else in the scratch slot
-  c.lds_x = (ld <= RF_LDS_X && !band) ? 1 : 0;
+  c.split = (!band && ld > RQ_MIN_LD) ? 1 : 0;
+  c.lds_x = (ld <= RF_LDS_X && !band && !c.split) ? 1 : 0;
   const size_t lds = c.lds_x ? ld * sizeof(double2) : 0;
   const dim3 g(n_spec, 2);
   if (band) {
@@ -1215,6 +1307,20 @@ int slk_refine_edges_mixed(const void* H, const void* S, int n, int ld, int batc
       hipLaunchKernelGGL(refine_band_kernel<2>, g, dim3(RB_THREADS), lds2, stream, c, TT, npan);
     else
       hipLaunchKernelGGL(refine_band_kernel<3>, g, dim3(RB_THREADS), lds2, stream, c, TT, npan);
+  } else if (c.split) {
+    // y on one wave, then x = Q y on 1024 threads (x in registers: m <= n <= NQ * 1024)
+    static_assert(EIGH_MAX_N <= 16 * RQ_THREADS, "refine_q1_kernel tiers");
+    hipLaunchKernelGGL(refine_tri_kernel<0>, g, dim3(64), 0, stream, c);
+    if (n <= 4 * RQ_THREADS)
+      hipLaunchKernelGGL((refine_q1_kernel<4, true>), g, dim3(RQ_THREADS), 0, stream, c);
+    else if (n <= 8 * RQ_THREADS)
+      hipLaunchKernelGGL((refine_q1_kernel<8, true>), g, dim3(RQ_THREADS), 0, stream, c);
+    else if (n <= 9 * RQ_THREADS)
+      hipLaunchKernelGGL((refine_q1_kernel<9, false>), g, dim3(RQ_THREADS), 0, stream, c);
+    else if (n <= 12 * RQ_THREADS)
+      hipLaunchKernelGGL((refine_q1_kernel<12, false>), g, dim3(RQ_THREADS), 0, stream, c);
+    else
+      hipLaunchKernelGGL((refine_q1_kernel<16, false>), g, dim3(RQ_THREADS), 0, stream, c);
   } else if (ld <= 64 * 4)
     hipLaunchKernelGGL(refine_tri_kernel<4>, g, dim3(64), lds, stream, c);
   else if (ld <= 64 * 8)

@@ -55,3 +55,29 @@ def test_rocksalt_64k_precision_form_vs_oracle():
         assert abs(ed["vbm"] - gap_ref["vbm"]) < EDGE_TOL_EV
         assert abs(ed["cbm"] - gap_ref["cbm"]) < EDGE_TOL_EV
     assert l1 < DOS_L1_REL_TOL
+
+
+@pytest.mark.parametrize("n_atoms,length", [(320, 16.0), (1000, 25.0)])
+def test_one_stage_edge_refinement_large(n_atoms, length):
+    """Above 2560 orbitals the precision form takes the one-stage reduction, and
+    the edge refinement's back-transform runs on 1024 threads per edge (2880
+    orbitals: reflector prefetch; 9000: without). Refined VBM / CBM against the
+    all-FP64 device path on the same H and S (Gamma only)."""
+    st = synth.large_cubic(n_atoms=n_atoms, length=length)
+    print(f"\ncase: cubic cell, {n_atoms} atoms, n = {9 * n_atoms}", flush=True)
+    species = sorted({int(z) for z in st.z})
+    tab = TableSet(synth.make_skf_set(species, seed=5), synth.shell_dict_spd(species), device="cuda")
+    inp = engine.StructureInput(st.z, st.positions * AA_BOHR, st.cell * AA_BOHR, st.kpoints[:1], np.ones(1))
+    plan = engine.plan_build(tab, [inp])
+    H, S = engine.run_build(tab, plan)
+    wsl = []
+    wm, _, _ = engine.eigh(H, S, precision="mixed", workspace=wsl)
+    raw = engine.edges_dict(engine.post(plan, wm.clone())[0][0].cpu().numpy())
+    em, _, _ = engine.post(plan, wm, refine=(H, S, wsl[0]))
+    wf, _, _ = engine.eigh(H, S, precision="fp64")
+    ef, _, _ = engine.post(plan, wf)
+    dm, df = engine.edges_dict(em[0].cpu().numpy()), engine.edges_dict(ef[0].cpu().numpy())
+    err_v, err_c = abs(dm["vbm"] - df["vbm"]), abs(dm["cbm"] - df["cbm"])
+    print(f"VBM err {err_v:.2e} eV, CBM err {err_c:.2e} eV (unrefined {abs(raw['vbm'] - df['vbm']):.2e}, "
+          f"{abs(raw['cbm'] - df['cbm']):.2e})")
+    assert err_v < 1e-8 and err_c < 1e-8
