@@ -352,6 +352,8 @@ __global__ __launch_bounds__(HS_ROWS) void hetrd_sum_kernel(HetrdCtx<T> c, int j
     const int k = tid < jj ? tid : NB + tid - jj;
     const C2* cp = c.C12p + (size_t)b * c.ncmax * 2 * NB;
     C2 acc = mk<T>(0, 0);
+    // unrolled: eight independent loads in flight per step of the (unchanged) sum order
+#pragma unroll 8
     for (int ch = 0; ch < nch; ++ch) acc = c_add(acc, cp[(size_t)ch * 2 * NB + k]);
     s_c12[k] = acc;
   }
@@ -364,6 +366,7 @@ __global__ __launch_bounds__(HS_ROWS) void hetrd_sum_kernel(HetrdCtx<T> c, int j
   const int r = j + 1 + blockIdx.x * HS_ROWS + tid;
   if (r >= n) return;
   C2 s = mk<T>(0, 0);
+#pragma unroll 8
   for (int ch = 0; ch < nch; ++ch) s = c_add(s, yp[(size_t)ch * ld + r]);
   C2 cl = next ? a[(size_t)r * ld + j + 1] : mk<T>(0, 0);
   for (int k = 0; k < jj; ++k) {
