@@ -517,6 +517,8 @@ __global__ __launch_bounds__(256) void band_w_kernel(BandCtx<T> c, int r0, int n
   const int tid = threadIdx.x;
   const int i = tid / BW, kk = tid % BW;
   C2 z = mk<T>(0, 0);
+  // unrolled: eight independent loads in flight per step of the (unchanged) sum order
+#pragma unroll 8
   for (int q = 0; q < nrbk; ++q) z = c_add(z, c.Zp[((size_t)b * c.nrb + q) * BW * BW + tid]);
   s_Z[i][kk] = z;
   s_T[i][kk] = c.TT[((size_t)b * c.npan + (r0 - BW) / BW) * BW * BW + tid];
