@@ -230,7 +230,7 @@ int slk_refine_edges_mixed(const void* H, const void* S, int n, int ld, int batc
  * tridiagonalisation mat-vec (bytes: one triangle of the trailing Hermitian
  * block per matrix), 1 = H/S build (bytes from slk_sampler_hint, per calling
  * thread: the H and S elements written), 2 = FP64 MFMA GEMM (flops of the
- * executed 64x64 output tiles), 3 = band chase (flops of the steps of one
+ * executed output tiles, 64x64 or 32x32), 3 = band chase (flops of the steps of one
  * matrix, i.e. per workgroup).
  * enable = 0 disables and frees. Thread-safe. */
 int slk_sampler_reset(int enable, int every, int capacity);
