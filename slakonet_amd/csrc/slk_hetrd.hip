@@ -259,7 +259,7 @@ __global__ __launch_bounds__(HT_THREADS) void hetrd_step_kernel(HetrdCtx<T> c, i
   const C2 zero = mk<T>(0, 0);
   const C2 vA = cA < n ? v[cA] : zero, vB = cB < n ? v[cB] : zero;
   C2 colA = zero, colB = zero;
-#pragma unroll 4
+#pragma unroll 2
   for (int rr = 0; rr < HT_RPW; ++rr) {
     const int rl = wv * HT_RPW + rr;
     const int r = row0 + rl;
