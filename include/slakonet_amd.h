@@ -119,7 +119,9 @@ int slk_pad_launch(const SlkStructDesc* structs_dev, int s, int nk, int n_orb, i
  * per-k loop of SimpleDftb.__call__ (slakonet/main.py:165-196).
  * w: [batch][wld] ascending; V (nullable): eigenvectors in columns;
  * info: [2 * batch]; info[b] > 0: S not positive definite at that pivot,
- * info[batch + b] > 0: QL iteration did not converge. */
+ * info[batch + b] > 0: the tridiagonal stage failed (QL iteration did not
+ * converge: the sweep index + 1 for vectors; 2: the bulge chase gave up a
+ * bounded wait, eigenvalues NaN). */
 size_t slk_eigh_workspace_bytes(int n, int ld, int batch, int vectors);
 int slk_eigh_f64(const void* H, const void* S, int n, int ld, int batch, double* w, int wld, void* V,
                  void* ws, size_t ws_bytes, int* info, hipStream_t stream);
@@ -141,6 +143,13 @@ int slk_eigh_mixed_timed(const void* H, const void* S, int n, int ld, int batch,
  * (slakonet/utils.py:585-816); the reference exposes no such switch.
  * Returns 1 for an unknown mode. */
 int slk_set_tridiag_mode(int mode);
+/* Polls per wait of the bulge chase (two-stage tridiagonalisation) before a
+ * wave gives up; a negative value gives up at the first wait. A matrix whose
+ * chase gave up gets info[batch + b] = 2 and NaN eigenvalues; the kernel still
+ * runs to its end. Process-wide; returns the previous limit (default 2^22).
+ * No reference counterpart (the reference's eigensolver has no bounded wait);
+ * it lets a caller or test exercise the failure path without a hang. */
+int slk_set_chase_spin_limit(int polls);
 /* Eigenvalues only for n <= 32: the whole solve of one matrix in LDS of one
  * wave64 workgroup, one launch per batch (slk_eigh_f64 dispatches here for
  * values-only solves of that size). info: [batch]. */

@@ -104,15 +104,22 @@ def properties(eig, k_weights, n_electrons, get_fermi=True, fermi_method="bisect
         mu = fermi_search(eig, n_electrons, k_weights)
     else:
         mu = fermi_bisect(eig, n_electrons, k_weights)
-    bg = band_gap(eig, mu)
+    try:
+        bg = band_gap(eig, mu)
+        mu_dict = mu
+    except ValueError:
+        # no occupied or no unoccupied state: the reference's except branch
+        # (main.py:1677-1687) reports a zero Fermi level and zero edges
+        bg = dict(gap=0.0, vbm=0.0, cbm=0.0, direct=True, vbm_kpoint=0, cbm_kpoint=0, metal=False)
+        mu_dict = 0.0
     bs = band_structure(eig, mu)
     grid, d = dos(eig, k_weights, mu)
     fi = int(np.argmin(np.abs(grid)))
     region = np.abs(grid) < 2.0
     gc = grid[region][int(np.argmin(d[region]))]
     p = {
-        "fermi_energy_eV": mu * H2E,
-        "fermi_energy_Ha": mu,
+        "fermi_energy_eV": mu_dict * H2E,
+        "fermi_energy_Ha": mu_dict,
         "band_gap_eV": bg["gap"],
         "vbm_eV": bg["vbm"],
         "cbm_eV": bg["cbm"],

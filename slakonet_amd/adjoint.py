@@ -165,7 +165,9 @@ def attach_table_gradient(calc, tables: TableSet):
 
 def gap_tensor(w: torch.Tensor, edges_row: dict) -> torch.Tensor:
     """Band gap in eV as cbm - vbm of the two eigenvalues the edge kernel picked."""
-    if edges_row["metal"] > 0:
+    if edges_row["metal"] > 0 or edges_row["vbm"] < -1e299 or edges_row["cbm"] > 1e299:
+        # metal, or no occupied / no unoccupied state (the properties dict then
+        # reports a zero gap, slakonet/main.py:1677-1687)
         return torch.tensor(0.0, dtype=torch.float64, device=w.device)
     kv, kc = int(edges_row["vbm_k"]), int(edges_row["cbm_k"])
     wd = w.detach()

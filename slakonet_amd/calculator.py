@@ -178,7 +178,9 @@ class SimpleDftb:
                                           C.c_void_p(jd.data_ptr()), 1, C.c_void_p(grid.data_ptr()), num_points,
                                           float(sigma), C.c_void_p(edges.data_ptr()), C.c_void_p(dos.data_ptr()),
                                           None, _lib.stream_handle()), "slk_dos_launch")
-        self._dos_stats = stats
+        # the statistics (DOS at E_F, gap centre, total) belong to the
+        # Fermi-shifted DOS only
+        self._dos_stats = stats if fermi_shift else None
         return grid, dos[0]
 
     def calculate_atom_projected_dos(self, sigma=0.1, energy_range=(-8, 6), num_points=1000):
@@ -235,6 +237,8 @@ class SimpleDftb:
         return self._forces
 
     def _calculate_electronic_energy(self):
+        if self.eigenvalue is None:
+            self()                          # slakonet/main.py:1433-1435
         mu = self.get_fermi_energy()
         kT = 0.025 / H2E
         occ = 1.0 / (1.0 + torch.exp((self.eigenvalue - mu.view(1, 1, 1)) / kT))

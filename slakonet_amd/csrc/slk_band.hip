@@ -32,6 +32,7 @@
 #include "slk_dense.h"
 #include <hip/hip_ext.h>
 
+#include <atomic>
 #include <mutex>
 #include <unordered_map>
 
@@ -59,6 +60,8 @@ struct BandCtx {
   C2* BND;       // [batch][ld][BND_LD]: compact band for the chase
   double *D, *E;
   int nch, nrb;  // chunk / row-block counts used for the strides of Yp, Zp
+  int* info;     // nullable [batch]: set to 2 when the chase gave up a bounded wait
+  int spin_max;  // polls per wait before giving up; < 0: give up at the first wait
 };
 
 template <typename T> struct MF;
@@ -811,10 +814,11 @@ __device__ __forceinline__ int chase_nsteps(int n, int s) {
 // each wave publishes its progress (sweep << 12 | completed steps, 4095 when
 // the sweep is done) in LDS with release semantics and waits on its
 // predecessor's counter with acquire semantics. A wait is bounded: past
-// CH_SPIN_MAX polls the kernel stops waiting and marks the result NaN.
+// c.spin_max polls (set by the host, slk_set_chase_spin_limit) the wave stops
+// waiting, the matrix is flagged in c.info (2) and its tridiagonal is set to
+// NaN; the kernel still runs to its end.
 // LDSB: the compact band lives in LDS (dynamic shared memory); otherwise in
 // global memory.
-#define CH_SPIN_MAX (1 << 22)
 template <typename T, bool LDSB>
 __global__ __launch_bounds__(CH_THREADS) void band_chase_kernel(BandCtx<T> c, unsigned long long* clk) {
   using C2 = typename C2T<T>::t;
@@ -852,9 +856,10 @@ __global__ __launch_bounds__(CH_THREADS) void band_chase_kernel(BandCtx<T> c, un
       if (s > 0 && ok) {
         const int need = ((s - 1) << 12) + min(cs + 3, 4095);
         int spins = 0;
-        while (__hip_atomic_load(&s_prog[pw], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP) < need) {
+        if (c.spin_max < 0) ok = false;
+        while (ok && __hip_atomic_load(&s_prog[pw], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP) < need) {
           __builtin_amdgcn_s_sleep(1);
-          if (++spins > CH_SPIN_MAX) {
+          if (++spins > c.spin_max) {
             ok = false;
             break;
           }
@@ -894,8 +899,10 @@ __global__ __launch_bounds__(CH_THREADS) void band_chase_kernel(BandCtx<T> c, un
   if (tid == 0) {
     D[n - 1] = (double)bnd[(size_t)(n - 1) * BND_LD].x;
     E[n - 1] = 0.0;
-    if (s_fail) D[0] = __builtin_nan("");
+    if (s_fail && c.info) c.info[b] = 2;
   }
+  if (s_fail)   // the tridiagonal is unordered garbage: NaN it so no caller takes it as a result
+    for (int i = tid; i < n; i += CH_THREADS) D[i] = __builtin_nan("");
   sample_end(clk);
 }
 
@@ -1173,12 +1180,17 @@ static void launch_panel_qr(hipStream_t st, const BandCtx<T>& c, int batch, int 
   else hipLaunchKernelGGL((panel_qr_kernel<T, 0>), g, blk, 0, st, c, j0);
 }
 
+// Polls per chase wait before the wave gives up (process-wide).
+static std::atomic<int> g_chase_spin_max{1 << 22};
+
 template <typename T>
 static void band_impl(hipStream_t st, typename C2T<T>::t* A, int n, int ld, int batch, double* D, double* E,
                       typename C2T<T>::t* TAU, typename C2T<T>::t* Vc, typename C2T<T>::t* Wc,
-                      typename C2T<T>::t* Yp, typename C2T<T>::t* scratch, typename C2T<T>::t* VT) {
+                      typename C2T<T>::t* Yp, typename C2T<T>::t* scratch, typename C2T<T>::t* VT, int* info) {
   using C2 = typename C2T<T>::t;
   BandCtx<T> c;
+  c.info = info;
+  c.spin_max = g_chase_spin_max.load();
   c.A = A;
   c.n = n;
   c.ld = ld;
@@ -1254,11 +1266,15 @@ static void band_impl(hipStream_t st, typename C2T<T>::t* A, int n, int ld, int 
 }
 
 void launch_band_tridiag(hipStream_t st, double2* C, int n, int ld, int batch, double* D, double* E, double2* TAU,
-                         double2* Vc, double2* Wc, double2* Yp, double2* scratch, double2* VT) {
-  band_impl<double>(st, C, n, ld, batch, D, E, TAU, Vc, Wc, Yp, scratch, VT);
+                         double2* Vc, double2* Wc, double2* Yp, double2* scratch, double2* VT, int* info) {
+  band_impl<double>(st, C, n, ld, batch, D, E, TAU, Vc, Wc, Yp, scratch, VT, info);
 }
 
 void launch_band_tridiag_f32(hipStream_t st, float2* C, int n, int ld, int batch, double* D, double* E, float2* TAU,
-                             float2* Vc, float2* Wc, float2* Yp, float2* scratch, float2* VT) {
-  band_impl<float>(st, C, n, ld, batch, D, E, TAU, Vc, Wc, Yp, scratch, VT);
+                             float2* Vc, float2* Wc, float2* Yp, float2* scratch, float2* VT, int* info) {
+  band_impl<float>(st, C, n, ld, batch, D, E, TAU, Vc, Wc, Yp, scratch, VT, info);
+}
+
+extern "C" int slk_set_chase_spin_limit(int polls) {
+  return g_chase_spin_max.exchange(polls);
 }

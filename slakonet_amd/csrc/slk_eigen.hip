@@ -58,12 +58,13 @@ __device__ __forceinline__ int sturm_count(const double* d, const double* e2, in
 // agree between kernels.
 struct TriInterval {
   double lo, hi, piv;
+  bool bad;   // some d or e is not finite (e.g. a chase that gave up): no eigenvalues
 };
 template <bool LDS>
 __device__ TriInterval tri_interval(const double* d, const double* e, int n, double* sd, double* se) {
-  __shared__ double s_red[3][EIG_THREADS / 64];
+  __shared__ double s_red[4][EIG_THREADS / 64];
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-  double lo = 1e300, hi = -1e300, emax = 0.0;
+  double lo = 1e300, hi = -1e300, emax = 0.0, bad = 0.0;
   for (int i = tid; i < n; i += blockDim.x) {
     const double di = d[i];
     const double ei = i < n - 1 ? e[i] : 0.0;
@@ -76,23 +77,27 @@ __device__ TriInterval tri_interval(const double* d, const double* e, int n, dou
     lo = fmin(lo, di - r);
     hi = fmax(hi, di + r);
     emax = fmax(emax, ei * ei);
+    if (!isfinite(di) || !isfinite(ei)) bad = 1.0;
   }
   lo = wave_min(lo);
   hi = wave_max(hi);
   emax = wave_max(emax);
-  if (lane == 0) { s_red[0][wv] = lo; s_red[1][wv] = hi; s_red[2][wv] = emax; }
+  bad = wave_max(bad);
+  if (lane == 0) { s_red[0][wv] = lo; s_red[1][wv] = hi; s_red[2][wv] = emax; s_red[3][wv] = bad; }
   __syncthreads();
-  lo = 1e300; hi = -1e300; emax = 0.0;
+  lo = 1e300; hi = -1e300; emax = 0.0; bad = 0.0;
   for (int q = 0; q < EIG_THREADS / 64; ++q) {
     lo = fmin(lo, s_red[0][q]);
     hi = fmax(hi, s_red[1][q]);
     emax = fmax(emax, s_red[2][q]);
+    bad = fmax(bad, s_red[3][q]);
   }
   const double span = fmax(fabs(lo), fabs(hi));
   TriInterval t;
   t.lo = lo - 2.0 * 2.2e-16 * span * n - 1e-300;
   t.hi = hi + 2.0 * 2.2e-16 * span * n + 1e-300;
   t.piv = fmax(2.2e-308, 2.2e-308 * emax) * 1e2;
+  t.bad = bad > 0.0;
   return t;
 }
 
@@ -149,6 +154,10 @@ __global__ __launch_bounds__(EIG_THREADS) void bisect_kernel(const double* D, co
   const int grp = lane / G, sub = lane % G;
   const int idx = (blockIdx.x * (EIG_THREADS / 64) + wv) * per_wave + grp;
   if ((blockIdx.x * (EIG_THREADS / 64) + wv) * per_wave >= n) return;   // whole wave idle
+  if (t.bad) {   // uniform over the workgroup
+    if (sub == 0 && idx < n) Wout[(size_t)b * wld + idx] = __builtin_nan("");
+    return;
+  }
   double lo = t.lo, hi = t.hi;
   const double piv = t.piv;
   if (idx < n) {
@@ -1141,7 +1150,8 @@ static int eigh_impl(const void* H, const void* S, int n, int ld, int batch, dou
     if (ev) (void)hipEventRecord(ev[3], stream);
     if (band)
       // B2 (W H) is free after the reduction: it holds the split-K partials
-      launch_band_tridiag(stream, s.B0, n, ld, batch, s.D, s.E, s.TAU, s.Vc, s.Wc, s.B2, s.Band, nullptr);
+      launch_band_tridiag(stream, s.B0, n, ld, batch, s.D, s.E, s.TAU, s.Vc, s.Wc, s.B2, s.Band, nullptr,
+                          info + batch);
     else
       launch_hetrd_blocked(stream, s.B0, n, ld, batch, s.D, s.E, s.TAU, s.Vc, s.Wc, s.Y, s.Yp, s.C12);
   } else {
@@ -1155,7 +1165,8 @@ static int eigh_impl(const void* H, const void* S, int n, int ld, int batch, dou
     // F0 (W in FP32) is free after the reduction: it receives the reflector
     // rows; F2 (W32 H32) holds the split-K partials of the band reduction
     if (band)
-      launch_band_tridiag_f32(stream, s.F1, n, ld, batch, s.D, s.E, s.fTAU, s.fVc, s.fWc, s.F2, s.fBand, s.F0);
+      launch_band_tridiag_f32(stream, s.F1, n, ld, batch, s.D, s.E, s.fTAU, s.fVc, s.fWc, s.F2, s.fBand, s.F0,
+                              info + batch);
     else
       launch_hetrd_blocked_f32(stream, s.F1, n, ld, batch, s.D, s.E, s.fTAU, s.fVc, s.fWc, s.fY, s.fYp, s.fC12,
                                s.F0);

@@ -244,7 +244,14 @@ def check_info(info: torch.Tensor, batch: int, k_offset: int = 0):
                            f"(Cholesky pivot {int(inf[k])})")
     bad = np.nonzero(inf[batch:2 * batch])[0]
     if len(bad):
-        raise EigenFailure(f"tridiagonal QL iteration did not converge at k-point index {int(bad[0]) + k_offset}")
+        raise EigenFailure(tridiag_failure(int(inf[batch + bad[0]]), int(bad[0]) + k_offset))
+
+
+def tridiag_failure(code: int, k: int) -> str:
+    """Message for a non-zero info[batch + b] of the eigensolver."""
+    if code == 2:
+        return f"bulge chase of the tridiagonalisation gave up a bounded wait at k-point index {k}"
+    return f"tridiagonal QL iteration did not converge at k-point index {k}"
 
 
 def spectra(plan: BuildPlan, w_ld: int) -> np.ndarray:

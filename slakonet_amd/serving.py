@@ -28,7 +28,7 @@ from typing import Dict, List, Optional, Sequence
 import numpy as np
 import torch
 
-from . import engine
+from . import _lib, engine
 from .tables import TableSet
 
 
@@ -91,16 +91,29 @@ class ServingEngine:
         self.bucket = bucket
         self.nf = len(engine.EDGE_FIELDS)
         self._rr = 0
+        self._bytes_cache = {}
 
     # ------------------------------------------------------------------
     def _orbitals(self, inp: engine.StructureInput) -> int:
         sp = self.tables.species_index(inp.z)
         return int(self.tables.norb[sp].sum())
 
-    @staticmethod
-    def batch_bytes(ld: int, nmat: int) -> int:
-        # H, S + eigensolver workspace (3 matrices + panels) + eigenvalues
-        return int(nmat * ld * ld * 16 * 5.3) + nmat * ld * 8
+    def batch_bytes(self, ld: int, nmat: int) -> int:
+        """Device bytes a batch holds while in flight: H and S, the exact
+        eigensolver workspace (slk_eigh_*_workspace_bytes), eigenvalues, the
+        band-edge refinement scratch and the per-structure outputs."""
+        key = (ld, nmat)
+        got = self._bytes_cache.get(key)
+        if got is None:
+            lib = _lib.host_lib()
+            ws = (lib.slk_eigh_mixed_workspace_bytes(ld, ld, nmat) if self.precision == "mixed" else
+                  lib.slk_eigh_workspace_bytes(ld, ld, nmat, 0))
+            ref = lib.slk_refine_scratch_bytes(ld, nmat) if self.precision == "mixed" else 0
+            ne = 0 if self.grid is None else int(self.grid.numel())
+            got = 2 * nmat * ld * ld * 16 + int(ws) + int(ref) + nmat * ld * 8 + 8 * nmat + \
+                nmat * (ne + 3 + len(engine.EDGE_FIELDS)) * 8
+            self._bytes_cache[key] = got
+        return got
 
     @staticmethod
     def batch_cost(ld: int, nmat: int) -> float:
@@ -121,6 +134,7 @@ class ServingEngine:
         fut = self.launchers[i].submit(self._launch, reqs, ld, self.streams[i])
         fut.stream_index = i
         fut.cost = cost
+        fut.reqs = reqs
         return fut
 
     def close(self):
@@ -162,11 +176,14 @@ class ServingEngine:
         for s, r in enumerate(b.reqs):
             nk = len(r.inp.kpoints)
             bad = np.nonzero(info[m0:m0 + nk])[0]
+            bad_t = np.nonzero(info[nmat + m0:nmat + m0 + nk])[0]
             res = engine.edges_dict(edges[s])
             res["gap_eV"] = 0.0 if res["metal"] > 0 else res["cbm"] - res["vbm"]
-            res["ok"] = len(bad) == 0
+            res["ok"] = len(bad) == 0 and len(bad_t) == 0
             if len(bad):
                 res["error"] = f"S(k) not positive definite at k-point index {int(bad[0])}"
+            elif len(bad_t):
+                res["error"] = engine.tridiag_failure(int(info[nmat + m0 + bad_t[0]]), int(bad_t[0]))
             if stats is not None:
                 res["dos_at_fermi"], res["dos_gap_center_eV"], res["dos_total_states"] = stats[s]
             if self.keep_outputs:
@@ -206,7 +223,19 @@ class ServingEngine:
                 fut, nbytes, t_sub = item
                 if not fut.done():
                     continue
-                b = fut.result()          # re-raises a launch failure
+                try:
+                    b = fut.result()
+                except Exception as exc:   # this batch failed to launch: fail its requests only
+                    self._pending[fut.stream_index] -= 1
+                    self._pending_cost[fut.stream_index] -= fut.cost
+                    now_f = time.perf_counter() - t0
+                    for r in fut.reqs:
+                        results[r.idx] = {"ok": False, "error": f"{type(exc).__name__}: {exc}"}
+                        done_t[r.idx] = now_f
+                    inflight.remove(item)
+                    used -= nbytes
+                    progressed = True
+                    continue
                 if b.event.query():
                     self._pending[fut.stream_index] -= 1
                     self._pending_cost[fut.stream_index] -= fut.cost
@@ -226,8 +255,11 @@ class ServingEngine:
                 nmat = 0
                 while q and len(take) < self.max_batch:
                     nk = len(q[0].inp.kpoints)
-                    if take and (nmat + nk > self.max_batch or
-                                 used + self.batch_bytes(ld, nmat + nk) > self.budget):
+                    if take and nmat + nk > self.max_batch:
+                        break
+                    # the budget holds for the first request too while other
+                    # batches are in flight (one alone on an idle device runs)
+                    if (take or inflight) and used + self.batch_bytes(ld, nmat + nk) > self.budget:
                         break
                     take.append(q.popleft())
                     nmat += nk

@@ -35,8 +35,10 @@ class Skf:
         self.hamiltonian = {_key(k): _t(v) for k, v in hamiltonian.items()}
         self.overlap = {_key(k): _t(v) for k, v in overlap.items()}
         self.grid = _t(grid)
-        if not (self.grid.diff().diff().abs() < 1e-5).all():
-            raise ValueError("Electronic integral grid spacing is not uniform")
+        steps = self.grid[1:] - self.grid[:-1]
+        if len(steps) and float((steps - steps[0]).abs().max()) >= 1e-5:
+            raise ValueError(f"SK table grid must be uniform (spacings vary from {float(steps.min())} "
+                             f"to {float(steps.max())} Bohr)")
         self.hs_cutoff = hs_cut
         check = [v is not None for v in (on_sites, occupations)]
         if any(check) and not all(check):

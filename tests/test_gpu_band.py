@@ -129,3 +129,29 @@ def test_degenerate_spectrum(precision):
     err = float(np.abs(w[:n] - ref).max())
     print(f"|w - exact| {err:.2e} Ha (tol {tol:.1e})")
     assert err < tol
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_chase_give_up_is_reported(precision):
+    """A bulge chase that gives up its bounded wait (forced from the host with
+    a negative spin limit; every kernel still runs to its end, the waves
+    merely stop waiting) flags the matrix with info[batch + b] = 2 and NaN
+    eigenvalues, and the binding raises instead of returning them."""
+    print(f"\ncase: chase give-up {precision}", flush=True)
+    lib = _lib.lib()
+    h, s = _problem(200, 2, seed=7)
+    H = torch.from_numpy(h).to("cuda")
+    S = torch.from_numpy(s).to("cuda")
+    prev = lib.slk_set_chase_spin_limit(-1)
+    try:
+        w, _, info = engine.eigh(H, S, precision=precision, check=False)
+        torch.cuda.synchronize()
+        inf = info.cpu().numpy()
+        assert (inf[:2] == 0).all() and (inf[2:] == 2).all()
+        assert torch.isnan(w[:, :200]).all()
+        with pytest.raises(engine.EigenFailure, match="bulge chase"):
+            engine.eigh(H, S, precision=precision)
+    finally:
+        lib.slk_set_chase_spin_limit(prev)
+    w, _, info = engine.eigh(H, S, precision=precision)
+    assert np.abs(w.cpu().numpy() - _lapack(h, s)).max() < (FP64_ATOL if precision == "fp64" else 1e-5)

@@ -129,3 +129,27 @@ def test_rocksalt_576_refined_edges():
     assert raw_err < 1e-3
     for key in ("vbm", "cbm"):
         assert abs(ref[key] - e64[key]) < REFINED_TOL_EV, (key, ref[key], e64[key])
+
+
+def test_failed_requests_are_reported_not_ok():
+    """A chase forced to give up its wait (negative spin limit) fails the
+    requests of its batches; the others (n <= 32, solved without a chase) stay
+    ok, and the replay finishes."""
+    from slakonet_amd import _lib
+    tab, inputs = _stream(10, seed=43, n_max=24)
+    grid = torch.linspace(-20.0, 10.0, 2001, dtype=torch.float64, device="cuda")
+    eng = ServingEngine(tab, dos_grid=grid, precision="mixed")
+    lib = _lib.lib()
+    prev = lib.slk_set_chase_spin_limit(-1)
+    try:
+        out = eng.replay(inputs, np.zeros(len(inputs)))
+    finally:
+        lib.slk_set_chase_spin_limit(prev)
+    n_orb = [eng._orbitals(i) for i in inputs]
+    assert any(n > 32 for n in n_orb) and any(n <= 32 for n in n_orb)
+    for i, n in enumerate(n_orb):
+        res = out["results"][i]
+        if bucket_dim(n) > 32:
+            assert not res["ok"] and "bulge chase" in res["error"], (i, n, res)
+        else:
+            assert res["ok"], (i, n)

@@ -218,3 +218,48 @@ def test_kpts_to_klines_pairs():
     assert np.array_equal(kl[0], [0, 0, 0, 0.5, 0, 0, 7]) and np.array_equal(kl[1], [0.5, 0.5, 0, 0, 0, 0, 7])
     k, w = klines_points(kl)
     assert len(k) == 14 and np.allclose(k[6], [0.5, 0, 0]) and np.allclose(w, 1 / 14)
+
+
+def test_skf_parameter_optimizer_from_path_and_clamp_rule(tmp_path):
+    """SkfParameterOptimizer(skf_path) (slakonet/optim.py:1356-1363) reads the
+    .skf file; apply_constraints (:1392-1434) clamps H to [c, 2 - c] x original
+    with c = 0.9 (l1 == l2) / 0.7 (l1 != l2), S to [max(0.95 x, 0.1), 1.05 x]
+    (l1 == l2) / [0.9, 1.1] x original."""
+    import torch
+    from slakonet_amd import skf_io
+    from slakonet_amd.optim import SkfParameterOptimizer
+    d = synth.make_skf(14, 14, seed=0)
+    path = tmp_path / "Si-Si.skf"
+    skf_io.write_skf(str(path), d)
+    opt = SkfParameterOptimizer(str(path))
+    assert set(opt.h_params) == {"0-0", "0-1", "0-2", "1-1", "1-2", "2-2"}
+    for key, v in d["hamiltonian"].items():
+        assert torch.allclose(opt.h_params[key].detach(), torch.as_tensor(np.asarray(v)), rtol=1e-14, atol=0)
+    assert SkfParameterOptimizer(d).h_params.keys() == opt.h_params.keys()
+    with torch.no_grad():
+        for p in list(opt.h_params.values()) + list(opt.s_params.values()):
+            p.mul_(3.0)
+    opt.apply_constraints()
+    for key, p in opt.h_params.items():
+        o = opt.original_h_params[key]
+        c = 0.9 if key[0] == key[2] else 0.7
+        # original H integrals are negative: 3 x original lies below the bounds
+        # [c o, (2 - c) o] -> swapped by sign; torch.clamp returns max(min(x, hi), lo)
+        want = torch.clamp(3.0 * o, o * c, o * (2 - c))
+        assert torch.equal(p.detach(), want), key
+        assert (p.detach() <= 0).all()
+    for key, p in opt.s_params.items():
+        o = opt.original_s_params[key]
+        if key[0] == key[2]:
+            want = torch.minimum(torch.maximum(3.0 * o, torch.maximum(0.95 * o, torch.tensor(0.1))), 1.05 * o)
+        else:
+            want = 1.1 * o
+        assert torch.allclose(p.detach(), want, rtol=1e-15, atol=0), key
+    with torch.no_grad():
+        for p in opt.s_params.values():
+            p.zero_()
+    opt.apply_constraints()
+    for key, p in opt.s_params.items():
+        o = opt.original_s_params[key]
+        lo = torch.maximum(0.95 * o, torch.tensor(0.1)) if key[0] == key[2] else 0.9 * o
+        assert torch.allclose(p.detach(), torch.minimum(lo, 1.05 * o if key[0] == key[2] else 1.1 * o)), key
