@@ -118,7 +118,7 @@ int slk_pad_launch(const SlkStructDesc* structs_dev, int s, int nk, int n_orb, i
  * Replaces eighb(h, s, scheme="chol") (slakonet/utils.py:585-816) and the
  * per-k loop of SimpleDftb.__call__ (slakonet/main.py:165-196).
  * w: [batch][wld] ascending; V (nullable): eigenvectors in columns;
- * info: [2 * batch]; info[b] > 0: S not positive definite at that pivot,
+ * info: [2 * batch], cleared by the call; info[b] > 0: S not positive definite at that pivot,
  * info[batch + b] > 0: the tridiagonal stage failed (QL iteration did not
  * converge: the sweep index + 1 for vectors; 2: the bulge chase gave up a
  * bounded wait, eigenvalues NaN). */
@@ -134,6 +134,18 @@ int slk_eigh_mixed(const void* H, const void* S, int n, int ld, int batch, doubl
                    size_t ws_bytes, int* info, hipStream_t stream);
 int slk_eigh_mixed_timed(const void* H, const void* S, int n, int ld, int batch, double* w, int wld, void* ws,
                          size_t ws_bytes, int* info, hipStream_t stream, float* stage_ms);
+/* slk_eigh_mixed without a workspace argument and without any allocation:
+ * the workspace is a static device array of slk_noalloc_scratch_bytes()
+ * (64 MiB) per GPU, held from the library's load until the process ends.
+ * Returns 4 when slk_eigh_mixed_workspace_bytes(n, ld, batch) exceeds it
+ * (e.g. one 1024 x 1024 matrix or three 576 x 576 fit). Calls that overlap
+ * on one GPU share the array and race: the caller serialises them. info is
+ * cleared by the call. No counterpart in the reference; it is for callers
+ * that cannot allocate device memory (HIP graph capture, allocation-free
+ * serving loops). */
+size_t slk_noalloc_scratch_bytes(void);
+int slk_eigh_mixed_noalloc(const void* H, const void* S, int n, int ld, int batch, double* w, int wld, int* info,
+                           hipStream_t stream);
 /* Tridiagonalisation used by the eigenvalue-only solves (slk_eigh_f64 with
  * V null, slk_eigh_mixed, and the matching slk_refine_edges_mixed): 0
  * automatic (two-stage: dense -> band of width 16 -> tridiagonal by bulge

@@ -80,6 +80,9 @@ SIGNATURES = {
                                  C.c_void_p, C.c_size_t, C.c_void_p, C.c_void_p]),
     "slk_eigh_mixed_timed": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_int,
                                        C.c_void_p, C.c_size_t, C.c_void_p, C.c_void_p, C.POINTER(C.c_float)]),
+    "slk_noalloc_scratch_bytes": (C.c_size_t, []),
+    "slk_eigh_mixed_noalloc": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_int,
+                                         C.c_void_p, C.c_void_p]),
     "slk_set_tridiag_mode": (C.c_int, [C.c_int]),
     "slk_set_chase_spin_limit": (C.c_int, [C.c_int]),
     "slk_eigh_small_f64": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_int,

@@ -1101,6 +1101,12 @@ static bool use_band(int n, int vectors) {
   return g_tridiag_mode == 2 || n <= BAND_AUTO_MAX;
 }
 
+// info[0 .. n) = 0: the solver's failure flags start cleared (a fill kernel,
+// so a captured graph replays it too).
+__global__ void zero_info_kernel(int* info, int n) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) info[i] = 0;
+}
+
 // Stage boundaries for optional timing: potrf | trinv | reduce (2 GEMMs) |
 // hetrd | tridiagonal solve | back-transform. ev (nullable) holds 7 events.
 // mixed: FP64 Cholesky and inverse, FP32 reduction and tridiagonalisation,
@@ -1115,6 +1121,7 @@ static int eigh_impl(const void* H, const void* S, int n, int ld, int batch, dou
   // values: any n up to EIGH_MAX_N (LDS-staged Sturm counts up to 4096, global beyond);
   // vectors: the QL sweep keeps 4n doubles in LDS (n <= 2048)
   if (n > EIGH_MAX_N || (V != nullptr && n > 2048)) return 3;
+  hipLaunchKernelGGL(zero_info_kernel, dim3((2 * batch + 255) / 256), dim3(256), 0, stream, info, 2 * batch);
   if (V == nullptr && n <= 32) {
     // whole solve in LDS, one launch (stages collapse into the first interval)
     if (ev) (void)hipEventRecord(ev[0], stream);
