@@ -252,15 +252,21 @@ int slk_refine_edges_mixed(const void* H, const void* S, int n, int ld, int batc
  * block per matrix), 1 = H/S build (bytes from slk_sampler_hint, per calling
  * thread: the H and S elements written), 2 = FP64 MFMA GEMM (flops of the
  * executed output tiles, 64x64 or 32x32), 3 = band chase (flops of the steps of one
- * matrix, i.e. per workgroup).
+ * matrix, i.e. per workgroup), 4 = FP32 fused stage-1 update and next A22 V
+ * (flops: 8 (m^2/2) 32 + 8 m^2 16 per matrix of trailing size m), 5 = DOS
+ * (bytes from slk_sampler_hint: eigenvalues, k-weights, grid read and DOS
+ * written; auxiliary amount: Gaussians of the full sum, nk nb n_E).
  * enable = 0 disables and frees. Thread-safe. */
 int slk_sampler_reset(int enable, int every, int capacity);
-int slk_sampler_hint(int kind, double bytes);
-/* out[0] = sampled launches, out[1] = their summed execution-window
+/* Amount (and auxiliary amount) of the calling thread's next launch of a
+ * hinted kind (1, 5). */
+int slk_sampler_hint(int kind, double bytes, double aux);
+/* out[9]: out[0] = sampled launches, out[1] = their summed execution-window
  * milliseconds, out[2] = their summed algorithmic amount, out[3] = launches
  * counted, out[4] = amount of all counted launches, out[5] = summed event-pair
- * milliseconds of the sampled launches, out[6] = wall clock rate (kHz);
- * waits for the sampled launches to complete. */
+ * milliseconds of the sampled launches, out[6] = wall clock rate (kHz),
+ * out[7] = summed auxiliary amount of the sampled launches, out[8] = of all
+ * counted launches; waits for the sampled launches to complete. */
 int slk_sampler_read(int kind, double* out);
 
 /* Atom-type projected DOS from the eigenvectors of one structure (V: the

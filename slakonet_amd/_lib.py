@@ -110,7 +110,7 @@ SIGNATURES = {
                                   C.c_void_p, C.c_int, C.c_double, C.c_double, C.c_double, C.c_void_p, C.c_void_p,
                                   C.c_void_p]),
     "slk_sampler_reset": (C.c_int, [C.c_int, C.c_int, C.c_int]),
-    "slk_sampler_hint": (C.c_int, [C.c_int, C.c_double]),
+    "slk_sampler_hint": (C.c_int, [C.c_int, C.c_double, C.c_double]),
     "slk_sampler_read": (C.c_int, [C.c_int, C.c_void_p]),
     "slk_dos_adjoint_launch": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_int,
                                          C.c_double, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),

@@ -1022,11 +1022,13 @@ __global__ __launch_bounds__(256) void fused_update_av_kernel(const typename C2T
                                                               const typename C2T<T>::t* __restrict__ Vn,
                                                               typename C2T<T>::t* __restrict__ A,
                                                               typename C2T<T>::t* __restrict__ Yp, int n, int ld,
-                                                              int r0n, int nch, long long sP, long long sM) {
+                                                              int r0n, int nch, long long sP, long long sM,
+                                                              unsigned long long* clk) {
   using C2 = typename C2T<T>::t;
   using V4 = typename MF<T>::v4;
   constexpr int KK = 2 * BW / 4;
   const int I = blockIdx.y, J = blockIdx.x;
+  sample_start(clk);
   if (J > I) return;
   const int mn = n - r0n;
   const int b = blockIdx.z;
@@ -1155,6 +1157,10 @@ __global__ __launch_bounds__(256) void fused_update_av_kernel(const typename C2T
       if (r < mn) yp[(size_t)(r0n + r) * BW + li] = mk<T>(yR[v], yI[v]);
     }
   }
+  if (clk != nullptr) {
+    __syncthreads();
+    sample_end(clk);
+  }
 }
 
 #define CH_LDS_MAX (150 * 1024)
@@ -1236,8 +1242,18 @@ static void band_impl(hipStream_t st, typename C2T<T>::t* A, int n, int ld, int 
       if (mn < 1) break;
       launch_panel_qr<T>(st, cn, batch, r0, mn);
       const int t = (mn + 63) / 64;
-      hipLaunchKernelGGL(fused_update_av_kernel<T>, dim3(t, t, batch), dim3(256), 0, st, cp.Vc + r0n, cp.Wc + r0n,
-                         cn.Vc + r0n, A, c.Yp, n, ld, r0n, c.nch, sP, sM);
+      // FP32 launches are sampled: 8 (m^2/2) 32 flops for the rank-2k on the
+      // lower triangle and 8 m^2 16 for A22 V, per matrix
+      SampleSlot fs;
+      const bool fsam = sizeof(T) == 4 && sampler_enabled() &&
+                        sampler_take(SLK_SAMPLE_FUSED_AV, (double)batch * 256.0 * mn * mn, &fs);
+      if (fsam)
+        hipExtLaunchKernelGGL(fused_update_av_kernel<T>, dim3(t, t, batch), dim3(256), 0, st, fs.e0, fs.e1, 0,
+                              cp.Vc + r0n, cp.Wc + r0n, cn.Vc + r0n, A, c.Yp, n, ld, r0n, c.nch, sP, sM, fs.clk);
+      else
+        hipLaunchKernelGGL(fused_update_av_kernel<T>, dim3(t, t, batch), dim3(256), 0, st, cp.Vc + r0n,
+                           cp.Wc + r0n, cn.Vc + r0n, A, c.Yp, n, ld, r0n, c.nch, sP, sM,
+                           (unsigned long long*)nullptr);
       hipLaunchKernelGGL(band_x_kernel<T>, dim3(t, batch), dim3(256), 0, st, cn, r0n, t);
       hipLaunchKernelGGL(band_w_kernel<T>, dim3(t, batch), dim3(256), 0, st, cn, r0n, t);
     }

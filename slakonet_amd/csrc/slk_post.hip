@@ -13,6 +13,8 @@
 #include "slakonet_amd.h"
 
 #include "slk_edges.h"
+#include "slk_sampler.h"
+#include <hip/hip_ext.h>
 
 #define POST_THREADS 256
 
@@ -181,34 +183,83 @@ __global__ __launch_bounds__(POST_THREADS) void edges_kernel(const double* __res
 }
 
 #define DOS_CHUNK 1024
+// Gaussians of levels more than DOS_CUT sigma from every grid point of a
+// workgroup are skipped: each is below exp(-50) ~ 2e-22 of its peak.
+#define DOS_CUT 10.0
+// One thread per grid point, one workgroup per 256 consecutive points. The
+// levels of the structure are streamed through in order; those within
+// DOS_CUT sigma of the workgroup's energy window are compacted into LDS
+// (order kept by a ballot prefix, so each point sums its terms in level
+// order) and summed by every thread of the workgroup.
 __global__ __launch_bounds__(POST_THREADS) void dos_kernel(const double* __restrict__ W, const double* __restrict__ KW,
                                                            const SlkDosJob* jobs, const double* __restrict__ grid,
-                                                           int ne, double sigma, const double* edges, double* dos) {
+                                                           int ne, double sigma, const double* edges, double* dos,
+                                                           unsigned long long* clk) {
+  sample_start(clk);
   const SlkDosJob jb = jobs[blockIdx.y];
-  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int e = blockIdx.x * blockDim.x + tid;
   const double E = e < ne ? grid[e] : 0.0;
+  __shared__ double s_lo[POST_THREADS / 64], s_hi[POST_THREADS / 64];
+  __shared__ int s_cnt[POST_THREADS / 64];
+  __shared__ double se[DOS_CHUNK];
+  __shared__ double sw[DOS_CHUNK];
+  {
+    const double lo = wave_min(e < ne ? E : 1e300), hi = wave_max(e < ne ? E : -1e300);
+    if (lane == 0) { s_lo[wv] = lo; s_hi[wv] = hi; }
+  }
+  __syncthreads();
+  double blo = 1e300, bhi = -1e300;
+  for (int q = 0; q < POST_THREADS / 64; ++q) {
+    blo = fmin(blo, s_lo[q]);
+    bhi = fmax(bhi, s_hi[q]);
+  }
+  const double wlo = blo - DOS_CUT * sigma, whi = bhi + DOS_CUT * sigma;
   const double mu = jb.edge >= 0 ? edges[(size_t)jb.edge * SLK_EDGE_FIELDS + E_MU] : 0.0;
   const double inv = 1.0 / sigma;
   const double norm = 1.0 / (sigma * sqrt(2.0 * 3.141592653589793));
-  __shared__ double se[DOS_CHUNK];
-  __shared__ double sw[DOS_CHUNK];
   const int tot = jb.nk * jb.nb;
   double acc = 0.0;
-  for (int c0 = 0; c0 < tot; c0 += DOS_CHUNK) {
-    const int cn = min(DOS_CHUNK, tot - c0);
-    __syncthreads();
-    for (int q = threadIdx.x; q < cn; q += blockDim.x) {
-      const int g = c0 + q, k = g / jb.nb, b = g % jb.nb;
-      se[q] = (W[jb.w0 + (long long)k * jb.wld + b] - mu) * H2E_OUT;
-      sw[q] = KW[jb.kw0 + k];
+  int fill = 0;   // kept levels staged in se / sw (the same on every thread)
+  for (int c0 = 0; c0 < tot; c0 += POST_THREADS) {
+    const int g = c0 + tid;
+    double val = 0.0, wk = 0.0;
+    bool keep = false;
+    if (g < tot) {
+      const int k = g / jb.nb, b = g % jb.nb;
+      val = (W[jb.w0 + (long long)k * jb.wld + b] - mu) * H2E_OUT;
+      keep = val >= wlo && val <= whi;
+      if (keep) wk = KW[jb.kw0 + k];
     }
+    const unsigned long long mask = __ballot(keep);
+    const int pre = __popcll(mask & ((1ull << lane) - 1ull));
+    if (lane == 0) s_cnt[wv] = __popcll(mask);
     __syncthreads();
-    for (int q = 0; q < cn; ++q) {
-      const double x = (E - se[q]) * inv;
-      acc += sw[q] * (exp(-0.5 * x * x) * norm);
+    int off = fill, total = 0;
+    for (int q = 0; q < POST_THREADS / 64; ++q) {
+      if (q < wv) off += s_cnt[q];
+      total += s_cnt[q];
+    }
+    if (keep) {
+      se[off + pre] = val;
+      sw[off + pre] = wk;
+    }
+    fill += total;
+    __syncthreads();
+    if (fill > DOS_CHUNK - POST_THREADS || c0 + POST_THREADS >= tot) {
+      for (int q = 0; q < fill; ++q) {
+        const double x = (E - se[q]) * inv;
+        acc += sw[q] * (exp(-0.5 * x * x) * norm);
+      }
+      fill = 0;
+      __syncthreads();
     }
   }
   if (e < ne) dos[jb.out0 + e] = acc;
+  if (clk != nullptr) {
+    __syncthreads();
+    sample_end(clk);
+  }
 }
 
 // Adjoint of dos_kernel: for every eigenvalue e_kb of a job,
@@ -364,8 +415,14 @@ int slk_edges_launch(const double* W, const SlkSpectrum* spec, int n_spec, doubl
 int slk_dos_launch(const double* W, const double* KW, const SlkDosJob* jobs, int n_jobs, const double* grid, int ne,
                    double sigma, const double* edges, double* dos, double* stats, hipStream_t stream) {
   if (n_jobs <= 0) return 0;
-  hipLaunchKernelGGL(dos_kernel, dim3((ne + POST_THREADS - 1) / POST_THREADS, n_jobs), dim3(POST_THREADS), 0, stream,
-                     W, KW, jobs, grid, ne, sigma, edges, dos);
+  const dim3 g((ne + POST_THREADS - 1) / POST_THREADS, n_jobs);
+  SampleSlot sl;
+  if (sampler_enabled() && sampler_take(SLK_SAMPLE_DOS, -1.0, &sl))
+    hipExtLaunchKernelGGL(dos_kernel, g, dim3(POST_THREADS), 0, stream, sl.e0, sl.e1, 0, W, KW, jobs, grid, ne, sigma,
+                          edges, dos, sl.clk);
+  else
+    hipLaunchKernelGGL(dos_kernel, g, dim3(POST_THREADS), 0, stream, W, KW, jobs, grid, ne, sigma, edges, dos,
+                       (unsigned long long*)nullptr);
   if (stats)
     hipLaunchKernelGGL(dos_stats_kernel, dim3(n_jobs), dim3(POST_THREADS), 0, stream, jobs, grid, ne, dos, stats);
   return (int)hipGetLastError();

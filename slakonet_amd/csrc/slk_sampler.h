@@ -11,13 +11,17 @@
 #include <hip/hip_runtime.h>
 
 // Kinds and their algorithmic amount: bytes for the memory-bound kinds (0, 1),
-// flops for the FP64 MFMA GEMM (2) and the bulge chase (3).
+// flops for the FP64 MFMA GEMM (2), the bulge chase (3) and the FP32 fused
+// stage-1 update (4); bytes for the DOS (5), whose auxiliary amount is the
+// number of Gaussians of the full sum (nk * nb * n_E per structure).
 enum {
   SLK_SAMPLE_HETRD_STEP = 0,
   SLK_SAMPLE_BUILD = 1,
   SLK_SAMPLE_GEMM_F64 = 2,
   SLK_SAMPLE_CHASE = 3,
-  SLK_SAMPLE_KINDS = 4
+  SLK_SAMPLE_FUSED_AV = 4,
+  SLK_SAMPLE_DOS = 5,
+  SLK_SAMPLE_KINDS = 6
 };
 
 // Workgroups of a sampled launch spread their clock atomics over
@@ -32,8 +36,8 @@ struct SampleSlot {
 };
 
 // Counts one launch of `kind` moving `bytes` algorithmic bytes (bytes < 0:
-// the pending hint of that kind). Returns true and a slot when this launch
-// is sampled.
+// the pending hint of that kind, with its auxiliary amount). Returns true and
+// a slot when this launch is sampled.
 bool sampler_take(int kind, double bytes, SampleSlot* slot);
 // Cheap check before computing an expensive algorithmic amount.
 bool sampler_enabled();

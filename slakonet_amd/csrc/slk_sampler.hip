@@ -15,7 +15,7 @@ __global__ void clk_init_kernel(unsigned long long* clk, size_t n) {
 
 struct Sample {
   hipEvent_t e0, e1;
-  double bytes;
+  double bytes, aux;
   int kind;
 };
 struct Sampler {
@@ -23,6 +23,7 @@ struct Sampler {
   int every = 0;
   long long launches[SLK_SAMPLE_KINDS] = {};
   double bytes[SLK_SAMPLE_KINDS] = {};
+  double aux[SLK_SAMPLE_KINDS] = {};
   std::vector<Sample> pool;
   unsigned long long* clk = nullptr;   // device [capacity][2]
   size_t used = 0;
@@ -31,6 +32,7 @@ Sampler g;
 std::mutex g_mu;
 // byte hint of the next launch of each kind, per launching thread
 thread_local double t_hint[SLK_SAMPLE_KINDS] = {};
+thread_local double t_aux[SLK_SAMPLE_KINDS] = {};
 
 void free_pool() {
   for (auto& s : g.pool) {
@@ -48,14 +50,20 @@ bool sampler_enabled() { return g.enabled; }
 
 bool sampler_take(int kind, double bytes, SampleSlot* slot) {
   if (!g.enabled || kind < 0 || kind >= SLK_SAMPLE_KINDS) return false;
-  if (bytes < 0) bytes = t_hint[kind];
+  double aux = 0.0;
+  if (bytes < 0) {
+    bytes = t_hint[kind];
+    aux = t_aux[kind];
+  }
   std::lock_guard<std::mutex> lock(g_mu);
   if (!g.enabled) return false;
   const long long idx = g.launches[kind]++;
   g.bytes[kind] += bytes;
+  g.aux[kind] += aux;
   if (g.every <= 0 || idx % g.every != 0 || g.used >= g.pool.size()) return false;
   Sample& s = g.pool[g.used];
   s.bytes = bytes;
+  s.aux = aux;
   s.kind = kind;
   slot->clk = g.clk + (size_t)g.used * SLOT_ELEMS;
   slot->e0 = s.e0;
@@ -73,6 +81,7 @@ int slk_sampler_reset(int enable, int every, int capacity) {
   for (int k = 0; k < SLK_SAMPLE_KINDS; ++k) {
     g.launches[k] = 0;
     g.bytes[k] = 0.0;
+    g.aux[k] = 0.0;
   }
   g.enabled = enable;
   g.every = every;
@@ -101,18 +110,20 @@ int slk_sampler_reset(int enable, int every, int capacity) {
   return 0;
 }
 
-int slk_sampler_hint(int kind, double bytes) {
+int slk_sampler_hint(int kind, double bytes, double aux) {
   if (kind < 0 || kind >= SLK_SAMPLE_KINDS) return 1;
   t_hint[kind] = bytes;
+  t_aux[kind] = aux;
   return 0;
 }
 
 int slk_sampler_read(int kind, double* out) {
   if (kind < 0 || kind >= SLK_SAMPLE_KINDS) return 1;
   std::lock_guard<std::mutex> lock(g_mu);
-  for (int i = 0; i < 7; ++i) out[i] = 0.0;
+  for (int i = 0; i < 9; ++i) out[i] = 0.0;
   out[3] = (double)g.launches[kind];
   out[4] = g.bytes[kind];
+  out[8] = g.aux[kind];
   if (g.used == 0) return 0;
   hipError_t e = hipDeviceSynchronize();
   if (e != hipSuccess) return (int)e;
@@ -123,7 +134,7 @@ int slk_sampler_read(int kind, double* out) {
   (void)hipGetDevice(&dev);
   e = hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev);
   if (e != hipSuccess || khz <= 0) return e != hipSuccess ? (int)e : 1;
-  double n = 0.0, ms = 0.0, bytes = 0.0, ev_ms = 0.0;
+  double n = 0.0, ms = 0.0, bytes = 0.0, ev_ms = 0.0, aux = 0.0;
   for (size_t i = 0; i < g.used; ++i) {
     const Sample& s = g.pool[i];
     if (s.kind != kind) continue;
@@ -140,6 +151,7 @@ int slk_sampler_read(int kind, double* out) {
     n += 1.0;
     ms += (double)(t1 - t0) / (double)khz;
     bytes += s.bytes;
+    aux += s.aux;
     ev_ms += t;
   }
   out[0] = n;
@@ -147,6 +159,7 @@ int slk_sampler_read(int kind, double* out) {
   out[2] = bytes;
   out[5] = ev_ms;
   out[6] = (double)khz;
+  out[7] = aux;
   return 0;
 }
 

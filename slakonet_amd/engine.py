@@ -171,7 +171,8 @@ def run_build(tables: TableSet, plan: BuildPlan, out_f32: bool = False, cutoff: 
     if SAMPLING:
         # algorithmic bytes of this launch for the live sampler: the H and S
         # elements the build writes (padding excluded)
-        lib.slk_sampler_hint(1, float(sum(nk * 2 * n * n for nk, n in zip(plan.nk, plan.n_orb)) * H.element_size()))
+        lib.slk_sampler_hint(1, float(sum(nk * 2 * n * n for nk, n in zip(plan.nk, plan.n_orb)) * H.element_size()),
+                             0.0)
     _lib.check(lib.slk_build_hs_launch(C.byref(args), sh), "slk_build_hs_launch")
     for s in range(plan.n_struct):
         if plan.n_orb[s] < plan.ld:
@@ -316,6 +317,12 @@ def post(plan: BuildPlan, w: torch.Tensor, fermi_mode: int = 0, kT: float = FERM
         ne = int(dos_grid.numel())
         jd = _upload(dos_jobs(plan, w.shape[1], ne), dev)
         dos = torch.empty(plan.n_struct, ne, dtype=torch.float64, device=dev)
+        if SAMPLING:
+            # algorithmic bytes (eigenvalues, k-weights and grid read, DOS
+            # written) and Gaussians of the full sum, for the live sampler
+            states = sum(nk * nb for nk, nb in zip(plan.nk, plan.n_orb))
+            nbytes = 8.0 * sum(nk * nb + nk + 2 * ne for nk, nb in zip(plan.nk, plan.n_orb))
+            lib.slk_sampler_hint(5, nbytes, float(states) * ne)
         stats = torch.empty(plan.n_struct, 3, dtype=torch.float64, device=dev)
         _lib.check(lib.slk_dos_launch(C.c_void_p(w.data_ptr()), C.c_void_p(kw.data_ptr()), C.c_void_p(jd.data_ptr()),
                                       plan.n_struct, C.c_void_p(dos_grid.data_ptr()), ne, float(sigma),
