@@ -504,6 +504,16 @@ def main():
         runner(tables, inputs, arrivals, grid)
         print("# warmup done", file=sys.stderr, flush=True)
     torch.cuda.synchronize()
+    if args.steps == 0:
+        # counter runs: only the sampled pass (launch counts and algorithmic
+        # amounts per kind, to divide the counters by), no timed region
+        start_sampling(HETRD_SAMPLE_EVERY, args.sample_capacity)
+        run_serve(tables, inputs, arrivals, grid, quiet=True)
+        torch.cuda.synchronize()
+        samples = stop_sampling()
+        print(json.dumps({"sampler_counts": {k: [int(v[3]), v[4], v[8]] for k, v in samples.items()},
+                          **roofline_fields(samples)}))
+        return
     walls, lats, n_ok, n_failed, results = [], [], 0, 0, None
     t_start = time.perf_counter()
     for s in range(args.steps):

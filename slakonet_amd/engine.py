@@ -208,6 +208,7 @@ def eigh(H: torch.Tensor, S: torch.Tensor, vectors: bool = False, stream=None, c
     if mixed and vectors:
         raise ValueError("the mixed-precision solve returns eigenvalues only")
     lib = _lib.lib()
+    check_pair(H, S)
     batch, ld, _ = H.shape
     n = ld
     ws_bytes = int(lib.slk_eigh_mixed_workspace_bytes(n, ld, batch) if mixed else
@@ -234,6 +235,18 @@ def eigh(H: torch.Tensor, S: torch.Tensor, vectors: bool = False, stream=None, c
     if check:
         check_info(info, batch)
     return w, V, info
+
+
+def check_pair(H: torch.Tensor, S: torch.Tensor):
+    """The kernels read H and S as contiguous [batch, ld, ld] complex128 on
+    the GPU; anything else is refused before a launch."""
+    for name, t in (("H", H), ("S", S)):
+        if t.dtype != torch.complex128:
+            raise TypeError(f"{name} must be complex128, got {t.dtype}")
+        if t.dim() != 3 or t.shape[1] != t.shape[2] or not t.is_contiguous() or not t.is_cuda:
+            raise ValueError(f"{name} must be a contiguous [batch, ld, ld] GPU tensor, got {tuple(t.shape)}")
+    if H.shape != S.shape or H.device != S.device:
+        raise ValueError("H and S differ in shape or device")
 
 
 def check_info(info: torch.Tensor, batch: int, k_offset: int = 0):
