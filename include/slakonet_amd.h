@@ -119,9 +119,11 @@ int slk_pad_launch(const SlkStructDesc* structs_dev, int s, int nk, int n_orb, i
  * per-k loop of SimpleDftb.__call__ (slakonet/main.py:165-196).
  * w: [batch][wld] ascending; V (nullable): eigenvectors in columns;
  * info: [2 * batch], cleared by the call; info[b] > 0: S not positive definite at that pivot,
- * info[batch + b] > 0: the tridiagonal stage failed (QL iteration did not
- * converge: the sweep index + 1 for vectors; 2: the bulge chase gave up a
- * bounded wait, eigenvalues NaN). */
+ * info[batch + b] = 2: the bulge chase of the two-stage tridiagonalisation
+ * gave up a bounded wait (eigenvalues NaN). Eigenvalues come from bisection
+ * on the FP64 tridiagonal; eigenvectors (V non-null) from inverse iteration on
+ * it (LAPACK dstein method, Gram-Schmidt inside clusters of close
+ * eigenvalues), then the reflectors and W^H. Any n <= 16384. */
 size_t slk_eigh_workspace_bytes(int n, int ld, int batch, int vectors);
 int slk_eigh_f64(const void* H, const void* S, int n, int ld, int batch, double* w, int wld, void* V,
                  void* ws, size_t ws_bytes, int* info, hipStream_t stream);

@@ -8,6 +8,8 @@
 * band_gap: calculate_band_gap (slakonet/main.py:637-707).
 * band_structure: calculate_band_structure_properties (slakonet/main.py:1539-1642).
 * dos: calculate_dos (slakonet/main.py:709-797), Fermi-shifted, eV.
+* occupations / density: fermi (slakonet/slaterkoster.py:25-74) and the
+  density of SimpleDftb.solve_kpoint (slakonet/main.py:157-161).
 * properties: get_properties_dict(include_dos_data=True)
   (slakonet/main.py:1657-1819) plus the get_fermi block of
   compute_multi_element_properties (slakonet/optim.py:1011-1059).
@@ -182,3 +184,24 @@ def atom_projected_dos(eig, vecs, orb_type, n_types, mu, sigma=0.1, energy_range
         g = norm * np.exp(-0.5 * ((grid_ev[None, :] - eig[k][:, None] * H2E) / sigma) ** 2)   # [nb, E]
         out += w @ g
     return grid, out / nk
+
+
+def occupations(n_states, n_electrons):
+    """fermi(eigenvalue, nelectron) without spin or smearing
+    (slakonet/slaterkoster.py:25-74): 2 for the floor(Ne/2) lowest states, 1
+    for the next one when Ne is odd."""
+    if n_electrons < 1:
+        raise ValueError("at least one electron")
+    n_pairs = int(n_electrons // 2)
+    occ = np.zeros(n_states)
+    occ[:min(n_pairs, n_states)] = 2.0
+    if n_electrons % 2 > 0 and n_pairs < n_states:
+        occ[n_pairs] = 1.0
+    return occ
+
+
+def density(vecs, occ):
+    """rho = conj(C sqrt(f)) (C sqrt(f))^T per k (slakonet/main.py:158-161);
+    vecs [nk, n, nb] eigenvectors in columns, occ [nb] -> [nk, n, n]."""
+    c = np.asarray(vecs) * np.sqrt(np.asarray(occ))[None, None, :]
+    return np.conj(c) @ np.swapaxes(c, -1, -2)

@@ -73,6 +73,9 @@ class SimpleDftb:
                                           float(self.nelectron.item()))
         self._edges = None
         self._band_gap = None
+        self._density = None
+        self._occ = None
+        self._V = None
         self.eigenvalue = None
 
     # ------------------------------------------------------------------ solve
@@ -91,7 +94,64 @@ class SimpleDftb:
             self.eigenvectors = V.permute(1, 2, 0).unsqueeze(0)   # [1, n, nb, nk]
         self._edges = None
         self._band_gap = None
+        self._density = None
+        self._occ = None
         return self.eigenvalue
+
+    # ------------------------------------------------------------ density
+    @property
+    def _occupations(self):
+        """Aufbau occupations [1, nk, nb] (slakonet/slaterkoster.py:25-74, as
+        called by SimpleDftb.solve_kpoint, slakonet/main.py:157): 2 for the
+        floor(Ne/2) lowest states of every k-point, 1 for the next one when
+        Ne is odd, 0 above (no smearing)."""
+        if self._occ is None:
+            ne = float(self.nelectron.item())
+            if ne < 1:
+                raise ValueError("occupations need at least one electron")
+            nb = int(self.eigenvalue.shape[-1])
+            nk = int(self.eigenvalue.shape[1])
+            n_pairs = int(ne // 2)
+            occ = torch.zeros(nb, dtype=torch.float64)
+            occ[:min(n_pairs, nb)] = 2.0
+            if ne % 2 > 0 and n_pairs < nb:
+                occ[n_pairs] = 1.0
+            self._occ = occ.to(self.device).expand(1, nk, nb).contiguous()
+        return self._occ
+
+    @property
+    def density(self):
+        """Density matrices [1, n, n, nk] (slakonet/main.py:158-161):
+        rho_k = conj(C sqrt(f)) (C sqrt(f))^T, i.e. rho[mu][nu] =
+        sum_j f_j conj(C[mu][j]) C[nu][j], with C the S-orthonormal
+        eigenvectors of k in columns. Formed on the device on first access
+        (column scaling, then one batched complex GEMM over all k); when the
+        solve ran without eigenvectors, they are computed first."""
+        if self._density is None:
+            import ctypes as C
+            from . import _lib
+            if self.eigenvalue is None:
+                self()
+            V = self._V
+            if V is None:
+                _, V, _ = engine.eigh(self.ham, self.overlap, vectors=True)
+            batch, ld, _ = V.shape
+            n = self._plan.n_orb[0]
+            occ = self._occupations[0]                                   # [nk, nb]
+            g = torch.zeros(batch, ld, dtype=torch.float64, device=V.device)
+            g[:, :occ.shape[-1]] = torch.sqrt(occ)
+            lib = _lib.lib()
+            sh = _lib.stream_handle()
+            Y = torch.empty_like(V)
+            R = torch.empty_like(V)
+            st = ld * ld
+            _lib.check(lib.slk_scale_columns(_lib.ptr(V), _lib.ptr(g), ld, ld, batch, ld, _lib.ptr(Y), sh),
+                       "slk_scale_columns")
+            # R = conj(Y) Y^T: op bit1 conjugates A, bit0 transposes B
+            _lib.check(lib.slk_zgemm(batch, ld, ld, ld, 1.0, _lib.ptr(Y), ld, st, 2, _lib.ptr(Y), ld, st, 1,
+                                     0.0, _lib.ptr(R), ld, st, sh), "slk_zgemm")
+            self._density = R[:, :n, :n].permute(1, 2, 0).unsqueeze(0)
+        return self._density
 
     # ------------------------------------------------------------------ fermi
     def _run_post(self, dos_grid=None, sigma=0.1):

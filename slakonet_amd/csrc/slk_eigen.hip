@@ -7,8 +7,9 @@
 //   reduce  C = W H W^H                        (batched complex GEMMs)
 //   hetrd   C = Q T Q^H, T real tridiagonal    (Householder, lower)
 //   values  bisection on Sturm counts of T     (FP64, thread per eigenvalue)
-//   vectors implicit QL on T with rotations accumulated into Z, then
-//           Y = Q Z (reflectors) and z = W^H Y (GEMM)
+//   vectors inverse iteration on T (thread per eigenvalue, Gram-Schmidt
+//           inside clusters) into Z, then Y = Q Z (reflectors) and
+//           z = W^H Y (GEMM)
 // Matrices are row-major complex (interleaved re/im) with leading dimension
 // ld and batch stride ld*ld.
 #include "slk_common.h"
@@ -216,130 +217,174 @@ static int bisect_group(int n, int batch) {
 }
 
 // ---------------------------------------------------------------------------
-// Implicit QL with Wilkinson shift on (d, e); rotations accumulated into the
-// real matrix Z (column-major: Zc[col*ld + row]), then sorted ascending.
-// Lane 0 computes each sweep's rotation sequence; all threads apply it to
-// their rows of Z with a carried column.
+// Eigenvectors of the FP64 tridiagonal T (d, e) by inverse iteration from the
+// bisection eigenvalues (the LAPACK dstein method: LU of T - sigma I with
+// partial pivoting, dgttrf / dgttrs, a fixed number of solves from a
+// deterministic start vector). One thread per eigenvalue, all eigenvalues of
+// all matrices in parallel. Eigenvalues closer than STEIN_CLUSTER * |T| to
+// their neighbour form a cluster; its first thread computes all of its
+// vectors in order, orthogonalising each against the cluster's earlier
+// vectors (modified Gram-Schmidt) after every solve. Outside clusters the
+// vectors are orthogonal to eps |T| / gap < 3e-10. A split tridiagonal
+// (e_i = 0 after a degenerate spectrum) needs nothing special: degenerate
+// eigenvalues are a cluster and the random starts plus Gram-Schmidt give an
+// orthonormal basis of their space.
+// Z is row-major here, Z[row * ld + col] (column = eigenvector), so that the
+// threads of a wave write consecutive addresses. Per-thread scratch (dl, dd,
+// du, du2, pivots, y: 6 n doubles) is interleaved with stride nthr (all
+// eigenvalue threads of the matrix), so that step i of every thread of a wave
+// touches consecutive addresses too.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(EIG_THREADS) void tql2_kernel(const double* D, const double* Ed, int n, int ld,
-                                                           double* Zc, double* Wout, int wld, int* info) {
-  const int b = blockIdx.x;
-  double* z = Zc + (size_t)b * ld * ld;
-  extern __shared__ double sm[];
-  double* d = sm;
-  double* e = sm + n;
-  double* cs = sm + 2 * n;
-  double* sn = sm + 3 * n;
-  __shared__ int s_l, s_m, s_last, s_done, s_fail;
-  const int tid = threadIdx.x;
-  for (int i = tid; i < n; i += blockDim.x) {
-    d[i] = D[(size_t)b * ld + i];
-    e[i] = i < n - 1 ? Ed[(size_t)b * ld + i] : 0.0;
+#define STEIN_THREADS 256
+#define STEIN_CLUSTER 1e-6
+#define STEIN_SCR 6
+
+__global__ __launch_bounds__(STEIN_THREADS) void stein_kernel(const double* D, const double* Ed, const double* W,
+                                                              int wld, int n, int ld, double* Zr, double* scr,
+                                                              int nthr) {
+  const int b = blockIdx.y;
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  const double* d = D + (size_t)b * ld;
+  const double* e = Ed + (size_t)b * ld;
+  const double* w = W + (size_t)b * wld;
+  double* z = Zr + (size_t)b * ld * ld;
+  // |T| = max row sum, reduced over the workgroup
+  __shared__ double s_red[STEIN_THREADS / 64];
+  double tn = 0.0;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    const double ei = i + 1 < n ? e[i] : 0.0;
+    const double ep = i > 0 ? e[i - 1] : 0.0;
+    tn = fmax(tn, fabs(d[i]) + fabs(ei) + fabs(ep));
   }
-  for (size_t q = tid; q < (size_t)n * ld; q += blockDim.x) {
-    const int c = (int)(q / ld), r = (int)(q % ld);
-    z[q] = (r == c) ? 1.0 : 0.0;
-  }
-  if (tid == 0) s_fail = 0;
+  tn = wave_max(tn);
+  if ((threadIdx.x & 63) == 0) s_red[threadIdx.x >> 6] = tn;
   __syncthreads();
-  for (int l = 0; l < n; ++l) {
-    int iter = 0;
-    while (true) {
-      if (tid == 0) {
-        int m;
-        for (m = l; m < n - 1; ++m) {
-          const double dd = fabs(d[m]) + fabs(d[m + 1]);
-          if (fabs(e[m]) <= 2.2e-16 * dd) break;
-        }
-        s_m = m;
-        s_done = (m == l) || (iter >= 60);
-        if (iter >= 60 && m != l) s_fail = l + 1;
-        if (!s_done) {
-          double g = (d[l + 1] - d[l]) / (2.0 * e[l]);
-          double r = hypot(g, 1.0);
-          g = d[m] - d[l] + e[l] / (g + (g >= 0.0 ? fabs(r) : -fabs(r)));
-          double s = 1.0, c = 1.0, p = 0.0;
-          int i;
-          int last = l;
-          for (i = m - 1; i >= l; --i) {
-            double f = s * e[i];
-            const double bb = c * e[i];
-            r = hypot(f, g);
-            e[i + 1] = r;
-            if (r == 0.0) {
-              d[i + 1] -= p;
-              e[m] = 0.0;
-              last = i + 1;
-              break;
-            }
-            s = f / r;
-            c = g / r;
-            g = d[i + 1] - p;
-            r = (d[i] - g) * s + 2.0 * c * bb;
-            p = s * r;
-            d[i + 1] = g + p;
-            g = c * r - bb;
-            cs[i] = c;
-            sn[i] = s;
-          }
-          if (i < l) {   // completed sweep
-            d[l] -= p;
-            e[l] = g;
-            e[m] = 0.0;
-            last = l;
-          }
-          s_last = last;
+  tn = 0.0;
+  for (int q = 0; q < STEIN_THREADS / 64; ++q) tn = fmax(tn, s_red[q]);
+  if (j >= n) return;
+  const double thr = STEIN_CLUSTER * fmax(tn, 1e-300);
+  if (j > 0 && w[j] - w[j - 1] <= thr) return;          // not the first of its cluster
+  int last = j;
+  while (last + 1 < n && w[last + 1] - w[last] <= thr) ++last;
+  const double tiny = 2.2e-16 * fmax(tn, 1e-300);
+  const size_t st = (size_t)nthr;
+  double* base = scr + (size_t)b * STEIN_SCR * nthr * n + j;
+  double *dl = base, *dd = base + st * n, *du = base + 2 * st * n, *du2 = base + 3 * st * n;
+  double *pv = base + 4 * st * n, *y = base + 5 * st * n;
+  const int its = last > j ? 4 : 3;
+  // Rows split off as 1 x 1 blocks (e = 0 on both sides, e.g. identity
+  // padding) are exact eigenvectors: a cluster member whose eigenvalue equals
+  // such a row's diagonal (to the bisection's accuracy) takes that unit
+  // vector, so a large degenerate cluster costs O(n) per member instead of
+  // Gram-Schmidt. Rows are scanned once, in order; a member without a match
+  // goes through inverse iteration.
+  const double tol_eig = 16.0 * 2.2e-16 * fmax(tn, 1e-300);
+  int ri = 0;
+  double sig_prev = -1e300;
+  for (int m = j; m <= last; ++m) {
+    if (last > j) {
+      int irow = -1;
+      for (; ri < n; ++ri) {
+        const bool iso = (ri == 0 || e[ri - 1] == 0.0) && (ri + 1 == n || e[ri] == 0.0);
+        if (iso && fabs(d[ri] - w[m]) <= tol_eig) {
+          irow = ri++;
+          break;
         }
       }
-      __syncthreads();
-      if (s_done) break;
-      ++iter;
-      const int m = s_m, last = s_last;
-      // rotations i = m-1 .. last act on columns (i, i+1)
-      for (int row = tid; row < n; row += blockDim.x) {
-        if (last > m - 1) break;
-        double carry = z[(size_t)m * ld + row];
-        for (int i = m - 1; i >= last; --i) {
-          const double zi = z[(size_t)i * ld + row];
-          const double c = cs[i], s = sn[i];
-          z[(size_t)(i + 1) * ld + row] = s * zi + c * carry;
-          carry = c * zi - s * carry;
-        }
-        z[(size_t)last * ld + row] = carry;
+      if (irow >= 0) {
+        for (int r = 0; r < n; ++r) z[(size_t)r * ld + m] = r == irow ? 1.0 : 0.0;
+        continue;
       }
-      __syncthreads();
     }
-    __syncthreads();
+    // separate the shifts of a cluster by a few ulps (dstein's perturbation)
+    double sig = w[m];
+    const double pert = 10.0 * 2.2e-16 * fmax(fabs(sig), tn);
+    if (sig - sig_prev < pert) sig = sig_prev + pert;
+    sig_prev = sig;
+    // LU with partial pivoting of T - sig I (dgttrf); the diagonal carried in a register
+    double dcur = d[0] - sig;
+    double ucur = n > 1 ? e[0] : 0.0;   // current super-diagonal of row i
+    for (int i = 0; i + 1 < n; ++i) {
+      const double li = e[i];                       // sub-diagonal below row i
+      double dnext = d[i + 1] - sig;
+      const double unext = i + 2 < n ? e[i + 1] : 0.0;
+      if (fabs(dcur) >= fabs(li)) {
+        if (dcur == 0.0) dcur = tiny;
+        const double f = li / dcur;
+        dl[i * st] = f;
+        dd[i * st] = dcur;
+        du[i * st] = ucur;
+        du2[i * st] = 0.0;
+        pv[i * st] = 0.0;
+        dnext -= f * ucur;
+        ucur = unext;
+      } else {
+        const double f = dcur / li;
+        dd[i * st] = li;
+        dl[i * st] = f;
+        du[i * st] = dnext;
+        du2[i * st] = unext;
+        pv[i * st] = 1.0;
+        dnext = ucur - f * dnext;
+        ucur = -f * unext;
+      }
+      dcur = dnext;
+    }
+    if (dcur == 0.0) dcur = tiny;
+    dd[(size_t)(n - 1) * st] = dcur;
+    // deterministic start vector, different for each member of a cluster
+    const double ph = 0.1 + 1.3 * (m - j);
+    for (int i = 0; i < n; ++i) y[i * st] = 1.0 + 0.37 * sin(0.7 * i + ph);
+    for (int it = 0; it < its; ++it) {
+      // forward: row interchanges and multipliers
+      double ycur = y[0];
+      for (int i = 0; i + 1 < n; ++i) {
+        double ynext = y[(i + 1) * st];
+        const double f = dl[i * st];
+        if (pv[i * st] == 0.0) {
+          ynext -= f * ycur;
+          y[i * st] = ycur;
+        } else {
+          y[i * st] = ynext;
+          ynext = ycur - f * ynext;
+        }
+        ycur = ynext;
+      }
+      y[(size_t)(n - 1) * st] = ycur;
+      // back substitution with U (dd, du, du2); scaled so that nothing overflows
+      double y1 = 0.0, y2 = 0.0, amax = 0.0;
+      for (int i = n - 1; i >= 0; --i) {
+        const double ui = i + 1 < n ? du[i * st] : 0.0;
+        const double u2 = i + 2 < n ? du2[i * st] : 0.0;
+        const double yi = (y[i * st] - ui * y1 - u2 * y2) / dd[i * st];
+        y[i * st] = yi;
+        amax = fmax(amax, fabs(yi));
+        y2 = y1;
+        y1 = yi;
+      }
+      const double sc = amax > 0.0 ? 1.0 / amax : 1.0;
+      // orthogonalise against the cluster's earlier vectors (columns j .. m-1)
+      for (int k = j; k < m; ++k) {
+        double dot = 0.0;
+        for (int i = 0; i < n; ++i) dot += z[(size_t)i * ld + k] * y[i * st];
+        for (int i = 0; i < n; ++i) y[i * st] -= dot * z[(size_t)i * ld + k];
+      }
+      double nr = 0.0;
+      for (int i = 0; i < n; ++i) {
+        const double v = y[i * st] * sc;
+        y[i * st] = v;
+        nr += v * v;
+      }
+      const double inv = nr > 0.0 ? 1.0 / sqrt(nr) : 0.0;
+      for (int i = 0; i < n; ++i) y[i * st] *= inv;
+    }
+    for (int i = 0; i < n; ++i) z[(size_t)i * ld + m] = y[i * st];
   }
-  // selection sort ascending, swapping columns of Z
-  for (int i = 0; i < n - 1; ++i) {
-    __shared__ int s_k;
-    if (tid == 0) {
-      int k = i;
-      double p = d[i];
-      for (int j = i + 1; j < n; ++j)
-        if (d[j] < p) { k = j; p = d[j]; }
-      s_k = k;
-      if (k != i) { d[k] = d[i]; d[i] = p; }
-    }
-    __syncthreads();
-    const int k = s_k;
-    if (k != i) {
-      for (int row = tid; row < n; row += blockDim.x) {
-        const double t = z[(size_t)i * ld + row];
-        z[(size_t)i * ld + row] = z[(size_t)k * ld + row];
-        z[(size_t)k * ld + row] = t;
-      }
-    }
-    __syncthreads();
-  }
-  for (int i = tid; i < n; i += blockDim.x) Wout[(size_t)b * wld + i] = d[i];
-  if (tid == 0) info[b] = s_fail;
 }
 
 // ---------------------------------------------------------------------------
 // Y = Q Z with Q = H_0 H_1 ... H_{n-2}: apply H_j = I - tau_j v_j v_j^H for
-// j = n-2 .. 0. Z is real column-major, Y complex row-major. One workgroup
+// j = n-2 .. 0. Z is real row-major (columns are eigenvectors), Y complex row-major. One workgroup
 // per (matrix, 64-column block of Z).
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(EIG_THREADS) void apply_q_kernel(const double2* __restrict__ Cref,
@@ -357,7 +402,7 @@ __global__ __launch_bounds__(EIG_THREADS) void apply_q_kernel(const double2* __r
   // load the column block into Y (complex)
   for (int q = tid; q < n * 64; q += blockDim.x) {
     const int r = q / 64, c = q % 64;
-    if (c < nc) y[(size_t)r * ld + c0 + c] = make_double2(z[(size_t)(c0 + c) * ld + r], 0.0);
+    if (c < nc) y[(size_t)r * ld + c0 + c] = make_double2(z[(size_t)r * ld + c0 + c], 0.0);
   }
   __syncthreads();
   __shared__ double2 s_dot[4][64];
@@ -1033,6 +1078,7 @@ struct EighWs {
   double2 *B0, *B1, *B2, *Vc, *Wc, *Y, *Yp, *C12, *TAU, *Band;
   float2 *F0, *F1, *F2, *fVc, *fWc, *fY, *fYp, *fC12, *fTAU, *fBand;
   double *D, *E, *Z;
+  double* SV;   // vectors: inverse-iteration scratch, STEIN_SCR ld^2 doubles per matrix
   int* SG;   // Sturm counts on the bisection start grid, 2 ld + 1 per matrix
 };
 
@@ -1061,6 +1107,7 @@ static size_t carve(char* base, int ld, int batch, int vectors, int mixed, EighW
     w->TAU = reinterpret_cast<double2*>(take((size_t)ld * batch * sizeof(double2)));
     w->Band = reinterpret_cast<double2*>(take(band_scratch_elems(ld) * batch * sizeof(double2)));
     w->Z = vectors ? reinterpret_cast<double*>(take(matel * batch * sizeof(double))) : nullptr;
+    w->SV = vectors ? reinterpret_cast<double*>(take(STEIN_SCR * matel * batch * sizeof(double))) : nullptr;
   } else {
     w->F0 = reinterpret_cast<float2*>(take(matel * batch * sizeof(float2)));
     w->F1 = reinterpret_cast<float2*>(take(matel * batch * sizeof(float2)));
@@ -1118,9 +1165,9 @@ static int eigh_impl(const void* H, const void* S, int n, int ld, int batch, dou
   const size_t need = mixed ? slk_eigh_mixed_workspace_bytes(n, ld, batch)
                             : slk_eigh_workspace_bytes(n, ld, batch, V != nullptr);
   if (ws_bytes < need) return 2;
-  // values: any n up to EIGH_MAX_N (LDS-staged Sturm counts up to 4096, global beyond);
-  // vectors: the QL sweep keeps 4n doubles in LDS (n <= 2048)
-  if (n > EIGH_MAX_N || (V != nullptr && n > 2048)) return 3;
+  // any n up to EIGH_MAX_N (LDS-staged Sturm counts up to 4096, global
+  // beyond; vectors by inverse iteration from global scratch)
+  if (n > EIGH_MAX_N) return 3;
   hipLaunchKernelGGL(zero_info_kernel, dim3((2 * batch + 255) / 256), dim3(256), 0, stream, info, 2 * batch);
   if (V == nullptr && n <= 32) {
     // whole solve in LDS, one launch (stages collapse into the first interval)
@@ -1179,7 +1226,7 @@ static int eigh_impl(const void* H, const void* S, int n, int ld, int batch, dou
                                s.F0);
   }
   if (ev) (void)hipEventRecord(ev[4], stream);
-  if (V == nullptr) {
+  {
     const int G = bisect_group(n, batch);
     const int eig_per_block = (EIG_THREADS / 64) * (64 / G);
     const dim3 grid((n + eig_per_block - 1) / eig_per_block, batch);
@@ -1220,11 +1267,14 @@ static int eigh_impl(const void* H, const void* S, int n, int ld, int batch, dou
                            (const double*)e2, (const int*)s.SG, M, rtol);
       }
     }
-    if (ev) (void)hipEventRecord(ev[5], stream);
-  } else {
-    hipLaunchKernelGGL(tql2_kernel, dim3(batch), dim3(EIG_THREADS), 4 * n * sizeof(double), stream, s.D, s.E, n, ld,
-                       s.Z, w, wld, info + batch);
-    if (ev) (void)hipEventRecord(ev[5], stream);
+  }
+  if (V != nullptr) {
+    // eigenvectors of T by inverse iteration from the eigenvalues just found
+    hipLaunchKernelGGL(stein_kernel, dim3((n + STEIN_THREADS - 1) / STEIN_THREADS, batch), dim3(STEIN_THREADS), 0,
+                       stream, s.D, s.E, w, wld, n, ld, s.Z, s.SV, n);
+  }
+  if (ev) (void)hipEventRecord(ev[5], stream);
+  if (V != nullptr) {
     hipLaunchKernelGGL(apply_q_kernel, dim3((n + 63) / 64, batch), dim3(EIG_THREADS), 0, stream, s.B0, s.TAU, s.Z,
                        s.B2, n, ld);
     // V = W^H Y (W^H upper)

@@ -229,8 +229,7 @@ def eigh(H: torch.Tensor, S: torch.Tensor, vectors: bool = False, stream=None, c
         rc = (lib.slk_eigh_mixed_timed if mixed else lib.slk_eigh_f64_timed)(*args, ms)
         stage_ms.append(dict(zip(EIGH_STAGES, list(ms))))
     if rc == 3:
-        raise NotImplementedError(f"matrix dimension {n} is beyond the solver's limits (values: n <= 16384, "
-                                  "vectors: n <= 2048)")
+        raise NotImplementedError(f"matrix dimension {n} is beyond the solver's limit (n <= 16384)")
     _lib.check(rc, "slk_eigh_f64")
     if check:
         check_info(info, batch)
@@ -265,7 +264,7 @@ def tridiag_failure(code: int, k: int) -> str:
     """Message for a non-zero info[batch + b] of the eigensolver."""
     if code == 2:
         return f"bulge chase of the tridiagonalisation gave up a bounded wait at k-point index {k}"
-    return f"tridiagonal QL iteration did not converge at k-point index {k}"
+    return f"tridiagonal stage failed (code {code}) at k-point index {k}"
 
 
 def spectra(plan: BuildPlan, w_ld: int) -> np.ndarray:

@@ -158,3 +158,38 @@ def test_build_through_interpolation_tail_matches_oracle():
         err = np.abs(g - want).max()
         print(f"tail build: max err {err:.3e} (|want| max {np.abs(want).max():.3e}, y_end {y_end:.3e})")
         assert err < 1e-10 * np.abs(want).max() + 1e-6 * y_end
+
+
+@pytest.mark.parametrize("vectors", [False, True])
+@pytest.mark.parametrize("name", ["diamond", "random3"])
+def test_density_and_occupations_match_oracle(name, vectors):
+    """calc.density [1, n, n, nk] and calc._occupations [1, nk, nb]
+    (main.py:157-161, 187-189) against the oracle on LAPACK eigenvectors.
+    rho is invariant to the phase of each eigenvector and to rotations inside
+    fully occupied degenerate subspaces, so the comparison is basis-free."""
+    from slakonet_amd.calculator import SimpleDftb
+    z, pos, cell, species, mesh, seed = _case(name)
+    skfs = synth.make_skf_set(species, seed=seed)
+    tab = TableSet(skfs, synth.shell_dict_spd(species), device="cuda")
+    k = np.array([[0.13, -0.21, 0.37], [0.31, 0.07, -0.19], [0.27, 0.41, 0.11]])
+    calc = SimpleDftb(Geometry(z, pos, cell=cell, units="angstrom"), shell_dict=synth.shell_dict_spd(species),
+                      tables=tab, kpoints_frac=k, with_eigenvectors=vectors)
+    calc()
+    ref = obuild.build_hs(z, pos * AA_BOHR, cell * AA_BOHR, synth.shell_dict_spd(species), skfs, k)
+    wr, vr = osolve.solve_all(ref["H"], ref["S"], vectors=True)
+    ne = obuild.n_electrons(z, skfs)
+    n = wr.shape[-1]
+    occ = opost.occupations(n, ne)
+    # a partly filled degenerate level would make rho basis-dependent: the case must not have one
+    top = int(ne // 2) - (0 if ne % 2 else 1)
+    assert (np.abs(wr[:, top + 1] - wr[:, top]) > 1e-6).all()
+    if ne % 2:
+        assert (np.abs(wr[:, top] - wr[:, top - 1]) > 1e-6).all()
+    rho_ref = opost.density(vr, occ)
+    got_occ = calc._occupations.cpu().numpy()
+    assert got_occ.shape == (1, len(k), n) and (got_occ[0] == occ[None]).all()
+    rho = calc.density.cpu().numpy()
+    assert rho.shape == (1, n, n, len(k))
+    err = np.abs(rho[0].transpose(2, 0, 1) - rho_ref).max()
+    print(f"density {name}: max err {err:.3e}")
+    assert err < 1e-9 * max(1.0, np.abs(rho_ref).max())

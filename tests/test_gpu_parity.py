@@ -272,3 +272,48 @@ def test_atom_projected_dos_matches_oracle(case):
                                        n_points=700)
     for i, s in enumerate(uniq):
         assert np.allclose(pd[s], pref[i], rtol=1e-7, atol=1e-9 * pref.max()), s
+
+
+def _residuals(H, S, w, V, n):
+    """max |H V - S V diag(w)| and max |V^H S V - I| on the device (complex128)."""
+    Hn, Sn, Vn = H[0, :n, :n], S[0, :n, :n], V[0, :n, :n]
+    wn = w[0, :n]
+    r = (Hn @ Vn - (Sn @ Vn) * wn[None, :]).abs().max().item()
+    g = (Vn.conj().transpose(0, 1) @ (Sn @ Vn) - torch.eye(n, dtype=Vn.dtype, device=Vn.device)).abs().max().item()
+    return r, g
+
+
+@pytest.mark.parametrize("case", ["stream-2304", "random-5200"])
+def test_eigh_vectors_large(case):
+    """Eigenvectors beyond the former n <= 2048 limit: a 256-atom random cell
+    as in the serving stream (n = 2304, complex k, QL arrays in LDS) and a random pair with n = 5200
+    (beyond 5000, QL arrays in global memory). Eigenvalues against the
+    values-only solve (two-stage / bisection, an independent path), residual
+    and S-orthonormality of the vectors."""
+    print(f"\ncase: vectors {case}", flush=True)
+    if case == "stream-2304":
+        species = [6, 14, 31]
+        z, pos, cell = synth.random_cell(np.random.default_rng(2304), 256, species, 1.8)
+        tab = _tables(synth.make_skf_set(species, seed=7), species)
+        inp = engine.StructureInput(z, pos * AA_BOHR, cell * AA_BOHR, np.array([[0.5, 0.25, 0.0]]), np.ones(1))
+        plan = engine.plan_build(tab, [inp])
+        H, S = engine.run_build(tab, plan)
+    else:
+        n = 5200
+        g = torch.Generator(device="cuda").manual_seed(5)
+        x = torch.randn(n, n, dtype=torch.complex128, device="cuda", generator=g)
+        H = ((x + x.conj().transpose(0, 1)) / (2 * n ** 0.5)).unsqueeze(0).contiguous()
+        y = torch.randn(n, n, dtype=torch.complex128, device="cuda", generator=g)
+        S = (torch.eye(n, dtype=torch.complex128, device="cuda") + 0.05 * (y @ y.conj().transpose(0, 1)) / n)
+        S = S.unsqueeze(0).contiguous()
+    n = H.shape[-1]
+    w, V, _ = engine.eigh(H, S, vectors=True)
+    w0, _, _ = engine.eigh(H, S)
+    torch.cuda.synchronize()
+    dw = (w - w0).abs().max().item()
+    r, o = _residuals(H, S, w, V, n)
+    scale = w.abs().max().item()
+    print(f"n={n}: |w_vec - w_val| {dw:.2e}, residual {r:.2e}, orthonormality {o:.2e}", flush=True)
+    assert dw < 1e-9 * max(1.0, scale)
+    assert r < 1e-9 * max(1.0, scale)
+    assert o < 1e-9

@@ -435,3 +435,27 @@ def test_properties_without_occupied_states_falls_back():
     assert p["band_gap_eV"] == 0.0 and p["is_direct_gap"]
     assert p["efermi"] < -0.1 * 27.211 and p["bandgap"] == np.inf
     assert p["valence_band_width_eV"] == 0.0 and p["minimum_direct_gap_eV"] is None
+
+
+def test_occupations_and_density_invariants():
+    """fermi occupations (slaterkoster.py:25-74) and the density of
+    main.py:158-161 on the 2-atom diamond cell at complex k: P = conj(rho) is
+    the physical density C f C^H, so Tr(P S) = Ne and P S P = 2 P (f in {0, 2});
+    rho is Hermitian. An odd Ne puts one electron in the next state."""
+    assert post.occupations(5, 7.0).tolist() == [2.0, 2.0, 2.0, 1.0, 0.0]
+    assert post.occupations(3, 4.0).tolist() == [2.0, 2.0, 0.0]
+    st = synth.diamond_si()
+    skfs = synth.make_skf_set([14], seed=0)
+    k = np.array([[0.13, -0.21, 0.37], [0.5, 0.25, 0.0]])
+    hs = build.build_hs(st.z, st.positions * AA_BOHR, st.cell * AA_BOHR, synth.shell_dict_spd([14]), skfs, k)
+    w, v = solve.solve_all(hs["H"], hs["S"], vectors=True)
+    ne = build.n_electrons(st.z, skfs)
+    occ = post.occupations(18, ne)
+    rho = post.density(v, occ)
+    for ik in range(len(k)):
+        p = np.conj(rho[ik])
+        s = hs["S"][ik]
+        np.testing.assert_allclose(p, p.conj().T, atol=1e-12)
+        assert abs(np.trace(p @ s) - ne) < 1e-10
+        np.testing.assert_allclose(p @ s @ p, 2.0 * p, atol=1e-10)
+        assert np.abs(p.imag).max() > 1e-6          # complex k: conj matters
