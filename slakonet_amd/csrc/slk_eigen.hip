@@ -12,6 +12,8 @@
 //           z = W^H Y (GEMM)
 // Matrices are row-major complex (interleaved re/im) with leading dimension
 // ld and batch stride ld*ld.
+#include <algorithm>
+
 #include "slk_common.h"
 #include "slakonet_amd.h"
 #include "slk_dense.h"
@@ -383,64 +385,96 @@ __global__ __launch_bounds__(STEIN_THREADS) void stein_kernel(const double* D, c
 }
 
 // ---------------------------------------------------------------------------
-// Y = Q Z with Q = H_0 H_1 ... H_{n-2}: apply H_j = I - tau_j v_j v_j^H for
-// j = n-2 .. 0. Z is real row-major (columns are eigenvectors), Y complex row-major. One workgroup
-// per (matrix, 64-column block of Z).
+// Y = Q Z with Q = H_0 H_1 ... H_{n-2}, H_j = I - tau_j v_j v_j^H (v_j[j+1] = 1,
+// entries below in column j of C): blocks of QB reflectors as compact WY,
+// Q_b = I - V_b T_b V_b^H, applied last block first with three MFMA GEMMs
+// each (W = V^H Y, W = T W, Y -= V W). Z is real row-major (columns are
+// eigenvectors); Y complex row-major.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(EIG_THREADS) void apply_q_kernel(const double2* __restrict__ Cref,
-                                                              const double2* __restrict__ TAU,
-                                                              const double* __restrict__ Zc, double2* Y, int n,
-                                                              int ld) {
+#define QB 32
+
+// Y = complex(Z) on the n x n block.
+__global__ void z_to_y_kernel(const double* __restrict__ Zr, double2* __restrict__ Y, int n, int ld) {
   const int b = blockIdx.y;
-  const int c0 = blockIdx.x * 64;
-  const double2* a = Cref + (size_t)b * ld * ld;
-  const double2* tau = TAU + (size_t)b * ld;
-  const double* z = Zc + (size_t)b * ld * ld;
-  double2* y = Y + (size_t)b * ld * ld;
-  const int tid = threadIdx.x;
-  const int nc = min(64, n - c0);
-  // load the column block into Y (complex)
-  for (int q = tid; q < n * 64; q += blockDim.x) {
-    const int r = q / 64, c = q % 64;
-    if (c < nc) y[(size_t)r * ld + c0 + c] = make_double2(z[(size_t)r * ld + c0 + c], 0.0);
+  const size_t off = (size_t)b * ld * ld;
+  for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < (size_t)n * n; q += (size_t)gridDim.x * blockDim.x) {
+    const int r = (int)(q / n), c = (int)(q % n);
+    Y[off + (size_t)r * ld + c] = make_double2(Zr[off + (size_t)r * ld + c], 0.0);
   }
+}
+
+// V_b [m x QB] (row stride QB) of reflectors j0 .. j0+nb-1: row r is matrix row
+// j0+1+r; reflector c has its unit at local row c, zeros above, C below.
+__global__ void qb_pack_kernel(const double2* __restrict__ Cm, int ld, long long sM, int j0, int nb, int m,
+                               double2* __restrict__ Vb, long long sV) {
+  const int b = blockIdx.y;
+  const double2* a = Cm + (size_t)b * sM;
+  double2* v = Vb + (size_t)b * sV;
+  for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < m * QB; q += gridDim.x * blockDim.x) {
+    const int r = q / QB, c = q % QB;
+    double2 x = make_double2(0.0, 0.0);
+    if (c < nb) {
+      if (r == c) x = make_double2(1.0, 0.0);
+      else if (r > c) x = a[(size_t)(j0 + 1 + r) * ld + j0 + c];
+    }
+    v[q] = x;
+  }
+}
+
+// T_b (QB x QB, upper) from G = V^H V and tau (LAPACK zlarft, forward,
+// columnwise): T[j][j] = tau_j, T[0:j, j] = -tau_j T[0:j, 0:j] G[0:j, j].
+// One workgroup of QB threads per matrix; thread i owns row i.
+__global__ void qb_tfactor_kernel(const double2* __restrict__ G, const double2* __restrict__ TAU, int ld, int j0,
+                                  int nb, double2* __restrict__ T) {
+  const int b = blockIdx.x;
+  const double2* g = G + (size_t)b * QB * QB;
+  double2* t = T + (size_t)b * QB * QB;
+  const double2* tau = TAU + (size_t)b * ld + j0;
+  __shared__ double2 s_t[QB][QB + 1];
+  const int i = threadIdx.x;
+  for (int c = 0; c < QB; ++c) s_t[i][c] = make_double2(0.0, 0.0);
   __syncthreads();
-  __shared__ double2 s_dot[4][64];
-  const int lane = tid & 63, wv = tid >> 6;
-  for (int j = n - 2; j >= 0; --j) {
-    const double2 t = tau[j];
-    if (t.x == 0.0 && t.y == 0.0) continue;
-    const int m = n - j - 1;
-    // dot[c] = v^H Y[j+1:, c]
-    double2 acc = make_double2(0.0, 0.0);
-    if (lane < nc) {
-      for (int r = wv; r < m; r += 4) {
-        const double2 vv = r == 0 ? make_double2(1.0, 0.0) : a[(size_t)(j + 1 + r) * ld + j];
-        const double2 yy = y[(size_t)(j + 1 + r) * ld + c0 + lane];
-        acc.x += vv.x * yy.x + vv.y * yy.y;
-        acc.y += vv.x * yy.y - vv.y * yy.x;
-      }
-    }
-    s_dot[wv][lane] = acc;
-    __syncthreads();
-    if (wv == 0) {
-      double2 s = s_dot[0][lane];
-      for (int q = 1; q < 4; ++q) s = c_add(s, s_dot[q][lane]);
-      s_dot[0][lane] = c_mul(t, s);
+  for (int j = 0; j < nb; ++j) {
+    const double2 tj = tau[j];
+    if (i < j) {
+      double2 acc = make_double2(0.0, 0.0);
+      for (int k = i; k < j; ++k) acc = c_add(acc, c_mul(s_t[i][k], g[k * QB + j]));
+      const double2 v = c_mul(tj, acc);
+      s_t[i][j] = make_double2(-v.x, -v.y);
+    } else if (i == j) {
+      s_t[i][j] = tj;
     }
     __syncthreads();
-    for (int q = tid; q < m * 64; q += blockDim.x) {
-      const int r = q / 64, c = q % 64;
-      if (c < nc) {
-        const double2 vv = r == 0 ? make_double2(1.0, 0.0) : a[(size_t)(j + 1 + r) * ld + j];
-        const double2 f = c_mul(vv, s_dot[0][c]);
-        double2 yv = y[(size_t)(j + 1 + r) * ld + c0 + c];
-        yv.x -= f.x;
-        yv.y -= f.y;
-        y[(size_t)(j + 1 + r) * ld + c0 + c] = yv;
-      }
-    }
-    __syncthreads();
+  }
+  for (int c = 0; c < QB; ++c) t[i * QB + c] = s_t[i][c];
+}
+
+// Scratch per matrix: V_b (ld x QB), G and T (QB x QB each), W and W2 (QB x ld each).
+static size_t apply_q_scratch_elems(int ld) { return (size_t)ld * QB + 2 * QB * QB + 2 * (size_t)QB * ld; }
+
+static void launch_apply_q(hipStream_t st, const double2* Cm, const double2* TAU, const double* Zr, double2* Y,
+                           int n, int ld, int batch, double2* scr) {
+  const long long sM = (long long)ld * ld;
+  const long long sV = (long long)ld * QB, sG = (long long)QB * QB, sW = (long long)QB * ld;
+  double2* Vb = scr;                          // [batch][ld][QB]
+  double2* G = Vb + (size_t)batch * sV;       // [batch][QB][QB]
+  double2* T = G + (size_t)batch * sG;        // [batch][QB][QB]
+  double2* W = T + (size_t)batch * sG;        // [batch][QB][ld]
+  double2* W2 = W + (size_t)batch * sW;       // [batch][QB][ld]
+  hipLaunchKernelGGL(z_to_y_kernel, dim3(256, batch), dim3(256), 0, st, Zr, Y, n, ld);
+  const int nref = n - 1;   // reflectors 0 .. n-2
+  if (nref <= 0) return;
+  for (int j0 = ((nref - 1) / QB) * QB; j0 >= 0; j0 -= QB) {
+    const int nb = min(QB, nref - j0);
+    const int m = n - j0 - 1;
+    double2* yr = Y + (size_t)(j0 + 1) * ld;
+    hipLaunchKernelGGL(qb_pack_kernel, dim3((m * QB + 255) / 256, batch), dim3(256), 0, st, Cm, ld, sM, j0, nb, m, Vb,
+                       sV);
+    launch_zgemm(st, batch, QB, QB, m, 1.0, Vb, QB, sV, OP_C, Vb, QB, sV, OP_N, 0.0, G, QB, sG, 0, 0);
+    hipLaunchKernelGGL(qb_tfactor_kernel, dim3(batch), dim3(QB), 0, st, G, TAU, ld, j0, nb, T);
+    launch_zgemm(st, batch, QB, n, m, 1.0, Vb, QB, sV, OP_C, yr, ld, sM, OP_N, 0.0, W, ld, sW, 0, 0);
+    launch_zgemm(st, batch, QB, n, QB, 1.0, T, QB, sG, OP_N, W, ld, sW, OP_N, 0.0, W2, ld, sW, 0, 0);
+    launch_zgemm(st, batch, m, n, QB, -1.0, Vb, QB, sV, OP_N, W2, ld, sW, OP_N, 1.0, yr, ld, sM, 0, 0);
   }
 }
 
@@ -1078,7 +1112,7 @@ struct EighWs {
   double2 *B0, *B1, *B2, *Vc, *Wc, *Y, *Yp, *C12, *TAU, *Band;
   float2 *F0, *F1, *F2, *fVc, *fWc, *fY, *fYp, *fC12, *fTAU, *fBand;
   double *D, *E, *Z;
-  double* SV;   // vectors: inverse-iteration scratch, STEIN_SCR ld^2 doubles per matrix
+  double* SV;   // vectors: inverse-iteration scratch (STEIN_SCR ld^2 doubles per matrix), then the back-transform's
   int* SG;   // Sturm counts on the bisection start grid, 2 ld + 1 per matrix
 };
 
@@ -1107,7 +1141,9 @@ static size_t carve(char* base, int ld, int batch, int vectors, int mixed, EighW
     w->TAU = reinterpret_cast<double2*>(take((size_t)ld * batch * sizeof(double2)));
     w->Band = reinterpret_cast<double2*>(take(band_scratch_elems(ld) * batch * sizeof(double2)));
     w->Z = vectors ? reinterpret_cast<double*>(take(matel * batch * sizeof(double))) : nullptr;
-    w->SV = vectors ? reinterpret_cast<double*>(take(STEIN_SCR * matel * batch * sizeof(double))) : nullptr;
+    // inverse-iteration scratch, reused by the back-transform's blocks
+    const size_t sv = std::max((size_t)STEIN_SCR * matel, 2 * apply_q_scratch_elems(ld));
+    w->SV = vectors ? reinterpret_cast<double*>(take(sv * batch * sizeof(double))) : nullptr;
   } else {
     w->F0 = reinterpret_cast<float2*>(take(matel * batch * sizeof(float2)));
     w->F1 = reinterpret_cast<float2*>(take(matel * batch * sizeof(float2)));
@@ -1275,8 +1311,9 @@ static int eigh_impl(const void* H, const void* S, int n, int ld, int batch, dou
   }
   if (ev) (void)hipEventRecord(ev[5], stream);
   if (V != nullptr) {
-    hipLaunchKernelGGL(apply_q_kernel, dim3((n + 63) / 64, batch), dim3(EIG_THREADS), 0, stream, s.B0, s.TAU, s.Z,
-                       s.B2, n, ld);
+    // the inverse-iteration scratch (STEIN_SCR ld^2 doubles per matrix) is
+    // free again: it holds the back-transform's blocks
+    launch_apply_q(stream, s.B0, s.TAU, s.Z, s.B2, n, ld, batch, reinterpret_cast<double2*>(s.SV));
     // V = W^H Y (W^H upper)
     launch_zgemm(stream, batch, n, n, n, 1.0, s.B1, ld, sM, OP_C, s.B2, ld, sM, OP_N, 0.0,
                  reinterpret_cast<double2*>(V), ld, sM, KLIM_A_UPPER, 0);
