@@ -50,6 +50,8 @@ def parse():
                    help="fit: the fitting workload (32 cells of 8-64 atoms, 3 species, DOS + gap loss, Adam "
                         "steps); large: one 1024-atom cubic cell (9216 orbitals) at Gamma + 4 complex k")
     p.add_argument("--fit-steps", type=int, default=200, help="Adam steps timed in --mode fit")
+    p.add_argument("--fit-lr", type=float, default=1e-5,
+                   help="AdamW learning rate in --mode fit (the reference's default, optim.py:1575)")
     p.add_argument("--sample-every", type=int, default=8,
                    help="sample every n-th tridiagonalisation mat-vec launch for the roofline")
     p.add_argument("--sample-capacity", type=int, default=8192,
@@ -312,12 +314,12 @@ def run_fit(args):
     ds = fit.teacher_datasets(structs, teacher, grid)
     model = MultiElementSkfParameterOptimizer(skf_dicts=synth.make_skf_set(sp, seed=TABLE_SEED), device=dev).to(dev)
     shells = synth.shell_dict_spd(sp)
-    fit.train_skf_parameters(model, ds, shells, num_epochs=max(1, args.warmup), learning_rate=1e-3, loss="dos_gap",
-                             log=None)
+    fit.train_skf_parameters(model, ds, shells, num_epochs=max(1, args.warmup), learning_rate=args.fit_lr,
+                             loss="dos_gap", log=None)
     torch.cuda.synchronize()
     torch.cuda.reset_peak_memory_stats(dev)
     t0 = time.perf_counter()
-    _, hist = fit.train_skf_parameters(model, ds, shells, num_epochs=args.fit_steps, learning_rate=1e-3,
+    _, hist = fit.train_skf_parameters(model, ds, shells, num_epochs=args.fit_steps, learning_rate=args.fit_lr,
                                        loss="dos_gap", early_stopping_patience=10 ** 9,
                                        log=lambda m: print("# " + m, file=sys.stderr, flush=True), log_every=20)
     torch.cuda.synchronize()
@@ -329,10 +331,12 @@ def run_fit(args):
         "ms_per_step": 1000.0 * total / max(1, len(hist)), "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "loss_first": hist[0]["total_loss"], "loss_last": hist[-1]["total_loss"],
+        "loss_min": min(h["total_loss"] for h in hist),
+        "loss_every_20": [round(h["total_loss"], 6) for h in hist[::20]],
         "peak_hbm_bytes": int(torch.cuda.max_memory_allocated(dev)),
         "config": {"workload": f"fitting: 32 random cells 8-64 atoms, species {species}, {n_mat} k-matrices, "
                                "teacher tables (A,b,B,c x (1+N(0,0.1))), loss MSE(DOS 2001 pts) + 0.1 gap err^2, "
-                               "AdamW lr 1e-3, clip 2.0"},
+                               f"AdamW lr {args.fit_lr:g}, clip 2.0"},
     }
     print(json.dumps(out))
 

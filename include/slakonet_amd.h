@@ -294,6 +294,16 @@ int slk_dos_adjoint_launch(const double* W, const double* KW, const SlkDosJob* j
                            const double* grid, int ne, double sigma, const double* edges, const double* gdos,
                            double* gW, hipStream_t stream);
 
+/* Chain rule through the bisection Fermi level of slk_fermi_launch (mode 0):
+ * gW (dL/de at fixed mu, layout of W) += (dL/dmu) dmu/de with dL/dmu =
+ * -sum gW and dmu/de_kb = w_k f'_kb / sum w f' (implicit function theorem on
+ * N(mu) = Ne, f' = f (1 - f) at kT). For losses of e - mu such as the
+ * Fermi-shifted DOS. No reference counterpart: the reference's fermi_search
+ * returns a detached leaf (slakonet/fermi.py:198), so its DOS gradient holds
+ * mu fixed; this is the derivative of the loss as evaluated. */
+int slk_fermi_chain_launch(const double* W, const double* KW, const SlkSpectrum* spec, int n_spec, double kT,
+                           const double* edges, double* gW, hipStream_t stream);
+
 /* Y[b] = C[b] diag(g[b]) (complex128 C [batch, ld, ld] row-major, g real
  * with row stride gld; columns >= n get 0). With Y C^H on slk_zgemm this
  * forms the weighted density Q = C diag(g) C^H of the eigenvalue adjoint

@@ -114,6 +114,8 @@ SIGNATURES = {
     "slk_sampler_read": (C.c_int, [C.c_int, C.c_void_p]),
     "slk_dos_adjoint_launch": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_int,
                                          C.c_double, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "slk_fermi_chain_launch": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_double, C.c_void_p,
+                                         C.c_void_p, C.c_void_p]),
     "slk_scale_columns": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_void_p,
                                     C.c_void_p]),
     "slk_adjoint_tables_launch": (C.c_int, [C.POINTER(SlkBuildArgs), C.c_void_p, C.c_void_p, C.c_void_p,

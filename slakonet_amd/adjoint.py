@@ -19,8 +19,11 @@ eigenvectors are real (Gamma) and differs from the true derivative at
 complex k. The formula here is the true derivative everywhere.
 
 Fermi level. As in the reference (fermi_search returns a detached leaf,
-slakonet/fermi.py:139-215), mu carries no gradient: the DOS and gap
-gradients are taken at fixed mu.
+slakonet/fermi.py:139-215), mu carries no gradient by default: the DOS and
+gap gradients are taken at fixed mu. GaussianDos(..., fermi_grad=True) adds
+mu's own dependence on the eigenvalues (implicit function theorem on
+N(mu) = Ne, slk_fermi_chain_launch), the derivative of the Fermi-shifted DOS
+as evaluated; the fitting loop uses it for its DOS loss.
 
 DOS. slk_dos_adjoint_launch is the adjoint of the Gaussian DOS kernel.
 """
@@ -33,7 +36,7 @@ import torch
 
 from . import _lib, engine
 from .tables import TableSet
-from .units import H2E, HS_CUTOFF_BOHR
+from .units import FERMI_KT_HA, H2E, HS_CUTOFF_BOHR
 
 OP_N, OP_C = 0, 3
 
@@ -122,10 +125,16 @@ class TableEigenvalues(torch.autograd.Function):
 
 
 class GaussianDos(torch.autograd.Function):
-    """DOS [n_struct, ne] of a plan's eigenvalues at the Fermi levels in ``edges``."""
+    """DOS [n_struct, ne] of a plan's eigenvalues at the Fermi levels in ``edges``.
+
+    fermi_grad False: the Fermi level is held fixed in the backward, as the
+    reference's detached fermi_search (slakonet/fermi.py:198) does. True: the
+    backward also carries the Fermi level's own dependence on the eigenvalues
+    (slk_fermi_chain_launch; the edges must come from the bisection), which
+    makes it the derivative of the Fermi-shifted DOS as evaluated."""
 
     @staticmethod
-    def forward(ctx, w, plan, edges, grid, sigma):
+    def forward(ctx, w, plan, edges, grid, sigma, fermi_grad=False):
         lib = _lib.lib()
         ne = int(grid.numel())
         jobs = engine._upload(engine.dos_jobs(plan, w.shape[1], ne), w.device)
@@ -135,7 +144,7 @@ class GaussianDos(torch.autograd.Function):
                                       _lib.ptr(grid), ne, float(sigma), _lib.ptr(edges), _lib.ptr(dos), None,
                                       _lib.stream_handle()), "slk_dos_launch")
         ctx.save_for_backward(wc, edges, grid)
-        ctx.plan, ctx.jobs, ctx.sigma = plan, jobs, float(sigma)
+        ctx.plan, ctx.jobs, ctx.sigma, ctx.fermi_grad = plan, jobs, float(sigma), bool(fermi_grad)
         return dos
 
     @staticmethod
@@ -150,7 +159,12 @@ class GaussianDos(torch.autograd.Function):
                                               plan.n_struct, max_states, _lib.ptr(grid), int(grid.numel()),
                                               ctx.sigma, _lib.ptr(edges), _lib.ptr(gdos), _lib.ptr(gw),
                                               _lib.stream_handle()), "slk_dos_adjoint_launch")
-        return gw, None, None, None, None
+        if ctx.fermi_grad:
+            spd = engine._upload(engine.spectra(plan, w.shape[1]), w.device)
+            _lib.check(lib.slk_fermi_chain_launch(_lib.ptr(w), _lib.ptr(plan.k_weights), _lib.ptr(spd), plan.n_struct,
+                                                  FERMI_KT_HA, _lib.ptr(edges), _lib.ptr(gw), _lib.stream_handle()),
+                       "slk_fermi_chain_launch")
+        return gw, None, None, None, None, None
 
 
 def attach_table_gradient(calc, tables: TableSet):

@@ -298,6 +298,44 @@ __global__ __launch_bounds__(POST_THREADS) void dos_adjoint_kernel(
   if (q < tot) gW[jb.w0 + (long long)k * jb.wld + b] = acc * norm * inv * H2E_OUT * KW[jb.kw0 + k];
 }
 
+// Chain rule through the Fermi level of the bisection (N(mu) = Ne with
+// N = 2 sum_k w_k sum_b f((e_kb - mu) / kT)): by the implicit function
+// theorem dmu/de_kb = w_k f'_kb / sum w f', f' = f (1 - f). An incoming
+// gradient g = dL/de at fixed mu plus dL/dmu = -sum g (every consumer here
+// depends on e - mu) gives g_kb -= (sum g) dmu/de_kb. One workgroup per
+// structure.
+__global__ __launch_bounds__(POST_THREADS) void fermi_chain_kernel(const double* __restrict__ W,
+                                                                   const double* __restrict__ KW,
+                                                                   const SlkSpectrum* spec, double kT,
+                                                                   const double* edges, double* gW) {
+  const SlkSpectrum sp = spec[blockIdx.x];
+  const double mu = edges[(size_t)blockIdx.x * SLK_EDGE_FIELDS + E_MU];
+  const int tid = threadIdx.x;
+  const int tot = sp.nk * sp.nb;
+  __shared__ double red[POST_THREADS / 64];
+  double gs = 0.0, ps = 0.0;
+  for (int q = tid; q < tot; q += blockDim.x) {
+    const int k = q / sp.nb, b = q % sp.nb;
+    const long long o = sp.w0 + (long long)k * sp.wld + b;
+    const double x = (W[o] - mu) / kT;
+    const double ex = exp(-fabs(x));
+    const double f1 = ex / ((1.0 + ex) * (1.0 + ex));   // f (1 - f), symmetric in x
+    gs += gW[o];
+    ps += KW[sp.kw0 + k] * f1;
+  }
+  const double G = block_sum(gs, red);
+  const double P = block_sum(ps, red);
+  if (!(P > 0.0)) return;   // every level far from mu: mu does not move with them
+  for (int q = tid; q < tot; q += blockDim.x) {
+    const int k = q / sp.nb, b = q % sp.nb;
+    const long long o = sp.w0 + (long long)k * sp.wld + b;
+    const double x = (W[o] - mu) / kT;
+    const double ex = exp(-fabs(x));
+    const double f1 = ex / ((1.0 + ex) * (1.0 + ex));
+    gW[o] -= G * KW[sp.kw0 + k] * f1 / P;
+  }
+}
+
 // Atom-type projected DOS (after predict_slakonet.py:119-210): per state the
 // weight of type t is sum over its orbitals of |psi_mu|^2 (coefficients of the
 // S-orthonormal eigenvector, no overlap), Gaussians on E + shift, averaged
@@ -425,6 +463,13 @@ int slk_dos_launch(const double* W, const double* KW, const SlkDosJob* jobs, int
                        (unsigned long long*)nullptr);
   if (stats)
     hipLaunchKernelGGL(dos_stats_kernel, dim3(n_jobs), dim3(POST_THREADS), 0, stream, jobs, grid, ne, dos, stats);
+  return (int)hipGetLastError();
+}
+
+int slk_fermi_chain_launch(const double* W, const double* KW, const SlkSpectrum* spec, int n_spec, double kT,
+                           const double* edges, double* gW, hipStream_t stream) {
+  if (n_spec <= 0) return 0;
+  hipLaunchKernelGGL(fermi_chain_kernel, dim3(n_spec), dim3(POST_THREADS), 0, stream, W, KW, spec, kT, edges, gW);
   return (int)hipGetLastError();
 }
 

@@ -67,7 +67,7 @@ class BatchedProperties:
             plan = engine.plan_build(tables, [inputs[i] for i in idx], ld=ld)
             self.buckets.append((idx, plan))
 
-    def __call__(self, table_tensor: torch.Tensor, with_grad: bool = True):
+    def __call__(self, table_tensor: torch.Tensor, with_grad: bool = True, fermi_grad: bool = False):
         """Returns (dos [n, ne], gaps [n] eV, edges rows per structure)."""
         dos_out: List[Optional[torch.Tensor]] = [None] * self.n
         gap_out: List[Optional[torch.Tensor]] = [None] * self.n
@@ -79,7 +79,7 @@ class BatchedProperties:
             del H, S
             edges, _, _ = engine.post(plan, w)
             wg = TableEigenvalues.apply(table_tensor, self.tables, plan, V, w) if with_grad else w
-            dos = GaussianDos.apply(wg, plan, edges, self.grid, self.sigma)
+            dos = GaussianDos.apply(wg, plan, edges, self.grid, self.sigma, fermi_grad)
             host = edges.cpu().numpy()
             for s, i in enumerate(idx):
                 row = engine.edges_dict(host[s])
@@ -106,10 +106,19 @@ def train_skf_parameters(model, datasets: Sequence[FitDataset], shell_dict, num_
                          learning_rate: float = 1e-5, loss: str = "reference", save_directory: Optional[str] = None,
                          weight_by_system_size: bool = True, early_stopping_patience: int = 20,
                          energy_range=(-20.0, 10.0), num_points: int = 2001, sigma: float = 0.1,
-                         device="cuda", log: Optional[Callable[[str], None]] = print, log_every: int = 5):
+                         device="cuda", log: Optional[Callable[[str], None]] = print, log_every: int = 5,
+                         fermi_grad: Optional[bool] = None):
     """Fit ``model`` (a MultiElementSkfParameterOptimizer) to the datasets.
 
+    ``fermi_grad`` (default: on for loss "dos_gap") carries the Fermi level's
+    dependence on the eigenvalues into the DOS gradient. Held fixed, as the
+    reference's detached Fermi level does, the Fermi-shifted DOS gradient
+    asks for shifts that the Fermi level then follows, and the DOS loss of
+    the fitting workload rises at every learning rate tried (1e-5 to 1e-3).
+
     Returns (model, loss_history). Each history entry carries the step time."""
+    if fermi_grad is None:
+        fermi_grad = loss == "dos_gap"
     dev = torch.device(device)
     grid = torch.linspace(energy_range[0], energy_range[1], num_points, dtype=torch.float64, device=dev)
     tables = model.table_set(shell_dict, dev)
@@ -130,7 +139,7 @@ def train_skf_parameters(model, datasets: Sequence[FitDataset], shell_dict, num_
         optimizer.zero_grad()
         model.apply_constraints()
         tables = model.table_set(shell_dict, dev)
-        dos, gaps, _ = props(tables.tables)
+        dos, gaps, _ = props(tables.tables, fermi_grad=fermi_grad)
         per = dataset_losses(dos, gaps, datasets, loss)
         batch_loss = sum(l * w for l, w in zip(per, weights)) / sum(weights)
         reg = 1e-10 * sum(torch.sum(p ** 2) for p in params)

@@ -6,6 +6,7 @@ import torch
 from slakonet_amd import engine, fit, synth
 from slakonet_amd.optim import MultiElementSkfParameterOptimizer
 from slakonet_amd.tables import TableSet
+from slakonet_amd.units import AA_BOHR
 
 pytestmark = pytest.mark.gpu
 
@@ -51,3 +52,60 @@ def test_training_reduces_loss_and_saves(tmp_path):
     back = MultiElementSkfParameterOptimizer.load_model(tmp_path / "final_model", method="compact")
     a = {k: v.cpu() for k, v in model.state_dict().items()}
     assert all(torch.equal(a[k], v) for k, v in back.state_dict().items())
+
+
+@pytest.mark.parametrize("cells,fermi_grad", [("small3", True), ("small3", False), ("largest", True)])
+def test_dos_loss_gradient_with_fermi_level(cells, fermi_grad):
+    """Table gradient of a Fermi-shifted DOS loss against central differences
+    (step 1e-4 along a random direction of the packed tables) of the loss as
+    evaluated, with mu re-solved by bisection at every point. With the Fermi
+    level's chain rule (slk_fermi_chain_launch) the two agree to 1e-6; held
+    fixed, as the reference's detached mu is, they do not."""
+    if cells == "small3":
+        sts = _small_batch(3)
+    else:   # the largest cell of the 32-cell fitting batch
+        sts = [max(synth.fitting_batch(seed=3, n_cells=32, species=SPECIES), key=lambda st: len(st.z))]
+    sp = sorted(SPECIES)
+    tab = TableSet(synth.make_skf_set(sp, seed=7), synth.shell_dict_spd(sp), device="cuda")
+    grid = torch.linspace(-20.0, 10.0, 2001, dtype=torch.float64, device="cuda")
+    inputs = [engine.StructureInput(st.z, st.positions * AA_BOHR, st.cell * AA_BOHR, st.kpoints, st.k_weights)
+              for st in sts]
+    props = fit.BatchedProperties(tab, inputs, grid)
+    c = torch.from_numpy(np.random.default_rng(5).normal(size=(len(inputs), grid.numel()))).cuda()
+
+    def loss(t, grad):
+        dos, _, _ = props(t, with_grad=grad, fermi_grad=fermi_grad)
+        return (dos * c).sum()
+
+    t0 = tab.tables.detach().clone().requires_grad_(True)
+    loss(t0, True).backward()
+    D = torch.from_numpy(np.random.default_rng(6).normal(size=tuple(t0.shape))).cuda() * t0.detach().abs()
+    an = float((t0.grad * D).sum())
+    h = 1e-4
+    with torch.no_grad():
+        lp = float(loss((t0 + h * D).contiguous(), False))
+        lm = float(loss((t0 - h * D).contiguous(), False))
+    fd = (lp - lm) / (2 * h)
+    rel = abs(an - fd) / abs(fd)
+    print(f"{cells} ({[len(st.z) for st in sts]} atoms) fermi_grad={fermi_grad}: analytic {an:.8e} "
+          f"fd {fd:.8e} rel {rel:.2e}")
+    if fermi_grad:
+        assert rel < 1e-6
+    else:
+        assert rel > 1e-3
+
+
+def test_fitting_batch_32_cells_learns():
+    """The 32-cell fitting workload (8-64 atoms, 3 species, teacher tables
+    scaled by 1 + N(0, 0.1)): a few AdamW steps of the DOS + gap loss lower it."""
+    sts = synth.fitting_batch(seed=3, n_cells=32, species=SPECIES)
+    sp = sorted(SPECIES)
+    grid = torch.linspace(-20.0, 10.0, 2001, dtype=torch.float64, device="cuda")
+    teacher = TableSet(synth.make_skf_set(sp, seed=7, teacher_seed=8), synth.shell_dict_spd(sp), device="cuda")
+    ds = fit.teacher_datasets(sts, teacher, grid)
+    model = MultiElementSkfParameterOptimizer(skf_dicts=synth.make_skf_set(sp, seed=7), device="cuda").to("cuda")
+    _, hist = fit.train_skf_parameters(model, ds, synth.shell_dict_spd(sp), num_epochs=5, learning_rate=1e-4,
+                                       loss="dos_gap", log=None)
+    losses = [h["total_loss"] for h in hist]
+    print("32-cell losses", losses)
+    assert all(b < a for a, b in zip(losses, losses[1:]))
