@@ -127,3 +127,55 @@ def build_hs(z, positions, cell, shell_dict, skfs, kpoints, cutoff=10.0, kinds=(
 def n_electrons(z, skfs):
     """Sum of homonuclear occupations per atom (slakonet/optim.py:1132-1163)."""
     return float(sum(sum(skfs[(int(zi), int(zi))]["atomic_data"]["occupations"]) for zi in z))
+
+
+def build_hs_rows(z, positions, cell, shell_dict, skfs, kpoints, row_atoms, cutoff=10.0, kinds=("H", "S")):
+    """Rows of the selected atoms of H(k) / S(k), [nk, rows, n] (hs_matrix,
+    slakonet/slaterkoster.py:103-441, restricted to row atoms). Each block
+    (i, j) is evaluated the way the full build obtains it: directly from the
+    ordered pair (i, j) when l_i < l_j, or l_i == l_j and i <= j; otherwise as
+    the conjugate transpose of block (j, i) from the pair (j, i) (:358-403),
+    i.e. the SK block of the reversed table and direction, transposed."""
+    z = np.asarray(z)
+    positions = np.asarray(positions, dtype=np.float64)
+    cell = np.asarray(cell, dtype=np.float64)
+    kpoints = np.asarray(kpoints, dtype=np.float64).reshape(-1, 3)
+    layout, n = orbital_layout(z, shell_dict)
+    t, tcart = reference_images(cell, positions)
+    phase = np.exp(1j * ((2.0 * np.pi * kpoints) @ t.T))     # [nk, c]
+    nk = len(kpoints)
+    rows = []
+    for i in row_atoms:
+        rows.extend(range(layout[i][0][1], layout[i][-1][1] + 2 * layout[i][-1][0] + 1))
+    rmap = {r: q for q, r in enumerate(rows)}
+    out = {}
+    for kind in kinds:
+        mat = np.zeros((nk, len(rows), n), dtype=np.complex128)
+        for i in row_atoms:
+            vec = tcart[:, None, :] + positions[None, :, :] - positions[i][None, None, :]   # [c, j, 3]
+            dist = np.sqrt((vec ** 2).sum(-1))                                              # [c, j]
+            for j in range(len(z)):
+                cs = np.nonzero((dist[:, j] < cutoff) & (dist[:, j] > 0))[0]
+                if len(cs) == 0:
+                    continue
+                d = dist[cs, j]
+                u = vec[cs, j] / d[:, None]
+                for (la, oa) in layout[i]:
+                    for (lb, ob) in layout[j]:
+                        if la < lb or (la == lb and i <= j):
+                            yy, step = _table(skfs, z[i], z[j], kind, la, lb)
+                            blk = sub_block_rot(la, lb, u, poly_interp_u(yy, step, d))
+                        else:
+                            yy, step = _table(skfs, z[j], z[i], kind, lb, la)
+                            blk = sub_block_rot(lb, la, -u, poly_interp_u(yy, step, d)).transpose(0, 2, 1)
+                        acc = np.einsum("kc,cab->kab", phase[:, cs], blk)
+                        r0 = rmap[oa]
+                        mat[:, r0:r0 + 2 * la + 1, ob:ob + 2 * lb + 1] += acc
+        for i in row_atoms:
+            on = skfs[(int(z[i]), int(z[i]))]["atomic_data"]["on_sites"]
+            for (l, o) in layout[i]:
+                for q in range(2 * l + 1):
+                    mat[:, rmap[o + q], o + q] += on[l] if kind == "H" else 1.0
+        out[kind] = mat
+    out["rows"] = np.array(rows)
+    return out

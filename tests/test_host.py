@@ -263,3 +263,14 @@ def test_skf_parameter_optimizer_from_path_and_clamp_rule(tmp_path):
         o = opt.original_s_params[key]
         lo = torch.maximum(0.95 * o, torch.tensor(0.1)) if key[0] == key[2] else 0.9 * o
         assert torch.allclose(p.detach(), torch.minimum(lo, 1.05 * o if key[0] == key[2] else 1.1 * o)), key
+
+
+def test_large_cubic_fixture_is_the_generator_output():
+    """tests/data/large_cubic_1024.npz (read by the GPU tests of the
+    1024-atom cell) is exactly synth.large_cubic(): seeded, 1024 atoms in a
+    22 A cube, minimum distance 1.9 A, species 14/6/8."""
+    d = np.load(os.path.join(ROOT, "tests", "data", "large_cubic_1024.npz"))
+    st = synth.large_cubic()
+    assert np.array_equal(d["z"], st.z) and np.array_equal(d["positions"], st.positions)
+    assert np.array_equal(d["cell"], st.cell) and np.array_equal(d["kpoints"], st.kpoints)
+    assert len(st.z) == 1024 and set(st.z.tolist()) == {6, 8, 14}

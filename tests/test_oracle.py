@@ -459,3 +459,20 @@ def test_occupations_and_density_invariants():
         assert abs(np.trace(p @ s) - ne) < 1e-10
         np.testing.assert_allclose(p @ s @ p, 2.0 * p, atol=1e-10)
         assert np.abs(p.imag).max() > 1e-6          # complex k: conj matters
+
+
+def test_row_restricted_build_equals_full_build():
+    """oracle.build.build_hs_rows (the rows of a few atoms of a large cell)
+    reproduces the corresponding rows of the full oracle build on a 3-species
+    cell at complex k, including hetero (l1 != l2) and i > j blocks."""
+    rng = np.random.default_rng(17)
+    sp = [1, 22, 31]
+    z, pos, cell = synth.random_cell(rng, 7, sp, 1.8)
+    skfs = synth.make_skf_set(sp, seed=5)
+    k = np.array([[0.1, -0.2, 0.3], [0.0, 0.0, 0.0]])
+    sd = synth.shell_dict_spd(sp)
+    full = build.build_hs(z, pos * AA_BOHR, cell * AA_BOHR, sd, skfs, k)
+    part = build.build_hs_rows(z, pos * AA_BOHR, cell * AA_BOHR, sd, skfs, k, [0, 3, 6])
+    for kind in ("H", "S"):
+        np.testing.assert_allclose(part[kind], full[kind][:, part["rows"], :], rtol=0,
+                                   atol=1e-13 * np.abs(full[kind]).max())
