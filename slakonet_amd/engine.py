@@ -18,7 +18,7 @@ import torch
 from . import _lib
 from .geometry import lattice_translations
 from .tables import TableSet
-from .units import FERMI_KT_HA, FERMI_MAX_ITER, FERMI_TOL, HS_CUTOFF_BOHR, METAL_THRESHOLD_EV
+from .units import FERMI_KT_HA, FERMI_MAX_ITER, FERMI_TOL, H2E, HS_CUTOFF_BOHR, METAL_THRESHOLD_EV
 
 KC = 4          # k-points per build workgroup (slk_build.hip)
 MAXCOL = 64     # columns per build workgroup
@@ -341,6 +341,17 @@ def post(plan: BuildPlan, w: torch.Tensor, fermi_mode: int = 0, kT: float = FERM
                                       C.c_void_p(edges.data_ptr()), C.c_void_p(dos.data_ptr()),
                                       C.c_void_p(stats.data_ptr()), sh), "slk_dos_launch")
     return edges, dos, stats
+
+
+def reaches_padding(row: dict) -> bool:
+    """True when a structure's highest band (mu + its highest shifted level)
+    comes within 1e-6 Ha of the identity-padding level PAD_H: the padded
+    eigenvalues could then sort among the structure's own bands."""
+    if row.get("n_con", 0) <= 0:
+        top = row["mu"] + row["s_vbm"] / H2E
+    else:
+        top = row["mu"] + (row["s_cbm"] + row["cb_width"]) / H2E
+    return top >= PAD_H - 1e-6
 
 
 def edges_dict(row: np.ndarray) -> dict:

@@ -83,6 +83,9 @@ class BatchedProperties:
             host = edges.cpu().numpy()
             for s, i in enumerate(idx):
                 row = engine.edges_dict(host[s])
+                if plan.n_orb[s] < plan.ld and engine.reaches_padding(row):
+                    raise engine.EigenFailure(f"structure {i}: its spectrum reaches the identity-padding level "
+                                              f"({engine.PAD_H} Ha) of bucket {plan.ld}")
                 m0, nk, n = plan.mat0[s], plan.nk[s], plan.n_orb[s]
                 dos_out[i] = dos[s]
                 gap_out[i] = gap_tensor(wg[m0:m0 + nk, :n], row)

@@ -184,6 +184,10 @@ class ServingEngine:
                 res["error"] = f"S(k) not positive definite at k-point index {int(bad[0])}"
             elif len(bad_t):
                 res["error"] = engine.tridiag_failure(int(info[nmat + m0 + bad_t[0]]), int(bad_t[0]))
+            elif r.n_orb < b.ld and engine.reaches_padding(res):
+                res["ok"] = False
+                res["error"] = (f"the spectrum reaches the identity-padding level ({engine.PAD_H} Ha) of "
+                                f"bucket {b.ld}")
             if stats is not None:
                 res["dos_at_fermi"], res["dos_gap_center_eV"], res["dos_total_states"] = stats[s]
             if self.keep_outputs:

@@ -274,3 +274,13 @@ def test_large_cubic_fixture_is_the_generator_output():
     assert np.array_equal(d["z"], st.z) and np.array_equal(d["positions"], st.positions)
     assert np.array_equal(d["cell"], st.cell) and np.array_equal(d["kpoints"], st.kpoints)
     assert len(st.z) == 1024 and set(st.z.tolist()) == {6, 8, 14}
+
+
+def test_padding_level_guard():
+    """A structure whose highest band reaches the identity-padding level is
+    flagged (its padded eigenvalues could sort among its own bands)."""
+    base = dict(mu=0.1, n_con=5, s_cbm=0.5, cb_width=10.0, s_vbm=-0.3)
+    assert not engine.reaches_padding(base)
+    top_ev = (engine.PAD_H - 0.1) * 27.211 - 0.5 + 1.0
+    assert engine.reaches_padding(dict(base, cb_width=top_ev))
+    assert not engine.reaches_padding(dict(base, n_con=0))
