@@ -26,6 +26,13 @@ import os
 import sys
 import time
 
+# The serving engine runs one batch per HIP stream; HIP maps streams onto this
+# many hardware queues per process (its default, 4, serialises the engine's 8
+# streams pairwise). Read once at HIP initialisation, so it is set before torch
+# is imported.
+SERVE_STREAMS = 8
+os.environ["GPU_MAX_HW_QUEUES"] = str(SERVE_STREAMS)
+
 import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
@@ -60,6 +67,7 @@ def parse():
                    help="run only a plain device copy of this many MiB (counter calibration) and exit")
     p.add_argument("--cpu-seconds", type=float, default=20.0, help="CPU baseline sample budget (0 = skip)")
     p.add_argument("--no-roofline", action="store_true")
+    p.add_argument("--streams", type=int, default=SERVE_STREAMS, help="HIP streams of the serving engine")
     p.add_argument("--plain-seconds", type=float, default=10.0,
                    help="budget of the plain per-structure pass over a prefix of the stream (0 = skip)")
     p.add_argument("--precision", choices=["mixed", "fp64"], default="mixed",
@@ -121,9 +129,12 @@ def run_plain(tables, inputs, arrivals, grid, budget_s=None):
     return time.perf_counter() - t0, np.array(lat), results, len(lat)
 
 
+N_STREAMS = SERVE_STREAMS
+
+
 def run_serve(tables, inputs, arrivals, grid, precision=None, quiet=False):
     from slakonet_amd.serving import ServingEngine
-    eng = ServingEngine(tables, dos_grid=grid, precision=precision or PRECISION)
+    eng = ServingEngine(tables, dos_grid=grid, precision=precision or PRECISION, n_streams=N_STREAMS)
     res = eng.replay(inputs, arrivals)
     eng.close()
     if not quiet:
@@ -465,8 +476,9 @@ def cpu_baseline(structs, skfs, budget_s):
 
 def main():
     args = parse()
-    global PRECISION
+    global PRECISION, N_STREAMS
     PRECISION = args.precision
+    N_STREAMS = args.streams
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported (the stream is data-parallel "
               "per request and would shard by request across GPUs)", file=sys.stderr)

@@ -68,7 +68,7 @@ class Batch:
 
 
 class ServingEngine:
-    def __init__(self, tables: TableSet, dos_grid: Optional[torch.Tensor] = None, n_streams: int = 4,
+    def __init__(self, tables: TableSet, dos_grid: Optional[torch.Tensor] = None, n_streams: int = 8,
                  max_inflight: int = 8, hbm_budget_bytes: float = 64e9, max_batch_matrices: int = 512,
                  fermi_mode: int = 0, keep_outputs: bool = False, precision: str = "mixed",
                  refine_edges: bool = True, bucket=bucket_dim):
