@@ -1023,15 +1023,25 @@ __global__ __launch_bounds__(256) void fused_update_av_kernel(const typename C2T
                                                               typename C2T<T>::t* __restrict__ A,
                                                               typename C2T<T>::t* __restrict__ Yp, int n, int ld,
                                                               int r0n, int nch, long long sP, long long sM,
-                                                              unsigned long long* clk) {
+                                                              int ntri, int batch, unsigned long long* clk) {
   using C2 = typename C2T<T>::t;
   using V4 = typename MF<T>::v4;
   constexpr int KK = 2 * BW / 4;
-  const int I = blockIdx.y, J = blockIdx.x;
   sample_start(clk);
-  if (J > I) return;
+  // One workgroup per lower tile (I, J), J <= I, of every matrix. Workgroups
+  // are dealt round-robin to the 8 XCDs, so the launch order is remapped to
+  // give each XCD a contiguous run of tiles (row-major in the triangle): tiles
+  // of one row block share their V / W rows in that XCD's L2.
+  const int G = ntri * batch;
+  const int per = (G + 7) / 8;
+  const int lin = (int)(blockIdx.x % 8) * per + (int)(blockIdx.x / 8);
+  if (lin >= G) return;
+  const int b = lin / ntri, q = lin % ntri;
+  int I = (int)((sqrt(8.0 * q + 1.0) - 1.0) * 0.5);
+  while ((I + 1) * (I + 2) / 2 <= q) ++I;
+  while (I * (I + 1) / 2 > q) --I;
+  const int J = q - I * (I + 1) / 2;
   const int mn = n - r0n;
-  const int b = blockIdx.z;
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const int li = lane & 15, lk = lane >> 4;
   const C2* vp = Vp + (size_t)b * sP;
@@ -1247,12 +1257,15 @@ static void band_impl(hipStream_t st, typename C2T<T>::t* A, int n, int ld, int 
       SampleSlot fs;
       const bool fsam = sizeof(T) == 4 && sampler_enabled() &&
                         sampler_take(SLK_SAMPLE_FUSED_AV, (double)batch * 256.0 * mn * mn, &fs);
+      const int ntri = t * (t + 1) / 2;
+      const int nwg = (ntri * batch + 7) / 8 * 8;
       if (fsam)
-        hipExtLaunchKernelGGL(fused_update_av_kernel<T>, dim3(t, t, batch), dim3(256), 0, st, fs.e0, fs.e1, 0,
-                              cp.Vc + r0n, cp.Wc + r0n, cn.Vc + r0n, A, c.Yp, n, ld, r0n, c.nch, sP, sM, fs.clk);
+        hipExtLaunchKernelGGL(fused_update_av_kernel<T>, dim3(nwg), dim3(256), 0, st, fs.e0, fs.e1, 0,
+                              cp.Vc + r0n, cp.Wc + r0n, cn.Vc + r0n, A, c.Yp, n, ld, r0n, c.nch, sP, sM, ntri,
+                              batch, fs.clk);
       else
-        hipLaunchKernelGGL(fused_update_av_kernel<T>, dim3(t, t, batch), dim3(256), 0, st, cp.Vc + r0n,
-                           cp.Wc + r0n, cn.Vc + r0n, A, c.Yp, n, ld, r0n, c.nch, sP, sM,
+        hipLaunchKernelGGL(fused_update_av_kernel<T>, dim3(nwg), dim3(256), 0, st, cp.Vc + r0n,
+                           cp.Wc + r0n, cn.Vc + r0n, A, c.Yp, n, ld, r0n, c.nch, sP, sM, ntri, batch,
                            (unsigned long long*)nullptr);
       hipLaunchKernelGGL(band_x_kernel<T>, dim3(t, batch), dim3(256), 0, st, cn, r0n, t);
       hipLaunchKernelGGL(band_w_kernel<T>, dim3(t, batch), dim3(256), 0, st, cn, r0n, t);
