@@ -41,7 +41,9 @@
 #define BW SLK_BW
 #define AV_KT 64                      // k per LDS tile of band_av
 #define AV_KCH 128                    // k per workgroup of band_av (split K)
+#ifndef CH_THREADS
 #define CH_THREADS 1024
+#endif
 #define CH_WAVES (CH_THREADS / 64)
 #define BND_LD (2 * BW)               // compact band: entries per column
 
@@ -836,7 +838,9 @@ __global__ __launch_bounds__(CH_THREADS) void band_chase_kernel(BandCtx<T> c, un
   C2* bnd = LDSB ? reinterpret_cast<C2*>(ch_smem) : c.BND + (size_t)b * ld * BND_LD;
   double* D = c.D + (size_t)b * ld;
   double* E = c.E + (size_t)b * ld;
-  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  // the wave index is uniform: readfirstlane lets the compiler keep the
+  // sweep and step indices (and every address built from them) in SGPRs
+  const int tid = threadIdx.x, lane = tid & 63, w = __builtin_amdgcn_readfirstlane(tid >> 6);
   if (tid < CH_WAVES) s_prog[tid] = -1;
   if (tid == 0) s_fail = 0;
   for (int e = tid; e < n * BND_LD; e += CH_THREADS) {
