@@ -84,28 +84,60 @@ template <> struct MF<double> {
 };
 
 template <typename C2>
-__device__ __forceinline__ C2 shfl_c(C2 v, int src) {
-  v.x = __shfl(v.x, src, 64);
-  v.y = __shfl(v.y, src, 64);
-  return v;
-}
-template <typename C2>
-__device__ __forceinline__ C2 shfl_xor_c(C2 v, int o) {
-  v.x = __shfl_xor(v.x, o, 64);
-  v.y = __shfl_xor(v.y, o, 64);
-  return v;
-}
-template <typename C2>
 __device__ __forceinline__ C2 sum16c(C2 v) {
   v.x = sum16(v.x);
   v.y = sum16(v.y);
   return v;
 }
-// sum over the 4 lane groups of a wave (same lane % 16)
+// Pair sums across rows (lane ^ 16) and halves (lane ^ 32) of a wave with the
+// gfx950 permlane swaps (VALU, no LDS round trip): given v twice, the swap
+// returns the lane's own value and its partner's, one in each result.
+__device__ __forceinline__ unsigned u32_of(float v) { return __builtin_bit_cast(unsigned, v); }
+__device__ __forceinline__ float f32_of(unsigned u) { return __builtin_bit_cast(float, u); }
+template <int ROWS>
+__device__ __forceinline__ float add_partner(float v) {
+  const unsigned u = u32_of(v);
+  if constexpr (ROWS == 16) {
+    const auto p = __builtin_amdgcn_permlane16_swap(u, u, false, false);
+    return f32_of(p[0]) + f32_of(p[1]);
+  } else {
+    const auto p = __builtin_amdgcn_permlane32_swap(u, u, false, false);
+    return f32_of(p[0]) + f32_of(p[1]);
+  }
+}
+template <int ROWS>
+__device__ __forceinline__ double add_partner(double v) {
+  const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
+  const unsigned lo = (unsigned)u, hi = (unsigned)(u >> 32);
+  unsigned a0, a1, b0, b1;
+  if constexpr (ROWS == 16) {
+    const auto p = __builtin_amdgcn_permlane16_swap(lo, lo, false, false);
+    const auto q = __builtin_amdgcn_permlane16_swap(hi, hi, false, false);
+    a0 = p[0], a1 = p[1], b0 = q[0], b1 = q[1];
+  } else {
+    const auto p = __builtin_amdgcn_permlane32_swap(lo, lo, false, false);
+    const auto q = __builtin_amdgcn_permlane32_swap(hi, hi, false, false);
+    a0 = p[0], a1 = p[1], b0 = q[0], b1 = q[1];
+  }
+  const double x = __builtin_bit_cast(double, ((unsigned long long)b0 << 32) | a0);
+  const double y = __builtin_bit_cast(double, ((unsigned long long)b1 << 32) | a1);
+  return x + y;
+}
+// sum over the 4 lane groups of a wave (same lane % 16): (g0 + g1) + (g2 + g3)
+// in every lane
 template <typename C2>
 __device__ __forceinline__ C2 sum4g(C2 v) {
-  v = c_add(v, shfl_xor_c(v, 16));
-  return c_add(v, shfl_xor_c(v, 32));
+  v.x = add_partner<32>(add_partner<16>(v.x));
+  v.y = add_partner<32>(add_partner<16>(v.y));
+  return v;
+}
+// value of lane 0 (v_readlane: a scalar read, no LDS round trip)
+__device__ __forceinline__ float lane0(float v) { return f32_of((unsigned)__builtin_amdgcn_readlane((int)u32_of(v), 0)); }
+__device__ __forceinline__ double lane0(double v) {
+  const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
+  const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)u, 0);
+  const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(u >> 32), 0);
+  return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
 }
 
 // Householder reflector of x = (alpha, rest) with |rest|^2 = xn2 (LAPACK
@@ -602,7 +634,7 @@ __device__ __forceinline__ T chase_step(typename C2T<T>::t* bnd, typename C2T<T>
   }
   // (a)
   const T xn2 = sum16((li >= 1 && (FULL || li < L)) ? x.x * x.x + x.y * x.y : T(0));
-  const C2 alpha = mk<T>(__shfl(x.x, 0, 64), __shfl(x.y, 0, 64));
+  const C2 alpha = mk<T>(lane0(x.x), lane0(x.y));
   const House<T> h = house<T>(alpha, xn2);
   const C2 tau = h.tau;
   const C2 v = li == 0 ? mk<T>(1, 0) : (((FULL || li < L) && !h.skip) ? c_mul(x, h.scale) : zero);
@@ -701,12 +733,28 @@ __device__ __forceinline__ ChaseLane chase_lane(int lane) {
 __device__ __forceinline__ float rcp_t(float x) { return __builtin_amdgcn_rcpf(x); }
 __device__ __forceinline__ double rcp_t(double x) { return 1.0 / x; }
 
+// FP32 global band through buffer instructions: a uniform descriptor for the
+// matrix's band, one per-step uniform byte offset per block (soffset) and the
+// lane's constant byte offset (voffset; the t strides become immediates). The
+// descriptor's range is the band (n BND_LD elements).
+typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ float2 band_ld(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff) {
+  const u32x2 u = __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0);
+  return make_float2(f32_of(u[0]), f32_of(u[1]));
+}
+__device__ __forceinline__ void band_st(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff, float2 v) {
+  const u32x2 u = {u32_of(v.x), u32_of(v.y)};
+  __builtin_amdgcn_raw_buffer_store_b64(u, r, voff, soff, 0);
+}
+
 // A full step (L = BW reflector rows, BW-1 bulge columns, BW rows below),
-// with the addressing reduced to uniform bases plus ChaseLane offsets.
-template <typename T>
+// with the addressing reduced to uniform bases plus ChaseLane offsets. BUF
+// (FP32 global band): every band access is a buffer instruction on rb.
+template <typename T, bool BUF>
 __device__ __forceinline__ T chase_step_full(typename C2T<T>::t* bnd, typename C2T<T>::t* v2row,
                                              typename C2T<T>::t* sv, int c1, int q1, int lane, const ChaseLane& o,
-                                             typename C2T<T>::t (*stg)[BW + 1], bool from_stage, bool to_stage) {
+                                             typename C2T<T>::t (*stg)[BW + 1], bool from_stage, bool to_stage,
+                                             __amdgpu_buffer_rsrc_t rb) {
   using C2 = typename C2T<T>::t;
   const int li = lane & 15, lg = lane >> 4;
   const C2 zero = mk<T>(0, 0);
@@ -714,24 +762,44 @@ __device__ __forceinline__ T chase_step_full(typename C2T<T>::t* bnd, typename C
   C2* bb = bnd + 31 * (c1 + 1) + q1;      // bulge block
   C2* bd = bnd + 32 * q1;                 // diagonal block
   C2* be = bnd + 32 * q1 + BW;            // block below
+  // byte offsets of the same four bases (BUF)
+  const unsigned sx = 8u * (31 * c1 + q1), sb = 8u * (31 * (c1 + 1) + q1), sd = 8u * 32 * q1, se = sd + 8u * BW;
   // from_stage: the previous step of this sweep left its updated block below
   // (this step's column c1 and bulge block) in the wave's stage, not in the band
-  const C2 x = from_stage ? stg[li][0] : bx[li];
+  C2 x;
+  if (from_stage)
+    x = stg[li][0];
+  else if constexpr (BUF)
+    x = band_ld(rb, 8u * li, sx);
+  else
+    x = bx[li];
   C2 eb[4], ed[4], ee[4];
 #pragma unroll
   for (int t = 0; t < 4; ++t) {
-    if (from_stage)
+    if (from_stage) {
       eb[t] = li < BW - 1 ? stg[lg + 4 * t][1 + li] : zero;
-    else
+    } else if constexpr (BUF) {
+      // lane li = 15 reads column q1 (in the band) and drops it
+      const C2 e = band_ld(rb, 8u * cl_ob(o, t), sb);
+      eb[t] = li < BW - 1 ? e : zero;
+    } else {
       eb[t] = load_or_zero(&at_off(bb, cl_ob(o, t)), li < BW - 1);
-    C2 d = at_off(bd, cl_od(o, t));
+    }
+    C2 d, e;
+    if constexpr (BUF) {
+      d = band_ld(rb, 8u * cl_od(o, t), sd);
+      e = band_ld(rb, 8u * cl_oe(o, t), se);
+    } else {
+      d = at_off(bd, cl_od(o, t));
+      e = at_off(be, cl_oe(o, t));
+    }
     if (cl_up(o, t)) d.y = -d.y;
     ed[t] = d;
-    ee[t] = at_off(be, cl_oe(o, t));
+    ee[t] = e;
   }
   // (a) reflector
   const T xn2 = sum16(li >= 1 ? x.x * x.x + x.y * x.y : T(0));
-  const C2 alpha = mk<T>(__shfl(x.x, 0, 64), __shfl(x.y, 0, 64));
+  const C2 alpha = mk<T>(lane0(x.x), lane0(x.y));
   C2 tau, scale;
   T beta;
   bool skip;
@@ -751,7 +819,11 @@ __device__ __forceinline__ T chase_step_full(typename C2T<T>::t* bnd, typename C
   }
   const C2 v = li == 0 ? mk<T>(1, 0) : (skip ? zero : c_mul(x, scale));
   if (lg == 0) {
-    bx[li] = li == 0 ? mk<T>(beta, 0) : zero;
+    const C2 xn = li == 0 ? mk<T>(beta, 0) : zero;
+    if constexpr (BUF)
+      band_st(rb, 8u * li, sx, xn);
+    else
+      bx[li] = xn;
     if (v2row != nullptr) v2row[q1 + li] = li == 0 ? tau : v;
     sv[li] = v;
   }
@@ -772,7 +844,13 @@ __device__ __forceinline__ T chase_step_full(typename C2T<T>::t* bnd, typename C
   const C2 fb = c_mul(c_conj(tau), db);
 #pragma unroll
   for (int t = 0; t < 4; ++t)
-    if (li < BW - 1) at_off(bb, cl_ob(o, t)) = c_sub(eb[t], c_mul(vr[t], fb));
+    if (li < BW - 1) {
+      const C2 nb = c_sub(eb[t], c_mul(vr[t], fb));
+      if constexpr (BUF)
+        band_st(rb, 8u * cl_ob(o, t), sb, nb);
+      else
+        at_off(bb, cl_ob(o, t)) = nb;
+    }
   const T vhy = sum16(v.x * y.x + v.y * y.y);
   const T t2 = T(0.5) * (tau.x * tau.x + tau.y * tau.y) * vhy;
   const C2 ty = c_mul(tau, y);
@@ -790,15 +868,21 @@ __device__ __forceinline__ T chase_step_full(typename C2T<T>::t* bnd, typename C
       e.x -= t1.x + t3.x;
       e.y -= t1.y + t3.y;
       if (r == li) e.y = T(0);
-      at_off(bd, cl_od(o, t)) = e;
+      if constexpr (BUF)
+        band_st(rb, 8u * cl_od(o, t), sd, e);
+      else
+        at_off(bd, cl_od(o, t)) = e;
     }
     // to_stage: the next step of this sweep (full, same wave) takes the
     // block below from the stage and writes it back to the band itself
     const C2 ev = c_sub(ee[t], c_mulc(tz, vr[t]));
-    if (to_stage)
+    if (to_stage) {
       stg[li][r] = ev;
-    else
+    } else if constexpr (BUF) {
+      band_st(rb, 8u * cl_oe(o, t), se, ev);
+    } else {
       at_off(be, cl_oe(o, t)) = ev;
+    }
   }
   __builtin_amdgcn_wave_barrier();
   return beta;
@@ -851,6 +935,10 @@ __global__ __launch_bounds__(CH_THREADS) void band_chase_kernel(BandCtx<T> c, un
   const int ns = n - 1;   // sweeps 0 .. n-2
   const int pw = (w + CH_WAVES - 1) % CH_WAVES;   // wave of the predecessor sweep
   const ChaseLane lo = chase_lane(lane);
+  // FP32 global band: full steps address it through a buffer descriptor
+  constexpr bool BUF = !LDSB && sizeof(T) == 4;
+  const __amdgpu_buffer_rsrc_t rb =
+      __builtin_amdgcn_make_buffer_rsrc(bnd, 0, BUF ? n * BND_LD * (int)sizeof(C2) : 0, 0x00020000);
   bool ok = true;
   for (int s = w; s < ns; s += CH_WAVES) {
     const int nst = chase_nsteps(n, s);
@@ -886,8 +974,8 @@ __global__ __launch_bounds__(CH_THREADS) void band_chase_kernel(BandCtx<T> c, un
       T beta;
       if (full) {
         const bool to_stage = !LDSB && cs + 1 < nst && q1 + 3 * BW - 1 <= n - 1;
-        beta = chase_step_full<T>(bnd, a + (size_t)s * ld, s_vw[w], c1, q1, lane, lo, s_stage[LDSB ? 0 : w],
-                                  staged, to_stage);
+        beta = chase_step_full<T, BUF>(bnd, a + (size_t)s * ld, s_vw[w], c1, q1, lane, lo, s_stage[LDSB ? 0 : w],
+                                       staged, to_stage, rb);
         staged = to_stage;
       } else {
         beta = chase_step<T, false>(bnd, a + (size_t)s * ld, s_vw[w], n, c1, q1, q2, lane);
