@@ -123,14 +123,63 @@ __device__ __forceinline__ double add_partner(double v) {
   const double y = __builtin_bit_cast(double, ((unsigned long long)b1 << 32) | a1);
   return x + y;
 }
-// sum over the 4 lane groups of a wave (same lane % 16): (g0 + g1) + (g2 + g3)
-// in every lane
+// Sums over the 4 lane groups of two values at once (3 swaps for 2 values):
+// a row-pair swap of a against b leaves each row pair holding the pair sums
+// of a in one row and of b in the other, the half swap completes both totals,
+// and a last row-pair swap hands every lane both. Each total is
+// (g0 + g1) + (g2 + g3).
+__device__ __forceinline__ void sum4g_pair(float& a, float& b) {
+  const auto p = __builtin_amdgcn_permlane16_swap(u32_of(a), u32_of(b), false, false);
+  const float c = f32_of(p[0]) + f32_of(p[1]);
+  const auto q = __builtin_amdgcn_permlane32_swap(u32_of(c), u32_of(c), false, false);
+  const float d = f32_of(q[0]) + f32_of(q[1]);
+  const auto r = __builtin_amdgcn_permlane16_swap(u32_of(d), u32_of(d), false, false);
+  a = f32_of(r[0]);
+  b = f32_of(r[1]);
+}
+__device__ __forceinline__ void sum4g_pair(double& a, double& b) {
+  a = add_partner<32>(add_partner<16>(a));
+  b = add_partner<32>(add_partner<16>(b));
+}
 template <typename C2>
-__device__ __forceinline__ C2 sum4g(C2 v) {
-  v.x = add_partner<32>(add_partner<16>(v.x));
-  v.y = add_partner<32>(add_partner<16>(v.y));
+__device__ __forceinline__ C2 sum4g_c(C2 v) {
+  sum4g_pair(v.x, v.y);
   return v;
 }
+
+// Complex multiply-accumulate as explicit FMA chains (4 FMAs each instead of
+// the products' 2 multiplies + 2 FMAs + 2 adds).
+__device__ __forceinline__ float fmat(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ __forceinline__ double fmat(double a, double b, double c) { return __builtin_fma(a, b, c); }
+// acc + a b
+template <typename C2>
+__device__ __forceinline__ C2 cfma(C2 a, C2 b, C2 acc) {
+  acc.x = fmat(-a.y, b.y, fmat(a.x, b.x, acc.x));
+  acc.y = fmat(a.y, b.x, fmat(a.x, b.y, acc.y));
+  return acc;
+}
+// acc + conj(a) b
+template <typename C2>
+__device__ __forceinline__ C2 ccfma(C2 a, C2 b, C2 acc) {
+  acc.x = fmat(a.y, b.y, fmat(a.x, b.x, acc.x));
+  acc.y = fmat(-a.y, b.x, fmat(a.x, b.y, acc.y));
+  return acc;
+}
+// e - a b
+template <typename C2>
+__device__ __forceinline__ C2 cfms(C2 a, C2 b, C2 e) {
+  e.x = fmat(a.y, b.y, fmat(-a.x, b.x, e.x));
+  e.y = fmat(-a.y, b.x, fmat(-a.x, b.y, e.y));
+  return e;
+}
+// e - a conj(b)
+template <typename C2>
+__device__ __forceinline__ C2 cfmsc(C2 a, C2 b, C2 e) {
+  e.x = fmat(-a.y, b.y, fmat(-a.x, b.x, e.x));
+  e.y = fmat(a.x, b.y, fmat(-a.y, b.x, e.y));
+  return e;
+}
+
 // value of lane 0 (v_readlane: a scalar read, no LDS round trip)
 __device__ __forceinline__ float lane0(float v) { return f32_of((unsigned)__builtin_amdgcn_readlane((int)u32_of(v), 0)); }
 __device__ __forceinline__ double lane0(double v) {
@@ -652,22 +701,22 @@ __device__ __forceinline__ T chase_step(typename C2T<T>::t* bnd, typename C2T<T>
   // (b) d_cb = sum_r conj(v_r) B[r][cb]; B[r][cb] -= conj(tau) v_r d_cb
   C2 db = zero;
 #pragma unroll
-  for (int t = 0; t < 4; ++t) db = c_add(db, c_cmul(vr[t], eb[t]));
+  for (int t = 0; t < 4; ++t) db = ccfma(vr[t], eb[t], db);
   // (c) y_li = sum_c D[li][c] v_c;  (d) z_li = sum_c E[li][c] v_c
   C2 y = zero, z = zero;
 #pragma unroll
   for (int t = 0; t < 4; ++t) {
-    y = c_add(y, c_mul(ed[t], vr[t]));
-    z = c_add(z, c_mul(ee[t], vr[t]));
+    y = cfma(ed[t], vr[t], y);
+    z = cfma(ee[t], vr[t], z);
   }
-  db = sum4g(db);
-  y = sum4g(y);
-  z = sum4g(z);
+  db = sum4g_c(db);
+  y = sum4g_c(y);
+  z = sum4g_c(z);
   const C2 fb = c_mul(c_conj(tau), db);
 #pragma unroll
   for (int t = 0; t < 4; ++t) {
     const int r = lg + 4 * t;
-    if ((FULL || r < L) && li < nbc) bnd[cb * BND_LD + q1 + r - cb] = c_sub(eb[t], c_mul(vr[t], fb));
+    if ((FULL || r < L) && li < nbc) bnd[cb * BND_LD + q1 + r - cb] = cfms(vr[t], fb, eb[t]);
   }
   const T vhy = sum16((FULL || li < L) ? v.x * y.x + v.y * y.y : T(0));
   const T t2 = T(0.5) * (tau.x * tau.x + tau.y * tau.y) * vhy;
@@ -681,15 +730,12 @@ __device__ __forceinline__ T chase_step(typename C2T<T>::t* bnd, typename C2T<T>
     const int r = lg + 4 * t;
     const C2 wr = sv[BW + r];
     if ((FULL || li < L) && r <= li) {
-      const C2 t1 = c_mulc(v, wr), t3 = c_mulc(w, vr[t]);
-      C2 e = ed[t];
-      e.x -= t1.x + t3.x;
-      e.y -= t1.y + t3.y;
+      C2 e = cfmsc(v, wr, cfmsc(w, vr[t], ed[t]));
       if (r == li) e.y = T(0);
       bnd[(q1 + r) * BND_LD + li - r] = e;
     }
     if ((FULL || r < L) && (FULL || li < Le))
-      bnd[(q1 + r) * BND_LD + q2 + 1 + li - q1 - r] = c_sub(ee[t], c_mulc(tz, vr[t]));
+      bnd[(q1 + r) * BND_LD + q2 + 1 + li - q1 - r] = cfmsc(tz, vr[t], ee[t]);
   }
   return h.beta;
 }
@@ -834,18 +880,18 @@ __device__ __forceinline__ T chase_step_full(typename C2T<T>::t* bnd, typename C
   C2 db = zero, y = zero, z = zero;
 #pragma unroll
   for (int t = 0; t < 4; ++t) {
-    db = c_add(db, c_cmul(vr[t], eb[t]));
-    y = c_add(y, c_mul(ed[t], vr[t]));
-    z = c_add(z, c_mul(ee[t], vr[t]));
+    db = ccfma(vr[t], eb[t], db);
+    y = cfma(ed[t], vr[t], y);
+    z = cfma(ee[t], vr[t], z);
   }
-  db = sum4g(db);
-  y = sum4g(y);
-  z = sum4g(z);
+  db = sum4g_c(db);
+  y = sum4g_c(y);
+  z = sum4g_c(z);
   const C2 fb = c_mul(c_conj(tau), db);
 #pragma unroll
   for (int t = 0; t < 4; ++t)
     if (li < BW - 1) {
-      const C2 nb = c_sub(eb[t], c_mul(vr[t], fb));
+      const C2 nb = cfms(vr[t], fb, eb[t]);
       if constexpr (BUF)
         band_st(rb, 8u * cl_ob(o, t), sb, nb);
       else
@@ -863,10 +909,7 @@ __device__ __forceinline__ T chase_step_full(typename C2T<T>::t* bnd, typename C
     const int r = lg + 4 * t;
     const C2 wr = sv[BW + r];
     if (!cl_up(o, t)) {
-      const C2 t1 = c_mulc(v, wr), t3 = c_mulc(w, vr[t]);
-      C2 e = ed[t];
-      e.x -= t1.x + t3.x;
-      e.y -= t1.y + t3.y;
+      C2 e = cfmsc(v, wr, cfmsc(w, vr[t], ed[t]));
       if (r == li) e.y = T(0);
       if constexpr (BUF)
         band_st(rb, 8u * cl_od(o, t), sd, e);
@@ -875,7 +918,7 @@ __device__ __forceinline__ T chase_step_full(typename C2T<T>::t* bnd, typename C
     }
     // to_stage: the next step of this sweep (full, same wave) takes the
     // block below from the stage and writes it back to the band itself
-    const C2 ev = c_sub(ee[t], c_mulc(tz, vr[t]));
+    const C2 ev = cfmsc(tz, vr[t], ee[t]);
     if (to_stage) {
       stg[li][r] = ev;
     } else if constexpr (BUF) {
