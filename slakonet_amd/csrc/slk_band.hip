@@ -992,8 +992,10 @@ __global__ __launch_bounds__(CH_THREADS) void band_chase_kernel(BandCtx<T> c, un
         const int need = ((s - 1) << 12) + min(cs + 3, 4095);
         int spins = 0;
         if (c.spin_max < 0) ok = false;
+        // sleep (8 x 64 cycles at most) until a wave of the workgroup
+        // publishes a step and wakes the sleepers (s_wakeup below)
         while (ok && __hip_atomic_load(&s_prog[pw], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP) < need) {
-          __builtin_amdgcn_s_sleep(1);
+          __builtin_amdgcn_s_sleep(8);
           if (++spins > c.spin_max) {
             ok = false;
             break;
@@ -1027,6 +1029,7 @@ __global__ __launch_bounds__(CH_THREADS) void band_chase_kernel(BandCtx<T> c, un
       if (cs == 0 && lane == 0) E[s] = (double)beta;
       __hip_atomic_store(&s_prog[w], (s << 12) + (cs + 1 == nst ? 4095 : cs + 1), __ATOMIC_RELEASE,
                          __HIP_MEMORY_SCOPE_WORKGROUP);
+      asm volatile("s_wakeup");
     }
   }
   if (!ok && lane == 0) s_fail = 1;
