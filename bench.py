@@ -68,6 +68,8 @@ def parse():
     p.add_argument("--cpu-seconds", type=float, default=20.0, help="CPU baseline sample budget (0 = skip)")
     p.add_argument("--no-roofline", action="store_true")
     p.add_argument("--streams", type=int, default=SERVE_STREAMS, help="HIP streams of the serving engine")
+    p.add_argument("--graph-max-ld", type=int, default=256,
+                   help="buckets up to this dimension replay their eigensolve from a HIP graph (0 = never)")
     p.add_argument("--plain-seconds", type=float, default=10.0,
                    help="budget of the plain per-structure pass over a prefix of the stream (0 = skip)")
     p.add_argument("--precision", choices=["mixed", "fp64"], default="mixed",
@@ -130,13 +132,18 @@ def run_plain(tables, inputs, arrivals, grid, budget_s=None):
 
 
 N_STREAMS = SERVE_STREAMS
+GRAPH_MAX_LD = 256
+GRAPH_STATS = {"captures": 0, "replays": 0}
 
 
 def run_serve(tables, inputs, arrivals, grid, precision=None, quiet=False):
     from slakonet_amd.serving import ServingEngine
-    eng = ServingEngine(tables, dos_grid=grid, precision=precision or PRECISION, n_streams=N_STREAMS)
+    eng = ServingEngine(tables, dos_grid=grid, precision=precision or PRECISION, n_streams=N_STREAMS,
+                        graphs=GRAPH_MAX_LD > 0, graph_max_ld=GRAPH_MAX_LD)
     res = eng.replay(inputs, arrivals)
     eng.close()
+    GRAPH_STATS["captures"] += eng.graph_captures
+    GRAPH_STATS["replays"] += eng.graph_replays
     if not quiet:
         by_ld = {}
         for ld, nmat, ns, t0, t1 in eng.batch_log:
@@ -476,9 +483,10 @@ def cpu_baseline(structs, skfs, budget_s):
 
 def main():
     args = parse()
-    global PRECISION, N_STREAMS
+    global PRECISION, N_STREAMS, GRAPH_MAX_LD
     PRECISION = args.precision
     N_STREAMS = args.streams
+    GRAPH_MAX_LD = args.graph_max_ld
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported (the stream is data-parallel "
               "per request and would shard by request across GPUs)", file=sys.stderr)
@@ -559,6 +567,10 @@ def main():
                    "mode": args.mode, "requests": args.requests, "rate_per_s": args.rate,
                    "latency": "arrival to result on the host, pooled over the timed steps"},
     }
+    if args.mode == "serve":
+        out["config"]["serving"] = {"streams": N_STREAMS, "graph_max_ld": GRAPH_MAX_LD,
+                                    "graph_captures": GRAPH_STATS["captures"],
+                                    "graph_replays": GRAPH_STATS["replays"]}
     print(json.dumps({"partial": True, "value": value, "p99_latency_ms": out["p99_latency_ms"]}), file=sys.stderr,
           flush=True)
     if args.mode == "serve":

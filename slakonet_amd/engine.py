@@ -159,12 +159,23 @@ def build_args(tables: TableSet, plan: BuildPlan, H_ptr: int, S_ptr: int, out_f3
 
 
 def run_build(tables: TableSet, plan: BuildPlan, out_f32: bool = False, cutoff: float = HS_CUTOFF_BOHR,
-              table_tensor: Optional[torch.Tensor] = None, stream=None):
-    """H, S [n_mat, ld, ld] complex128 (or complex64)."""
+              table_tensor: Optional[torch.Tensor] = None, stream=None, out=None):
+    """H, S [n_mat, ld, ld] complex128 (or complex64). ``out`` = (H, S): write
+    into these buffers (every element is written: the build or the padding)."""
     lib = _lib.lib()
     dt = torch.complex64 if out_f32 else torch.complex128
-    H = torch.empty(plan.n_mat, plan.ld, plan.ld, dtype=dt, device=tables.device)
-    S = torch.empty_like(H)
+    if out is not None:
+        H, S = out
+        shape = (plan.n_mat, plan.ld, plan.ld)
+        dev = torch.device(tables.device)
+        if dev.index is None:
+            dev = torch.device(dev.type, torch.cuda.current_device())
+        for t in (H, S):
+            if tuple(t.shape) != shape or t.dtype != dt or not t.is_contiguous() or t.device != dev:
+                raise ValueError(f"out buffers must be contiguous {dt} {shape} on {dev}")
+    else:
+        H = torch.empty(plan.n_mat, plan.ld, plan.ld, dtype=dt, device=tables.device)
+        S = torch.empty_like(H)
     d = plan.dev
     args = build_args(tables, plan, H.data_ptr(), S.data_ptr(), out_f32, cutoff, table_tensor)
     sh = _lib.stream_handle(stream)

@@ -15,12 +15,25 @@ hardware queue is full blocks only its own thread while the admission loop
 and the other streams keep going (ctypes releases the GIL in the library
 calls).
 
+Small buckets (ld <= graph_max_ld) replay their eigensolve from a HIP graph:
+the second batch of a (ld, matrices) shape captures the solve over that
+batch's H and S buffers, which become persistent; later batches of the shape
+(on any stream, after the previous user's completion event) build straight
+into those buffers and replay the graph; a shape whose graph is busy on
+another launch thread is solved eagerly. The cached
+graphs (least recently used evicted beyond graph_cache_bytes, once their last
+replay has completed) hold their buffers outside the per-batch HBM budget.
+For small solves the launch chain is comparable to the work, so a replay is
+5-15 % shorter; from ld 512 up it gains nothing measurable and the eager path
+is kept.
+
 SK tables stay in HBM for the lifetime of the engine.
 """
 from __future__ import annotations
 
+import threading
 import time
-from collections import deque
+from collections import OrderedDict, deque
 from concurrent.futures import Future, ThreadPoolExecutor
 from dataclasses import dataclass, field
 from typing import Dict, List, Optional, Sequence
@@ -67,11 +80,27 @@ class Batch:
     t_launch: float = 0.0
 
 
+@dataclass
+class SolveGraph:
+    """A captured eigensolve of one (ld, matrices) shape and the
+    persistent buffers it reads (H, S) and writes (w, info, workspace)."""
+    graph: torch.cuda.CUDAGraph
+    H: torch.Tensor
+    S: torch.Tensor
+    w: torch.Tensor
+    info: torch.Tensor
+    ws: torch.Tensor
+    nbytes: int
+    done: Optional[torch.cuda.Event] = None
+    busy: bool = True   # a batch is between its replay and the record of ``done``
+
+
 class ServingEngine:
     def __init__(self, tables: TableSet, dos_grid: Optional[torch.Tensor] = None, n_streams: int = 8,
                  max_inflight: int = 8, hbm_budget_bytes: float = 64e9, max_batch_matrices: int = 512,
                  fermi_mode: int = 0, keep_outputs: bool = False, precision: str = "mixed",
-                 refine_edges: bool = True, bucket=bucket_dim):
+                 refine_edges: bool = True, bucket=bucket_dim, graphs: bool = True, graph_max_ld: int = 256,
+                 graph_cache_bytes: float = 8e9):
         self.tables = tables
         self.dev = tables.device
         self.dev_index = self.dev.index if self.dev.index is not None else torch.cuda.current_device()
@@ -79,6 +108,9 @@ class ServingEngine:
         self.streams = [torch.cuda.Stream(device=self.dev) for _ in range(n_streams)]
         # one launch thread per stream keeps each stream's launch order
         self.launchers = [ThreadPoolExecutor(max_workers=1) for _ in range(n_streams)]
+        # graph captures run on streams of their own: a capturing stream must
+        # not be one whose earlier events the admission loop still queries
+        self.capture_streams = [torch.cuda.Stream(device=self.dev) for _ in range(n_streams)]
         self._pending = [0] * n_streams
         self._pending_cost = [0.0] * n_streams
         self.max_inflight = max_inflight
@@ -92,6 +124,15 @@ class ServingEngine:
         self.nf = len(engine.EDGE_FIELDS)
         self._rr = 0
         self._bytes_cache = {}
+        self.graphs = graphs
+        self.graph_max_ld = graph_max_ld
+        self.graph_cache_bytes = graph_cache_bytes
+        self._graphs: "OrderedDict[tuple, SolveGraph]" = OrderedDict()
+        self._graph_seen: Dict[tuple, int] = {}
+        self._graph_bytes = 0
+        self._graph_lock = threading.Lock()
+        self.graph_replays = 0
+        self.graph_captures = 0
 
     # ------------------------------------------------------------------
     def _orbitals(self, inp: engine.StructureInput) -> int:
@@ -131,7 +172,7 @@ class ServingEngine:
         self._rr += 1
         self._pending[i] += 1
         self._pending_cost[i] += cost
-        fut = self.launchers[i].submit(self._launch, reqs, ld, self.streams[i])
+        fut = self.launchers[i].submit(self._launch, reqs, ld, self.streams[i], i)
         fut.stream_index = i
         fut.cost = cost
         fut.reqs = reqs
@@ -141,16 +182,99 @@ class ServingEngine:
         for ex in self.launchers:
             ex.shutdown(wait=True)
 
-    def _launch(self, reqs: List[Request], ld: int, stream) -> Batch:
+    # ------------------------------------------------------------------
+    def _graph_get(self, key) -> Optional[SolveGraph]:
+        with self._graph_lock:
+            e = self._graphs.get(key)
+            if e is not None:
+                if e.busy:     # another launch thread is between its replay and the record of ``done``
+                    return None
+                self._graphs.move_to_end(key)
+                e.busy = True
+                return e
+            self._graph_seen[key] = self._graph_seen.get(key, 0) + 1
+            return None
+
+    def _graph_wanted(self, key, nbytes: int) -> bool:
+        """Capture on the second batch of a shape, if it fits the cache and
+        no other launch thread holds a graph of the shape."""
+        with self._graph_lock:
+            return (key not in self._graphs and self._graph_seen.get(key, 0) >= 2 and
+                    nbytes <= self.graph_cache_bytes / 8)
+
+    def _graph_insert(self, key, e: SolveGraph):
+        with self._graph_lock:
+            self._graphs[key] = e
+            self._graph_bytes += e.nbytes
+            # evict the least recently used entries whose last replay is done
+            for k in list(self._graphs):
+                if self._graph_bytes <= self.graph_cache_bytes:
+                    break
+                old = self._graphs[k]
+                if k == key or old.busy or (old.done is not None and not old.done.query()):
+                    continue
+                del self._graphs[k]
+                self._graph_bytes -= old.nbytes
+
+    def _capture(self, H: torch.Tensor, S: torch.Tensor, si: int) -> SolveGraph:
+        """Capture the eigensolve over H, S on the launch thread's capture
+        stream; the capture records without executing, the caller replays on
+        its serving stream (after the build that fills H and S)."""
+        g = torch.cuda.CUDAGraph()
+        wsl = []
+        with torch.cuda.stream(self.capture_streams[si]):
+            g.capture_begin(capture_error_mode="thread_local")
+            try:
+                w, _, info = engine.eigh(H, S, check=False, precision=self.precision, workspace=wsl)
+            finally:
+                g.capture_end()
+        nbytes = 2 * H.numel() * H.element_size() + wsl[0].numel() + w.numel() * 8 + info.numel() * 4
+        return SolveGraph(g, H, S, w, info, wsl[0], nbytes)
+
+    def _solve(self, plan, ld: int, nmat: int, stream, si: int):
+        """Build and solve one batch on the current stream: from a cached
+        graph for small buckets, else eagerly. Returns H, S, w, info, workspace."""
+        key = (ld, nmat)
+        # no capture while the kernel sampler is on (its launches record events)
+        graphs = self.graphs and ld <= self.graph_max_ld and not engine.SAMPLING
+        e = self._graph_get(key) if graphs else None
+        if e is not None:
+            if e.done is not None:
+                # the previous batch of this shape (maybe on another stream)
+                # has finished reading the buffers before they are rebuilt
+                torch.cuda.current_stream().wait_event(e.done)
+            H, S = engine.run_build(self.tables, plan, out=(e.H, e.S))
+        else:
+            H, S = engine.run_build(self.tables, plan)
+            if graphs and self._graph_wanted(key, 3 * H.numel() * 16):
+                e = self._capture(H, S, si)
+                self._graph_insert(key, e)
+                self.graph_captures += 1
+        if e is not None:
+            e.graph.replay()
+            e.done = torch.cuda.Event()
+            self.graph_replays += 1
+            return H, S, e.w, e.info, e.ws, e
+        wsl = []
+        w, _, info = engine.eigh(H, S, check=False, precision=self.precision, workspace=wsl)
+        return H, S, w, info, wsl[0], None
+
+    def _launch(self, reqs: List[Request], ld: int, stream, si: int = 0) -> Batch:
         torch.cuda.set_device(self.dev_index)
         nmat = sum(len(r.inp.kpoints) for r in reqs)
         with torch.cuda.stream(stream):
             plan = engine.plan_build(self.tables, [r.inp for r in reqs], ld=ld)
-            H, S = engine.run_build(self.tables, plan)
-            wsl = []
-            w, _, info = engine.eigh(H, S, check=False, precision=self.precision, workspace=wsl)
-            refine = (H, S, wsl[0]) if (self.refine_edges and self.precision == "mixed") else None
+            H, S, w, info, ws, ge = self._solve(plan, ld, nmat, stream, si)
+            refine = (H, S, ws) if (self.refine_edges and self.precision == "mixed") else None
             edges, dos, stats = engine.post(plan, w, fermi_mode=self.fermi_mode, dos_grid=self.grid, refine=refine)
+            if ge is not None:
+                # the graph's buffers are free for the next replay once this
+                # batch's readers (refinement, post) have run
+                ge.done.record(stream)
+                with self._graph_lock:
+                    ge.busy = False
+                if self.keep_outputs:
+                    w = w.clone()
             parts = [edges.reshape(-1), info.to(torch.float64)]
             if stats is not None:
                 parts.append(stats.reshape(-1))
@@ -159,7 +283,7 @@ class ServingEngine:
             host.copy_(flat, non_blocking=True)
             ev = torch.cuda.Event()
             ev.record(stream)
-        b = Batch(reqs, ld, stream, ev, host, self.batch_bytes(ld, nmat), keep=[H, S, w, edges, info, flat, wsl])
+        b = Batch(reqs, ld, stream, ev, host, self.batch_bytes(ld, nmat), keep=[H, S, w, edges, info, flat, ws])
         if self.keep_outputs:
             b.dos, b.w, b.plan = dos, w, plan
         return b

@@ -153,3 +153,33 @@ def test_failed_requests_are_reported_not_ok():
             assert not res["ok"] and "bulge chase" in res["error"], (i, n, res)
         else:
             assert res["ok"], (i, n)
+
+
+@pytest.mark.parametrize("precision", ["mixed", "fp64"])
+def test_graph_replays_equal_eager_serving(precision):
+    """Small buckets replay their eigensolve from a captured graph from the
+    second batch of a shape on: every request's eigenvalues, edges and DOS
+    equal those of the same stream served with graphs off, bit for bit."""
+    tab, base = _stream(3, seed=57, n_max=24)
+    inputs = [base[i % 3] for i in range(12)]
+    arrivals = np.arange(12) * 0.015     # one request per batch (max_batch_matrices=1): shapes repeat
+    grid = torch.linspace(-20.0, 10.0, 2001, dtype=torch.float64, device="cuda")
+    outs = {}
+    for graphs in (False, True):
+        eng = ServingEngine(tab, dos_grid=grid, keep_outputs=True, precision=precision, n_streams=1,
+                            max_batch_matrices=1, graphs=graphs)
+        outs[graphs] = eng.replay(inputs, arrivals)["results"]
+        eng.close()
+        bad = [r.get("error") for r in outs[graphs].values() if not r["ok"]]
+        assert not bad, bad
+        if graphs:
+            # per shape: eager, capture (+ replay), then two replays into the persistent buffers
+            print(f"captures {eng.graph_captures}, replays {eng.graph_replays}")
+            assert eng.graph_captures == 3 and eng.graph_replays == 9
+    for i in range(len(inputs)):
+        a, b = outs[False][i], outs[True][i]
+        assert a["ok"] and b["ok"]
+        assert torch.equal(a["eigenvalues"], b["eigenvalues"]), i
+        assert torch.equal(a["dos"], b["dos"]), i
+        for key in ("vbm", "cbm", "mu", "gap_eV"):
+            assert a[key] == b[key], (i, key)
