@@ -219,8 +219,8 @@ __device__ __forceinline__ House<T> house(typename C2T<T>::t alpha, T xn2) {
 
 // ---------------------------------------------------------------------------
 // Stage 1, panel QR: 16 waves, wave w owns column w of the m x BW panel,
-// lane l owns rows l + 64 q (registers for RPT > 0; RPT = 0 re-reads the
-// panel from global memory). Column k: wave k reduces its norm, forms the
+// lane l owns rows l + 64 q (registers for q < RPT, LDS for the QT rows after
+// those; RPT = 0 re-reads the panel from global memory). Column k: wave k reduces its norm, forms the
 // reflector and publishes v_k (LDS, or column k of Vc for RPT = 0); after one
 // barrier, waves j > k apply it to their columns (inner product and update
 // within the wave), wave k+1 also forms the next norm, and waves l < k form
@@ -261,10 +261,12 @@ __device__ __forceinline__ void for_rows(int nq, F f) {
   }
 }
 
-template <typename T, int RPT>
+template <typename T, int RPT, int QT = 0>
 __global__ __launch_bounds__(1024) void panel_qr_kernel(BandCtx<T> c, int j0) {
   using C2 = typename C2T<T>::t;
+  static_assert(QT == 0 || RPT > 0, "LDS rows extend register rows");
   constexpr int NR = RPT > 0 ? RPT : 1;
+  constexpr int FR = RPT > 0 ? RPT + QT : 0;   // row slots per lane (0: global)
   const int b = blockIdx.x;
   const int n = c.n, ld = c.ld;
   const int r0 = j0 + BW, m = n - r0;
@@ -274,24 +276,38 @@ __global__ __launch_bounds__(1024) void panel_qr_kernel(BandCtx<T> c, int j0) {
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   // published reflector, double-buffered by column parity: wave k+1 writes
   // v_{k+1} while other waves may still read v_k (one barrier per column)
-  __shared__ C2 s_vb[2][RPT > 0 ? 64 * RPT : 1];
+  __shared__ C2 s_vb[2][RPT > 0 ? 64 * FR : 1];
   __shared__ C2 s_tau[BW];
   __shared__ C2 s_G[BW][BW];   // s_G[k][l] = v_l^H v_k, l < k
   __shared__ C2 s_T[BW][BW];
-  const int nq = RPT > 0 ? RPT : (m + 63) / 64;
+  // rows lane + 64 (RPT + t) of column w at s_tl[t][lane][w]
+  __shared__ C2 s_tl[QT > 0 ? QT : 1][QT > 0 ? 64 : 1][BW + 1];
+  const int nq = RPT > 0 ? FR : (m + 63) / 64;
   C2* col = a + (size_t)(r0 + lane) * ld + j0 + w;   // this lane's rows of column w
   C2 p[NR];
   if constexpr (RPT > 0) {
 #pragma unroll
     for (int q = 0; q < RPT; ++q) p[q] = load_or_zero(col + (size_t)64 * q * ld, lane + 64 * q < m);
+#pragma unroll
+    for (int t = 0; t < QT; ++t)
+      s_tl[t][lane][w] = load_or_zero(col + (size_t)64 * (RPT + t) * ld, lane + 64 * (RPT + t) < m);
   }
+  // q is a compile-time constant in the unrolled row loops (RPT > 0)
   auto get = [&](int q) -> C2 {
-    if constexpr (RPT > 0) return p[q];
-    else return load_or_zero(col + (size_t)64 * q * ld, lane + 64 * q < m);
+    if constexpr (RPT > 0) {
+      if (QT > 0 && q >= RPT) return s_tl[q - RPT][lane][w];
+      return p[q];
+    } else {
+      return load_or_zero(col + (size_t)64 * q * ld, lane + 64 * q < m);
+    }
   };
   auto put = [&](int q, C2 v) {
-    if constexpr (RPT > 0) p[q] = v;
-    else if (lane + 64 * q < m) col[(size_t)64 * q * ld] = v;
+    if constexpr (RPT > 0) {
+      if (QT > 0 && q >= RPT) s_tl[q - RPT][lane][w] = v;
+      else p[q] = v;
+    } else if (lane + 64 * q < m) {
+      col[(size_t)64 * q * ld] = v;
+    }
   };
   // T (Q = I - V T V^H): T[k][k] = tau_k, T[i][k] = -tau_k sum_{l=i}^{k-1} T[i][l] g_l^(k);
   // lane i of wave 0 owns row i
@@ -311,7 +327,7 @@ __global__ __launch_bounds__(1024) void panel_qr_kernel(BandCtx<T> c, int j0) {
   };
   T nrm = 0;
   if (w == 0)
-    for_rows<RPT>(nq, [&](int q) {
+    for_rows<FR>(nq, [&](int q) {
         const int r = lane + 64 * q;
         const C2 e = get(q);
         if (r > 0 && r < m) nrm += e.x * e.x + e.y * e.y;
@@ -324,7 +340,7 @@ __global__ __launch_bounds__(1024) void panel_qr_kernel(BandCtx<T> c, int j0) {
       C2 a0 = get(0);   // row k lives in lane k, q = 0 (k < BW <= 64)
       const C2 alpha = mk<T>(__shfl(a0.x, k, 64), __shfl(a0.y, k, 64));
       const House<T> h = house<T>(alpha, xn2);
-      for_rows<RPT>(nq, [&](int q) {
+      for_rows<FR>(nq, [&](int q) {
           const int r = lane + 64 * q;
           if (r >= m) return;
           C2 e = get(q);
@@ -345,7 +361,7 @@ __global__ __launch_bounds__(1024) void panel_qr_kernel(BandCtx<T> c, int j0) {
     if (w > k) {
       const C2 tau = s_tau[k];
       C2 u = mk<T>(0, 0);
-      for_rows<RPT>(nq, [&](int q) {
+      for_rows<FR>(nq, [&](int q) {
           const int r = lane + 64 * q;
           if (r >= k && r < m) {
             const C2 vr = RPT > 0 ? s_v[r] : vk[r];
@@ -357,7 +373,7 @@ __global__ __launch_bounds__(1024) void panel_qr_kernel(BandCtx<T> c, int j0) {
       u = wave_sum_c(u);
       const C2 cu = c_mul(c_conj(tau), u);
       nrm = 0;
-      for_rows<RPT>(nq, [&](int q) {
+      for_rows<FR>(nq, [&](int q) {
           const int r = lane + 64 * q;
           if (r >= k && r < m) {
             const C2 vr = RPT > 0 ? s_v[r] : vk[r];
@@ -372,7 +388,7 @@ __global__ __launch_bounds__(1024) void panel_qr_kernel(BandCtx<T> c, int j0) {
     } else if (w < k) {
       // g_w = v_w^H v_k over rows >= k (column w holds v_w below row w)
       C2 g = mk<T>(0, 0);
-      for_rows<RPT>(nq, [&](int q) {
+      for_rows<FR>(nq, [&](int q) {
           const int r = lane + 64 * q;
           if (r >= k && r < m) {
             const C2 vr = RPT > 0 ? s_v[r] : vk[r];
@@ -392,7 +408,7 @@ __global__ __launch_bounds__(1024) void panel_qr_kernel(BandCtx<T> c, int j0) {
   __syncthreads();
   // outputs: panel column w (R in the band, v_w below), V in Vc[w] and Wc[BW + w], VT row j0 + w
   C2* wcv = c.Wc + (size_t)b * 2 * BW * ld + (size_t)BW * ld;
-  for_rows<RPT>(nq, [&](int q) {
+  for_rows<FR>(nq, [&](int q) {
       const int r = lane + 64 * q;
       if (r >= m) return;
       const C2 e = get(q);
@@ -1331,6 +1347,10 @@ static void launch_panel_qr(hipStream_t st, const BandCtx<T>& c, int batch, int 
   else if (nq <= 16) hipLaunchKernelGGL((panel_qr_kernel<T, 16>), g, blk, 0, st, c, j0);
   else if (nq <= 24 && sizeof(T) == 4) hipLaunchKernelGGL((panel_qr_kernel<T, sizeof(T) == 4 ? 24 : 16>), g, blk, 0, st, c, j0);
   else if (nq <= 32 && sizeof(T) == 4) hipLaunchKernelGGL((panel_qr_kernel<T, sizeof(T) == 4 ? 32 : 16>), g, blk, 0, st, c, j0);
+  // beyond the register tiers: 4 more rows per lane in LDS (FP32: m <= 2304,
+  // every serving matrix; FP64: m <= 1280)
+  else if (nq <= 36 && sizeof(T) == 4) hipLaunchKernelGGL((panel_qr_kernel<T, sizeof(T) == 4 ? 32 : 16, 4>), g, blk, 0, st, c, j0);
+  else if (nq <= 20 && sizeof(T) == 8) hipLaunchKernelGGL((panel_qr_kernel<T, 16, 4>), g, blk, 0, st, c, j0);
   else hipLaunchKernelGGL((panel_qr_kernel<T, 0>), g, blk, 0, st, c, j0);
 }
 
