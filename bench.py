@@ -163,12 +163,15 @@ def edge_errors(mixed, ref):
     the requests that succeeded in both and are insulators in the reference."""
     gap = vbm = cbm = 0.0
     n = flips = 0
+    flip_gaps = []
     for i, r in ref.items():
         m = mixed.get(i)
         if m is None or not m.get("ok") or not r.get("ok"):
             continue
         if (r["metal"] > 0) != (m["metal"] > 0):
+            # a gap at the metal threshold (1 meV) can land on either side
             flips += 1
+            flip_gaps.append([r["cbm"] - r["vbm"], m["cbm"] - m["vbm"]])
             continue
         if r["metal"] > 0:
             continue
@@ -177,7 +180,7 @@ def edge_errors(mixed, ref):
         cbm = max(cbm, abs(m["cbm"] - r["cbm"]))
         gap = max(gap, abs((m["cbm"] - m["vbm"]) - (r["cbm"] - r["vbm"])))
     return {"max_gap_err_eV": gap, "max_vbm_err_eV": vbm, "max_cbm_err_eV": cbm, "insulators_compared": n,
-            "metal_flag_flips": flips}
+            "metal_flag_flips": flips, "flip_edge_gaps_eV_fp64_mixed": flip_gaps[:8]}
 
 
 HETRD_SAMPLE_EVERY = 8
