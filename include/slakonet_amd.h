@@ -136,6 +136,21 @@ int slk_eigh_mixed(const void* H, const void* S, int n, int ld, int batch, doubl
                    size_t ws_bytes, int* info, hipStream_t stream);
 int slk_eigh_mixed_timed(const void* H, const void* S, int n, int ld, int batch, double* w, int wld, void* ws,
                          size_t ws_bytes, int* info, hipStream_t stream, float* stage_ms);
+/* Both solves behind one entry point with flags: SLK_EIGH_MIXED selects the
+ * precision form (V must then be null), SLK_EIGH_REAL declares that every H
+ * and S of the batch is real symmetric, stored as complex with zero imaginary
+ * parts (k at a point where every Bloch phase is +-1, e.g. Gamma or the
+ * Monkhorst-Pack point (1/2, 1/2, 1/2) of a 1x1x1 mesh): the kernels then
+ * skip the imaginary arithmetic. Results are those of the complex solve up to
+ * rounding; with imaginary parts present under SLK_EIGH_REAL they are
+ * undefined. stage_ms (nullable, 6 floats): per-stage times as in the _timed
+ * variants (the call then waits for the solve). Returns 1 for unknown flags.
+ * Replaces the same reference operators as slk_eigh_f64 (eighb,
+ * slakonet/utils.py:585-816); the reference has no real-symmetric case. */
+#define SLK_EIGH_MIXED 1
+#define SLK_EIGH_REAL 2
+int slk_eigh_ex(const void* H, const void* S, int n, int ld, int batch, double* w, int wld, void* V, void* ws,
+                size_t ws_bytes, int* info, int flags, float* stage_ms, hipStream_t stream);
 /* slk_eigh_mixed without a workspace argument and without any allocation:
  * the workspace is a static device array of slk_noalloc_scratch_bytes()
  * (64 MiB) per GPU, held from the library's load until the process ends.

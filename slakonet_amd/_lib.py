@@ -80,6 +80,8 @@ SIGNATURES = {
                                  C.c_void_p, C.c_size_t, C.c_void_p, C.c_void_p]),
     "slk_eigh_mixed_timed": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_int,
                                        C.c_void_p, C.c_size_t, C.c_void_p, C.c_void_p, C.POINTER(C.c_float)]),
+    "slk_eigh_ex": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_int, C.c_void_p,
+                              C.c_void_p, C.c_size_t, C.c_void_p, C.c_int, C.POINTER(C.c_float), C.c_void_p]),
     "slk_noalloc_scratch_bytes": (C.c_size_t, []),
     "slk_eigh_mixed_noalloc": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_int,
                                          C.c_void_p, C.c_void_p]),

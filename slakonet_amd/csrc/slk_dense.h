@@ -8,23 +8,25 @@ enum { OP_N = 0, OP_T = 1, OP_CONJ = 2, OP_C = 3 };
 enum { KLIM_A_LOWER = 1, KLIM_A_UPPER = 2, KLIM_B_UPPER = 4, KLIM_B_LOWER = 8 };
 
 // C = alpha opA(A) opB(B) + beta C for `batch` problems with strides sA/sB/sC.
-// tri = 1 skips 64x64 output tiles strictly above the diagonal.
+// tri = 1 skips 64x64 output tiles strictly above the diagonal. re: A and B
+// are real (zero imaginary parts); C gets zero imaginary parts.
 void launch_zgemm(hipStream_t st, int batch, int M, int N, int K, double alpha, const double2* A, int lda,
                   long long sA, int opA, const double2* B, int ldb, long long sB, int opB, double beta, double2* C,
-                  int ldc, long long sC, int klim, int tri);
+                  int ldc, long long sC, int klim, int tri, bool re = false);
 
 void launch_cgemm(hipStream_t st, int batch, int M, int N, int K, float alpha, const float2* A, int lda,
                   long long sA, int opA, const float2* B, int ldb, long long sB, int opB, float beta, float2* C,
-                  int ldc, long long sC, int klim, int tri);
+                  int ldc, long long sC, int klim, int tri, bool re = false);
 
 // Zero-fill (no memset nodes so that the solve can be graph-captured).
 void launch_zero(hipStream_t st, void* p, size_t bytes);
 
 // Cholesky S = L L^H (into L, a copy of S; lower part) and W = L^-1 together
 // (recursive halving, large GEMMs); info[b] = first failing pivot (1-based)
-// or 0. T: temporary of ld * ld / 2 complex128 per matrix.
+// or 0. T: temporary of ld * ld / 2 complex128 per matrix. re: S is real
+// (zero imaginary parts), and so are L and W.
 void launch_chol_inv(hipStream_t st, const double2* S, double2* L, double2* W, double2* T, int n, int ld, int batch,
-                     int* info);
+                     int* info, bool re = false);
 
 // Blocked Householder tridiagonalisation of the Hermitian C (in place): d, e
 // (real tridiagonal), tau, reflectors in the strict lower part of C.

@@ -48,7 +48,10 @@ template <> struct CT<float> {
 // The next K tile is fetched into registers while the current one is
 // multiplied from LDS (measured faster than fetch-stage-multiply in turn at
 // every size from 64^3 x 8 to 2048^3, FP64 and FP32).
-template <typename T, int TM>
+// RE: the operands are real (zero imaginary parts in complex storage): one
+// real MFMA per block instead of four, no imaginary LDS planes, and the
+// result's imaginary parts are written as zero.
+template <typename T, int TM, bool RE = false>
 __global__ __launch_bounds__(256) void gemm_mfma_kernel(int M, int N, int K, T alpha,
                                                         const typename CT<T>::c2* __restrict__ A, int lda,
                                                         long long sA, int opA,
@@ -66,7 +69,7 @@ __global__ __launch_bounds__(256) void gemm_mfma_kernel(int M, int N, int K, T a
     sample_end(clk);
     return;
   }
-  __shared__ T Ar[KD][TM + 4], Ai[KD][TM + 4], Br[KD][TM + 4], Bi[KD][TM + 4];
+  __shared__ T Ar[KD][TM + 4], Ai[RE ? 1 : KD][TM + 4], Br[KD][TM + 4], Bi[RE ? 1 : KD][TM + 4];
   const int b = blockIdx.z;
   const C2* a = A + (size_t)b * sA;
   const C2* bb = B + (size_t)b * sB;
@@ -123,18 +126,22 @@ __global__ __launch_bounds__(256) void gemm_mfma_kernel(int M, int N, int K, T a
       int mm, kk;
       a_idx(tid + 256 * i, mm, kk);
       const bool ok = m0 + mm < M && k0 + kk < K;
-      const T im = ok ? pa[i].y : T(0);
       Ar[kk][mm] = ok ? pa[i].x : T(0);
-      Ai[kk][mm] = (opA & OP_CONJ) ? -im : im;
+      if constexpr (!RE) {
+        const T im = ok ? pa[i].y : T(0);
+        Ai[kk][mm] = (opA & OP_CONJ) ? -im : im;
+      }
     }
 #pragma unroll
     for (int i = 0; i < NL; ++i) {
       int nn, kk;
       b_idx(tid + 256 * i, nn, kk);
       const bool ok = n0 + nn < N && k0 + kk < K;
-      const T im = ok ? pb[i].y : T(0);
       Br[kk][nn] = ok ? pb[i].x : T(0);
-      Bi[kk][nn] = (opB & OP_CONJ) ? -im : im;
+      if constexpr (!RE) {
+        const T im = ok ? pb[i].y : T(0);
+        Bi[kk][nn] = (opB & OP_CONJ) ? -im : im;
+      }
     }
   };
   if (kbeg < kend) fetch(kbeg);
@@ -148,22 +155,26 @@ __global__ __launch_bounds__(256) void gemm_mfma_kernel(int M, int N, int K, T a
 #pragma unroll
       for (int i = 0; i < BQ; ++i) {
         ar[i] = Ar[k4 + lk][wr + 16 * i + li];
-        ai[i] = Ai[k4 + lk][wr + 16 * i + li];
-        nai[i] = -ai[i];
+        if constexpr (!RE) {
+          ai[i] = Ai[k4 + lk][wr + 16 * i + li];
+          nai[i] = -ai[i];
+        }
       }
 #pragma unroll
       for (int j = 0; j < BQ; ++j) {
         br[j] = Br[k4 + lk][wc + 16 * j + li];
-        bi[j] = Bi[k4 + lk][wc + 16 * j + li];
+        if constexpr (!RE) bi[j] = Bi[k4 + lk][wc + 16 * j + li];
       }
 #pragma unroll
       for (int i = 0; i < BQ; ++i)
 #pragma unroll
         for (int j = 0; j < BQ; ++j) {
           accR[i][j] = CT<T>::mfma(ar[i], br[j], accR[i][j]);
-          accR[i][j] = CT<T>::mfma(nai[i], bi[j], accR[i][j]);
-          accI[i][j] = CT<T>::mfma(ar[i], bi[j], accI[i][j]);
-          accI[i][j] = CT<T>::mfma(ai[i], br[j], accI[i][j]);
+          if constexpr (!RE) {
+            accR[i][j] = CT<T>::mfma(nai[i], bi[j], accR[i][j]);
+            accI[i][j] = CT<T>::mfma(ar[i], bi[j], accI[i][j]);
+            accI[i][j] = CT<T>::mfma(ai[i], br[j], accI[i][j]);
+          }
         }
     }
     __syncthreads();
@@ -178,11 +189,11 @@ __global__ __launch_bounds__(256) void gemm_mfma_kernel(int M, int N, int K, T a
         if (gm < M && gn < N) {
           C2 r;
           r.x = alpha * accR[i][j][v];
-          r.y = alpha * accI[i][j][v];
+          r.y = RE ? T(0) : alpha * accI[i][j][v];
           if (beta != T(0)) {
             const C2 o = c[(size_t)gm * ldc + gn];
             r.x += beta * o.x;
-            r.y += beta * o.y;
+            if constexpr (!RE) r.y += beta * o.y;
           }
           c[(size_t)gm * ldc + gn] = r;
         }
@@ -223,14 +234,17 @@ static int gemm_tile(int batch, int M, int N, int tri) {
 
 void launch_zgemm(hipStream_t st, int batch, int M, int N, int K, double alpha, const double2* A, int lda,
                   long long sA, int opA, const double2* B, int ldb, long long sB, int opB, double beta, double2* C,
-                  int ldc, long long sC, int klim, int tri) {
+                  int ldc, long long sC, int klim, int tri, bool re) {
   if (M <= 0 || N <= 0 || batch <= 0) return;
   const int tm = gemm_tile(batch, M, N, tri);
   dim3 g((N + tm - 1) / tm, (M + tm - 1) / tm, batch);
   SampleSlot sl;
-  const bool sampled =
-      sampler_enabled() && sampler_take(SLK_SAMPLE_GEMM_F64, gemm_tile_flops(batch, M, N, K, klim, tri, tm), &sl);
-  auto kern = tm == GT ? gemm_mfma_kernel<double, GT> : gemm_mfma_kernel<double, 32>;
+  // a real product is a quarter of the complex flops
+  const bool sampled = sampler_enabled() &&
+                       sampler_take(SLK_SAMPLE_GEMM_F64,
+                                    gemm_tile_flops(batch, M, N, K, klim, tri, tm) * (re ? 0.25 : 1.0), &sl);
+  auto kern = re ? (tm == GT ? gemm_mfma_kernel<double, GT, true> : gemm_mfma_kernel<double, 32, true>)
+                 : (tm == GT ? gemm_mfma_kernel<double, GT> : gemm_mfma_kernel<double, 32>);
   if (sampled)
     hipExtLaunchKernelGGL(kern, g, dim3(256), 0, st, sl.e0, sl.e1, 0, M, N, K, alpha, A, lda, sA, opA, B, ldb, sB,
                           opB, beta, C, ldc, sC, klim, tri, sl.clk);
@@ -241,11 +255,12 @@ void launch_zgemm(hipStream_t st, int batch, int M, int N, int K, double alpha, 
 
 void launch_cgemm(hipStream_t st, int batch, int M, int N, int K, float alpha, const float2* A, int lda,
                   long long sA, int opA, const float2* B, int ldb, long long sB, int opB, float beta, float2* C,
-                  int ldc, long long sC, int klim, int tri) {
+                  int ldc, long long sC, int klim, int tri, bool re) {
   if (M <= 0 || N <= 0 || batch <= 0) return;
   const int tm = gemm_tile(batch, M, N, tri);
   dim3 g((N + tm - 1) / tm, (M + tm - 1) / tm, batch);
-  auto kern = tm == GT ? gemm_mfma_kernel<float, GT> : gemm_mfma_kernel<float, 32>;
+  auto kern = re ? (tm == GT ? gemm_mfma_kernel<float, GT, true> : gemm_mfma_kernel<float, 32, true>)
+                 : (tm == GT ? gemm_mfma_kernel<float, GT> : gemm_mfma_kernel<float, 32>);
   hipLaunchKernelGGL(kern, g, dim3(256), 0, st, M, N, K, alpha, A, lda, sA, opA, B, ldb, sB, opB, beta, C, ldc, sC,
                      klim, tri, (unsigned long long*)nullptr);
 }
@@ -364,7 +379,7 @@ __global__ void copy_block_kernel(const double2* __restrict__ src, int lds, long
 // and inverted in LDS by potrf_diag_kernel. The GEMMs have K = n1 instead of
 // the NB of a blocked right-looking loop. T: ld * ld / 2 per matrix.
 static void chol_inv_rec(hipStream_t st, double2* L, double2* W, double2* T, int o, int m, int n, int ld, int batch,
-                         long long sM, long long sT, int* info) {
+                         long long sM, long long sT, int* info, bool re) {
   // 32-row leaves: a 64-row leaf (64 dependent steps in one workgroup) measured
   // 7% slower over the whole recursion than two 32-row leaves with the four
   // GEMMs and the copy between them
@@ -374,31 +389,31 @@ static void chol_inv_rec(hipStream_t st, double2* L, double2* W, double2* T, int
   }
   const int n1 = ((m + 1) / 2 + NB - 1) / NB * NB, n2 = m - n1;
   double2* L21 = L + (size_t)(o + n1) * ld + o;
-  chol_inv_rec(st, L, W, T, o, n1, n, ld, batch, sM, sT, info);
+  chol_inv_rec(st, L, W, T, o, n1, n, ld, batch, sM, sT, info, re);
   // L21 = A21 W11^H (W11^H upper: K ends at the column block); via T, then back
   launch_zgemm(st, batch, n2, n1, n1, 1.0, L21, ld, sM, OP_N, W + (size_t)o * ld + o, ld, sM, OP_C, 0.0, T, ld, sT,
-               KLIM_B_UPPER, 0);
+               KLIM_B_UPPER, 0, re);
   const long long cells = (long long)n2 * n1;
   const long long cblk = (cells + 255) / 256;
   hipLaunchKernelGGL(copy_block_kernel, dim3((unsigned)(cblk < 1024 ? cblk : 1024), batch), dim3(256), 0, st, T, ld, sT,
                      L21, ld, sM, n2, n1);
   // A22 -= L21 L21^H over the lower tiles
   launch_zgemm(st, batch, n2, n2, n1, -1.0, L21, ld, sM, OP_N, L21, ld, sM, OP_C, 1.0, L + (size_t)(o + n1) * ld + o + n1,
-               ld, sM, 0, 1);
-  chol_inv_rec(st, L, W, T, o + n1, n2, n, ld, batch, sM, sT, info);
+               ld, sM, 0, 1, re);
+  chol_inv_rec(st, L, W, T, o + n1, n2, n, ld, batch, sM, sT, info, re);
   // W21 = -W22 (L21 W11)
   launch_zgemm(st, batch, n2, n1, n1, 1.0, L21, ld, sM, OP_N, W + (size_t)o * ld + o, ld, sM, OP_N, 0.0, T, ld, sT,
-               KLIM_B_LOWER, 0);
+               KLIM_B_LOWER, 0, re);
   launch_zgemm(st, batch, n2, n1, n2, -1.0, W + (size_t)(o + n1) * ld + o + n1, ld, sM, OP_N, T, ld, sT, OP_N, 0.0,
-               W + (size_t)(o + n1) * ld + o, ld, sM, KLIM_A_LOWER, 0);
+               W + (size_t)(o + n1) * ld + o, ld, sM, KLIM_A_LOWER, 0, re);
 }
 
 void launch_chol_inv(hipStream_t st, const double2* S, double2* L, double2* W, double2* T, int n, int ld, int batch,
-                     int* info) {
+                     int* info, bool re) {
   // L = S (lower part used), W = 0 (diagonal blocks filled by the leaves)
   (void)hipMemcpyAsync(L, S, (size_t)batch * ld * ld * sizeof(double2), hipMemcpyDeviceToDevice, st);
   launch_zero(st, W, (size_t)batch * ld * ld * sizeof(double2));
-  chol_inv_rec(st, L, W, T, 0, n, n, ld, batch, (long long)ld * ld, (long long)ld * ld / 2, info);
+  chol_inv_rec(st, L, W, T, 0, n, n, ld, batch, (long long)ld * ld, (long long)ld * ld / 2, info, re);
 }
 
 

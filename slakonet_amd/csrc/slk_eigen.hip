@@ -453,7 +453,7 @@ __global__ void qb_tfactor_kernel(const double2* __restrict__ G, const double2* 
 static size_t apply_q_scratch_elems(int ld) { return (size_t)ld * QB + 2 * QB * QB + 2 * (size_t)QB * ld; }
 
 static void launch_apply_q(hipStream_t st, const double2* Cm, const double2* TAU, const double* Zr, double2* Y,
-                           int n, int ld, int batch, double2* scr) {
+                           int n, int ld, int batch, double2* scr, bool re) {
   const long long sM = (long long)ld * ld;
   const long long sV = (long long)ld * QB, sG = (long long)QB * QB, sW = (long long)QB * ld;
   double2* Vb = scr;                          // [batch][ld][QB]
@@ -470,11 +470,11 @@ static void launch_apply_q(hipStream_t st, const double2* Cm, const double2* TAU
     double2* yr = Y + (size_t)(j0 + 1) * ld;
     hipLaunchKernelGGL(qb_pack_kernel, dim3((m * QB + 255) / 256, batch), dim3(256), 0, st, Cm, ld, sM, j0, nb, m, Vb,
                        sV);
-    launch_zgemm(st, batch, QB, QB, m, 1.0, Vb, QB, sV, OP_C, Vb, QB, sV, OP_N, 0.0, G, QB, sG, 0, 0);
+    launch_zgemm(st, batch, QB, QB, m, 1.0, Vb, QB, sV, OP_C, Vb, QB, sV, OP_N, 0.0, G, QB, sG, 0, 0, re);
     hipLaunchKernelGGL(qb_tfactor_kernel, dim3(batch), dim3(QB), 0, st, G, TAU, ld, j0, nb, T);
-    launch_zgemm(st, batch, QB, n, m, 1.0, Vb, QB, sV, OP_C, yr, ld, sM, OP_N, 0.0, W, ld, sW, 0, 0);
-    launch_zgemm(st, batch, QB, n, QB, 1.0, T, QB, sG, OP_N, W, ld, sW, OP_N, 0.0, W2, ld, sW, 0, 0);
-    launch_zgemm(st, batch, m, n, QB, -1.0, Vb, QB, sV, OP_N, W2, ld, sW, OP_N, 1.0, yr, ld, sM, 0, 0);
+    launch_zgemm(st, batch, QB, n, m, 1.0, Vb, QB, sV, OP_C, yr, ld, sM, OP_N, 0.0, W, ld, sW, 0, 0, re);
+    launch_zgemm(st, batch, QB, n, QB, 1.0, T, QB, sG, OP_N, W, ld, sW, OP_N, 0.0, W2, ld, sW, 0, 0, re);
+    launch_zgemm(st, batch, m, n, QB, -1.0, Vb, QB, sV, OP_N, W2, ld, sW, OP_N, 1.0, yr, ld, sM, 0, 0, re);
   }
 }
 
@@ -1195,8 +1195,13 @@ __global__ void zero_info_kernel(int* info, int n) {
 // mixed: FP64 Cholesky and inverse, FP32 reduction and tridiagonalisation,
 // FP64 bisection (values only).
 static int eigh_impl(const void* H, const void* S, int n, int ld, int batch, double* w, int wld, void* V,
-                     void* ws, size_t ws_bytes, int* info, hipStream_t stream, hipEvent_t* ev, int mixed) {
+                     void* ws, size_t ws_bytes, int* info, hipStream_t stream, hipEvent_t* ev, int mixed,
+                     int flags = 0) {
   if (batch <= 0 || n <= 0) return 0;
+  // SLK_EIGH_REAL: every H and S of the batch is real symmetric (zero
+  // imaginary parts, e.g. k at a time-reversal-invariant point): the kernels
+  // that know it skip the imaginary arithmetic and write exact zeros
+  const bool re = (flags & SLK_EIGH_REAL) != 0;
   if (mixed && V != nullptr) return 3;
   const size_t need = mixed ? slk_eigh_mixed_workspace_bytes(n, ld, batch)
                             : slk_eigh_workspace_bytes(n, ld, batch, V != nullptr);
@@ -1222,7 +1227,7 @@ static int eigh_impl(const void* H, const void* S, int n, int ld, int batch, dou
   // Cholesky and L^-1 in one recursion (the "potrf" stage; "trinv" is empty).
   // The temporary (ld * ld / 2 complex128 per matrix) borrows the product
   // buffer of the reduction, which is free until then.
-  launch_chol_inv(stream, sp, s.B0, s.B1, mixed ? reinterpret_cast<double2*>(s.F2) : s.B2, n, ld, batch, info);
+  launch_chol_inv(stream, sp, s.B0, s.B1, mixed ? reinterpret_cast<double2*>(s.F2) : s.B2, n, ld, batch, info, re);
   if (ev) (void)hipEventRecord(ev[1], stream);
   if (ev) (void)hipEventRecord(ev[2], stream);
   // Both tridiagonalisations (band and one-stage) read only the lower
@@ -1234,9 +1239,9 @@ static int eigh_impl(const void* H, const void* S, int n, int ld, int batch, dou
   if (!mixed) {
     // B2 = W H (W lower), B0 = B2 W^H (W^H upper)
     launch_zgemm(stream, batch, n, n, n, 1.0, s.B1, ld, sM, OP_N, h, ld, sM, OP_N, 0.0, s.B2, ld, sM, KLIM_A_LOWER,
-                 tri);
+                 tri, re);
     launch_zgemm(stream, batch, n, n, n, 1.0, s.B2, ld, sM, OP_N, s.B1, ld, sM, OP_C, 0.0, s.B0, ld, sM,
-                 KLIM_B_UPPER, tri);
+                 KLIM_B_UPPER, tri, re);
     if (ev) (void)hipEventRecord(ev[3], stream);
     if (band)
       // B2 (W H) is free after the reduction: it holds the split-K partials
@@ -1248,9 +1253,9 @@ static int eigh_impl(const void* H, const void* S, int n, int ld, int batch, dou
     launch_to_c64(stream, s.B1, s.F0, (size_t)batch * ld * ld);
     launch_to_c64(stream, h, s.F1, (size_t)batch * ld * ld);
     launch_cgemm(stream, batch, n, n, n, 1.0f, s.F0, ld, sM, OP_N, s.F1, ld, sM, OP_N, 0.0f, s.F2, ld, sM,
-                 KLIM_A_LOWER, tri);
+                 KLIM_A_LOWER, tri, re);
     launch_cgemm(stream, batch, n, n, n, 1.0f, s.F2, ld, sM, OP_N, s.F0, ld, sM, OP_C, 0.0f, s.F1, ld, sM,
-                 KLIM_B_UPPER, tri);
+                 KLIM_B_UPPER, tri, re);
     if (ev) (void)hipEventRecord(ev[3], stream);
     // F0 (W in FP32) is free after the reduction: it receives the reflector
     // rows; F2 (W32 H32) holds the split-K partials of the band reduction
@@ -1313,21 +1318,21 @@ static int eigh_impl(const void* H, const void* S, int n, int ld, int batch, dou
   if (V != nullptr) {
     // the inverse-iteration scratch (STEIN_SCR ld^2 doubles per matrix) is
     // free again: it holds the back-transform's blocks
-    launch_apply_q(stream, s.B0, s.TAU, s.Z, s.B2, n, ld, batch, reinterpret_cast<double2*>(s.SV));
+    launch_apply_q(stream, s.B0, s.TAU, s.Z, s.B2, n, ld, batch, reinterpret_cast<double2*>(s.SV), re);
     // V = W^H Y (W^H upper)
     launch_zgemm(stream, batch, n, n, n, 1.0, s.B1, ld, sM, OP_C, s.B2, ld, sM, OP_N, 0.0,
-                 reinterpret_cast<double2*>(V), ld, sM, KLIM_A_UPPER, 0);
+                 reinterpret_cast<double2*>(V), ld, sM, KLIM_A_UPPER, 0, re);
   }
   if (ev) (void)hipEventRecord(ev[6], stream);
   return (int)hipGetLastError();
 }
 
 static int eigh_timed(const void* H, const void* S, int n, int ld, int batch, double* w, int wld, void* V, void* ws,
-                      size_t ws_bytes, int* info, hipStream_t stream, float* stage_ms, int mixed) {
+                      size_t ws_bytes, int* info, hipStream_t stream, float* stage_ms, int mixed, int flags = 0) {
   hipEvent_t ev[7];
   for (int i = 0; i < 7; ++i)
     if (hipEventCreate(&ev[i]) != hipSuccess) return 4;
-  int rc = eigh_impl(H, S, n, ld, batch, w, wld, V, ws, ws_bytes, info, stream, ev, mixed);
+  int rc = eigh_impl(H, S, n, ld, batch, w, wld, V, ws, ws_bytes, info, stream, ev, mixed, flags);
   if (rc == 0) {
     rc = (int)hipEventSynchronize(ev[6]);
     for (int i = 0; i < 6 && rc == 0; ++i) rc = (int)hipEventElapsedTime(&stage_ms[i], ev[i], ev[i + 1]);
@@ -1445,6 +1450,15 @@ int slk_refine_edges_mixed(const void* H, const void* S, int n, int ld, int batc
 int slk_eigh_mixed_timed(const void* H, const void* S, int n, int ld, int batch, double* w, int wld, void* ws,
                          size_t ws_bytes, int* info, hipStream_t stream, float* stage_ms) {
   return eigh_timed(H, S, n, ld, batch, w, wld, nullptr, ws, ws_bytes, info, stream, stage_ms, 1);
+}
+
+int slk_eigh_ex(const void* H, const void* S, int n, int ld, int batch, double* w, int wld, void* V, void* ws,
+                size_t ws_bytes, int* info, int flags, float* stage_ms, hipStream_t stream) {
+  if (flags & ~(SLK_EIGH_MIXED | SLK_EIGH_REAL)) return 1;
+  const int mixed = (flags & SLK_EIGH_MIXED) ? 1 : 0;
+  if (stage_ms != nullptr)
+    return eigh_timed(H, S, n, ld, batch, w, wld, V, ws, ws_bytes, info, stream, stage_ms, mixed, flags);
+  return eigh_impl(H, S, n, ld, batch, w, wld, V, ws, ws_bytes, info, stream, nullptr, mixed, flags);
 }
 
 }  // extern "C"

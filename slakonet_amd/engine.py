@@ -203,12 +203,18 @@ EIGH_STAGES = ["potrf", "trinv", "reduce", "hetrd", "tridiag_solve", "back_trans
 PRECISIONS = ("fp64", "mixed")
 
 
+EIGH_MIXED, EIGH_REAL = 1, 2   # slk_eigh_ex flags
+
+
 def eigh(H: torch.Tensor, S: torch.Tensor, vectors: bool = False, stream=None, check: bool = True,
-         stage_ms: Optional[list] = None, precision: str = "fp64", workspace: Optional[list] = None):
+         stage_ms: Optional[list] = None, precision: str = "fp64", workspace: Optional[list] = None,
+         real: bool = False):
     """Batched generalized eigensolve. Returns w [batch, ld], V or None, info.
 
     precision "fp64": every stage in FP64. "mixed": FP64 Cholesky and L^-1,
     FP32 reduction and tridiagonalisation, FP64 bisection (values only).
+    real: every H and S is real symmetric (zero imaginary parts; k-points whose
+    Bloch phases are all +-1): the solve skips the imaginary arithmetic.
     With ``stage_ms`` (a list) the per-stage milliseconds measured with HIP
     events on the launch stream are appended (one synchronisation). With
     ``workspace`` (a list) the solver workspace is appended (the band-edge
@@ -230,14 +236,12 @@ def eigh(H: torch.Tensor, S: torch.Tensor, vectors: bool = False, stream=None, c
     w = torch.empty(batch, ld, dtype=torch.float64, device=H.device)
     info = torch.zeros(2 * batch, dtype=torch.int32, device=H.device)
     V = torch.empty_like(H) if vectors else None
-    head = (C.c_void_p(H.data_ptr()), C.c_void_p(S.data_ptr()), n, ld, batch, C.c_void_p(w.data_ptr()), ld)
-    tail = (C.c_void_p(ws.data_ptr()), ws_bytes, C.c_void_p(info.data_ptr()), _lib.stream_handle(stream))
-    args = head + tail if mixed else head + (_lib.ptr(V),) + tail
-    if stage_ms is None:
-        rc = (lib.slk_eigh_mixed if mixed else lib.slk_eigh_f64)(*args)
-    else:
-        ms = (C.c_float * 6)()
-        rc = (lib.slk_eigh_mixed_timed if mixed else lib.slk_eigh_f64_timed)(*args, ms)
+    flags = (EIGH_MIXED if mixed else 0) | (EIGH_REAL if real else 0)
+    ms = (C.c_float * 6)() if stage_ms is not None else None
+    rc = lib.slk_eigh_ex(C.c_void_p(H.data_ptr()), C.c_void_p(S.data_ptr()), n, ld, batch, C.c_void_p(w.data_ptr()),
+                         ld, _lib.ptr(V) if V is not None else None, C.c_void_p(ws.data_ptr()), ws_bytes,
+                         C.c_void_p(info.data_ptr()), flags, ms, _lib.stream_handle(stream))
+    if stage_ms is not None:
         stage_ms.append(dict(zip(EIGH_STAGES, list(ms))))
     if rc == 3:
         raise NotImplementedError(f"matrix dimension {n} is beyond the solver's limit (n <= 16384)")
