@@ -86,7 +86,8 @@ class SimpleDftb:
 
     def __call__(self):
         self.compute_hs_matrices()
-        w, V, _ = engine.eigh(self.ham, self.overlap, vectors=self.with_eigenvectors)
+        w, V, _ = engine.eigh(self.ham, self.overlap, vectors=self.with_eigenvectors,
+                              real=engine.real_kpoints(self._inp.kpoints))
         self._w = w
         self._V = V
         self.eigenvalue = w.unsqueeze(0)                       # [1, nk, nb]
@@ -134,7 +135,7 @@ class SimpleDftb:
                 self()
             V = self._V
             if V is None:
-                _, V, _ = engine.eigh(self.ham, self.overlap, vectors=True)
+                _, V, _ = engine.eigh(self.ham, self.overlap, vectors=True, real=engine.real_kpoints(self._inp.kpoints))
             batch, ld, _ = V.shape
             n = self._plan.n_orb[0]
             occ = self._occupations[0]                                   # [nk, nb]

@@ -180,6 +180,45 @@ __device__ __forceinline__ C2 cfmsc(C2 a, C2 b, C2 e) {
   return e;
 }
 
+// The same products for a real step (RE: every operand has a zero imaginary
+// part, which is not read; the result's stays as it is, zero).
+template <bool RE, typename C2>
+__device__ __forceinline__ C2 cfma_r(C2 a, C2 b, C2 acc) {
+  if constexpr (RE) {
+    acc.x = fmat(a.x, b.x, acc.x);
+    return acc;
+  } else {
+    return cfma(a, b, acc);
+  }
+}
+template <bool RE, typename C2>
+__device__ __forceinline__ C2 ccfma_r(C2 a, C2 b, C2 acc) {
+  if constexpr (RE) {
+    acc.x = fmat(a.x, b.x, acc.x);
+    return acc;
+  } else {
+    return ccfma(a, b, acc);
+  }
+}
+template <bool RE, typename C2>
+__device__ __forceinline__ C2 cfms_r(C2 a, C2 b, C2 e) {
+  if constexpr (RE) {
+    e.x = fmat(-a.x, b.x, e.x);
+    return e;
+  } else {
+    return cfms(a, b, e);
+  }
+}
+template <bool RE, typename C2>
+__device__ __forceinline__ C2 cfmsc_r(C2 a, C2 b, C2 e) {
+  if constexpr (RE) {
+    e.x = fmat(-a.x, b.x, e.x);
+    return e;
+  } else {
+    return cfmsc(a, b, e);
+  }
+}
+
 // value of lane 0 (v_readlane: a scalar read, no LDS round trip)
 __device__ __forceinline__ float lane0(float v) { return f32_of((unsigned)__builtin_amdgcn_readlane((int)u32_of(v), 0)); }
 __device__ __forceinline__ double lane0(double v) {
@@ -812,7 +851,7 @@ __device__ __forceinline__ void band_st(__amdgpu_buffer_rsrc_t r, unsigned voff,
 // A full step (L = BW reflector rows, BW-1 bulge columns, BW rows below),
 // with the addressing reduced to uniform bases plus ChaseLane offsets. BUF
 // (FP32 global band): every band access is a buffer instruction on rb.
-template <typename T, bool BUF>
+template <typename T, bool BUF, bool RE>
 __device__ __forceinline__ T chase_step_full(typename C2T<T>::t* bnd, typename C2T<T>::t* v2row,
                                              typename C2T<T>::t* sv, int c1, int q1, int lane, const ChaseLane& o,
                                              typename C2T<T>::t (*stg)[BW + 1], bool from_stage, bool to_stage,
@@ -860,8 +899,8 @@ __device__ __forceinline__ T chase_step_full(typename C2T<T>::t* bnd, typename C
     ee[t] = e;
   }
   // (a) reflector
-  const T xn2 = sum16(li >= 1 ? x.x * x.x + x.y * x.y : T(0));
-  const C2 alpha = mk<T>(lane0(x.x), lane0(x.y));
+  const T xn2 = sum16(li >= 1 ? (RE ? x.x * x.x : x.x * x.x + x.y * x.y) : T(0));
+  const C2 alpha = mk<T>(lane0(x.x), RE ? T(0) : lane0(x.y));
   C2 tau, scale;
   T beta;
   bool skip;
@@ -870,6 +909,14 @@ __device__ __forceinline__ T chase_step_full(typename C2T<T>::t* bnd, typename C
     beta = alpha.x;
     tau = zero;
     scale = zero;
+  } else if constexpr (RE) {
+    skip = false;
+    const T nrm = sqrt(alpha.x * alpha.x + xn2);
+    beta = alpha.x >= T(0) ? -nrm : nrm;
+    const T ib = rcp_t(beta);
+    tau = mk<T>((beta - alpha.x) * ib, T(0));
+    const T dx = alpha.x - beta, idd = rcp_t(dx * dx);
+    scale = mk<T>(dx * idd, T(0));
   } else {
     skip = false;
     const T nrm = sqrt(alpha.x * alpha.x + alpha.y * alpha.y + xn2);
@@ -879,7 +926,7 @@ __device__ __forceinline__ T chase_step_full(typename C2T<T>::t* bnd, typename C
     const T dx = alpha.x - beta, dy = alpha.y, idd = rcp_t(dx * dx + dy * dy);
     scale = mk<T>(dx * idd, -dy * idd);
   }
-  const C2 v = li == 0 ? mk<T>(1, 0) : (skip ? zero : c_mul(x, scale));
+  const C2 v = li == 0 ? mk<T>(1, 0) : (skip ? zero : (RE ? mk<T>(x.x * scale.x, T(0)) : c_mul(x, scale)));
   if (lg == 0) {
     const C2 xn = li == 0 ? mk<T>(beta, 0) : zero;
     if constexpr (BUF)
@@ -896,36 +943,41 @@ __device__ __forceinline__ T chase_step_full(typename C2T<T>::t* bnd, typename C
   C2 db = zero, y = zero, z = zero;
 #pragma unroll
   for (int t = 0; t < 4; ++t) {
-    db = ccfma(vr[t], eb[t], db);
-    y = cfma(ed[t], vr[t], y);
-    z = cfma(ee[t], vr[t], z);
+    db = ccfma_r<RE>(vr[t], eb[t], db);
+    y = cfma_r<RE>(ed[t], vr[t], y);
+    z = cfma_r<RE>(ee[t], vr[t], z);
   }
-  db = sum4g_c(db);
-  y = sum4g_c(y);
-  z = sum4g_c(z);
-  const C2 fb = c_mul(c_conj(tau), db);
+  if constexpr (RE) {
+    sum4g_pair(db.x, y.x);
+    z.x = add_partner<32>(add_partner<16>(z.x));
+  } else {
+    db = sum4g_c(db);
+    y = sum4g_c(y);
+    z = sum4g_c(z);
+  }
+  const C2 fb = RE ? mk<T>(tau.x * db.x, T(0)) : c_mul(c_conj(tau), db);
 #pragma unroll
   for (int t = 0; t < 4; ++t)
     if (li < BW - 1) {
-      const C2 nb = cfms(vr[t], fb, eb[t]);
+      const C2 nb = cfms_r<RE>(vr[t], fb, eb[t]);
       if constexpr (BUF)
         band_st(rb, 8u * cl_ob(o, t), sb, nb);
       else
         at_off(bb, cl_ob(o, t)) = nb;
     }
-  const T vhy = sum16(v.x * y.x + v.y * y.y);
-  const T t2 = T(0.5) * (tau.x * tau.x + tau.y * tau.y) * vhy;
-  const C2 ty = c_mul(tau, y);
-  const C2 w = mk<T>(ty.x - t2 * v.x, ty.y - t2 * v.y);
+  const T vhy = sum16(RE ? v.x * y.x : v.x * y.x + v.y * y.y);
+  const T t2 = T(0.5) * (RE ? tau.x * tau.x : tau.x * tau.x + tau.y * tau.y) * vhy;
+  const C2 ty = RE ? mk<T>(tau.x * y.x, T(0)) : c_mul(tau, y);
+  const C2 w = RE ? mk<T>(ty.x - t2 * v.x, T(0)) : mk<T>(ty.x - t2 * v.x, ty.y - t2 * v.y);
   if (lg == 0) sv[BW + li] = w;
   __builtin_amdgcn_wave_barrier();
-  const C2 tz = c_mul(tau, z);
+  const C2 tz = RE ? mk<T>(tau.x * z.x, T(0)) : c_mul(tau, z);
 #pragma unroll
   for (int t = 0; t < 4; ++t) {
     const int r = lg + 4 * t;
     const C2 wr = sv[BW + r];
     if (!cl_up(o, t)) {
-      C2 e = cfmsc(v, wr, cfmsc(w, vr[t], ed[t]));
+      C2 e = cfmsc_r<RE>(v, wr, cfmsc_r<RE>(w, vr[t], ed[t]));
       if (r == li) e.y = T(0);
       if constexpr (BUF)
         band_st(rb, 8u * cl_od(o, t), sd, e);
@@ -934,7 +986,7 @@ __device__ __forceinline__ T chase_step_full(typename C2T<T>::t* bnd, typename C
     }
     // to_stage: the next step of this sweep (full, same wave) takes the
     // block below from the stage and writes it back to the band itself
-    const C2 ev = cfmsc(tz, vr[t], ee[t]);
+    const C2 ev = cfmsc_r<RE>(tz, vr[t], ee[t]);
     if (to_stage) {
       stg[li][r] = ev;
     } else if constexpr (BUF) {
@@ -964,7 +1016,7 @@ __device__ __forceinline__ int chase_nsteps(int n, int s) {
 // NaN; the kernel still runs to its end.
 // LDSB: the compact band lives in LDS (dynamic shared memory); otherwise in
 // global memory.
-template <typename T, bool LDSB>
+template <typename T, bool LDSB, bool RE = false>
 __global__ __launch_bounds__(CH_THREADS) void band_chase_kernel(BandCtx<T> c, unsigned long long* clk) {
   using C2 = typename C2T<T>::t;
   sample_start(clk);
@@ -1035,7 +1087,7 @@ __global__ __launch_bounds__(CH_THREADS) void band_chase_kernel(BandCtx<T> c, un
       T beta;
       if (full) {
         const bool to_stage = !LDSB && cs + 1 < nst && q1 + 3 * BW - 1 <= n - 1;
-        beta = chase_step_full<T, BUF>(bnd, a + (size_t)s * ld, s_vw[w], c1, q1, lane, lo, s_stage[LDSB ? 0 : w],
+        beta = chase_step_full<T, BUF, RE>(bnd, a + (size_t)s * ld, s_vw[w], c1, q1, lane, lo, s_stage[LDSB ? 0 : w],
                                        staged, to_stage, rb);
         staged = to_stage;
       } else {
@@ -1113,7 +1165,7 @@ size_t band_tt_offset(int ld, int batch) {
 // C[0:m, 0:16] -= [V | W] [W | V]^H restricted to the first 16 columns of
 // A22(p) (C at its origin, operands at its first row). One 16 x 16 block per
 // wave, 64 rows per workgroup.
-template <typename T>
+template <typename T, bool RE = false>
 __global__ __launch_bounds__(256) void strip_update_kernel(int m, const typename C2T<T>::t* __restrict__ Vc,
                                                            const typename C2T<T>::t* __restrict__ Wc,
                                                            typename C2T<T>::t* __restrict__ C, int ld, long long sP,
@@ -1148,14 +1200,16 @@ __global__ __launch_bounds__(256) void strip_update_kernel(int m, const typename
 #pragma unroll
   for (int q = 0; q < KK; ++q) {
     accR = MF<T>::mfma(-av[q].x, bw[q].x, accR);
-    accR = MF<T>::mfma(-av[q].y, bw[q].y, accR);
-    accI = MF<T>::mfma(-av[q].y, bw[q].x, accI);
-    accI = MF<T>::mfma(av[q].x, bw[q].y, accI);
+    if constexpr (!RE) {
+      accR = MF<T>::mfma(-av[q].y, bw[q].y, accR);
+      accI = MF<T>::mfma(-av[q].y, bw[q].x, accI);
+      accI = MF<T>::mfma(av[q].x, bw[q].y, accI);
+    }
   }
 #pragma unroll
   for (int v = 0; v < 4; ++v) {
     const int r = m0 + MF<T>::row(lk, v);
-    if (r < m && li < ncol) c[(size_t)r * ld + li] = mk<T>(accR[v], accI[v]);
+    if (r < m && li < ncol) c[(size_t)r * ld + li] = mk<T>(accR[v], RE ? T(0) : accI[v]);
   }
 }
 
@@ -1170,7 +1224,10 @@ __global__ __launch_bounds__(256) void strip_update_kernel(int m, const typename
 //      into chunk J, and (I != J) rows of block J from A_IJ^H V_I into chunk
 //      I. Every (row block, chunk) slot is written by exactly one tile.
 // Vp, Wp: panel p's [V | W], [W | V] at row r0n; Vn: V_{p+1} at row r0n.
-template <typename T>
+// RE: real data (zero imaginary parts): one real MFMA per block product,
+// no imaginary LDS planes (half the LDS, twice the workgroups per CU), and
+// zero imaginary parts written.
+template <typename T, bool RE = false>
 __global__ __launch_bounds__(256) void fused_update_av_kernel(const typename C2T<T>::t* __restrict__ Vp,
                                                               const typename C2T<T>::t* __restrict__ Wp,
                                                               const typename C2T<T>::t* __restrict__ Vn,
@@ -1202,8 +1259,8 @@ __global__ __launch_bounds__(256) void fused_update_av_kernel(const typename C2T
   const C2* wp = Wp + (size_t)b * sP;
   const C2* vn = Vn + (size_t)b * sP;
   C2* a = A + (size_t)b * sM + (size_t)r0n * ld + r0n;
-  __shared__ T Cr[64][65], Ci[64][65];
-  __shared__ T VJr[64][BW + 1], VJi[64][BW + 1], VIr[64][BW + 1], VIi[64][BW + 1];
+  __shared__ T Cr[64][65], Ci[RE ? 1 : 64][65];
+  __shared__ T VJr[64][BW + 1], VJi[RE ? 1 : 64][BW + 1], VIr[64][BW + 1], VIi[RE ? 1 : 64][BW + 1];
   // V_{p+1} rows of blocks J and I (zero beyond the matrix)
   for (int e = tid; e < 64 * BW; e += 256) {
     const int col = e / 64, k = e % 64;
@@ -1211,9 +1268,11 @@ __global__ __launch_bounds__(256) void fused_update_av_kernel(const typename C2T
     const C2 xj = load_or_zero(vn + (size_t)col * ld + rj, rj < mn);
     const C2 xi = load_or_zero(vn + (size_t)col * ld + ri, ri < mn);
     VJr[k][col] = xj.x;
-    VJi[k][col] = xj.y;
     VIr[k][col] = xi.x;
-    VIi[k][col] = xi.y;
+    if constexpr (!RE) {
+      VJi[k][col] = xj.y;
+      VIi[k][col] = xi.y;
+    }
   }
   // 1. rank-2k on this wave's 32 x 32 block
   const int wr = (wv >> 1) * 32, wc = (wv & 1) * 32;
@@ -1253,9 +1312,11 @@ __global__ __launch_bounds__(256) void fused_update_av_kernel(const typename C2T
 #pragma unroll
         for (int j = 0; j < 2; ++j) {
           accR[i][j] = MF<T>::mfma(-av[i][q].x, bw[j][q].x, accR[i][j]);
-          accR[i][j] = MF<T>::mfma(-av[i][q].y, bw[j][q].y, accR[i][j]);
-          accI[i][j] = MF<T>::mfma(-av[i][q].y, bw[j][q].x, accI[i][j]);
-          accI[i][j] = MF<T>::mfma(av[i][q].x, bw[j][q].y, accI[i][j]);
+          if constexpr (!RE) {
+            accR[i][j] = MF<T>::mfma(-av[i][q].y, bw[j][q].y, accR[i][j]);
+            accI[i][j] = MF<T>::mfma(-av[i][q].y, bw[j][q].x, accI[i][j]);
+            accI[i][j] = MF<T>::mfma(av[i][q].x, bw[j][q].y, accI[i][j]);
+          }
         }
 #pragma unroll
     for (int i = 0; i < 2; ++i)
@@ -1265,9 +1326,9 @@ __global__ __launch_bounds__(256) void fused_update_av_kernel(const typename C2T
         for (int v = 0; v < 4; ++v) {
           const int rl = wr + 16 * i + MF<T>::row(lk, v), cl = wc + 16 * j + li;
           const int r = 64 * I + rl, col = 64 * J + cl;
-          if (r < mn && col < mn) a[(size_t)r * ld + col] = mk<T>(accR[i][j][v], accI[i][j][v]);
+          if (r < mn && col < mn) a[(size_t)r * ld + col] = mk<T>(accR[i][j][v], RE ? T(0) : accI[i][j][v]);
           Cr[rl][cl] = accR[i][j][v];
-          Ci[rl][cl] = accI[i][j][v];
+          if constexpr (!RE) Ci[rl][cl] = accI[i][j][v];
         }
   }
   __syncthreads();
@@ -1277,7 +1338,7 @@ __global__ __launch_bounds__(256) void fused_update_av_kernel(const typename C2T
       const int r = e / 64, col = e % 64;
       if (col > r) {
         Cr[r][col] = Cr[col][r];
-        Ci[r][col] = -Ci[col][r];
+        if constexpr (!RE) Ci[r][col] = -Ci[col][r];
       }
     }
     __syncthreads();
@@ -1288,18 +1349,20 @@ __global__ __launch_bounds__(256) void fused_update_av_kernel(const typename C2T
     V4 yR = V4{0, 0, 0, 0}, yI = V4{0, 0, 0, 0};
 #pragma unroll
     for (int k4 = 0; k4 < 64; k4 += 4) {
-      const T ar = Cr[rw + li][k4 + lk], ai = Ci[rw + li][k4 + lk];
-      const T br = VJr[k4 + lk][li], bi = VJi[k4 + lk][li];
+      const T ar = Cr[rw + li][k4 + lk], br = VJr[k4 + lk][li];
       yR = MF<T>::mfma(ar, br, yR);
-      yR = MF<T>::mfma(-ai, bi, yR);
-      yI = MF<T>::mfma(ar, bi, yI);
-      yI = MF<T>::mfma(ai, br, yI);
+      if constexpr (!RE) {
+        const T ai = Ci[rw + li][k4 + lk], bi = VJi[k4 + lk][li];
+        yR = MF<T>::mfma(-ai, bi, yR);
+        yI = MF<T>::mfma(ar, bi, yI);
+        yI = MF<T>::mfma(ai, br, yI);
+      }
     }
     C2* yp = Yp + ((size_t)b * nch + J) * ld * BW;
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
       const int r = 64 * I + rw + MF<T>::row(lk, v);
-      if (r < mn) yp[(size_t)(r0n + r) * BW + li] = mk<T>(yR[v], yI[v]);
+      if (r < mn) yp[(size_t)(r0n + r) * BW + li] = mk<T>(yR[v], RE ? T(0) : yI[v]);
     }
   }
   if (I != J) {
@@ -1307,18 +1370,20 @@ __global__ __launch_bounds__(256) void fused_update_av_kernel(const typename C2T
 #pragma unroll
     for (int k4 = 0; k4 < 64; k4 += 4) {
       // (A^H)[i][k] = conj(A[k][i])
-      const T ar = Cr[k4 + lk][rw + li], ai = -Ci[k4 + lk][rw + li];
-      const T br = VIr[k4 + lk][li], bi = VIi[k4 + lk][li];
+      const T ar = Cr[k4 + lk][rw + li], br = VIr[k4 + lk][li];
       yR = MF<T>::mfma(ar, br, yR);
-      yR = MF<T>::mfma(-ai, bi, yR);
-      yI = MF<T>::mfma(ar, bi, yI);
-      yI = MF<T>::mfma(ai, br, yI);
+      if constexpr (!RE) {
+        const T ai = -Ci[k4 + lk][rw + li], bi = VIi[k4 + lk][li];
+        yR = MF<T>::mfma(-ai, bi, yR);
+        yI = MF<T>::mfma(ar, bi, yI);
+        yI = MF<T>::mfma(ai, br, yI);
+      }
     }
     C2* yp = Yp + ((size_t)b * nch + I) * ld * BW;
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
       const int r = 64 * J + rw + MF<T>::row(lk, v);
-      if (r < mn) yp[(size_t)(r0n + r) * BW + li] = mk<T>(yR[v], yI[v]);
+      if (r < mn) yp[(size_t)(r0n + r) * BW + li] = mk<T>(yR[v], RE ? T(0) : yI[v]);
     }
   }
   if (clk != nullptr) {
@@ -1329,9 +1394,9 @@ __global__ __launch_bounds__(256) void fused_update_av_kernel(const typename C2T
 
 #define CH_LDS_MAX (150 * 1024)
 // Dynamic LDS beyond the default limit has to be allowed per kernel once.
-template <typename T>
+template <typename T, bool RE>
 static bool chase_lds_ok() {
-  static const bool ok = hipFuncSetAttribute(reinterpret_cast<const void*>(&band_chase_kernel<T, true>),
+  static const bool ok = hipFuncSetAttribute(reinterpret_cast<const void*>(&band_chase_kernel<T, true, RE>),
                                              hipFuncAttributeMaxDynamicSharedMemorySize, CH_LDS_MAX) == hipSuccess;
   return ok;
 }
@@ -1357,7 +1422,7 @@ static void launch_panel_qr(hipStream_t st, const BandCtx<T>& c, int batch, int 
 // Polls per chase wait before the wave gives up (process-wide).
 static std::atomic<int> g_chase_spin_max{1 << 22};
 
-template <typename T>
+template <typename T, bool RE>
 static void band_impl(hipStream_t st, typename C2T<T>::t* A, int n, int ld, int batch, double* D, double* E,
                       typename C2T<T>::t* TAU, typename C2T<T>::t* Vc, typename C2T<T>::t* Wc,
                       typename C2T<T>::t* Yp, typename C2T<T>::t* scratch, typename C2T<T>::t* VT, int* info) {
@@ -1404,7 +1469,7 @@ static void band_impl(hipStream_t st, typename C2T<T>::t* A, int n, int ld, int 
       const BandCtx<T>& cp = cx[p & 1];
       const BandCtx<T>& cn = cx[(p + 1) & 1];
       const int r0 = BW * (p + 1), mp = n - r0;       // A22(p)
-      hipLaunchKernelGGL(strip_update_kernel<T>, dim3((mp + 63) / 64, 1, batch), dim3(256), 0, st, mp, cp.Vc + r0,
+      hipLaunchKernelGGL((strip_update_kernel<T, RE>), dim3((mp + 63) / 64, 1, batch), dim3(256), 0, st, mp, cp.Vc + r0,
                          cp.Wc + r0, A + (size_t)r0 * ld + r0, ld, sP, sM);
       const int r0n = r0 + BW, mn = n - r0n;          // A22(p + 1)
       if (mn < 1) break;
@@ -1418,11 +1483,11 @@ static void band_impl(hipStream_t st, typename C2T<T>::t* A, int n, int ld, int 
       const int ntri = t * (t + 1) / 2;
       const int nwg = (ntri * batch + 7) / 8 * 8;
       if (fsam)
-        hipExtLaunchKernelGGL(fused_update_av_kernel<T>, dim3(nwg), dim3(256), 0, st, fs.e0, fs.e1, 0,
+        hipExtLaunchKernelGGL((fused_update_av_kernel<T, RE>), dim3(nwg), dim3(256), 0, st, fs.e0, fs.e1, 0,
                               cp.Vc + r0n, cp.Wc + r0n, cn.Vc + r0n, A, c.Yp, n, ld, r0n, c.nch, sP, sM, ntri,
                               batch, fs.clk);
       else
-        hipLaunchKernelGGL(fused_update_av_kernel<T>, dim3(nwg), dim3(256), 0, st, cp.Vc + r0n,
+        hipLaunchKernelGGL((fused_update_av_kernel<T, RE>), dim3(nwg), dim3(256), 0, st, cp.Vc + r0n,
                            cp.Wc + r0n, cn.Vc + r0n, A, c.Yp, n, ld, r0n, c.nch, sP, sM, ntri, batch,
                            (unsigned long long*)nullptr);
       hipLaunchKernelGGL(band_x_kernel<T>, dim3(t, batch), dim3(256), 0, st, cn, r0n, t);
@@ -1436,30 +1501,36 @@ static void band_impl(hipStream_t st, typename C2T<T>::t* A, int n, int ld, int 
   // own compute unit, so flops / window is the per-CU rate
   const bool sampled = sampler_enabled() && sampler_take(SLK_SAMPLE_CHASE, chase_flops(n), &sl);
   unsigned long long* clk = sampled ? sl.clk : nullptr;
-  if (lds <= CH_LDS_MAX && chase_lds_ok<T>()) {
+  if (lds <= CH_LDS_MAX && chase_lds_ok<T, RE>()) {
     if (sampled)
-      hipExtLaunchKernelGGL((band_chase_kernel<T, true>), dim3(batch), dim3(CH_THREADS), lds, st, sl.e0, sl.e1, 0, c,
+      hipExtLaunchKernelGGL((band_chase_kernel<T, true, RE>), dim3(batch), dim3(CH_THREADS), lds, st, sl.e0, sl.e1, 0, c,
                             clk);
     else
-      hipLaunchKernelGGL((band_chase_kernel<T, true>), dim3(batch), dim3(CH_THREADS), lds, st, c, clk);
+      hipLaunchKernelGGL((band_chase_kernel<T, true, RE>), dim3(batch), dim3(CH_THREADS), lds, st, c, clk);
   } else {
     if (sampled)
-      hipExtLaunchKernelGGL((band_chase_kernel<T, false>), dim3(batch), dim3(CH_THREADS), 0, st, sl.e0, sl.e1, 0, c,
+      hipExtLaunchKernelGGL((band_chase_kernel<T, false, RE>), dim3(batch), dim3(CH_THREADS), 0, st, sl.e0, sl.e1, 0, c,
                             clk);
     else
-      hipLaunchKernelGGL((band_chase_kernel<T, false>), dim3(batch), dim3(CH_THREADS), 0, st, c, clk);
+      hipLaunchKernelGGL((band_chase_kernel<T, false, RE>), dim3(batch), dim3(CH_THREADS), 0, st, c, clk);
   }
   (void)sizeof(C2);
 }
 
 void launch_band_tridiag(hipStream_t st, double2* C, int n, int ld, int batch, double* D, double* E, double2* TAU,
-                         double2* Vc, double2* Wc, double2* Yp, double2* scratch, double2* VT, int* info) {
-  band_impl<double>(st, C, n, ld, batch, D, E, TAU, Vc, Wc, Yp, scratch, VT, info);
+                         double2* Vc, double2* Wc, double2* Yp, double2* scratch, double2* VT, int* info, bool re) {
+  if (re)
+    band_impl<double, true>(st, C, n, ld, batch, D, E, TAU, Vc, Wc, Yp, scratch, VT, info);
+  else
+    band_impl<double, false>(st, C, n, ld, batch, D, E, TAU, Vc, Wc, Yp, scratch, VT, info);
 }
 
 void launch_band_tridiag_f32(hipStream_t st, float2* C, int n, int ld, int batch, double* D, double* E, float2* TAU,
-                             float2* Vc, float2* Wc, float2* Yp, float2* scratch, float2* VT, int* info) {
-  band_impl<float>(st, C, n, ld, batch, D, E, TAU, Vc, Wc, Yp, scratch, VT, info);
+                             float2* Vc, float2* Wc, float2* Yp, float2* scratch, float2* VT, int* info, bool re) {
+  if (re)
+    band_impl<float, true>(st, C, n, ld, batch, D, E, TAU, Vc, Wc, Yp, scratch, VT, info);
+  else
+    band_impl<float, false>(st, C, n, ld, batch, D, E, TAU, Vc, Wc, Yp, scratch, VT, info);
 }
 
 extern "C" int slk_set_chase_spin_limit(int polls) {

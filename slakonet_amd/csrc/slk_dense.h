@@ -57,11 +57,15 @@ constexpr int SLK_BW = 16;   // band width of the two-stage reduction
 // triangle holds sweep s's reflectors, tau in place of each leading 1.
 // Workspace: Vc, Wc [2 SLK_BW][ld] and Yp [band_partials_elems(ld)] per
 // matrix, band scratch [band_scratch_elems(ld)] per matrix.
-// info (nullable, [batch]): set to 2 for a matrix whose bulge chase gave up a bounded wait
+// info (nullable, [batch]): set to 2 for a matrix whose bulge chase gave up a bounded wait.
+// re: C is real symmetric (zero imaginary parts); every output keeps zero
+// imaginary parts.
 void launch_band_tridiag(hipStream_t st, double2* C, int n, int ld, int batch, double* D, double* E, double2* TAU,
-                         double2* Vc, double2* Wc, double2* Yp, double2* scratch, double2* VT, int* info);
+                         double2* Vc, double2* Wc, double2* Yp, double2* scratch, double2* VT, int* info,
+                         bool re = false);
 void launch_band_tridiag_f32(hipStream_t st, float2* C, int n, int ld, int batch, double* D, double* E, float2* TAU,
-                             float2* Vc, float2* Wc, float2* Yp, float2* scratch, float2* VT, int* info);
+                             float2* Vc, float2* Wc, float2* Yp, float2* scratch, float2* VT, int* info,
+                             bool re = false);
 size_t band_partials_elems(int ld);
 size_t band_scratch_elems(int ld);
 // Panels of the band reduction per matrix, and the element offset inside the

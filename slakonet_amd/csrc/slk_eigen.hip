@@ -1246,7 +1246,7 @@ static int eigh_impl(const void* H, const void* S, int n, int ld, int batch, dou
     if (band)
       // B2 (W H) is free after the reduction: it holds the split-K partials
       launch_band_tridiag(stream, s.B0, n, ld, batch, s.D, s.E, s.TAU, s.Vc, s.Wc, s.B2, s.Band, nullptr,
-                          info + batch);
+                          info + batch, re);
     else
       launch_hetrd_blocked(stream, s.B0, n, ld, batch, s.D, s.E, s.TAU, s.Vc, s.Wc, s.Y, s.Yp, s.C12);
   } else {
@@ -1261,7 +1261,7 @@ static int eigh_impl(const void* H, const void* S, int n, int ld, int batch, dou
     // rows; F2 (W32 H32) holds the split-K partials of the band reduction
     if (band)
       launch_band_tridiag_f32(stream, s.F1, n, ld, batch, s.D, s.E, s.fTAU, s.fVc, s.fWc, s.F2, s.fBand, s.F0,
-                              info + batch);
+                              info + batch, re);
     else
       launch_hetrd_blocked_f32(stream, s.F1, n, ld, batch, s.D, s.E, s.fTAU, s.fVc, s.fWc, s.fY, s.fYp, s.fC12,
                                s.F0);

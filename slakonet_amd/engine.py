@@ -58,6 +58,7 @@ class BuildPlan:
     nelec: List[float]
     dev: dict = field(default_factory=dict)
     n_work: int = 0
+    real: bool = False               # every matrix real symmetric (set by callers that solve it so)
 
 
 def _upload(arr: np.ndarray, device) -> torch.Tensor:
@@ -204,6 +205,15 @@ PRECISIONS = ("fp64", "mixed")
 
 
 EIGH_MIXED, EIGH_REAL = 1, 2   # slk_eigh_ex flags
+
+
+def real_kpoints(kfrac) -> bool:
+    """True when every Bloch phase exp(2 pi i k.T) over integer T is +-1, i.e.
+    2k is integer for every k-point (Gamma and the other time-reversal
+    invariant points, such as the 1x1x1 Monkhorst-Pack point (1/2, 1/2, 1/2)):
+    H(k) and S(k) are then real symmetric."""
+    k2 = 2.0 * np.asarray(kfrac, dtype=np.float64)
+    return bool(np.all(np.abs(k2 - np.round(k2)) < 1e-12))
 
 
 def eigh(H: torch.Tensor, S: torch.Tensor, vectors: bool = False, stream=None, check: bool = True,

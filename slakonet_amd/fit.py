@@ -65,6 +65,8 @@ class BatchedProperties:
         self.buckets = []
         for ld, idx in sorted(groups.items()):
             plan = engine.plan_build(tables, [inputs[i] for i in idx], ld=ld)
+            # a bucket whose k-points all have real H and S is solved as real symmetric
+            plan.real = all(engine.real_kpoints(inputs[i].kpoints) for i in idx)
             self.buckets.append((idx, plan))
 
     def __call__(self, table_tensor: torch.Tensor, with_grad: bool = True, fermi_grad: bool = False):
@@ -75,7 +77,7 @@ class BatchedProperties:
         tt = table_tensor.detach()
         for idx, plan in self.buckets:
             H, S = engine.run_build(self.tables, plan, table_tensor=tt)
-            w, V, _ = engine.eigh(H, S, vectors=with_grad)
+            w, V, _ = engine.eigh(H, S, vectors=with_grad, real=plan.real)
             del H, S
             edges, _, _ = engine.post(plan, w)
             wg = TableEigenvalues.apply(table_tensor, self.tables, plan, V, w) if with_grad else w

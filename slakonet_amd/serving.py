@@ -63,6 +63,7 @@ class Request:
     n_orb: int
     arrival: float
     ld: int
+    real: bool = False   # every k-point's H and S are real symmetric
 
 
 @dataclass
@@ -100,7 +101,7 @@ class ServingEngine:
                  max_inflight: int = 8, hbm_budget_bytes: float = 64e9, max_batch_matrices: int = 512,
                  fermi_mode: int = 0, keep_outputs: bool = False, precision: str = "mixed",
                  refine_edges: bool = True, bucket=bucket_dim, graphs: bool = True, graph_max_ld: int = 256,
-                 graph_cache_bytes: float = 8e9):
+                 graph_cache_bytes: float = 8e9, real_symmetric: bool = True):
         self.tables = tables
         self.dev = tables.device
         self.dev_index = self.dev.index if self.dev.index is not None else torch.cuda.current_device()
@@ -124,6 +125,7 @@ class ServingEngine:
         self.nf = len(engine.EDGE_FIELDS)
         self._rr = 0
         self._bytes_cache = {}
+        self.real_symmetric = real_symmetric
         self.graphs = graphs
         self.graph_max_ld = graph_max_ld
         self.graph_cache_bytes = graph_cache_bytes
@@ -163,7 +165,7 @@ class ServingEngine:
         nmat * ld^3."""
         return float(ld) * ld + nmat * float(ld) ** 3 / 4096.0
 
-    def _submit(self, reqs: List[Request], ld: int) -> Future:
+    def _submit(self, reqs: List[Request], ld: int, real: bool = False) -> Future:
         """Queue a batch on the stream with the least pending cost, so that a
         small batch does not wait behind a large one on the same stream."""
         nmat = sum(len(r.inp.kpoints) for r in reqs)
@@ -172,7 +174,7 @@ class ServingEngine:
         self._rr += 1
         self._pending[i] += 1
         self._pending_cost[i] += cost
-        fut = self.launchers[i].submit(self._launch, reqs, ld, self.streams[i], i)
+        fut = self.launchers[i].submit(self._launch, reqs, ld, self.streams[i], i, real)
         fut.stream_index = i
         fut.cost = cost
         fut.reqs = reqs
@@ -216,7 +218,7 @@ class ServingEngine:
                 del self._graphs[k]
                 self._graph_bytes -= old.nbytes
 
-    def _capture(self, H: torch.Tensor, S: torch.Tensor, si: int) -> SolveGraph:
+    def _capture(self, H: torch.Tensor, S: torch.Tensor, si: int, real: bool) -> SolveGraph:
         """Capture the eigensolve over H, S on the launch thread's capture
         stream; the capture records without executing, the caller replays on
         its serving stream (after the build that fills H and S)."""
@@ -225,16 +227,16 @@ class ServingEngine:
         with torch.cuda.stream(self.capture_streams[si]):
             g.capture_begin(capture_error_mode="thread_local")
             try:
-                w, _, info = engine.eigh(H, S, check=False, precision=self.precision, workspace=wsl)
+                w, _, info = engine.eigh(H, S, check=False, precision=self.precision, workspace=wsl, real=real)
             finally:
                 g.capture_end()
         nbytes = 2 * H.numel() * H.element_size() + wsl[0].numel() + w.numel() * 8 + info.numel() * 4
         return SolveGraph(g, H, S, w, info, wsl[0], nbytes)
 
-    def _solve(self, plan, ld: int, nmat: int, stream, si: int):
+    def _solve(self, plan, ld: int, nmat: int, stream, si: int, real: bool = False):
         """Build and solve one batch on the current stream: from a cached
         graph for small buckets, else eagerly. Returns H, S, w, info, workspace."""
-        key = (ld, nmat)
+        key = (ld, nmat, real)
         # no capture while the kernel sampler is on (its launches record events)
         graphs = self.graphs and ld <= self.graph_max_ld and not engine.SAMPLING
         e = self._graph_get(key) if graphs else None
@@ -247,7 +249,7 @@ class ServingEngine:
         else:
             H, S = engine.run_build(self.tables, plan)
             if graphs and self._graph_wanted(key, 3 * H.numel() * 16):
-                e = self._capture(H, S, si)
+                e = self._capture(H, S, si, real)
                 self._graph_insert(key, e)
                 self.graph_captures += 1
         if e is not None:
@@ -256,15 +258,15 @@ class ServingEngine:
             self.graph_replays += 1
             return H, S, e.w, e.info, e.ws, e
         wsl = []
-        w, _, info = engine.eigh(H, S, check=False, precision=self.precision, workspace=wsl)
+        w, _, info = engine.eigh(H, S, check=False, precision=self.precision, workspace=wsl, real=real)
         return H, S, w, info, wsl[0], None
 
-    def _launch(self, reqs: List[Request], ld: int, stream, si: int = 0) -> Batch:
+    def _launch(self, reqs: List[Request], ld: int, stream, si: int = 0, real: bool = False) -> Batch:
         torch.cuda.set_device(self.dev_index)
         nmat = sum(len(r.inp.kpoints) for r in reqs)
         with torch.cuda.stream(stream):
             plan = engine.plan_build(self.tables, [r.inp for r in reqs], ld=ld)
-            H, S, w, info, ws, ge = self._solve(plan, ld, nmat, stream, si)
+            H, S, w, info, ws, ge = self._solve(plan, ld, nmat, stream, si, real)
             refine = (H, S, ws) if (self.refine_edges and self.precision == "mixed") else None
             edges, dos, stats = engine.post(plan, w, fermi_mode=self.fermi_mode, dos_grid=self.grid, refine=refine)
             if ge is not None:
@@ -345,7 +347,10 @@ class ServingEngine:
                 inp = inputs[i]
                 no = self._orbitals(inp)
                 ld = self.bucket(no)
-                queues.setdefault(ld, deque()).append(Request(i, inp, no, float(arrivals[i]), ld))
+                # batches are formed per (bucket, real): a real batch skips the
+                # imaginary arithmetic of the solve
+                real = self.real_symmetric and engine.real_kpoints(inp.kpoints)
+                queues.setdefault((ld, real), deque()).append(Request(i, inp, no, float(arrivals[i]), ld, real))
             progressed = False
             for item in list(inflight):
                 fut, nbytes, t_sub = item
@@ -374,11 +379,11 @@ class ServingEngine:
                     progressed = True
             # launch: oldest bucket first while slots and budget allow
             while len(inflight) < self.max_inflight:
-                live = [(q[0].arrival, ld) for ld, q in queues.items() if q]
+                live = [(q[0].arrival, key) for key, q in queues.items() if q]
                 if not live:
                     break
-                _, ld = min(live)
-                q = queues[ld]
+                _, (ld, real) = min(live)
+                q = queues[(ld, real)]
                 take = []
                 nmat = 0
                 while q and len(take) < self.max_batch:
@@ -394,7 +399,7 @@ class ServingEngine:
                 if not take:
                     break
                 nbytes = self.batch_bytes(ld, nmat)
-                inflight.append([self._submit(take, ld), nbytes, time.perf_counter() - t0])
+                inflight.append([self._submit(take, ld, real), nbytes, time.perf_counter() - t0])
                 used += nbytes
                 progressed = True
             if not progressed:
