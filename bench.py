@@ -200,7 +200,8 @@ FP64_MFMA_PEAK_TFS = 78.6
 FP32_MFMA_PEAK_TFS = 157.3
 # one compute unit's FP32 VALU rate: 4 SIMDs x 16 lanes x 2 flops (FMA) x 2.4 GHz
 CU_FP32_VALU_PEAK_TFS = 4 * 16 * 2 * 2.4e9 / 1e12
-KINDS = ("hetrd_step", "build", "gemm_f64", "chase", "fused_update_av", "dos")
+KINDS = ("hetrd_step", "build", "gemm_f64", "chase", "fused_update_av", "dos", "fused_update_av_real",
+         "chase_real")
 
 
 def start_sampling(every, capacity):
@@ -275,11 +276,19 @@ def roofline_fields(samples):
     bandwidth (the DOS also as Gaussians per second of the full sum)."""
     out = {}
     r = _mfma_roofline(samples["fused_update_av"], FP32_MFMA_PEAK_TFS,
-                       "fused_update_av_kernel<float> (stage-1 rank-2k update of the lower triangle + next A22 V)",
+                       "fused_update_av_kernel<float, false> (complex batches: stage-1 rank-2k update of the "
+                       "lower triangle + next A22 V)",
                        FUSED_TRAFFIC_FILE, "hbm_bytes_per_flop")
     if r is not None:
         r["algorithmic_bytes_per_flop"] = 1.0 / 32.0   # lower triangle read and written once, 8 B per element
         out["roofline"] = r
+    r = _mfma_roofline(samples["fused_update_av_real"], FP32_MFMA_PEAK_TFS,
+                       "fused_update_av_kernel<float, true> (real-symmetric batches: one real MFMA per block)",
+                       FUSED_TRAFFIC_FILE, "hbm_bytes_per_flop_real")
+    if r is not None:
+        # the real path keeps the complex64 layout (zero imaginary parts): same bytes, a quarter of the flops
+        r["algorithmic_bytes_per_flop"] = 1.0 / 8.0
+        out["roofline_fused_real"] = r
     r = _mfma_roofline(samples["gemm_f64"], FP64_MFMA_PEAK_TFS,
                        "gemm_mfma_kernel<double> (Cholesky trailing updates, L^-1)", GEMM_TRAFFIC_FILE,
                        "hbm_bytes_per_flop")
@@ -296,6 +305,16 @@ def roofline_fields(samples):
                                  "avg_launch_us": 1000.0 * ms, "avg_launch_us_events": 1000.0 * ev,
                                  "algorithmic_flops_per_workgroup": fl, "launches_sampled": int(n),
                                  "launches_total": int(n_all), "est_kernel_ms_total": ms * n_all}
+    ch = _window(samples["chase_real"])
+    if ch is not None:
+        n, ms, ev, fl, n_all, fl_all, _ = ch
+        ach = fl / (ms * 1e-3) / 1e12
+        out["roofline_chase_real"] = {"bound": "valu (one CU)", "achieved": ach, "peak": CU_FP32_VALU_PEAK_TFS,
+                                      "unit": "TFLOP/s", "frac": ach / CU_FP32_VALU_PEAK_TFS,
+                                      "kernel": "band_chase_kernel<float, *, true> (real-symmetric batches)",
+                                      "avg_launch_us": 1000.0 * ms, "avg_launch_us_events": 1000.0 * ev,
+                                      "algorithmic_flops_per_workgroup": fl, "launches_sampled": int(n),
+                                      "launches_total": int(n_all), "est_kernel_ms_total": ms * n_all}
     h = _window(samples["hetrd_step"])
     if h is not None:
         n, ms, ev, nbytes, n_all, _, _ = h

@@ -268,11 +268,15 @@ int slk_refine_edges_mixed(const void* H, const void* S, int n, int ld, int batc
  * tridiagonalisation mat-vec (bytes: one triangle of the trailing Hermitian
  * block per matrix), 1 = H/S build (bytes from slk_sampler_hint, per calling
  * thread: the H and S elements written), 2 = FP64 MFMA GEMM (flops of the
- * executed output tiles, 64x64 or 32x32), 3 = band chase (flops of the steps of one
+ * executed output tiles, 64x64 or 32x32; a real product counts a quarter),
+ * 3 = band chase of complex Hermitian batches (flops of the steps of one
  * matrix, i.e. per workgroup), 4 = FP32 fused stage-1 update and next A22 V
- * (flops: 8 (m^2/2) 32 + 8 m^2 16 per matrix of trailing size m), 5 = DOS
- * (bytes from slk_sampler_hint: eigenvalues, k-weights, grid read and DOS
- * written; auxiliary amount: Gaussians of the full sum, nk nb n_E).
+ * of complex Hermitian batches (flops: 8 (m^2/2) 32 + 8 m^2 16 per matrix of
+ * trailing size m), 5 = DOS (bytes from slk_sampler_hint: eigenvalues,
+ * k-weights, grid read and DOS written; auxiliary amount: Gaussians of the
+ * full sum, nk nb n_E), 6 = the fused update of real-symmetric batches
+ * (SLK_EIGH_REAL; 2 (m^2/2) 32 + 2 m^2 16 flops per matrix), 7 = the chase of
+ * real-symmetric batches (a quarter of the complex step flops).
  * enable = 0 disables and frees. Thread-safe. */
 int slk_sampler_reset(int enable, int every, int capacity);
 /* Amount (and auxiliary amount) of the calling thread's next launch of a

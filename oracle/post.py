@@ -10,6 +10,7 @@
 * dos: calculate_dos (slakonet/main.py:709-797), Fermi-shifted, eV.
 * occupations / density: fermi (slakonet/slaterkoster.py:25-74) and the
   density of SimpleDftb.solve_kpoint (slakonet/main.py:157-161).
+* electronic_energy: _calculate_electronic_energy (slakonet/main.py:1430-1448).
 * properties: get_properties_dict(include_dos_data=True)
   (slakonet/main.py:1657-1819) plus the get_fermi block of
   compute_multi_element_properties (slakonet/optim.py:1011-1059).
@@ -198,6 +199,20 @@ def occupations(n_states, n_electrons):
     if n_electrons % 2 > 0 and n_pairs < n_states:
         occ[n_pairs] = 1.0
     return occ
+
+
+def electronic_energy(eig, k_weights, mu, kT_eV=0.025):
+    """_calculate_electronic_energy (slakonet/main.py:1430-1448): sum over k
+    and bands of w_k f(e_kb) e_kb with Fermi-Dirac occupations
+    f = 1/(1+exp((e-mu)/kT)) at kT = 0.025 eV (fermi_smearing,
+    slakonet/fermi.py:54-96, which zeroes f where e == 0); no spin factor.
+    eig [nk, nb] Hartree, mu Hartree -> Hartree."""
+    eig = np.asarray(eig, dtype=np.float64)
+    w = np.asarray(k_weights, dtype=np.float64).reshape(-1)
+    kT = kT_eV / H2E
+    f = 1.0 / (1.0 + np.exp((eig - mu) / kT))
+    f = np.where(eig == 0.0, 0.0, f)
+    return float((f * eig * w[:, None]).sum())
 
 
 def density(vecs, occ):

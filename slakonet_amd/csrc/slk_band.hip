@@ -1476,10 +1476,12 @@ static void band_impl(hipStream_t st, typename C2T<T>::t* A, int n, int ld, int 
       launch_panel_qr<T>(st, cn, batch, r0, mn);
       const int t = (mn + 63) / 64;
       // FP32 launches are sampled: 8 (m^2/2) 32 flops for the rank-2k on the
-      // lower triangle and 8 m^2 16 for A22 V, per matrix
+      // lower triangle and 8 m^2 16 for A22 V, per complex matrix; a real
+      // matrix takes 2 flops per multiply-add instead of 8
       SampleSlot fs;
       const bool fsam = sizeof(T) == 4 && sampler_enabled() &&
-                        sampler_take(SLK_SAMPLE_FUSED_AV, (double)batch * 256.0 * mn * mn, &fs);
+                        sampler_take(RE ? SLK_SAMPLE_FUSED_AV_REAL : SLK_SAMPLE_FUSED_AV,
+                                     (double)batch * (RE ? 64.0 : 256.0) * mn * mn, &fs);
       const int ntri = t * (t + 1) / 2;
       const int nwg = (ntri * batch + 7) / 8 * 8;
       if (fsam)
@@ -1499,7 +1501,8 @@ static void band_impl(hipStream_t st, typename C2T<T>::t* A, int n, int ld, int 
   SampleSlot sl;
   // algorithmic amount per workgroup (one matrix): each workgroup runs on its
   // own compute unit, so flops / window is the per-CU rate
-  const bool sampled = sampler_enabled() && sampler_take(SLK_SAMPLE_CHASE, chase_flops(n), &sl);
+  const bool sampled = sampler_enabled() && sampler_take(RE ? SLK_SAMPLE_CHASE_REAL : SLK_SAMPLE_CHASE,
+                                                          RE ? 0.25 * chase_flops(n) : chase_flops(n), &sl);
   unsigned long long* clk = sampled ? sl.clk : nullptr;
   if (lds <= CH_LDS_MAX && chase_lds_ok<T, RE>()) {
     if (sampled)

@@ -12,8 +12,11 @@
 
 // Kinds and their algorithmic amount: bytes for the memory-bound kinds (0, 1),
 // flops for the FP64 MFMA GEMM (2), the bulge chase (3) and the FP32 fused
-// stage-1 update (4); bytes for the DOS (5), whose auxiliary amount is the
-// number of Gaussians of the full sum (nk * nb * n_E per structure).
+// stage-1 update (4) of complex Hermitian batches; bytes for the DOS (5),
+// whose auxiliary amount is the number of Gaussians of the full sum
+// (nk * nb * n_E per structure); flops of the real-symmetric instantiations of
+// the fused update (6) and the chase (7), a quarter of the complex counts (one
+// real multiply-add in place of four).
 enum {
   SLK_SAMPLE_HETRD_STEP = 0,
   SLK_SAMPLE_BUILD = 1,
@@ -21,7 +24,9 @@ enum {
   SLK_SAMPLE_CHASE = 3,
   SLK_SAMPLE_FUSED_AV = 4,
   SLK_SAMPLE_DOS = 5,
-  SLK_SAMPLE_KINDS = 6
+  SLK_SAMPLE_FUSED_AV_REAL = 6,
+  SLK_SAMPLE_CHASE_REAL = 7,
+  SLK_SAMPLE_KINDS = 8
 };
 
 // Workgroups of a sampled launch spread their clock atomics over

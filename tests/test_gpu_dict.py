@@ -193,3 +193,26 @@ def test_density_and_occupations_match_oracle(name, vectors):
     err = np.abs(rho[0].transpose(2, 0, 1) - rho_ref).max()
     print(f"density {name}: max err {err:.3e}")
     assert err < 1e-9 * max(1.0, np.abs(rho_ref).max())
+
+
+@pytest.mark.parametrize("name", ["diamond", "random3"])
+def test_electronic_energy_matches_oracle(name):
+    """_calculate_electronic_energy (main.py:1430-1448) on a fresh calculator
+    (it runs the solve itself, main.py:1433-1435) against
+    oracle.post.electronic_energy on the oracle's build, LAPACK eigenvalues and
+    the bisection Fermi level."""
+    from slakonet_amd.calculator import SimpleDftb
+    from slakonet_amd.geometry import monkhorst_pack
+    z, pos, cell, species, mesh, seed = _case(name)
+    skfs = synth.make_skf_set(species, seed=seed)
+    k, w = monkhorst_pack(mesh)
+    tab = TableSet(skfs, synth.shell_dict_spd(species), device="cuda")
+    calc = SimpleDftb(Geometry(z, pos, cell=cell, units="angstrom"), shell_dict=synth.shell_dict_spd(species),
+                      tables=tab, kpoints_frac=k, k_weights=w)
+    e = float(calc._calculate_electronic_energy())
+    ref = obuild.build_hs(z, pos * AA_BOHR, cell * AA_BOHR, synth.shell_dict_spd(species), skfs, k)
+    wr, _ = osolve.solve_all(ref["H"], ref["S"])
+    mu = opost.fermi_bisect(wr, obuild.n_electrons(z, skfs), w)
+    want = opost.electronic_energy(wr, w, mu)
+    # eigenvalues agree to 1e-9 Ha; each occupation moves by at most f' * 1e-9 < 1e-9 / (4 kT)
+    assert abs(e - want) < 1e-7 * max(1.0, abs(want)), (e, want)

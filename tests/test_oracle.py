@@ -476,3 +476,27 @@ def test_row_restricted_build_equals_full_build():
     for kind in ("H", "S"):
         np.testing.assert_allclose(part[kind], full[kind][:, part["rows"], :], rtol=0,
                                    atol=1e-13 * np.abs(full[kind]).max())
+
+
+def test_electronic_energy_closed_forms():
+    """Two levels mu -/+ a per k-point: f(-a) + f(a) = 1 and
+    f(-a) - f(a) = tanh(a / 2kT), so the energy is sum_k w_k (mu - a_k tanh(a_k / 2kT)).
+    A flipped exponent sign gives mu + a tanh(...); dropped weights change the sum.
+    At kT -> 0 it is the weighted sum of the levels below mu."""
+    mu = -0.21
+    a = np.array([0.001, 0.003, 0.02])
+    w = np.array([0.5, 0.3, 0.2])
+    eig = np.stack([mu - a, mu + a], axis=1)
+    kT = 0.025 / post.H2E
+    want = float((w * (mu - a * np.tanh(a / (2 * kT)))).sum())
+    assert abs(post.electronic_energy(eig, w, mu) - want) < 1e-14
+    rng = np.random.default_rng(3)
+    eig = np.sort(rng.uniform(-1.0, 0.5, (4, 7)), axis=1)
+    w = rng.uniform(0.1, 1.0, 4)
+    mu = 0.5 * (eig[:, 3].max() + eig[:, 4].min()) if eig[:, 3].max() < eig[:, 4].min() else -0.3
+    cold = float((np.where(eig < mu, eig, 0.0) * w[:, None]).sum())
+    assert abs(post.electronic_energy(eig, w, mu, kT_eV=1e-6) - cold) < 1e-12
+    # a level at exactly zero carries no occupation (fermi.py:95); its energy term is zero either way
+    eig0 = np.array([[-0.5, 0.0, 0.4]])
+    assert post.electronic_energy(eig0, [1.0], 0.1) == pytest.approx(-0.5 * 1.0 / (1 + np.exp(-0.6 / kT))
+                                                                      + 0.4 / (1 + np.exp(0.3 / kT)), abs=1e-15)
