@@ -13,9 +13,9 @@ Differences, all deliberate:
   once, since geometries do not change during fitting;
 * gradients come from the hand-written adjoints (slakonet_amd.adjoint);
   the Fermi level carries no gradient, as in the reference;
-* datasets are given directly (structure + target DOS + target gap); the
-  reference's vasprun.xml loader (MultiVaspDataLoader, :1446-1567) is not
-  part of the hot path;
+* datasets are given directly (structure + target DOS + target gap);
+  slakonet_amd.vasp builds them from vasprun.xml files (MultiVaspDataLoader,
+  :1446-1567) for train_multi_vasp_skf_parameters;
 * loss "reference" is the loss the reference trains with (:1701-1726,
   |gap - target| per dataset, weighted by atom count); loss "dos_gap" is
   MSE(DOS) + 0.1 * (gap error)^2 per dataset, averaged.
