@@ -65,6 +65,18 @@ typedef struct SlkBuildArgs {
   void* H;
   void* S;
   int out_f32;              /* 0: complex128, 1: complex64 */
+  /* Whole-matrix form (n_mat > 0): the build writes every element of the
+   * batch's n_mat matrices of dimension ld -- a bandwidth-bound fill (zeros,
+   * the on-site diagonal, identity padding of rows/columns >= n_orb with
+   * H = pad_h on the padded diagonal and S = 1), then the blocks of the atom
+   * pairs that have a pair-image within the cutoff, one workgroup per row
+   * panel (the n_row_work items that follow the n_work column tiles in work:
+   * one per structure, atom i and k chunk). n_mat = 0: one workgroup per
+   * column tile writes the n_orb x n_orb part and the padding is left to
+   * slk_pad_launch. */
+  double pad_h;
+  int n_struct, n_mat, ld; /* batch shape: structs entries, matrices, padded dimension */
+  int n_row_work;
 } SlkBuildArgs;
 
 /* Host-side batch planner (no GPU needed). Replaces the per-structure image
@@ -94,6 +106,7 @@ typedef struct SlkPlanOut {
   long long bytes[SLK_PLAN_SECTIONS];  /* used bytes of each section */
   long long total;
   int n_work, n_trans, n_mat, ld;
+  int n_row_work;           /* row-panel items after the n_work column tiles in the work section */
 } SlkPlanOut;
 /* 0 on success, 1 bad input (empty structure, unknown species, singular
  * cell), 2 ld smaller than the largest structure. */
@@ -107,6 +120,13 @@ int slk_plan_fill(const SlkPlanIn* in, const SlkPlanOut* out, void* blob, int* n
  * (slakonet/skfeed.py:285-359) and PolyInterpU.__call__
  * (slakonet/interpolation.py:107-238). */
 int slk_build_hs_launch(const SlkBuildArgs* args, hipStream_t stream);
+/* Blocks of the whole-matrix form of slk_build_hs_launch: 0 (default) by
+ * row panels for ld >= 448 and by column tiles below, 1 by column tiles
+ * (every tile, zero blocks included; the fill then writes the padding
+ * only). Both give the same bits. Process-wide; set
+ * it before building, not while builds are in flight. Returns 1 for an
+ * unknown mode. No reference counterpart (a measurement and test switch). */
+int slk_set_build_mode(int mode);
 
 /* Identity-padding of rows/columns [n_orb, ld) of a structure's matrices
  * (S = 1, H = pad_h on the padded diagonal). No reference counterpart: used to

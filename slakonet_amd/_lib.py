@@ -37,7 +37,9 @@ class SlkBuildArgs(C.Structure):
     _fields_ = [("tab", SlkTableSet), ("structs", C.c_void_p), ("work", C.c_void_p), ("n_work", C.c_int),
                 ("atom_pos", C.c_void_p), ("atom_spec", C.c_void_p), ("atom_orb", C.c_void_p),
                 ("trans_cart", C.c_void_p), ("trans_int", C.c_void_p), ("kfrac", C.c_void_p),
-                ("cutoff", C.c_double), ("H", C.c_void_p), ("S", C.c_void_p), ("out_f32", C.c_int)]
+                ("cutoff", C.c_double), ("H", C.c_void_p), ("S", C.c_void_p), ("out_f32", C.c_int),
+                ("pad_h", C.c_double), ("n_struct", C.c_int), ("n_mat", C.c_int), ("ld", C.c_int),
+                ("n_row_work", C.c_int)]
 
 
 class SlkSpectrum(C.Structure):
@@ -62,7 +64,7 @@ class SlkPlanIn(C.Structure):
 class SlkPlanOut(C.Structure):
     _fields_ = [("off", C.c_longlong * PLAN_SECTIONS), ("bytes", C.c_longlong * PLAN_SECTIONS),
                 ("total", C.c_longlong), ("n_work", C.c_int), ("n_trans", C.c_int), ("n_mat", C.c_int),
-                ("ld", C.c_int)]
+                ("ld", C.c_int), ("n_row_work", C.c_int)]
 
 
 # name -> (restype, argtypes)
@@ -70,6 +72,7 @@ SIGNATURES = {
     "slk_plan_size": (C.c_int, [C.POINTER(SlkPlanIn), C.POINTER(SlkPlanOut)]),
     "slk_plan_fill": (C.c_int, [C.POINTER(SlkPlanIn), C.POINTER(SlkPlanOut), C.c_void_p, C.c_void_p, C.c_void_p]),
     "slk_build_hs_launch": (C.c_int, [C.POINTER(SlkBuildArgs), C.c_void_p]),
+    "slk_set_build_mode": (C.c_int, [C.c_int]),
     "slk_pad_launch": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p,
                                  C.c_double, C.c_int, C.c_void_p]),
     "slk_eigh_workspace_bytes": (C.c_size_t, [C.c_int, C.c_int, C.c_int, C.c_int]),

@@ -27,6 +27,9 @@
 #define MAXPI 24        // pair-images per LDS batch
 #define SEG 2048        // candidates screened per segment
 #define MAXCOL 64
+#define ROWS_MAXSPEC 128     // row panels: species of the table set (slot map in LDS)
+#define ROWS_SEG 1024        // row panels: candidates screened per segment (<= 32 per thread)
+#define ROWS_MIN_LD 448      // row panels for padded dimensions from here up
 
 __constant__ int c_slot_l[9] = {0, 1, 1, 1, 2, 2, 2, 2, 2};
 __constant__ int c_slot_m[9] = {0, 0, 1, 2, 0, 1, 2, 3, 4};
@@ -254,9 +257,9 @@ __device__ __forceinline__ void store_c<double2>(double2* p, double2 v) { *p = v
 template <>
 __device__ __forceinline__ void store_c<float2>(float2* p, double2 v) { *p = make_float2((float)v.x, (float)v.y); }
 
+// One tile (work item wk) of the build, executed by the whole workgroup.
 template <typename OUT>
-__global__ __launch_bounds__(BUILD_THREADS) void slk_build_kernel(SlkBuildArgs A, unsigned long long* clk) {
-  sample_start(clk);
+__device__ __forceinline__ void build_tile(const SlkBuildArgs& A, const SlkBuildWork wk) {
   __shared__ int s_list[SEG];            // compacted candidates (packed jrel<<20 | t)
   __shared__ int s_total;
   __shared__ int pi_j[MAXPI];
@@ -275,7 +278,6 @@ __global__ __launch_bounds__(BUILD_THREADS) void slk_build_kernel(SlkBuildArgs A
   __shared__ int s_wsum[BUILD_THREADS / 64];
   __shared__ int s_rng[MAXCOL][2];       // per column atom: its pair-images in the batch
 
-  const SlkBuildWork wk = A.work[blockIdx.x];
   const SlkStructDesc sd = A.structs[wk.s];
   const int tid = threadIdx.x;
   const int ai = sd.atom0 + wk.i;
@@ -521,6 +523,328 @@ __global__ __launch_bounds__(BUILD_THREADS) void slk_build_kernel(SlkBuildArgs A
       }
     }
   }
+}
+
+// Column tiles: one workgroup per work item.
+template <typename OUT>
+__global__ __launch_bounds__(BUILD_THREADS) void slk_build_kernel(SlkBuildArgs A, unsigned long long* clk) {
+  sample_start(clk);
+  build_tile<OUT>(A, A.work[blockIdx.x]);
+  if (clk != nullptr) {
+    __syncthreads();
+    sample_end(clk);
+  }
+}
+
+// Whole-matrix fill of the batch: rows [FILL_ROWS * blockIdx.x, +FILL_ROWS)
+// of matrix blockIdx.y. Zeros off the diagonal, on-site energies (S = 1) on
+// the diagonal of rows < n_orb, and identity padding (H = pad_h, S = 1) on
+// the diagonal of rows >= n_orb. pad_only: only the padded rows and columns.
+#define FILL_ROWS 8
+template <typename OUT>
+__global__ __launch_bounds__(256) void slk_fill_kernel(SlkBuildArgs A, int pad_only, unsigned long long* clk) {
+  sample_start(clk);
+  const int m = blockIdx.y, ld = A.ld;
+  // structure of matrix m: the last s with mat0 <= m
+  int lo = 0, hi = A.n_struct - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (A.structs[mid].mat0 <= m) lo = mid; else hi = mid - 1;
+  }
+  const SlkStructDesc sd = A.structs[lo];
+  const int n = sd.n_orb;
+  OUT* H = reinterpret_cast<OUT*>(A.H) + (size_t)m * ld * ld;
+  OUT* S = reinterpret_cast<OUT*>(A.S) + (size_t)m * ld * ld;
+  const int r1 = min(ld, (int)(blockIdx.x + 1) * FILL_ROWS);
+  for (int row = blockIdx.x * FILL_ROWS; row < r1; ++row) {
+    const bool prow = row >= n;
+    const int c0 = (pad_only && !prow) ? n : 0;
+    for (int col = c0 + threadIdx.x; col < ld; col += 256) {
+      double h = 0.0, sv = 0.0;
+      if (col == row) {
+        sv = 1.0;
+        if (prow) {
+          h = A.pad_h;
+        } else {
+          // atom of orbital row: the last atom whose first orbital is <= row
+          int a0 = 0, a1 = sd.n_atoms - 1;
+          while (a0 < a1) {
+            const int mid = (a0 + a1 + 1) >> 1;
+            if (A.atom_orb[sd.atom0 + mid] <= row) a0 = mid; else a1 = mid - 1;
+          }
+          const int sp = A.atom_spec[sd.atom0 + a0];
+          const int ml = row - A.atom_orb[sd.atom0 + a0];
+          h = A.tab.spec_onsite[sp * 3 + c_slot_l[A.tab.spec_slot[sp * 9 + ml]]];
+        }
+      }
+      store_c<OUT>(H + (size_t)row * ld + col, make_double2(h, 0.0));
+      store_c<OUT>(S + (size_t)row * ld + col, make_double2(sv, 0.0));
+    }
+  }
+  if (clk != nullptr) {
+    __syncthreads();
+    sample_end(clk);
+  }
+}
+
+// Row panels: one workgroup per (structure, atom i, chunk of <= KC k-points)
+// writes the blocks (i, j) of every atom j that has a pair-image within the
+// cutoff, over the whole-matrix fill (zeros and the on-site diagonal) of the
+// same launch sequence. In a structure whose atoms are not in spatial order,
+// almost every 64-column tile holds some neighbour of atom i, so the tile
+// form runs one short, latency-bound workgroup per tile; here one workgroup
+// screens all (j, T) candidates of atom i in the tile form's order (j-major,
+// T-minor), computes weights, integrals, phases and SK blocks per batch of
+// pair-images exactly as the tile form does, and walks the pair-images in
+// order: each thread keeps the (slot pair, k) entries q = tid, tid + 256 of
+// the current atom j's block in registers and writes them when j changes
+// (on-site added when j == i). Every element gets the same terms in the same
+// order as in the tile form, so the two agree bit for bit.
+template <typename OUT>
+__global__ __launch_bounds__(BUILD_THREADS, 3) void slk_build_rows_kernel(SlkBuildArgs A, unsigned long long* clk) {
+  sample_start(clk);
+  constexpr int NQ = (81 * KC + BUILD_THREADS - 1) / BUILD_THREADS;
+  __shared__ int s_list[ROWS_SEG + MAXPI];   // pending pair-images (candidate index), in order
+  __shared__ int pi_j[MAXPI];
+  __shared__ int pi_t[MAXPI];
+  __shared__ double pi_u[MAXPI][3];
+  __shared__ int pi_base[MAXPI][2];
+  __shared__ double pi_w[MAXPI][2][8];
+  __shared__ double pi_int[MAXPI][2][20];
+  __shared__ double E[MAXPI][2][81];
+  __shared__ double2 ph[MAXPI][KC];
+  __shared__ int pi_mask[MAXPI];
+  __shared__ int pi_sj[MAXPI], pi_oj[MAXPI];        // species and first orbital of the pair-image's atom j
+  __shared__ int s_wsum[BUILD_THREADS / 64];
+  __shared__ signed char s_inv[ROWS_MAXSPEC * 9];   // species, slot -> local orbital (-1: absent)
+  __shared__ int s_smask[ROWS_MAXSPEC];              // slots present per species
+
+  const SlkBuildWork wk = A.work[A.n_work + blockIdx.x];
+  const SlkStructDesc sd = A.structs[wk.s];
+  const int tid = threadIdx.x;
+  const int nt = sd.nt;
+  const int ai = sd.atom0 + wk.i;
+  const int si = A.atom_spec[ai];
+  const int norb_i = A.tab.spec_norb[si];
+  const int oi = A.atom_orb[ai];
+  const double rix = A.atom_pos[3 * ai], riy = A.atom_pos[3 * ai + 1], riz = A.atom_pos[3 * ai + 2];
+  int mask_i = 0;
+  for (int o = 0; o < norb_i; ++o) mask_i |= 1 << A.tab.spec_slot[si * 9 + o];
+  for (int e = tid; e < A.tab.nspec * 9; e += BUILD_THREADS) s_inv[e] = -1;
+  __syncthreads();
+  for (int e = tid; e < A.tab.nspec * 9; e += BUILD_THREADS) {
+    const int sp = e / 9, o = e % 9;
+    if (o < A.tab.spec_norb[sp]) s_inv[sp * 9 + A.tab.spec_slot[sp * 9 + o]] = (signed char)o;
+  }
+  for (int sp = tid; sp < A.tab.nspec; sp += BUILD_THREADS) {
+    int m = 0;
+    for (int o = 0; o < A.tab.spec_norb[sp]; ++o) m |= 1 << A.tab.spec_slot[sp * 9 + o];
+    s_smask[sp] = m;
+  }
+  OUT* H = reinterpret_cast<OUT*>(A.H);
+  OUT* S = reinterpret_cast<OUT*>(A.S);
+  const size_t mat_el = (size_t)sd.ld * sd.ld;
+  const size_t kbase = (size_t)(sd.mat0 + wk.kc0) * mat_el;
+
+  double2 accH[NQ], accS[NQ];
+#pragma unroll
+  for (int u = 0; u < NQ; ++u) { accH[u] = make_double2(0.0, 0.0); accS[u] = make_double2(0.0, 0.0); }
+  int cur_j = -1;
+  // writes the block of the current atom j (species sj, first orbital oj;
+  // rows of atom i) and clears the accumulators
+  auto flush = [&](int sj, int oj) {
+#pragma unroll
+    for (int u = 0; u < NQ; ++u) {
+      const int q = tid + u * BUILD_THREADS;
+      if (q < 81 * KC) {
+        const int kk = q / 81, ab = q % 81, a = ab / 9, b = ab % 9;
+        const int ml = s_inv[si * 9 + a], mc = s_inv[sj * 9 + b];
+        if (kk < wk.nkc && ml >= 0 && mc >= 0) {
+          const int row = oi + ml, col = oj + mc;
+          double onh = 0.0, ons = 0.0;
+          if (row == col) {
+            onh = A.tab.spec_onsite[si * 3 + c_slot_l[a]];
+            ons = 1.0;
+          }
+          const size_t off = kbase + (size_t)kk * mat_el + (size_t)row * sd.ld + col;
+          store_c<OUT>(H + off, make_double2(accH[u].x + onh, accH[u].y));
+          store_c<OUT>(S + off, make_double2(accS[u].x + ons, accS[u].y));
+        }
+      }
+      accH[u] = make_double2(0.0, 0.0);
+      accS[u] = make_double2(0.0, 0.0);
+    }
+  };
+
+  const int ncand = sd.n_atoms * nt;   // < 2^30 (the planner makes no row panels otherwise)
+  int cur_sj = 0, cur_oj = 0;
+  // pair-images found by the screening wait in s_list and are processed in
+  // full batches of MAXPI (the last batch may be short); pend entries carry
+  // over from one screening segment to the next
+  int pend = 0;
+  for (int seg0 = 0; seg0 < ncand; seg0 += ROWS_SEG) {
+    const int segn = (ncand - seg0) < ROWS_SEG ? (ncand - seg0) : ROWS_SEG;
+    // each thread screens a contiguous run of <= ROWS_SEG / 256 candidates
+    // (deterministic order) and keeps its in-range flags as bits
+    const int per = (segn + BUILD_THREADS - 1) / BUILD_THREADS;
+    const int beg = tid * per;
+    const int end = min(beg + per, segn);
+    unsigned bits = 0u;
+    int mine = 0;
+    for (int c = beg; c < end; ++c) {
+      const int g = seg0 + c;
+      const int jr = g / nt, t = g % nt;
+      const int aj = sd.atom0 + jr;
+      const int tt = sd.t0 + t;
+      const double vx = (A.trans_cart[3 * tt] + A.atom_pos[3 * aj]) - rix;
+      const double vy = (A.trans_cart[3 * tt + 1] + A.atom_pos[3 * aj + 1]) - riy;
+      const double vz = (A.trans_cart[3 * tt + 2] + A.atom_pos[3 * aj + 2]) - riz;
+      const double r = sqrt(vx * vx + vy * vy + vz * vz);
+      if (r < A.cutoff && r > 0.0) {
+        bits |= 1u << (c - beg);
+        ++mine;
+      }
+    }
+    const int lane = tid & 63, wv = tid >> 6;
+    int inc = mine;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      const int v = __shfl_up(inc, d, 64);
+      if (lane >= d) inc += v;
+    }
+    if (lane == 63) s_wsum[wv] = inc;
+    __syncthreads();
+    int woff = 0, total_all = 0;
+#pragma unroll
+    for (int q = 0; q < BUILD_THREADS / 64; ++q) {
+      if (q < wv) woff += s_wsum[q];
+      total_all += s_wsum[q];
+    }
+    int pos = pend + woff + inc - mine;
+    for (int c = beg; c < end; ++c)
+      if ((bits >> (c - beg)) & 1u) s_list[pos++] = seg0 + c;
+    __syncthreads();
+    const int total = pend + total_all;
+    const bool last = seg0 + ROWS_SEG >= ncand;
+    int b0 = 0;
+    while (total - b0 >= MAXPI || (last && b0 < total)) {
+      const int np = min(MAXPI, total - b0);
+      // (a) geometry + interpolation weights per pair-image and table direction
+      if (tid < 2 * np) {
+        const int p = tid >> 1, dir = tid & 1;
+        const int g = s_list[b0 + p];
+        const int jr = g / nt, t = g % nt;
+        const int aj = sd.atom0 + jr;
+        const int tt = sd.t0 + t;
+        const double vx = (A.trans_cart[3 * tt] + A.atom_pos[3 * aj]) - rix;
+        const double vy = (A.trans_cart[3 * tt + 1] + A.atom_pos[3 * aj + 1]) - riy;
+        const double vz = (A.trans_cart[3 * tt + 2] + A.atom_pos[3 * aj + 2]) - riz;
+        const double r = sqrt(vx * vx + vy * vy + vz * vz);
+        const int sj = A.atom_spec[aj];
+        const int pid = dir == 0 ? A.tab.pair_index[si * A.tab.nspec + sj]
+                                 : A.tab.pair_index[sj * A.tab.nspec + si];
+        if (dir == 0) {
+          pi_j[p] = jr;
+          pi_t[p] = t;
+          pi_sj[p] = sj;
+          pi_oj[p] = A.atom_orb[aj];
+          pi_u[p][0] = vx / r; pi_u[p][1] = vy / r; pi_u[p][2] = vz / r;
+          pi_mask[p] = s_smask[sj];
+        }
+        int base = -1;
+        double w[8];
+        if (pid >= 0) {
+          interp_weights(r, A.tab.pair_ngrid[pid], A.tab.pair_step[pid], A.tab.tail, &base, w);
+          if (base >= 0) base += A.tab.pair_row0[pid];
+        }
+        pi_base[p][dir] = base;
+        for (int m = 0; m < 8; ++m) pi_w[p][dir][m] = (base >= 0) ? w[m] : 0.0;
+      }
+      __syncthreads();
+      // (b) integrals
+      for (int task = tid; task < np * 40; task += BUILD_THREADS) {
+        const int p = task / 40, rem = task % 40, dir = rem / 20, ig = rem % 20;
+        const int base = pi_base[p][dir];
+        double v = 0.0;
+        if (base >= 0) {
+          const double* tb = A.tab.tables + (size_t)base * 20 + ig;
+#pragma unroll
+          for (int m = 0; m < 8; ++m) v += pi_w[p][dir][m] * tb[(size_t)m * 20];
+        }
+        pi_int[p][dir][ig] = v;
+      }
+      // (c) phases exp(2 pi i k.T)
+      for (int task = tid; task < np * KC; task += BUILD_THREADS) {
+        const int p = task / KC, kk = task % KC;
+        double2 v = make_double2(0.0, 0.0);
+        if (kk < wk.nkc) {
+          const int kg = sd.k0 + wk.kc0 + kk;
+          const int tt = sd.t0 + pi_t[p];
+          const double arg = 2.0 * (A.kfrac[3 * kg] * A.trans_int[3 * tt] +
+                                    A.kfrac[3 * kg + 1] * A.trans_int[3 * tt + 1] +
+                                    A.kfrac[3 * kg + 2] * A.trans_int[3 * tt + 2]);
+          double sn, cs;
+          sincospi(arg, &sn, &cs);
+          v = make_double2(cs, sn);
+        }
+        ph[p][kk] = v;
+      }
+      __syncthreads();
+      // (d) effective SK blocks (canonical slots) per pair-image
+      for (int task = tid; task < np * 81; task += BUILD_THREADS) {
+        const int p = task / 81, ab = task % 81, a = ab / 9, b = ab % 9;
+        if (!((mask_i >> a) & (pi_mask[p] >> b) & 1)) continue;
+        const int la = c_slot_l[a], lb = c_slot_l[b];
+        const int j = pi_j[p];
+        const bool direct = (la < lb) || (la == lb && wk.i <= j);
+        double eh, es;
+        const double ux = pi_u[p][0], uy = pi_u[p][1], uz = pi_u[p][2];
+        if (direct) {
+          const int ib = lpair_base(la, lb);
+          eh = sk_elem(la, lb, c_slot_m[a], c_slot_m[b], ux, uy, uz, &pi_int[p][0][ib]);
+          es = sk_elem(la, lb, c_slot_m[a], c_slot_m[b], ux, uy, uz, &pi_int[p][0][10 + ib]);
+        } else {
+          const int ib = lpair_base(lb, la);
+          eh = sk_elem(lb, la, c_slot_m[b], c_slot_m[a], -ux, -uy, -uz, &pi_int[p][1][ib]);
+          es = sk_elem(lb, la, c_slot_m[b], c_slot_m[a], -ux, -uy, -uz, &pi_int[p][1][10 + ib]);
+        }
+        E[p][0][ab] = eh;
+        E[p][1][ab] = es;
+      }
+      __syncthreads();
+      // (e) accumulate; a new atom j writes the previous one's block first
+      for (int p = 0; p < np; ++p) {
+        const int j = pi_j[p];
+        if (j != cur_j) {
+          if (cur_j >= 0) flush(cur_sj, cur_oj);
+          cur_j = j;
+          cur_sj = pi_sj[p];
+          cur_oj = pi_oj[p];
+        }
+#pragma unroll
+        for (int u = 0; u < NQ; ++u) {
+          const int q = tid + u * BUILD_THREADS;
+          if (q < 81 * KC) {
+            const int kk = q / 81, ab = q % 81;
+            const double eh = E[p][0][ab], es = E[p][1][ab];
+            const double2 f = ph[p][kk];
+            accH[u].x += eh * f.x; accH[u].y += eh * f.y;
+            accS[u].x += es * f.x; accS[u].y += es * f.y;
+          }
+        }
+      }
+      __syncthreads();
+      b0 += np;
+    }
+    // carry the unprocessed tail (< MAXPI entries) to the front
+    const int rem = total - b0;
+    const int keep = tid < rem ? s_list[b0 + tid] : 0;
+    __syncthreads();
+    if (tid < rem) s_list[tid] = keep;
+    __syncthreads();
+    pend = rem;
+  }
+  if (cur_j >= 0) flush(cur_sj, cur_oj);
   if (clk != nullptr) {
     __syncthreads();
     sample_end(clk);
@@ -554,24 +878,61 @@ __global__ void slk_pad_kernel(const SlkStructDesc* structs, int s, void* Hv, vo
 
 extern "C" {
 
-int slk_build_hs_launch(const SlkBuildArgs* args, hipStream_t stream) {
-  if (args->n_work <= 0) return 0;
-  SampleSlot sl;
-  unsigned long long* none = nullptr;
-  const bool sampled = sampler_take(SLK_SAMPLE_BUILD, -1.0, &sl);
-  if (args->out_f32) {
+static int g_build_mode = 0;   // whole-matrix form: 0 row panels, 1 every column tile
+
+int slk_set_build_mode(int mode) {
+  if (mode < 0 || mode > 1) return 1;
+  g_build_mode = mode;
+  return 0;
+}
+
+extern "C++" {
+template <typename OUT>
+static void build_launch(const SlkBuildArgs& a, hipStream_t stream, bool sampled, const SampleSlot& sl) {
+  unsigned long long* clk = sampled ? sl.clk : nullptr;
+  if (a.n_mat <= 0) {
     if (sampled)
-      hipExtLaunchKernelGGL(slk_build_kernel<float2>, dim3(args->n_work), dim3(BUILD_THREADS), 0, stream, sl.e0,
-                            sl.e1, 0, *args, sl.clk);
+      hipExtLaunchKernelGGL(slk_build_kernel<OUT>, dim3(a.n_work), dim3(BUILD_THREADS), 0, stream, sl.e0, sl.e1, 0,
+                            a, clk);
     else
-      hipLaunchKernelGGL(slk_build_kernel<float2>, dim3(args->n_work), dim3(BUILD_THREADS), 0, stream, *args, none);
+      hipLaunchKernelGGL(slk_build_kernel<OUT>, dim3(a.n_work), dim3(BUILD_THREADS), 0, stream, a, clk);
+    return;
+  }
+  // whole-matrix form: the fill, then the blocks by row panels (or by every
+  // column tile); a sampled launch's window and events span both
+  // row panels from ROWS_MIN_LD up; below it a structure has few atoms, so
+  // few row panels, and the tiles measured faster
+  const bool rows = g_build_mode == 0 && a.n_row_work > 0 && a.tab.nspec <= ROWS_MAXSPEC && a.ld >= ROWS_MIN_LD;
+  const dim3 fgrid((a.ld + FILL_ROWS - 1) / FILL_ROWS, a.n_mat);
+  if (sampled)
+    hipExtLaunchKernelGGL(slk_fill_kernel<OUT>, fgrid, dim3(256), 0, stream, sl.e0, nullptr, 0, a, rows ? 0 : 1, clk);
+  else
+    hipLaunchKernelGGL(slk_fill_kernel<OUT>, fgrid, dim3(256), 0, stream, a, rows ? 0 : 1, clk);
+  if (rows) {
+    if (sampled)
+      hipExtLaunchKernelGGL(slk_build_rows_kernel<OUT>, dim3(a.n_row_work), dim3(BUILD_THREADS), 0, stream, nullptr,
+                            sl.e1, 0, a, clk);
+    else
+      hipLaunchKernelGGL(slk_build_rows_kernel<OUT>, dim3(a.n_row_work), dim3(BUILD_THREADS), 0, stream, a, clk);
   } else {
     if (sampled)
-      hipExtLaunchKernelGGL(slk_build_kernel<double2>, dim3(args->n_work), dim3(BUILD_THREADS), 0, stream, sl.e0,
-                            sl.e1, 0, *args, sl.clk);
+      hipExtLaunchKernelGGL(slk_build_kernel<OUT>, dim3(a.n_work), dim3(BUILD_THREADS), 0, stream, nullptr, sl.e1, 0,
+                            a, clk);
     else
-      hipLaunchKernelGGL(slk_build_kernel<double2>, dim3(args->n_work), dim3(BUILD_THREADS), 0, stream, *args, none);
+      hipLaunchKernelGGL(slk_build_kernel<OUT>, dim3(a.n_work), dim3(BUILD_THREADS), 0, stream, a, clk);
   }
+}
+}  // extern "C++"
+
+int slk_build_hs_launch(const SlkBuildArgs* args, hipStream_t stream) {
+  if (args->n_work <= 0) return 0;
+  if (args->n_mat > 0 && (args->n_struct <= 0 || args->ld <= 0)) return (int)hipErrorInvalidValue;
+  SampleSlot sl;
+  const bool sampled = sampler_take(SLK_SAMPLE_BUILD, -1.0, &sl);
+  if (args->out_f32)
+    build_launch<float2>(*args, stream, sampled, sl);
+  else
+    build_launch<double2>(*args, stream, sampled, sl);
   return (int)hipGetLastError();
 }
 

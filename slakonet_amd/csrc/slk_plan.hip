@@ -85,10 +85,11 @@ extern "C" {
 
 int slk_plan_size(const SlkPlanIn* in, SlkPlanOut* out) {
   std::memset(out, 0, sizeof(*out));
-  long long atoms = 0, kpts = 0, trans = 0, work = 0;
+  long long atoms = 0, kpts = 0, trans = 0, work = 0, rows = 0;
   int maxn = 0;
   std::vector<int> norb;
   std::vector<Tile> tiles;
+  bool rows_ok = true;
   for (int s = 0; s < in->n_struct; ++s) {
     const int na = in->n_atoms[s];
     const int nk = in->nk[s];
@@ -106,15 +107,19 @@ int slk_plan_size(const SlkPlanIn* in, SlkPlanOut* out) {
     trans += (long long)(2 * rng[0] + 1) * (2 * rng[1] + 1) * (2 * rng[2] + 1);
     column_tiles(norb.data(), na, &tiles);
     work += (long long)na * (long long)tiles.size() * ((nk + KC - 1) / KC);
+    rows += (long long)na * ((nk + KC - 1) / KC);
+    // a row panel screens na * nt candidates with 32-bit indices
+    if ((long long)na * (2 * rng[0] + 1) * (2 * rng[1] + 1) * (2 * rng[2] + 1) >= (1LL << 30)) rows_ok = false;
     maxn = n > maxn ? n : maxn;
     atoms += na;
     kpts += nk;
   }
   const int ld = in->ld > 0 ? in->ld : maxn;
   if (ld < maxn) return 2;
-  if (trans >= (1LL << 31) || work >= (1LL << 31)) return 1;
+  if (!rows_ok) rows = 0;
+  if (trans >= (1LL << 31) || work + rows >= (1LL << 31)) return 1;
   const size_t sizes[SLK_PLAN_SECTIONS] = {
-      (size_t)in->n_struct * sizeof(SlkStructDesc), (size_t)work * sizeof(SlkBuildWork),
+      (size_t)in->n_struct * sizeof(SlkStructDesc), (size_t)(work + rows) * sizeof(SlkBuildWork),
       (size_t)atoms * 3 * sizeof(double),           (size_t)atoms * sizeof(int),
       (size_t)atoms * sizeof(int),                  (size_t)trans * 3 * sizeof(double),
       (size_t)trans * 3 * sizeof(int),              (size_t)kpts * 3 * sizeof(double),
@@ -127,6 +132,7 @@ int slk_plan_size(const SlkPlanIn* in, SlkPlanOut* out) {
   }
   out->total = (long long)off;
   out->n_work = (int)work;
+  out->n_row_work = (int)rows;
   out->n_trans = (int)trans;
   out->n_mat = (int)kpts;
   out->ld = ld;
@@ -145,7 +151,7 @@ int slk_plan_fill(const SlkPlanIn* in, const SlkPlanOut* out, void* blob, int* n
   int* tint = reinterpret_cast<int*>(b + out->off[6]);
   double* kf = reinterpret_cast<double*>(b + out->off[7]);
   double* kw = reinterpret_cast<double*>(b + out->off[8]);
-  long long atom0 = 0, t0 = 0, k0 = 0, w0 = 0;
+  long long atom0 = 0, t0 = 0, k0 = 0, w0 = 0, r0 = out->n_work;
   std::vector<int> norb;
   std::vector<Tile> tiles;
   for (int s = 0; s < in->n_struct; ++s) {
@@ -194,6 +200,19 @@ int slk_plan_fill(const SlkPlanIn* in, const SlkPlanOut* out, void* blob, int* n
           w.kc0 = kc * KC;
           w.nkc = nk - kc * KC < KC ? nk - kc * KC : KC;
         }
+    if (out->n_row_work > 0)
+      for (int i = 0; i < na; ++i)
+        for (int kc = 0; kc < nkc; ++kc) {
+          SlkBuildWork& w = wk[r0++];
+          w.s = s;
+          w.i = i;
+          w.j0 = 0;
+          w.j1 = na;
+          w.c0 = 0;
+          w.ncol = n;
+          w.kc0 = kc * KC;
+          w.nkc = nk - kc * KC < KC ? nk - kc * KC : KC;
+        }
     SlkStructDesc& d = sd[s];
     d.atom0 = (int)atom0;
     d.n_atoms = na;
@@ -210,7 +229,7 @@ int slk_plan_fill(const SlkPlanIn* in, const SlkPlanOut* out, void* blob, int* n
     t0 += nt;
     k0 += nk;
   }
-  return (w0 == out->n_work && t0 == out->n_trans) ? 0 : 1;
+  return (w0 == out->n_work && t0 == out->n_trans && r0 == (long long)out->n_work + out->n_row_work) ? 0 : 1;
 }
 
 }  // extern "C"

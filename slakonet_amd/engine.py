@@ -59,6 +59,7 @@ class BuildPlan:
     dev: dict = field(default_factory=dict)
     n_work: int = 0
     real: bool = False               # every matrix real symmetric (set by callers that solve it so)
+    n_row_work: int = 0              # row-panel build items after the n_work column tiles
 
 
 def _upload(arr: np.ndarray, device) -> torch.Tensor:
@@ -136,7 +137,7 @@ def plan_build(tables: TableSet, inputs: Sequence[StructureInput], ld: Optional[
     kwd = views["kw"].view(torch.float64)
     nelec = [inp.nelec if inp.nelec is not None else tables.n_electrons(z) for inp, z in zip(inputs, zs)]
     return BuildPlan(len(inputs), int(out.n_mat), int(out.ld), n_orb.tolist(), nk.tolist(), mat0.tolist(), kwd,
-                     nelec, dev, int(out.n_work))
+                     nelec, dev, int(out.n_work), n_row_work=int(out.n_row_work))
 
 
 def _pinned_bytes(n: int) -> torch.Tensor:
@@ -156,7 +157,8 @@ def build_args(tables: TableSet, plan: BuildPlan, H_ptr: int, S_ptr: int, out_f3
     return _lib.SlkBuildArgs(tables.c_struct(table_tensor), d["structs"].data_ptr(), d["work"].data_ptr(),
                              plan.n_work, d["atom_pos"].data_ptr(), d["atom_spec"].data_ptr(),
                              d["atom_orb"].data_ptr(), d["trans_cart"].data_ptr(), d["trans_int"].data_ptr(),
-                             d["kfrac"].data_ptr(), float(cutoff), H_ptr, S_ptr, int(out_f32))
+                             d["kfrac"].data_ptr(), float(cutoff), H_ptr, S_ptr, int(out_f32), PAD_H,
+                             plan.n_struct, plan.n_mat, plan.ld, plan.n_row_work)
 
 
 def run_build(tables: TableSet, plan: BuildPlan, out_f32: bool = False, cutoff: float = HS_CUTOFF_BOHR,
@@ -185,12 +187,8 @@ def run_build(tables: TableSet, plan: BuildPlan, out_f32: bool = False, cutoff: 
         # elements the build writes (padding excluded)
         lib.slk_sampler_hint(1, float(sum(nk * 2 * n * n for nk, n in zip(plan.nk, plan.n_orb)) * H.element_size()),
                              0.0)
+    # the whole-matrix form writes every element, identity padding included
     _lib.check(lib.slk_build_hs_launch(C.byref(args), sh), "slk_build_hs_launch")
-    for s in range(plan.n_struct):
-        if plan.n_orb[s] < plan.ld:
-            _lib.check(lib.slk_pad_launch(C.c_void_p(d["structs"].data_ptr()), s, plan.nk[s], plan.n_orb[s],
-                                          plan.ld, C.c_void_p(H.data_ptr()), C.c_void_p(S.data_ptr()), PAD_H,
-                                          int(out_f32), sh), "slk_pad_launch")
     return H, S
 
 

@@ -204,7 +204,20 @@ def test_native_planner_matches_python_bookkeeping():
         assert np.array_equal(ws["i"], ii.ravel()) and np.array_equal(ws["kc0"], kk.ravel() * engine.KC)
         assert np.array_equal(ws["j0"], tiles[tt.ravel(), 0]) and np.array_equal(ws["ncol"], tiles[tt.ravel(), 3])
         w0 += ii.size
-    assert w0 == plan.n_work == len(work)
+    assert w0 == plan.n_work
+    # row-panel items follow the tiles: one per (structure, atom, k chunk), all columns
+    rows = work[plan.n_work:]
+    assert len(rows) == plan.n_row_work == sum(len(i.z) * ((len(i.kpoints) + engine.KC - 1) // engine.KC)
+                                               for i in inputs)
+    r0 = 0
+    for s, inp in enumerate(inputs):
+        nkc = (len(inp.kpoints) + engine.KC - 1) // engine.KC
+        rs = rows[r0:r0 + len(inp.z) * nkc]
+        assert np.all(rs["s"] == s) and np.all(rs["j0"] == 0) and np.all(rs["j1"] == len(inp.z))
+        assert np.all(rs["ncol"] == plan.n_orb[s])
+        assert np.array_equal(rs["i"], np.repeat(np.arange(len(inp.z)), nkc))
+        assert np.array_equal(rs["kc0"], np.tile(np.arange(nkc) * engine.KC, len(inp.z)))
+        r0 += len(rs)
     with pytest.raises(ValueError):
         engine.plan_build(tab, inputs, ld=8)
 
