@@ -264,21 +264,16 @@ __device__ __forceinline__ House<T> house(typename C2T<T>::t alpha, T xn2) {
 // barrier, waves j > k apply it to their columns (inner product and update
 // within the wave), wave k+1 also forms the next norm, and waves l < k form
 // g_l = v_l^H v_k for the block reflector factor T.
-template <typename C2>
-__device__ __forceinline__ C2 wave_sum_c(C2 v) {
-  v.x = sum16(v.x);
-  v.y = sum16(v.y);
-  v.x += __shfl_xor(v.x, 16, 64);
-  v.y += __shfl_xor(v.y, 16, 64);
-  v.x += __shfl_xor(v.x, 32, 64);
-  v.y += __shfl_xor(v.y, 32, 64);
-  return v;
-}
+// Wave sums: within rows of 16 lanes, then across rows and halves with the
+// permlane swaps (VALU, no LDS round trip).
 template <typename T>
 __device__ __forceinline__ T wave_sum_r(T v) {
-  v = sum16(v);
-  v += __shfl_xor(v, 16, 64);
-  v += __shfl_xor(v, 32, 64);
+  return add_partner<32>(add_partner<16>(sum16(v)));
+}
+template <typename C2>
+__device__ __forceinline__ C2 wave_sum_c(C2 v) {
+  v.x = wave_sum_r(v.x);
+  v.y = wave_sum_r(v.y);
   return v;
 }
 
@@ -300,7 +295,7 @@ __device__ __forceinline__ void for_rows(int nq, F f) {
   }
 }
 
-template <typename T, int RPT, int QT = 0>
+template <typename T, int RPT, int QT = 0, bool RE = false>
 __global__ __launch_bounds__(1024) void panel_qr_kernel(BandCtx<T> c, int j0) {
   using C2 = typename C2T<T>::t;
   static_assert(QT == 0 || RPT > 0, "LDS rows extend register rows");
@@ -369,7 +364,7 @@ __global__ __launch_bounds__(1024) void panel_qr_kernel(BandCtx<T> c, int j0) {
     for_rows<FR>(nq, [&](int q) {
         const int r = lane + 64 * q;
         const C2 e = get(q);
-        if (r > 0 && r < m) nrm += e.x * e.x + e.y * e.y;
+        if (r > 0 && r < m) nrm += RE ? e.x * e.x : e.x * e.x + e.y * e.y;
       });
   for (int k = 0; k < kb; ++k) {
     const C2* vk = vcb + (size_t)k * ld + r0;   // global copy of v_k (RPT = 0)
@@ -377,7 +372,7 @@ __global__ __launch_bounds__(1024) void panel_qr_kernel(BandCtx<T> c, int j0) {
     if (w == k) {
       const T xn2 = wave_sum_r(nrm);
       C2 a0 = get(0);   // row k lives in lane k, q = 0 (k < BW <= 64)
-      const C2 alpha = mk<T>(__shfl(a0.x, k, 64), __shfl(a0.y, k, 64));
+      const C2 alpha = mk<T>(__shfl(a0.x, k, 64), RE ? T(0) : __shfl(a0.y, k, 64));
       const House<T> h = house<T>(alpha, xn2);
       for_rows<FR>(nq, [&](int q) {
           const int r = lane + 64 * q;
@@ -388,7 +383,7 @@ __global__ __launch_bounds__(1024) void panel_qr_kernel(BandCtx<T> c, int j0) {
             vv = mk<T>(1, 0);
             put(q, mk<T>(h.beta, 0));
           } else if (r > k) {
-            vv = h.skip ? mk<T>(0, 0) : c_mul(e, h.scale);
+            vv = h.skip ? mk<T>(0, 0) : (RE ? mk<T>(e.x * h.scale.x, T(0)) : c_mul(e, h.scale));
             put(q, vv);
           }
           if constexpr (RPT > 0) s_v[r] = vv;
@@ -404,24 +399,33 @@ __global__ __launch_bounds__(1024) void panel_qr_kernel(BandCtx<T> c, int j0) {
           const int r = lane + 64 * q;
           if (r >= k && r < m) {
             const C2 vr = RPT > 0 ? s_v[r] : vk[r];
-            const C2 t = c_cmul(vr, get(q));
-            u.x += t.x;
-            u.y += t.y;
+            if constexpr (RE) {
+              u.x = fmat(vr.x, get(q).x, u.x);
+            } else {
+              const C2 t = c_cmul(vr, get(q));
+              u.x += t.x;
+              u.y += t.y;
+            }
           }
         });
-      u = wave_sum_c(u);
-      const C2 cu = c_mul(c_conj(tau), u);
+      if constexpr (RE) u.x = wave_sum_r(u.x);
+      else u = wave_sum_c(u);
+      const C2 cu = RE ? mk<T>(tau.x * u.x, T(0)) : c_mul(c_conj(tau), u);
       nrm = 0;
       for_rows<FR>(nq, [&](int q) {
           const int r = lane + 64 * q;
           if (r >= k && r < m) {
             const C2 vr = RPT > 0 ? s_v[r] : vk[r];
             C2 e = get(q);
-            const C2 t = c_mul(vr, cu);
-            e.x -= t.x;
-            e.y -= t.y;
+            if constexpr (RE) {
+              e.x = fmat(-vr.x, cu.x, e.x);
+            } else {
+              const C2 t = c_mul(vr, cu);
+              e.x -= t.x;
+              e.y -= t.y;
+            }
             put(q, e);
-            if (r > k + 1) nrm += e.x * e.x + e.y * e.y;
+            if (r > k + 1) nrm += RE ? e.x * e.x : e.x * e.x + e.y * e.y;
           }
         });
     } else if (w < k) {
@@ -431,12 +435,17 @@ __global__ __launch_bounds__(1024) void panel_qr_kernel(BandCtx<T> c, int j0) {
           const int r = lane + 64 * q;
           if (r >= k && r < m) {
             const C2 vr = RPT > 0 ? s_v[r] : vk[r];
-            const C2 t = c_cmul(get(q), vr);
-            g.x += t.x;
-            g.y += t.y;
+            if constexpr (RE) {
+              g.x = fmat(get(q).x, vr.x, g.x);
+            } else {
+              const C2 t = c_cmul(get(q), vr);
+              g.x += t.x;
+              g.y += t.y;
+            }
           }
         });
-      g = wave_sum_c(g);
+      if constexpr (RE) g.x = wave_sum_r(g.x);
+      else g = wave_sum_c(g);
       if (lane == 0) s_G[k][w] = g;
       // wave 0 meanwhile builds column k-1 of T (its g values are complete)
       if (w == 0) t_column(k - 1);
@@ -1401,22 +1410,23 @@ static bool chase_lds_ok() {
   return ok;
 }
 
-template <typename T>
+template <typename T, bool RE>
 static void launch_panel_qr(hipStream_t st, const BandCtx<T>& c, int batch, int j0, int m) {
   const dim3 g(batch), blk(1024);
   // rows per lane in registers: a complex value is 2 (FP32) or 4 (FP64) VGPRs
   const int nq = (m + 63) / 64;
-  if (nq <= 2) hipLaunchKernelGGL((panel_qr_kernel<T, 2>), g, blk, 0, st, c, j0);
-  else if (nq <= 4) hipLaunchKernelGGL((panel_qr_kernel<T, 4>), g, blk, 0, st, c, j0);
-  else if (nq <= 8) hipLaunchKernelGGL((panel_qr_kernel<T, 8>), g, blk, 0, st, c, j0);
-  else if (nq <= 16) hipLaunchKernelGGL((panel_qr_kernel<T, 16>), g, blk, 0, st, c, j0);
-  else if (nq <= 24 && sizeof(T) == 4) hipLaunchKernelGGL((panel_qr_kernel<T, sizeof(T) == 4 ? 24 : 16>), g, blk, 0, st, c, j0);
-  else if (nq <= 32 && sizeof(T) == 4) hipLaunchKernelGGL((panel_qr_kernel<T, sizeof(T) == 4 ? 32 : 16>), g, blk, 0, st, c, j0);
+  constexpr bool F = sizeof(T) == 4;
+  if (nq <= 2) hipLaunchKernelGGL((panel_qr_kernel<T, 2, 0, RE>), g, blk, 0, st, c, j0);
+  else if (nq <= 4) hipLaunchKernelGGL((panel_qr_kernel<T, 4, 0, RE>), g, blk, 0, st, c, j0);
+  else if (nq <= 8) hipLaunchKernelGGL((panel_qr_kernel<T, 8, 0, RE>), g, blk, 0, st, c, j0);
+  else if (nq <= 16) hipLaunchKernelGGL((panel_qr_kernel<T, 16, 0, RE>), g, blk, 0, st, c, j0);
+  else if (nq <= 24 && F) hipLaunchKernelGGL((panel_qr_kernel<T, F ? 24 : 16, 0, RE>), g, blk, 0, st, c, j0);
+  else if (nq <= 32 && F) hipLaunchKernelGGL((panel_qr_kernel<T, F ? 32 : 16, 0, RE>), g, blk, 0, st, c, j0);
   // beyond the register tiers: 4 more rows per lane in LDS (FP32: m <= 2304,
   // every serving matrix; FP64: m <= 1280)
-  else if (nq <= 36 && sizeof(T) == 4) hipLaunchKernelGGL((panel_qr_kernel<T, sizeof(T) == 4 ? 32 : 16, 4>), g, blk, 0, st, c, j0);
-  else if (nq <= 20 && sizeof(T) == 8) hipLaunchKernelGGL((panel_qr_kernel<T, 16, 4>), g, blk, 0, st, c, j0);
-  else hipLaunchKernelGGL((panel_qr_kernel<T, 0>), g, blk, 0, st, c, j0);
+  else if (nq <= 36 && F) hipLaunchKernelGGL((panel_qr_kernel<T, F ? 32 : 16, 4, RE>), g, blk, 0, st, c, j0);
+  else if (nq <= 20 && !F) hipLaunchKernelGGL((panel_qr_kernel<T, 16, 4, RE>), g, blk, 0, st, c, j0);
+  else hipLaunchKernelGGL((panel_qr_kernel<T, 0, 0, RE>), g, blk, 0, st, c, j0);
 }
 
 // Polls per chase wait before the wave gives up (process-wide).
@@ -1460,7 +1470,7 @@ static void band_impl(hipStream_t st, typename C2T<T>::t* A, int n, int ld, int 
   // and defines tau for every column whose reflector starts inside the matrix
   if (n - BW >= 1) {
     const int m = n - BW;
-    launch_panel_qr<T>(st, cx[0], batch, 0, m);
+    launch_panel_qr<T, RE>(st, cx[0], batch, 0, m);
     const int nrbk = (m + 63) / 64, nchk = (m + AV_KCH - 1) / AV_KCH;
     hipLaunchKernelGGL(band_av_kernel<T>, dim3(nrbk, nchk, batch), dim3(256), 0, st, cx[0], BW);
     hipLaunchKernelGGL(band_x_kernel<T>, dim3(nrbk, batch), dim3(256), 0, st, cx[0], BW, nchk);
@@ -1473,7 +1483,7 @@ static void band_impl(hipStream_t st, typename C2T<T>::t* A, int n, int ld, int 
                          cp.Wc + r0, A + (size_t)r0 * ld + r0, ld, sP, sM);
       const int r0n = r0 + BW, mn = n - r0n;          // A22(p + 1)
       if (mn < 1) break;
-      launch_panel_qr<T>(st, cn, batch, r0, mn);
+      launch_panel_qr<T, RE>(st, cn, batch, r0, mn);
       const int t = (mn + 63) / 64;
       // FP32 launches are sampled: 8 (m^2/2) 32 flops for the rank-2k on the
       // lower triangle and 8 m^2 16 for A22 V, per complex matrix; a real
