@@ -266,29 +266,49 @@ def roofline_fields(samples):
     """Rooflines from the live samples of the sampled pass.
 
     roofline: the FP32 fused stage-1 update with the next panel's A22 V
-    (fused_update_av_kernel<float>), the largest consumer of the device in the
-    serving stream, against the FP32 dense MFMA peak; traffic = measured HBM
-    bytes per algorithmic flop (counter runs) times the flops per launch.
+    (fused_update_av_kernel<float, true>, the real-symmetric batches, which
+    carry most of its flops; the complex instantiation is
+    roofline_fused_complex), the largest consumer of the device in the
+    serving stream. Real: 16 flop per algorithmic byte, against HBM
+    bandwidth; complex: 32 flop/B, against the FP32 dense MFMA peak.
+    traffic = measured HBM bytes per algorithmic flop (counter runs) times
+    the flops per launch.
     roofline_gemm_f64: the FP64 MFMA GEMM (Cholesky / L^-1 recursion, FP64
     reduction). roofline_chase: the bulge chase, one workgroup per matrix,
     against one CU's FP32 VALU rate (flops are counted per workgroup).
     build_roofline / dos_roofline: the H/S build and the DOS against HBM
     bandwidth (the DOS also as Gaussians per second of the full sum)."""
     out = {}
-    r = _mfma_roofline(samples["fused_update_av"], FP32_MFMA_PEAK_TFS,
-                       "fused_update_av_kernel<float, false> (complex batches: stage-1 rank-2k update of the "
-                       "lower triangle + next A22 V)",
-                       FUSED_TRAFFIC_FILE, "hbm_bytes_per_flop")
-    if r is not None:
-        r["algorithmic_bytes_per_flop"] = 1.0 / 32.0   # lower triangle read and written once, 8 B per element
-        out["roofline"] = r
-    r = _mfma_roofline(samples["fused_update_av_real"], FP32_MFMA_PEAK_TFS,
-                       "fused_update_av_kernel<float, true> (real-symmetric batches: one real MFMA per block)",
-                       FUSED_TRAFFIC_FILE, "hbm_bytes_per_flop_real")
-    if r is not None:
-        # the real path keeps the complex64 layout (zero imaginary parts): same bytes, a quarter of the flops
-        r["algorithmic_bytes_per_flop"] = 1.0 / 8.0
-        out["roofline_fused_real"] = r
+    rc = _mfma_roofline(samples["fused_update_av"], FP32_MFMA_PEAK_TFS,
+                        "fused_update_av_kernel<float, false> (complex batches: stage-1 rank-2k update of the "
+                        "lower triangle + next A22 V)",
+                        FUSED_TRAFFIC_FILE, "hbm_bytes_per_flop")
+    if rc is not None:
+        rc["algorithmic_bytes_per_flop"] = 1.0 / 32.0   # lower triangle read and written once, 8 B per element
+    rr = _mfma_roofline(samples["fused_update_av_real"], FP32_MFMA_PEAK_TFS,
+                        "fused_update_av_kernel<float, true> (real-symmetric batches: stage-1 rank-2k update of "
+                        "the lower triangle + next A22 V, one real MFMA per block)",
+                        FUSED_TRAFFIC_FILE, "hbm_bytes_per_flop_real")
+    if rr is not None:
+        # 64 m^2 flops per matrix against the real lower triangle read and written once in FP32
+        # (m^2/2 x 4 B each way): 16 flop/B, below the FP32 ridge (157.3 TF/s / 8 TB/s = 19.7 flop/B),
+        # so this instantiation is bound by HBM. The kernel keeps the complex64 layout (zero imaginary
+        # parts), so its measured traffic is at least twice these bytes.
+        fl = rr["algorithmic_flops_per_launch"]
+        t_s = rr["avg_launch_us"] * 1e-6
+        rr.update({"bound": "hbm", "achieved": fl / 16.0 / t_s / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                   "frac": fl / 16.0 / t_s / 1e9 / HBM_PEAK_GBS, "algorithmic_bytes_per_launch": fl / 16.0,
+                   "algorithmic_bytes_per_flop": 1.0 / 16.0, "achieved_tflops": fl / t_s / 1e12,
+                   "frac_of_fp32_mfma_peak": fl / t_s / 1e12 / FP32_MFMA_PEAK_TFS})
+    # the instantiation with more flops in the stream is the line's roofline
+    if rr is not None and (rc is None or rr["algorithmic_flops_total"] >= rc["algorithmic_flops_total"]):
+        out["roofline"] = rr
+        if rc is not None:
+            out["roofline_fused_complex"] = rc
+    elif rc is not None:
+        out["roofline"] = rc
+        if rr is not None:
+            out["roofline_fused_real"] = rr
     r = _mfma_roofline(samples["gemm_f64"], FP64_MFMA_PEAK_TFS,
                        "gemm_mfma_kernel<double> (Cholesky trailing updates, L^-1)", GEMM_TRAFFIC_FILE,
                        "hbm_bytes_per_flop")
@@ -330,7 +350,9 @@ def roofline_fields(samples):
         n, ms, ev, nbytes, n_all, _, _ = b
         ach = nbytes / (ms * 1e-3) / 1e9
         out["build_roofline"] = {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                                 "frac": ach / HBM_PEAK_GBS, "kernel": "slk_build_kernel<complex128>",
+                                 "frac": ach / HBM_PEAK_GBS,
+                                 "kernel": "slk_fill_kernel + slk_build_rows_kernel / slk_build_kernel "
+                                           "(complex128; the window spans the launch sequence)",
                                  "avg_launch_us": 1000.0 * ms, "avg_launch_us_events": 1000.0 * ev,
                                  "algorithmic_bytes_per_launch": nbytes,
                                  "launches_sampled": int(n), "launches_total": int(n_all)}
