@@ -361,50 +361,40 @@ __global__ __launch_bounds__(LEAF_THREADS) void potrf_diag_kernel(double2* L, do
   }
 }
 
-// dst = src for a rows x cols block (row strides lds / ldd, matrix strides sS / sD).
-__global__ void copy_block_kernel(const double2* __restrict__ src, int lds, long long sS, double2* __restrict__ dst,
-                                  int ldd, long long sD, int rows, int cols) {
-  const int b = blockIdx.y;
-  const long long total = (long long)rows * cols;
-  for (long long e = (long long)blockIdx.x * 256 + threadIdx.x; e < total; e += (long long)gridDim.x * 256) {
-    const int r = (int)(e / cols), c = (int)(e % cols);
-    dst[(size_t)b * sD + (size_t)r * ldd + c] = src[(size_t)b * sS + (size_t)r * lds + c];
-  }
-}
-
 // Cholesky S = L L^H together with W = L^-1, by recursive halving of the
 // block [o, o+m): factor the leading n1 columns, L21 = A21 W11^H, the
 // trailing update A22 -= L21 L21^H (lower tiles, K = n1), factor the
 // trailing block, then W21 = -W22 (L21 W11). The NB x NB leaves are factored
 // and inverted in LDS by potrf_diag_kernel. The GEMMs have K = n1 instead of
-// the NB of a blocked right-looking loop. T: ld * ld / 2 per matrix.
+// the NB of a blocked right-looking loop. L is scratch here (only W is read
+// afterwards): a level keeps its L21 compact (n2 x n1) at the start of its
+// scratch T while the trailing recursion works after it, and L21 W11 goes
+// into L's L21 block, which nothing reads any more. A level holds n2 n1 of T
+// and the trailing recursion at most its own share: under 2/3 of ld * ld / 2
+// per matrix.
 static void chol_inv_rec(hipStream_t st, double2* L, double2* W, double2* T, int o, int m, int n, int ld, int batch,
                          long long sM, long long sT, int* info, bool re) {
   // 32-row leaves: a 64-row leaf (64 dependent steps in one workgroup) measured
   // 7% slower over the whole recursion than two 32-row leaves with the four
-  // GEMMs and the copy between them
+  // GEMMs between them
   if (m <= NB) {
     hipLaunchKernelGGL(potrf_diag_kernel<NB>, dim3(batch), dim3(LEAF_THREADS), 0, st, L, W, m, ld, o, info);
     return;
   }
   const int n1 = ((m + 1) / 2 + NB - 1) / NB * NB, n2 = m - n1;
-  double2* L21 = L + (size_t)(o + n1) * ld + o;
+  double2* A21 = L + (size_t)(o + n1) * ld + o;
   chol_inv_rec(st, L, W, T, o, n1, n, ld, batch, sM, sT, info, re);
-  // L21 = A21 W11^H (W11^H upper: K ends at the column block); via T, then back
-  launch_zgemm(st, batch, n2, n1, n1, 1.0, L21, ld, sM, OP_N, W + (size_t)o * ld + o, ld, sM, OP_C, 0.0, T, ld, sT,
+  // L21 = A21 W11^H (W11^H upper: K ends at the column block), compact in T
+  launch_zgemm(st, batch, n2, n1, n1, 1.0, A21, ld, sM, OP_N, W + (size_t)o * ld + o, ld, sM, OP_C, 0.0, T, n1, sT,
                KLIM_B_UPPER, 0, re);
-  const long long cells = (long long)n2 * n1;
-  const long long cblk = (cells + 255) / 256;
-  hipLaunchKernelGGL(copy_block_kernel, dim3((unsigned)(cblk < 1024 ? cblk : 1024), batch), dim3(256), 0, st, T, ld, sT,
-                     L21, ld, sM, n2, n1);
   // A22 -= L21 L21^H over the lower tiles
-  launch_zgemm(st, batch, n2, n2, n1, -1.0, L21, ld, sM, OP_N, L21, ld, sM, OP_C, 1.0, L + (size_t)(o + n1) * ld + o + n1,
+  launch_zgemm(st, batch, n2, n2, n1, -1.0, T, n1, sT, OP_N, T, n1, sT, OP_C, 1.0, L + (size_t)(o + n1) * ld + o + n1,
                ld, sM, 0, 1, re);
-  chol_inv_rec(st, L, W, T, o + n1, n2, n, ld, batch, sM, sT, info, re);
-  // W21 = -W22 (L21 W11)
-  launch_zgemm(st, batch, n2, n1, n1, 1.0, L21, ld, sM, OP_N, W + (size_t)o * ld + o, ld, sM, OP_N, 0.0, T, ld, sT,
+  chol_inv_rec(st, L, W, T + (size_t)n2 * n1, o + n1, n2, n, ld, batch, sM, sT, info, re);
+  // W21 = -W22 (L21 W11), the product in L's A21 block
+  launch_zgemm(st, batch, n2, n1, n1, 1.0, T, n1, sT, OP_N, W + (size_t)o * ld + o, ld, sM, OP_N, 0.0, A21, ld, sM,
                KLIM_B_LOWER, 0, re);
-  launch_zgemm(st, batch, n2, n1, n2, -1.0, W + (size_t)(o + n1) * ld + o + n1, ld, sM, OP_N, T, ld, sT, OP_N, 0.0,
+  launch_zgemm(st, batch, n2, n1, n2, -1.0, W + (size_t)(o + n1) * ld + o + n1, ld, sM, OP_N, A21, ld, sM, OP_N, 0.0,
                W + (size_t)(o + n1) * ld + o, ld, sM, KLIM_A_LOWER, 0, re);
 }
 
