@@ -50,10 +50,18 @@ class FitDataset:
 class BatchedProperties:
     """Gap (eV) and Fermi-shifted Gaussian DOS of many structures, one batched
     device pass per size bucket, differentiable with respect to the packed
-    tables through the hand-written adjoints."""
+    tables through the hand-written adjoints.
+
+    The buckets run on ``n_streams`` HIP streams side by side: their solves
+    hold one workgroup per matrix in several stages (the unblocked finisher
+    of the tridiagonalisation, inverse iteration), so one bucket at a time
+    leaves most compute units idle. Every bucket is launched before any
+    result is read back; the backward of each bucket runs on its forward's
+    stream (autograd keeps forward streams), and the outputs are joined on
+    the caller's stream."""
 
     def __init__(self, tables: TableSet, inputs: Sequence[engine.StructureInput], grid: torch.Tensor,
-                 sigma: float = 0.1):
+                 sigma: float = 0.1, n_streams: int = 4):
         self.tables = tables
         self.grid = grid
         self.sigma = float(sigma)
@@ -68,6 +76,8 @@ class BatchedProperties:
             # a bucket whose k-points all have real H and S is solved as real symmetric
             plan.real = all(engine.real_kpoints(inputs[i].kpoints) for i in idx)
             self.buckets.append((idx, plan))
+        dev = torch.device(tables.device)
+        self.streams = [torch.cuda.Stream(device=dev) for _ in range(max(1, min(n_streams, len(self.buckets))))]
 
     def __call__(self, table_tensor: torch.Tensor, with_grad: bool = True, fermi_grad: bool = False):
         """Returns (dos [n, ne], gaps [n] eV, edges rows per structure)."""
@@ -75,23 +85,44 @@ class BatchedProperties:
         gap_out: List[Optional[torch.Tensor]] = [None] * self.n
         rows: List[Optional[dict]] = [None] * self.n
         tt = table_tensor.detach()
-        for idx, plan in self.buckets:
-            H, S = engine.run_build(self.tables, plan, table_tensor=tt)
-            w, V, _ = engine.eigh(H, S, vectors=with_grad, real=plan.real)
-            del H, S
-            edges, _, _ = engine.post(plan, w)
-            wg = TableEigenvalues.apply(table_tensor, self.tables, plan, V, w) if with_grad else w
-            dos = GaussianDos.apply(wg, plan, edges, self.grid, self.sigma, fermi_grad)
-            host = edges.cpu().numpy()
-            for s, i in enumerate(idx):
-                row = engine.edges_dict(host[s])
-                if plan.n_orb[s] < plan.ld and engine.reaches_padding(row):
-                    raise engine.EigenFailure(f"structure {i}: its spectrum reaches the identity-padding level "
-                                              f"({engine.PAD_H} Ha) of bucket {plan.ld}")
-                m0, nk, n = plan.mat0[s], plan.nk[s], plan.n_orb[s]
-                dos_out[i] = dos[s]
-                gap_out[i] = gap_tensor(wg[m0:m0 + nk, :n], row)
-                rows[i] = row
+        cur = torch.cuda.current_stream(tt.device)
+        launched = []
+        for bi, (idx, plan) in enumerate(self.buckets):
+            st = self.streams[bi % len(self.streams)]
+            st.wait_stream(cur)          # the tables of this step are ready
+            with torch.cuda.stream(st):
+                H, S = engine.run_build(self.tables, plan, table_tensor=tt)
+                # no host check here (it would wait for this bucket): info is read after every launch
+                w, V, info = engine.eigh(H, S, vectors=with_grad, real=plan.real, check=False)
+                del H, S
+                edges, _, _ = engine.post(plan, w)
+                wg = TableEigenvalues.apply(table_tensor, self.tables, plan, V, w) if with_grad else w
+                dos = GaussianDos.apply(wg, plan, edges, self.grid, self.sigma, fermi_grad)
+                host = torch.empty(edges.shape, dtype=edges.dtype, pin_memory=True)
+                host.copy_(edges, non_blocking=True)
+                done = torch.cuda.Event()
+                done.record(st)
+            launched.append((idx, plan, wg, dos, host, done, st, info))
+        for idx, plan, wg, dos, host, done, st, info in launched:
+            done.synchronize()
+            engine.check_info(info, plan.n_mat)
+            h = host.numpy()
+            with torch.cuda.stream(st):
+                for s, i in enumerate(idx):
+                    row = engine.edges_dict(h[s])
+                    if plan.n_orb[s] < plan.ld and engine.reaches_padding(row):
+                        raise engine.EigenFailure(f"structure {i}: its spectrum reaches the identity-padding level "
+                                                  f"({engine.PAD_H} Ha) of bucket {plan.ld}")
+                    m0, nk, n = plan.mat0[s], plan.nk[s], plan.n_orb[s]
+                    dos_out[i] = dos[s]
+                    gap_out[i] = gap_tensor(wg[m0:m0 + nk, :n], row)
+                    rows[i] = row
+            # the joined outputs are read on the caller's stream
+            cur.wait_stream(st)
+            dos.record_stream(cur)
+            wg.record_stream(cur)
+            for i in idx:
+                gap_out[i].record_stream(cur)
         return torch.stack(dos_out), torch.stack(gap_out), rows
 
 
