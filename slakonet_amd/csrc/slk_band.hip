@@ -66,6 +66,25 @@ struct BandCtx {
   int spin_max;  // polls per wait before giving up; < 0: give up at the first wait
 };
 
+// Element e = r * ld + c of a matrix of A (a: the matrix's origin). A complex
+// matrix stores it at a[e]; a real batch (RE) stores the real value alone at
+// real index e of the same slot, so the stage-1 passes move half the bytes.
+template <typename T, bool RE>
+__device__ __forceinline__ typename C2T<T>::t a_ld(const typename C2T<T>::t* a, size_t e, bool ok) {
+  if constexpr (RE) {
+    T x = 0;
+    if (ok) x = reinterpret_cast<const T*>(a)[e];
+    return mk<T>(x, T(0));
+  } else {
+    return load_or_zero(a + e, ok);
+  }
+}
+template <typename T, bool RE>
+__device__ __forceinline__ void a_st(typename C2T<T>::t* a, size_t e, typename C2T<T>::t v) {
+  if constexpr (RE) reinterpret_cast<T*>(a)[e] = v.x;
+  else a[e] = v;
+}
+
 template <typename T> struct MF;
 template <> struct MF<float> {
   using v4 = __attribute__((ext_vector_type(4))) float;
@@ -317,14 +336,14 @@ __global__ __launch_bounds__(1024) void panel_qr_kernel(BandCtx<T> c, int j0) {
   // rows lane + 64 (RPT + t) of column w at s_tl[t][lane][w]
   __shared__ C2 s_tl[QT > 0 ? QT : 1][QT > 0 ? 64 : 1][BW + 1];
   const int nq = RPT > 0 ? FR : (m + 63) / 64;
-  C2* col = a + (size_t)(r0 + lane) * ld + j0 + w;   // this lane's rows of column w
+  const size_t col = (size_t)(r0 + lane) * ld + j0 + w;   // this lane's rows of column w
   C2 p[NR];
   if constexpr (RPT > 0) {
 #pragma unroll
-    for (int q = 0; q < RPT; ++q) p[q] = load_or_zero(col + (size_t)64 * q * ld, lane + 64 * q < m);
+    for (int q = 0; q < RPT; ++q) p[q] = a_ld<T, RE>(a, col + (size_t)64 * q * ld, lane + 64 * q < m);
 #pragma unroll
     for (int t = 0; t < QT; ++t)
-      s_tl[t][lane][w] = load_or_zero(col + (size_t)64 * (RPT + t) * ld, lane + 64 * (RPT + t) < m);
+      s_tl[t][lane][w] = a_ld<T, RE>(a, col + (size_t)64 * (RPT + t) * ld, lane + 64 * (RPT + t) < m);
   }
   // q is a compile-time constant in the unrolled row loops (RPT > 0)
   auto get = [&](int q) -> C2 {
@@ -332,7 +351,7 @@ __global__ __launch_bounds__(1024) void panel_qr_kernel(BandCtx<T> c, int j0) {
       if (QT > 0 && q >= RPT) return s_tl[q - RPT][lane][w];
       return p[q];
     } else {
-      return load_or_zero(col + (size_t)64 * q * ld, lane + 64 * q < m);
+      return a_ld<T, RE>(a, col + (size_t)64 * q * ld, lane + 64 * q < m);
     }
   };
   auto put = [&](int q, C2 v) {
@@ -340,7 +359,7 @@ __global__ __launch_bounds__(1024) void panel_qr_kernel(BandCtx<T> c, int j0) {
       if (QT > 0 && q >= RPT) s_tl[q - RPT][lane][w] = v;
       else p[q] = v;
     } else if (lane + 64 * q < m) {
-      col[(size_t)64 * q * ld] = v;
+      a_st<T, RE>(a, col + (size_t)64 * q * ld, v);
     }
   };
   // T (Q = I - V T V^H): T[k][k] = tau_k, T[i][k] = -tau_k sum_{l=i}^{k-1} T[i][l] g_l^(k);
@@ -460,7 +479,7 @@ __global__ __launch_bounds__(1024) void panel_qr_kernel(BandCtx<T> c, int j0) {
       const int r = lane + 64 * q;
       if (r >= m) return;
       const C2 e = get(q);
-      if constexpr (RPT > 0) col[(size_t)64 * q * ld] = e;
+      if constexpr (RPT > 0) a_st<T, RE>(a, col + (size_t)64 * q * ld, e);
       C2 v = mk<T>(0, 0);
       if (w < kb) v = r > w ? e : (r == w ? mk<T>(1, 0) : mk<T>(0, 0));
       vcb[(size_t)w * ld + r0 + r] = v;
@@ -476,7 +495,7 @@ __global__ __launch_bounds__(1024) void panel_qr_kernel(BandCtx<T> c, int j0) {
 // rows x 16 columns, complex product as 4 real 16x16x4 MFMAs per k-step.
 // The next 64-column tile is fetched into registers while the current one is
 // multiplied from LDS.
-template <typename T>
+template <typename T, bool RE = false>
 __global__ __launch_bounds__(256, 3) void band_av_kernel(BandCtx<T> c, int r0) {
   using C2 = typename C2T<T>::t;
   using V4 = typename MF<T>::v4;
@@ -513,9 +532,9 @@ __global__ __launch_bounds__(256, 3) void band_av_kernel(BandCtx<T> c, int r0) {
     const int R = mode == 1 ? k0 : row0, Cc = mode == 1 ? row0 : k0;
     const int limR = mode == 1 ? kend : n, limC = mode == 1 ? n : kend;
     const int h0 = tid / 64, lo = tid % 64;
-    const C2* p = a + min(Cc + lo, limC - 1);
+    const size_t p = min(Cc + lo, limC - 1);
 #pragma unroll
-    for (int i = 0; i < NA; ++i) ra[i] = p[(size_t)min(R + h0 + 4 * i, limR - 1) * ld];
+    for (int i = 0; i < NA; ++i) ra[i] = a_ld<T, RE>(a, p + (size_t)min(R + h0 + 4 * i, limR - 1) * ld, true);
 #pragma unroll
     for (int i = 0; i < NBT; ++i) {
       const int e = tid + 256 * i;
@@ -1049,7 +1068,7 @@ __global__ __launch_bounds__(CH_THREADS) void band_chase_kernel(BandCtx<T> c, un
   if (tid == 0) s_fail = 0;
   for (int e = tid; e < n * BND_LD; e += CH_THREADS) {
     const int col = e / BND_LD, d = e % BND_LD, r = col + d;
-    bnd[e] = (d <= BW && r < n) ? a[(size_t)r * ld + col] : mk<T>(0, 0);
+    bnd[e] = a_ld<T, RE>(a, (size_t)r * ld + col, d <= BW && r < n);
   }
   __syncthreads();
   const int ns = n - 1;   // sweeps 0 .. n-2
@@ -1177,8 +1196,8 @@ size_t band_tt_offset(int ld, int batch) {
 template <typename T, bool RE = false>
 __global__ __launch_bounds__(256) void strip_update_kernel(int m, const typename C2T<T>::t* __restrict__ Vc,
                                                            const typename C2T<T>::t* __restrict__ Wc,
-                                                           typename C2T<T>::t* __restrict__ C, int ld, long long sP,
-                                                           long long sM) {
+                                                           typename C2T<T>::t* __restrict__ C, long long off, int ld,
+                                                           long long sP, long long sM) {
   using C2 = typename C2T<T>::t;
   using V4 = typename MF<T>::v4;
   constexpr int KK = 2 * BW / 4;
@@ -1195,7 +1214,7 @@ __global__ __launch_bounds__(256) void strip_update_kernel(int m, const typename
 #pragma unroll
   for (int v = 0; v < 4; ++v) {
     const int r = m0 + MF<T>::row(lk, v);
-    const C2 x = load_or_zero(c + (size_t)r * ld + li, r < m && li < ncol);
+    const C2 x = a_ld<T, RE>(c, off + (size_t)r * ld + li, r < m && li < ncol);
     accR[v] = x.x;
     accI[v] = x.y;
   }
@@ -1218,7 +1237,7 @@ __global__ __launch_bounds__(256) void strip_update_kernel(int m, const typename
 #pragma unroll
   for (int v = 0; v < 4; ++v) {
     const int r = m0 + MF<T>::row(lk, v);
-    if (r < m && li < ncol) c[(size_t)r * ld + li] = mk<T>(accR[v], RE ? T(0) : accI[v]);
+    if (r < m && li < ncol) a_st<T, RE>(c, off + (size_t)r * ld + li, mk<T>(accR[v], RE ? T(0) : accI[v]));
   }
 }
 
@@ -1267,7 +1286,8 @@ __global__ __launch_bounds__(256) void fused_update_av_kernel(const typename C2T
   const C2* vp = Vp + (size_t)b * sP;
   const C2* wp = Wp + (size_t)b * sP;
   const C2* vn = Vn + (size_t)b * sP;
-  C2* a = A + (size_t)b * sM + (size_t)r0n * ld + r0n;
+  C2* a = A + (size_t)b * sM;
+  const size_t o = (size_t)r0n * ld + r0n;   // origin of A22(p+1) in the matrix
   __shared__ T Cr[64][65], Ci[RE ? 1 : 64][65];
   __shared__ T VJr[64][BW + 1], VJi[RE ? 1 : 64][BW + 1], VIr[64][BW + 1], VIi[RE ? 1 : 64][BW + 1];
   // V_{p+1} rows of blocks J and I (zero beyond the matrix)
@@ -1310,7 +1330,7 @@ __global__ __launch_bounds__(256) void fused_update_av_kernel(const typename C2T
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
           const int r = m0 + 16 * i + MF<T>::row(lk, v), col = n0 + 16 * j + li;
-          const C2 x = load_or_zero(a + (size_t)r * ld + col, r < mn && col < mn);
+          const C2 x = a_ld<T, RE>(a, o + (size_t)r * ld + col, r < mn && col < mn);
           accR[i][j][v] = x.x;
           accI[i][j][v] = x.y;
         }
@@ -1335,7 +1355,7 @@ __global__ __launch_bounds__(256) void fused_update_av_kernel(const typename C2T
         for (int v = 0; v < 4; ++v) {
           const int rl = wr + 16 * i + MF<T>::row(lk, v), cl = wc + 16 * j + li;
           const int r = 64 * I + rl, col = 64 * J + cl;
-          if (r < mn && col < mn) a[(size_t)r * ld + col] = mk<T>(accR[i][j][v], RE ? T(0) : accI[i][j][v]);
+          if (r < mn && col < mn) a_st<T, RE>(a, o + (size_t)r * ld + col, mk<T>(accR[i][j][v], RE ? T(0) : accI[i][j][v]));
           Cr[rl][cl] = accR[i][j][v];
           if constexpr (!RE) Ci[rl][cl] = accI[i][j][v];
         }
@@ -1472,7 +1492,7 @@ static void band_impl(hipStream_t st, typename C2T<T>::t* A, int n, int ld, int 
     const int m = n - BW;
     launch_panel_qr<T, RE>(st, cx[0], batch, 0, m);
     const int nrbk = (m + 63) / 64, nchk = (m + AV_KCH - 1) / AV_KCH;
-    hipLaunchKernelGGL(band_av_kernel<T>, dim3(nrbk, nchk, batch), dim3(256), 0, st, cx[0], BW);
+    hipLaunchKernelGGL((band_av_kernel<T, RE>), dim3(nrbk, nchk, batch), dim3(256), 0, st, cx[0], BW);
     hipLaunchKernelGGL(band_x_kernel<T>, dim3(nrbk, batch), dim3(256), 0, st, cx[0], BW, nchk);
     hipLaunchKernelGGL(band_w_kernel<T>, dim3(nrbk, batch), dim3(256), 0, st, cx[0], BW, nrbk);
     for (int p = 0;; ++p) {
@@ -1480,7 +1500,7 @@ static void band_impl(hipStream_t st, typename C2T<T>::t* A, int n, int ld, int 
       const BandCtx<T>& cn = cx[(p + 1) & 1];
       const int r0 = BW * (p + 1), mp = n - r0;       // A22(p)
       hipLaunchKernelGGL((strip_update_kernel<T, RE>), dim3((mp + 63) / 64, 1, batch), dim3(256), 0, st, mp, cp.Vc + r0,
-                         cp.Wc + r0, A + (size_t)r0 * ld + r0, ld, sP, sM);
+                         cp.Wc + r0, A, (long long)r0 * ld + r0, ld, sP, sM);
       const int r0n = r0 + BW, mn = n - r0n;          // A22(p + 1)
       if (mn < 1) break;
       launch_panel_qr<T, RE>(st, cn, batch, r0, mn);

@@ -9,14 +9,16 @@ enum { KLIM_A_LOWER = 1, KLIM_A_UPPER = 2, KLIM_B_UPPER = 4, KLIM_B_LOWER = 8 };
 
 // C = alpha opA(A) opB(B) + beta C for `batch` problems with strides sA/sB/sC.
 // tri = 1 skips 64x64 output tiles strictly above the diagonal. re: A and B
-// are real (zero imaginary parts); C gets zero imaginary parts.
+// are real (zero imaginary parts); C gets zero imaginary parts, or with
+// c_real the real values alone, element (i, j) of problem b at real index
+// i * ldc + j of the complex slot C + b * sC (half the bytes).
 void launch_zgemm(hipStream_t st, int batch, int M, int N, int K, double alpha, const double2* A, int lda,
                   long long sA, int opA, const double2* B, int ldb, long long sB, int opB, double beta, double2* C,
-                  int ldc, long long sC, int klim, int tri, bool re = false);
+                  int ldc, long long sC, int klim, int tri, bool re = false, bool c_real = false);
 
 void launch_cgemm(hipStream_t st, int batch, int M, int N, int K, float alpha, const float2* A, int lda,
                   long long sA, int opA, const float2* B, int ldb, long long sB, int opB, float beta, float2* C,
-                  int ldc, long long sC, int klim, int tri, bool re = false);
+                  int ldc, long long sC, int klim, int tri, bool re = false, bool c_real = false);
 
 // Zero-fill (no memset nodes so that the solve can be graph-captured).
 void launch_zero(hipStream_t st, void* p, size_t bytes);
@@ -58,8 +60,9 @@ constexpr int SLK_BW = 16;   // band width of the two-stage reduction
 // Workspace: Vc, Wc [2 SLK_BW][ld] and Yp [band_partials_elems(ld)] per
 // matrix, band scratch [band_scratch_elems(ld)] per matrix.
 // info (nullable, [batch]): set to 2 for a matrix whose bulge chase gave up a bounded wait.
-// re: C is real symmetric (zero imaginary parts); every output keeps zero
-// imaginary parts.
+// re: C is real symmetric, given as its real values alone (element (i, j) of
+// matrix b at real index i * ld + j of the complex slot C + b * ld * ld, as
+// the GEMMs write it with c_real); every output keeps zero imaginary parts.
 void launch_band_tridiag(hipStream_t st, double2* C, int n, int ld, int batch, double* D, double* E, double2* TAU,
                          double2* Vc, double2* Wc, double2* Yp, double2* scratch, double2* VT, int* info,
                          bool re = false);

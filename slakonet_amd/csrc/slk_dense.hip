@@ -50,14 +50,15 @@ template <> struct CT<float> {
 // every size from 64^3 x 8 to 2048^3, FP64 and FP32).
 // RE: the operands are real (zero imaginary parts in complex storage): one
 // real MFMA per block instead of four, no imaginary LDS planes, and the
-// result's imaginary parts are written as zero.
+// result's imaginary parts are written as zero, or with creal only its real
+// parts, packed (element (gm, gn) at real index gm * ldc + gn of the slot).
 template <typename T, int TM, bool RE = false>
 __global__ __launch_bounds__(256) void gemm_mfma_kernel(int M, int N, int K, T alpha,
                                                         const typename CT<T>::c2* __restrict__ A, int lda,
                                                         long long sA, int opA,
                                                         const typename CT<T>::c2* __restrict__ B, int ldb,
                                                         long long sB, int opB, T beta, typename CT<T>::c2* C,
-                                                        int ldc, long long sC, int klim, int tri,
+                                                        int ldc, long long sC, int klim, int tri, int creal,
                                                         unsigned long long* clk) {
   using C2 = typename CT<T>::c2;
   using V4 = typename CT<T>::v4;
@@ -186,7 +187,12 @@ __global__ __launch_bounds__(256) void gemm_mfma_kernel(int M, int N, int K, T a
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
         const int gm = m0 + wr + 16 * i + CT<T>::row(lk, v), gn = n0 + wc + 16 * j + li;
-        if (gm < M && gn < N) {
+        if (RE && creal && gm < M && gn < N) {
+          T* cr = reinterpret_cast<T*>(c) + (size_t)gm * ldc + gn;
+          T r = alpha * accR[i][j][v];
+          if (beta != T(0)) r += beta * *cr;
+          *cr = r;
+        } else if (gm < M && gn < N) {
           C2 r;
           r.x = alpha * accR[i][j][v];
           r.y = RE ? T(0) : alpha * accI[i][j][v];
@@ -234,10 +240,11 @@ static int gemm_tile(int batch, int M, int N, int tri) {
 
 void launch_zgemm(hipStream_t st, int batch, int M, int N, int K, double alpha, const double2* A, int lda,
                   long long sA, int opA, const double2* B, int ldb, long long sB, int opB, double beta, double2* C,
-                  int ldc, long long sC, int klim, int tri, bool re) {
+                  int ldc, long long sC, int klim, int tri, bool re, bool c_real) {
   if (M <= 0 || N <= 0 || batch <= 0) return;
   const int tm = gemm_tile(batch, M, N, tri);
   dim3 g((N + tm - 1) / tm, (M + tm - 1) / tm, batch);
+  const int creal = re && c_real;
   SampleSlot sl;
   // a real product is a quarter of the complex flops
   const bool sampled = sampler_enabled() &&
@@ -247,22 +254,23 @@ void launch_zgemm(hipStream_t st, int batch, int M, int N, int K, double alpha, 
                  : (tm == GT ? gemm_mfma_kernel<double, GT> : gemm_mfma_kernel<double, 32>);
   if (sampled)
     hipExtLaunchKernelGGL(kern, g, dim3(256), 0, st, sl.e0, sl.e1, 0, M, N, K, alpha, A, lda, sA, opA, B, ldb, sB,
-                          opB, beta, C, ldc, sC, klim, tri, sl.clk);
+                          opB, beta, C, ldc, sC, klim, tri, creal, sl.clk);
   else
     hipLaunchKernelGGL(kern, g, dim3(256), 0, st, M, N, K, alpha, A, lda, sA, opA, B, ldb, sB, opB, beta, C, ldc, sC,
-                       klim, tri, (unsigned long long*)nullptr);
+                       klim, tri, creal, (unsigned long long*)nullptr);
 }
 
 void launch_cgemm(hipStream_t st, int batch, int M, int N, int K, float alpha, const float2* A, int lda,
                   long long sA, int opA, const float2* B, int ldb, long long sB, int opB, float beta, float2* C,
-                  int ldc, long long sC, int klim, int tri, bool re) {
+                  int ldc, long long sC, int klim, int tri, bool re, bool c_real) {
   if (M <= 0 || N <= 0 || batch <= 0) return;
   const int tm = gemm_tile(batch, M, N, tri);
   dim3 g((N + tm - 1) / tm, (M + tm - 1) / tm, batch);
+  const int creal = re && c_real;
   auto kern = re ? (tm == GT ? gemm_mfma_kernel<float, GT, true> : gemm_mfma_kernel<float, 32, true>)
                  : (tm == GT ? gemm_mfma_kernel<float, GT> : gemm_mfma_kernel<float, 32>);
   hipLaunchKernelGGL(kern, g, dim3(256), 0, st, M, N, K, alpha, A, lda, sA, opA, B, ldb, sB, opB, beta, C, ldc, sC,
-                     klim, tri, (unsigned long long*)nullptr);
+                     klim, tri, creal, (unsigned long long*)nullptr);
 }
 
 __global__ void zero_kernel(uint4* p, size_t n16) {

@@ -1240,8 +1240,9 @@ static int eigh_impl(const void* H, const void* S, int n, int ld, int batch, dou
     // B2 = W H (W lower), B0 = B2 W^H (W^H upper)
     launch_zgemm(stream, batch, n, n, n, 1.0, s.B1, ld, sM, OP_N, h, ld, sM, OP_N, 0.0, s.B2, ld, sM, KLIM_A_LOWER,
                  tri, re);
+    // a real batch bound for the band reduction gets C as packed real values
     launch_zgemm(stream, batch, n, n, n, 1.0, s.B2, ld, sM, OP_N, s.B1, ld, sM, OP_C, 0.0, s.B0, ld, sM,
-                 KLIM_B_UPPER, tri, re);
+                 KLIM_B_UPPER, tri, re, band);
     if (ev) (void)hipEventRecord(ev[3], stream);
     if (band)
       // B2 (W H) is free after the reduction: it holds the split-K partials
@@ -1255,7 +1256,7 @@ static int eigh_impl(const void* H, const void* S, int n, int ld, int batch, dou
     launch_cgemm(stream, batch, n, n, n, 1.0f, s.F0, ld, sM, OP_N, s.F1, ld, sM, OP_N, 0.0f, s.F2, ld, sM,
                  KLIM_A_LOWER, tri, re);
     launch_cgemm(stream, batch, n, n, n, 1.0f, s.F2, ld, sM, OP_N, s.F0, ld, sM, OP_C, 0.0f, s.F1, ld, sM,
-                 KLIM_B_UPPER, tri, re);
+                 KLIM_B_UPPER, tri, re, band);
     if (ev) (void)hipEventRecord(ev[3], stream);
     // F0 (W in FP32) is free after the reduction: it receives the reflector
     // rows; F2 (W32 H32) holds the split-K partials of the band reduction
