@@ -60,6 +60,7 @@ struct BandCtx {
   C2* TAU;       // [batch][ld]: stage-1 tau of column j
   C2* VT;        // nullable [batch][ld][ld]: stage-1 reflector of column j as row j
   C2* BND;       // [batch][ld][BND_LD]: compact band for the chase
+  C2* X16;       // [batch][BW][BW]: X of the first BW rows of the current A22, row-major
   double *D, *E;
   int nch, nrb;  // chunk / row-block counts used for the strides of Yp, Zp
   int* info;     // nullable [batch]: set to 2 when the chase gave up a bounded wait
@@ -603,12 +604,14 @@ __global__ __launch_bounds__(256, 3) void band_av_kernel(BandCtx<T> c, int r0) {
 #pragma unroll
   for (int v = 0; v < 4; ++v) {
     const int gr = row0 + 16 * wv + MF<T>::row(lk, v);
-    if (gr < n) yp[(size_t)gr * BW + li] = mk<T>(accR[v], accI[v]);
+    if (gr < n) a_st<T, RE>(yp, (size_t)gr * BW + li, mk<T>(accR[v], accI[v]));
   }
 }
 
 // Stage 1, X = Y T for a block of 64 rows, and the block's partial V^H X.
-template <typename T>
+// RE: the Y partials are packed real values (element e of a (matrix, chunk)
+// slot at real index e).
+template <typename T, bool RE = false>
 __global__ __launch_bounds__(256) void band_x_kernel(BandCtx<T> c, int r0, int nchk) {
   using C2 = typename C2T<T>::t;
   const int n = c.n, ld = c.ld;
@@ -632,13 +635,24 @@ __global__ __launch_bounds__(256) void band_x_kernel(BandCtx<T> c, int r0, int n
       v[q] = mk<T>(0, 0);
     }
     if (gr < n) {
-      const C2* yp = c.Yp + ((size_t)b * c.nch * ld + gr) * BW + c0;
       const size_t cstride = (size_t)ld * BW;
+      if constexpr (RE) {
+        // 4 adjacent reals: one 16-byte load per chunk
+        const T* yp = reinterpret_cast<const T*>(c.Yp + (size_t)b * c.nch * cstride) + (size_t)gr * BW + c0;
 #pragma unroll 8
-      for (int ch = 0; ch < nchk; ++ch) {
-        const C2* p = yp + ch * cstride;
+        for (int ch = 0; ch < nchk; ++ch) {
+          const T* p = yp + 2 * ch * cstride;
 #pragma unroll
-        for (int q = 0; q < 4; ++q) y[q] = c_add(y[q], p[q]);
+          for (int q = 0; q < 4; ++q) y[q].x += p[q];
+        }
+      } else {
+        const C2* yp = c.Yp + ((size_t)b * c.nch * ld + gr) * BW + c0;
+#pragma unroll 8
+        for (int ch = 0; ch < nchk; ++ch) {
+          const C2* p = yp + ch * cstride;
+#pragma unroll
+          for (int q = 0; q < 4; ++q) y[q] = c_add(y[q], p[q]);
+        }
       }
 #pragma unroll
       for (int q = 0; q < 4; ++q) v[q] = vc[(size_t)(c0 + q) * ld + gr];
@@ -662,6 +676,9 @@ __global__ __launch_bounds__(256) void band_x_kernel(BandCtx<T> c, int r0, int n
     }
     s_x[rl][cc] = x;
     if (gr < n) wc[(size_t)cc * ld + gr] = x;
+    // the strip rows of X, read by every band_w workgroup while workgroup 0
+    // replaces them in Wc by W
+    if (blockIdx.x == 0 && rl < BW) c.X16[(size_t)b * BW * BW + rl * BW + cc] = x;
   }
   __syncthreads();
   const int i = tid / BW, kk = tid % BW;
@@ -674,15 +691,22 @@ __global__ __launch_bounds__(256) void band_x_kernel(BandCtx<T> c, int r0, int n
   c.Zp[((size_t)b * c.nrb + blockIdx.x) * BW * BW + tid] = z;
 }
 
-// Stage 1, W = X - V M / 2 with M = T^H (V^H X) (Hermitian), 64 rows per block.
-template <typename T>
+// Stage 1, W = X - V M / 2 with M = T^H (V^H X) (Hermitian), 64 rows per
+// block, and the strip of the next panel: the first min(m, BW) columns of
+// A22 (origin r0, order m = n - r0) take, on this block's rows, the rank-2k
+// update -= V W_s^H + W V_s^H with V_s, W_s the strip's own rows of V and W.
+// Every workgroup forms W_s = X_s - V_s M / 2 itself from the strip rows X_s
+// that band_x saved, so the next panel's QR can follow this launch directly.
+template <typename T, bool RE = false>
 __global__ __launch_bounds__(256) void band_w_kernel(BandCtx<T> c, int r0, int nrbk) {
   using C2 = typename C2T<T>::t;
   const int n = c.n, ld = c.ld;
   const int b = blockIdx.y;
   const int row0 = r0 + blockIdx.x * 64;
+  const int m = n - r0, ncol = m < BW ? m : BW;
   __shared__ C2 s_T[BW][BW], s_Z[BW][BW], s_M[BW][BW];
-  __shared__ C2 s_v[64][BW + 1];
+  __shared__ C2 s_v[64][BW + 1], s_w[64][BW + 1];
+  __shared__ C2 s_vs[BW][BW + 1], s_ws[BW][BW + 1];
   const int tid = threadIdx.x;
   const int i = tid / BW, kk = tid % BW;
   C2 z = mk<T>(0, 0);
@@ -697,6 +721,8 @@ __global__ __launch_bounds__(256) void band_w_kernel(BandCtx<T> c, int r0, int n
   C2* wc = c.Wc + (size_t)b * 2 * BW * ld;
 #pragma unroll
   for (int q = 0; q < 4; ++q) s_v[rl][c0 + q] = gr < n ? vc[(size_t)(c0 + q) * ld + gr] : mk<T>(0, 0);
+  // strip row i of V (column kk)
+  s_vs[i][kk] = i < m ? vc[(size_t)kk * ld + r0 + i] : mk<T>(0, 0);
   __syncthreads();
   C2 mm = mk<T>(0, 0);
 #pragma unroll
@@ -707,20 +733,56 @@ __global__ __launch_bounds__(256) void band_w_kernel(BandCtx<T> c, int r0, int n
   }
   s_M[i][kk] = mm;
   __syncthreads();
-  if (gr >= n) return;
+  {
+    // strip row i of W
+    C2 x = i < m ? c.X16[(size_t)b * BW * BW + tid] : mk<T>(0, 0);
+#pragma unroll
+    for (int l = 0; l < BW; ++l) {
+      const C2 t = c_mul(s_vs[i][l], s_M[l][kk]);
+      x.x -= T(0.5) * t.x;
+      x.y -= T(0.5) * t.y;
+    }
+    s_ws[i][kk] = x;
+  }
   C2* vw = c.Vc + (size_t)b * 2 * BW * ld + (size_t)BW * ld;   // W also goes after V
 #pragma unroll
   for (int q = 0; q < 4; ++q) {
     const int cc = c0 + q;
-    C2 x = wc[(size_t)cc * ld + gr];
+    C2 x = gr < n ? wc[(size_t)cc * ld + gr] : mk<T>(0, 0);
 #pragma unroll
     for (int l = 0; l < BW; ++l) {
       const C2 t = c_mul(s_v[rl][l], s_M[l][cc]);
       x.x -= T(0.5) * t.x;
       x.y -= T(0.5) * t.y;
     }
-    wc[(size_t)cc * ld + gr] = x;
-    vw[(size_t)cc * ld + gr] = x;
+    s_w[rl][cc] = x;
+    if (gr < n) {
+      wc[(size_t)cc * ld + gr] = x;
+      vw[(size_t)cc * ld + gr] = x;
+    }
+  }
+  __syncthreads();
+  if (gr >= n) return;
+  // the strip: A[gr][r0 + j] -= sum_k V[gr][k] conj(W_s[j][k]) + W[gr][k] conj(V_s[j][k])
+  C2* a = c.A + (size_t)b * ld * ld;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int j = c0 + q;
+    if (j >= ncol) continue;
+    const size_t e = (size_t)gr * ld + r0 + j;
+    C2 acc = a_ld<T, RE>(a, e, true);
+#pragma unroll
+    for (int k = 0; k < BW; ++k) {
+      if constexpr (RE) {
+        acc.x -= s_v[rl][k].x * s_ws[j][k].x + s_w[rl][k].x * s_vs[j][k].x;
+      } else {
+        const C2 t1 = c_cmul(s_ws[j][k], s_v[rl][k]);
+        const C2 t2 = c_cmul(s_vs[j][k], s_w[rl][k]);
+        acc.x -= t1.x + t2.x;
+        acc.y -= t1.y + t2.y;
+      }
+    }
+    a_st<T, RE>(a, e, acc);
   }
 }
 
@@ -1176,7 +1238,7 @@ int band_panels(int ld) { return (ld + BW - 1) / BW; }
 // compact band, Zp, the panels' T factors, the second [V|W], [W|V] pair
 size_t band_scratch_elems(int ld) {
   return (size_t)ld * BND_LD + (size_t)((ld + 63) / 64) * BW * BW + (size_t)band_panels(ld) * BW * BW +
-         (size_t)2 * (2 * BW) * ld;
+         (size_t)2 * (2 * BW) * ld + (size_t)BW * BW;
 }
 size_t band_tt_offset(int ld, int batch) {
   return (size_t)batch * ld * BND_LD + (size_t)batch * ((ld + 63) / 64) * BW * BW;
@@ -1184,62 +1246,12 @@ size_t band_tt_offset(int ld, int batch) {
 
 // ---------------------------------------------------------------------------
 // Look-ahead form of stage 1. Panel p's update of A22(p) is split: the next
-// panel's 16 columns first (strip_update_kernel), so that its QR can run, then
-// the rest, A22(p+1), in one pass that also forms the next panel's product
-// Y = A22(p+1) V_{p+1} (fused_update_av_kernel). A22(p+1) is read and written
-// once per panel instead of read twice and written once.
+// panel's 16 columns first (in band_w_kernel, on the rows of each of its
+// blocks), so that its QR can run, then the rest, A22(p+1), in one pass that
+// also forms the next panel's product Y = A22(p+1) V_{p+1}
+// (fused_update_av_kernel). A22(p+1) is read and written once per panel
+// instead of read twice and written once.
 // ---------------------------------------------------------------------------
-
-// C[0:m, 0:16] -= [V | W] [W | V]^H restricted to the first 16 columns of
-// A22(p) (C at its origin, operands at its first row). One 16 x 16 block per
-// wave, 64 rows per workgroup.
-template <typename T, bool RE = false>
-__global__ __launch_bounds__(256) void strip_update_kernel(int m, const typename C2T<T>::t* __restrict__ Vc,
-                                                           const typename C2T<T>::t* __restrict__ Wc,
-                                                           typename C2T<T>::t* __restrict__ C, long long off, int ld,
-                                                           long long sP, long long sM) {
-  using C2 = typename C2T<T>::t;
-  using V4 = typename MF<T>::v4;
-  constexpr int KK = 2 * BW / 4;
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  const int m0 = blockIdx.x * 64 + 16 * wv;
-  if (m0 >= m) return;
-  const int b = blockIdx.z;
-  const C2* vc = Vc + (size_t)b * sP;
-  const C2* wc = Wc + (size_t)b * sP;
-  C2* c = C + (size_t)b * sM;
-  const int li = lane & 15, lk = lane >> 4;
-  const int ncol = m < BW ? m : BW;
-  V4 accR, accI;
-#pragma unroll
-  for (int v = 0; v < 4; ++v) {
-    const int r = m0 + MF<T>::row(lk, v);
-    const C2 x = a_ld<T, RE>(c, off + (size_t)r * ld + li, r < m && li < ncol);
-    accR[v] = x.x;
-    accI[v] = x.y;
-  }
-  C2 av[KK], bw[KK];
-#pragma unroll
-  for (int q = 0; q < KK; ++q) {
-    const size_t krow = (size_t)(4 * q + lk) * ld;
-    av[q] = load_or_zero(vc + krow + m0 + li, m0 + li < m);
-    bw[q] = load_or_zero(wc + krow + li, li < ncol);
-  }
-#pragma unroll
-  for (int q = 0; q < KK; ++q) {
-    accR = MF<T>::mfma(-av[q].x, bw[q].x, accR);
-    if constexpr (!RE) {
-      accR = MF<T>::mfma(-av[q].y, bw[q].y, accR);
-      accI = MF<T>::mfma(-av[q].y, bw[q].x, accI);
-      accI = MF<T>::mfma(av[q].x, bw[q].y, accI);
-    }
-  }
-#pragma unroll
-  for (int v = 0; v < 4; ++v) {
-    const int r = m0 + MF<T>::row(lk, v);
-    if (r < m && li < ncol) a_st<T, RE>(c, off + (size_t)r * ld + li, mk<T>(accR[v], RE ? T(0) : accI[v]));
-  }
-}
 
 // One 64 x 64 tile (I, J), J <= I, of A22(p+1) (origin r0n, order mn):
 //   1. A -= [V | W]_p [W | V]_p^H on its 32 x 32 blocks touching the lower
@@ -1391,7 +1403,7 @@ __global__ __launch_bounds__(256) void fused_update_av_kernel(const typename C2T
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
       const int r = 64 * I + rw + MF<T>::row(lk, v);
-      if (r < mn) yp[(size_t)(r0n + r) * BW + li] = mk<T>(yR[v], RE ? T(0) : yI[v]);
+      if (r < mn) a_st<T, RE>(yp, (size_t)(r0n + r) * BW + li, mk<T>(yR[v], RE ? T(0) : yI[v]));
     }
   }
   if (I != J) {
@@ -1412,7 +1424,7 @@ __global__ __launch_bounds__(256) void fused_update_av_kernel(const typename C2T
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
       const int r = 64 * J + rw + MF<T>::row(lk, v);
-      if (r < mn) yp[(size_t)(r0n + r) * BW + li] = mk<T>(yR[v], RE ? T(0) : yI[v]);
+      if (r < mn) a_st<T, RE>(yp, (size_t)(r0n + r) * BW + li, mk<T>(yR[v], RE ? T(0) : yI[v]));
     }
   }
   if (clk != nullptr) {
@@ -1474,12 +1486,13 @@ static void band_impl(hipStream_t st, typename C2T<T>::t* A, int n, int ld, int 
   c.nrb = (ld + 63) / 64;
   c.npan = band_panels(ld);
   // scratch: band, then Zp, then the panels' T, then the second [V|W], [W|V]
-  // pair (band_scratch_elems per matrix)
+  // pair, then X16 (band_scratch_elems per matrix)
   c.BND = scratch;
   c.Zp = scratch + (size_t)batch * ld * BND_LD;
   c.TT = c.Zp + (size_t)batch * c.nrb * BW * BW;
   C2* Vc2 = c.TT + (size_t)batch * c.npan * BW * BW;
   C2* Wc2 = Vc2 + (size_t)batch * 2 * BW * ld;
+  c.X16 = Wc2 + (size_t)batch * 2 * BW * ld;
   const long long sP = (long long)2 * BW * ld, sM = (long long)ld * ld;
   // panel p works in the buffer pair of its parity: the next panel's QR
   // writes its V while panel p's [V | W] is still needed
@@ -1493,14 +1506,14 @@ static void band_impl(hipStream_t st, typename C2T<T>::t* A, int n, int ld, int 
     launch_panel_qr<T, RE>(st, cx[0], batch, 0, m);
     const int nrbk = (m + 63) / 64, nchk = (m + AV_KCH - 1) / AV_KCH;
     hipLaunchKernelGGL((band_av_kernel<T, RE>), dim3(nrbk, nchk, batch), dim3(256), 0, st, cx[0], BW);
-    hipLaunchKernelGGL(band_x_kernel<T>, dim3(nrbk, batch), dim3(256), 0, st, cx[0], BW, nchk);
-    hipLaunchKernelGGL(band_w_kernel<T>, dim3(nrbk, batch), dim3(256), 0, st, cx[0], BW, nrbk);
+    hipLaunchKernelGGL((band_x_kernel<T, RE>), dim3(nrbk, batch), dim3(256), 0, st, cx[0], BW, nchk);
+    hipLaunchKernelGGL((band_w_kernel<T, RE>), dim3(nrbk, batch), dim3(256), 0, st, cx[0], BW, nrbk);
     for (int p = 0;; ++p) {
+      // the strip of A22(p) (origin BW (p + 1)) is current: the band_w
+      // launch that formed panel p's W applied it
       const BandCtx<T>& cp = cx[p & 1];
       const BandCtx<T>& cn = cx[(p + 1) & 1];
-      const int r0 = BW * (p + 1), mp = n - r0;       // A22(p)
-      hipLaunchKernelGGL((strip_update_kernel<T, RE>), dim3((mp + 63) / 64, 1, batch), dim3(256), 0, st, mp, cp.Vc + r0,
-                         cp.Wc + r0, A, (long long)r0 * ld + r0, ld, sP, sM);
+      const int r0 = BW * (p + 1);                    // A22(p)
       const int r0n = r0 + BW, mn = n - r0n;          // A22(p + 1)
       if (mn < 1) break;
       launch_panel_qr<T, RE>(st, cn, batch, r0, mn);
@@ -1522,8 +1535,8 @@ static void band_impl(hipStream_t st, typename C2T<T>::t* A, int n, int ld, int 
         hipLaunchKernelGGL((fused_update_av_kernel<T, RE>), dim3(nwg), dim3(256), 0, st, cp.Vc + r0n,
                            cp.Wc + r0n, cn.Vc + r0n, A, c.Yp, n, ld, r0n, c.nch, sP, sM, ntri, batch,
                            (unsigned long long*)nullptr);
-      hipLaunchKernelGGL(band_x_kernel<T>, dim3(t, batch), dim3(256), 0, st, cn, r0n, t);
-      hipLaunchKernelGGL(band_w_kernel<T>, dim3(t, batch), dim3(256), 0, st, cn, r0n, t);
+      hipLaunchKernelGGL((band_x_kernel<T, RE>), dim3(t, batch), dim3(256), 0, st, cn, r0n, t);
+      hipLaunchKernelGGL((band_w_kernel<T, RE>), dim3(t, batch), dim3(256), 0, st, cn, r0n, t);
     }
   }
   // the band (and the schedule) in LDS when it fits, else in global memory
