@@ -39,8 +39,10 @@ def _solve(h, s, precision, real, mode=0, vectors=False):
         lib.slk_set_tridiag_mode(0)
 
 
-CASES = [(24, 6, 0), (96, 4, 0), (640, 2, 0), (1100, 2, 0), (300, 2, 1), (2432, 1, 0)]
-IDS = ["small", "two-stage-lds-band", "two-stage-hbm-band", "two-stage-1100", "one-stage-forced", "one-stage"]
+# 33 and 97: the last trailing block has one row (a one-element reflector)
+CASES = [(24, 6, 0), (33, 4, 0), (96, 4, 0), (97, 3, 0), (640, 2, 0), (1100, 2, 0), (300, 2, 1), (2432, 1, 0)]
+IDS = ["small", "two-stage-33", "two-stage-lds-band", "two-stage-97", "two-stage-hbm-band", "two-stage-1100",
+       "one-stage-forced", "one-stage"]
 
 
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
