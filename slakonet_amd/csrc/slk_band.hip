@@ -1302,19 +1302,32 @@ __global__ __launch_bounds__(256) void fused_update_av_kernel(const typename C2T
   const size_t o = (size_t)r0n * ld + r0n;   // origin of A22(p+1) in the matrix
   __shared__ T Cr[64][65], Ci[RE ? 1 : 64][65];
   __shared__ T VJr[64][BW + 1], VJi[RE ? 1 : 64][BW + 1], VIr[64][BW + 1], VIi[RE ? 1 : 64][BW + 1];
-  // V_{p+1} rows of blocks J and I (zero beyond the matrix)
-  for (int e = tid; e < 64 * BW; e += 256) {
-    const int col = e / 64, k = e % 64;
+  // V_{p+1} rows of blocks J and I (zero beyond the matrix). Real data: stored
+  // to LDS after the rank-2k, so that these loads are in flight together with
+  // the rank-2k's instead of one round trip before them; complex data (twice
+  // the registers) is stored at once
+  constexpr int NV = 64 * BW / 256;
+  C2 xj[NV], xi[NV];
+#pragma unroll
+  for (int it = 0; it < NV; ++it) {
+    const int e = tid + 256 * it, col = e / 64, k = e % 64;
     const int rj = 64 * J + k, ri = 64 * I + k;
-    const C2 xj = load_or_zero(vn + (size_t)col * ld + rj, rj < mn);
-    const C2 xi = load_or_zero(vn + (size_t)col * ld + ri, ri < mn);
-    VJr[k][col] = xj.x;
-    VIr[k][col] = xi.x;
-    if constexpr (!RE) {
-      VJi[k][col] = xj.y;
-      VIi[k][col] = xi.y;
-    }
+    xj[it] = load_or_zero(vn + (size_t)col * ld + rj, rj < mn);
+    xi[it] = load_or_zero(vn + (size_t)col * ld + ri, ri < mn);
   }
+  auto store_v = [&]() {
+#pragma unroll
+    for (int it = 0; it < NV; ++it) {
+      const int e = tid + 256 * it, col = e / 64, k = e % 64;
+      VJr[k][col] = xj[it].x;
+      VIr[k][col] = xi[it].x;
+      if constexpr (!RE) {
+        VJi[k][col] = xj[it].y;
+        VIi[k][col] = xi[it].y;
+      }
+    }
+  };
+  if constexpr (!RE) store_v();
   // 1. rank-2k on this wave's 32 x 32 block
   const int wr = (wv >> 1) * 32, wc = (wv & 1) * 32;
   const int m0 = 64 * I + wr, n0 = 64 * J + wc;
@@ -1372,6 +1385,7 @@ __global__ __launch_bounds__(256) void fused_update_av_kernel(const typename C2T
           if constexpr (!RE) Ci[rl][cl] = accI[i][j][v];
         }
   }
+  if constexpr (RE) store_v();
   __syncthreads();
   // 2. Hermitian completion of a diagonal tile
   if (I == J) {
