@@ -635,6 +635,9 @@ __global__ __launch_bounds__(256) void band_x_kernel(BandCtx<T> c, int r0, int n
       v[q] = mk<T>(0, 0);
     }
     if (gr < n) {
+      // V rows first: in flight together with the partials
+#pragma unroll
+      for (int q = 0; q < 4; ++q) v[q] = vc[(size_t)(c0 + q) * ld + gr];
       const size_t cstride = (size_t)ld * BW;
       if constexpr (RE) {
         // 4 adjacent reals: one 16-byte load per chunk
@@ -654,8 +657,6 @@ __global__ __launch_bounds__(256) void band_x_kernel(BandCtx<T> c, int r0, int n
           for (int q = 0; q < 4; ++q) y[q] = c_add(y[q], p[q]);
         }
       }
-#pragma unroll
-      for (int q = 0; q < 4; ++q) v[q] = vc[(size_t)(c0 + q) * ld + gr];
     }
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
@@ -709,20 +710,31 @@ __global__ __launch_bounds__(256) void band_w_kernel(BandCtx<T> c, int r0, int n
   __shared__ C2 s_vs[BW][BW + 1], s_ws[BW][BW + 1];
   const int tid = threadIdx.x;
   const int i = tid / BW, kk = tid % BW;
+  const int rl = tid >> 2, c0 = (tid & 3) * 4;
+  const int gr = row0 + rl;
+  const C2* vc = c.Vc + (size_t)b * 2 * BW * ld;
+  C2* wc = c.Wc + (size_t)b * 2 * BW * ld;
+  // the loads that do not depend on the Zp sum go first, so that they are in
+  // flight together with it: T, this block's V and X rows, the strip row i of
+  // V (column kk) and of X
+  const C2 tt = c.TT[((size_t)b * c.npan + (r0 - BW) / BW) * BW * BW + tid];
+  C2 vr[4], xr[4];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    vr[q] = gr < n ? vc[(size_t)(c0 + q) * ld + gr] : mk<T>(0, 0);
+    xr[q] = gr < n ? wc[(size_t)(c0 + q) * ld + gr] : mk<T>(0, 0);
+  }
+  const C2 vs = i < m ? vc[(size_t)kk * ld + r0 + i] : mk<T>(0, 0);
+  const C2 xs = i < m ? c.X16[(size_t)b * BW * BW + tid] : mk<T>(0, 0);
   C2 z = mk<T>(0, 0);
   // unrolled: eight independent loads in flight per step of the (unchanged) sum order
 #pragma unroll 8
   for (int q = 0; q < nrbk; ++q) z = c_add(z, c.Zp[((size_t)b * c.nrb + q) * BW * BW + tid]);
   s_Z[i][kk] = z;
-  s_T[i][kk] = c.TT[((size_t)b * c.npan + (r0 - BW) / BW) * BW * BW + tid];
-  const int rl = tid >> 2, c0 = (tid & 3) * 4;
-  const int gr = row0 + rl;
-  const C2* vc = c.Vc + (size_t)b * 2 * BW * ld;
-  C2* wc = c.Wc + (size_t)b * 2 * BW * ld;
+  s_T[i][kk] = tt;
 #pragma unroll
-  for (int q = 0; q < 4; ++q) s_v[rl][c0 + q] = gr < n ? vc[(size_t)(c0 + q) * ld + gr] : mk<T>(0, 0);
-  // strip row i of V (column kk)
-  s_vs[i][kk] = i < m ? vc[(size_t)kk * ld + r0 + i] : mk<T>(0, 0);
+  for (int q = 0; q < 4; ++q) s_v[rl][c0 + q] = vr[q];
+  s_vs[i][kk] = vs;
   __syncthreads();
   C2 mm = mk<T>(0, 0);
 #pragma unroll
@@ -735,7 +747,7 @@ __global__ __launch_bounds__(256) void band_w_kernel(BandCtx<T> c, int r0, int n
   __syncthreads();
   {
     // strip row i of W
-    C2 x = i < m ? c.X16[(size_t)b * BW * BW + tid] : mk<T>(0, 0);
+    C2 x = xs;
 #pragma unroll
     for (int l = 0; l < BW; ++l) {
       const C2 t = c_mul(s_vs[i][l], s_M[l][kk]);
@@ -748,7 +760,7 @@ __global__ __launch_bounds__(256) void band_w_kernel(BandCtx<T> c, int r0, int n
 #pragma unroll
   for (int q = 0; q < 4; ++q) {
     const int cc = c0 + q;
-    C2 x = gr < n ? wc[(size_t)cc * ld + gr] : mk<T>(0, 0);
+    C2 x = xr[q];
 #pragma unroll
     for (int l = 0; l < BW; ++l) {
       const C2 t = c_mul(s_v[rl][l], s_M[l][cc]);
